@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark of the block-structured stencil path on one MI355X.
+
+Default workload (one "step" = one fused lab + 7-point Laplacian + forward
+Euler + FP64 L2-norm pass over the grid): 8x8x8 blocks of 32^3 cells (256^3),
+periodic, ghost width 1, seeded sum of 8 Fourier modes, dt = 0.1 h^2, run in
+FP32 and in FP64 from the same initial field.  `value` is the FP32 path in
+GCell-updates/s; the FP64 path, the FP32-vs-FP64 rel-L2 error, the roofline
+of the dominant kernel (HIP events on the launch stream) and the CPU baseline
+(the C++ oracle on the host cores) are reported in the same JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--workload diffusion|weno5|jacobi|batch]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GCell-updates/s + HBM GB/s % peak, FP32 vs FP64 path at stated rel-L2 error"
+HBM_PEAK_GBS = 8000.0
+
+
+class HipEvents:
+    """hipEvent timing on a given stream (the stream the kernels run on)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipDeviceSynchronize.argtypes = []
+
+    def new(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, stream.handle) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return float(ms.value)
+
+    def device_sync(self):
+        assert self.hip.hipDeviceSynchronize() == 0
+
+
+def roofline(bytes_per_launch, avg_s, traffic=None):
+    ach = bytes_per_launch / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+
+def measured_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+# ------------------------------------------------------------------ diffusion
+
+
+def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup):
+    nb, n, N = (8, 8, 8), (32, 32, 32), 256
+    h = 1.0 / N
+    dt = 0.1 * h * h
+    a = cnv.Cartesian(nb, n, dtype=dtype)
+    b = cnv.Cartesian(nb, n, dtype=dtype)
+    a.upload_global(0, u0.astype(dtype))
+    st = cnv.Stream()
+    norms = cnv.DeviceBuffer(8 * max(1, steps + warmup))
+    src, dst = a, b
+    for k in range(warmup):
+        cnv.diffusion_step(src, dst, dt, sumsq=norms.at(8 * k), stream=st)
+        src, dst = dst, src
+    st.synchronize()
+    # record exactly `steps` steps into one graph
+    g = cnv.Graph(st).capture()
+    s2, d2 = src, dst
+    for k in range(steps):
+        cnv.diffusion_step(s2, d2, dt, sumsq=norms.at(8 * (warmup + k)), stream=st)
+        s2, d2 = d2, s2
+    g.end()
+    e0, e1 = ev.new(), ev.new()
+    ev.device_sync()
+    t0 = time.perf_counter()
+    ev.record(e0, st)
+    g.launch()
+    ev.record(e1, st)
+    st.synchronize()
+    ev.device_sync()
+    wall = time.perf_counter() - t0
+    kern_ms = ev.elapsed_ms(e0, e1)
+    final = s2.download(0)
+    hist = norms.download(np.float64, steps + warmup)
+    cells = 256 ** 3
+    return {
+        "wall_s": wall, "event_ms": kern_ms, "final": final, "norms": hist,
+        "gcups": cells * steps / wall / 1e9,
+        "bytes_per_launch": cells * 2 * np.dtype(dtype).itemsize,
+        "avg_kernel_s": kern_ms / 1e3 / steps,
+    }
+
+
+def cpu_baseline_diffusion(u0, budget_s=10.0):
+    import oracle
+    from cubismnova_amd.layout import global_to_blocks
+    nb, n, N = (8, 8, 8), (32, 32, 32), 256
+    nt = max(1, min(16, os.cpu_count() or 1))
+    slab = global_to_blocks(u0, nb, n)
+    h = [1.0 / N] * 3
+    dt = 0.1 / N / N
+    cur = slab
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        cur, _ = oracle.diffusion_step(oracle.Field(cur, nb, n), h, dt, nthreads=nt)
+        done += 1
+        el = time.perf_counter() - t0
+        if (el >= budget_s and done >= 2) or done >= 400:
+            break
+    return {"value": round(N ** 3 * done / el / 1e9, 4), "unit": "GCell-updates/s", "cores": nt, "kind": "port",
+            "sample": "%d FP64 diffusion steps on the full 256^3 grid (C++ oracle: per-block lab + Laplacian + "
+                      "Euler, OpenMP over blocks with thread-local labs), %.1f s" % (done, el)}
+
+
+def bench_diffusion(args):
+    import cubismnova_amd as cnv
+    from cubismnova_amd import workloads
+    ev = HipEvents()
+    u0 = workloads.diffusion_large(256)
+    r32 = diffusion_gpu(cnv, ev, np.float32, u0, args.steps, args.warmup)
+    r64 = diffusion_gpu(cnv, ev, np.float64, u0, args.steps, args.warmup)
+    f32, f64 = r32["final"].astype(np.float64), r64["final"]
+    rel = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
+    nrel = float(np.max(np.abs(r32["norms"] - r64["norms"]) / r64["norms"]))
+    out = {
+        "metric": METRIC, "value": round(r32["gcups"], 3), "unit": "GCell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded sum of 8 Fourier modes, |k| in [1,16], amplitude 1/|k|)",
+        "config": {"workload": "diffusion: 8x8x8 blocks of 32^3 cells (256^3), periodic, ghost 1, fused "
+                               "lab + 7-point Laplacian + forward Euler (dt=0.1h^2) + FP64 L2 norm per step",
+                   "blocks": [8, 8, 8], "block_cells": [32, 32, 32], "steps_per_graph": args.steps},
+        "roofline": roofline(r32["bytes_per_launch"], r32["avg_kernel_s"],
+                             measured_traffic("stencil7_diffusion_f32_256.json")),
+        "fp64": {"value": round(r64["gcups"], 3), "unit": "GCell-updates/s",
+                 "ms_per_step": round(r64["wall_s"] * 1e3 / args.steps, 5),
+                 "roofline": roofline(r64["bytes_per_launch"], r64["avg_kernel_s"],
+                                      measured_traffic("stencil7_diffusion_f64_256.json"))},
+        "fp32_over_fp64": round(r32["gcups"] / r64["gcups"], 3),
+        "fp32_rel_l2_vs_fp64": rel,
+        "fp32_norm_history_max_rel_dev": nrel,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_diffusion(u0, args.cpu_seconds)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="diffusion", choices=["diffusion"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (got --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1:
+        sys.exit("--steps must be >= 1")
+    out = bench_diffusion(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,474 @@
+"""cubismnova_amd -- MI355X-native block-structured stencil path.
+
+Python mirror of CubismNova's host API for the stencil path (the names follow
+include/Cubism/{Core/Stencil.h, BC/*.h, Grid/Cartesian.h, Block/FieldLab.h});
+every compute call goes through the C ABI of libcubismnova_amd.so
+(include/cubismnova_amd.h).  Storage lives on the GPU in the reference's
+block layout; numpy arrays are only the host side of uploads and downloads.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import CnvError, call, lib
+from .layout import blocks_to_global, global_to_blocks
+
+__all__ = [
+    "Stencil", "Base", "Absorbing", "Dirichlet", "Symmetry", "BoundaryInfo", "Cartesian", "FieldLab",
+    "DeviceBuffer", "Stream", "Graph", "laplacian", "diffusion_step", "grid_sums", "CnvError", "lib",
+    "global_to_blocks", "blocks_to_global",
+]
+
+# ------------------------------------------------------------------ stencil
+
+
+class Stencil:
+    """Core::Stencil<3> (include/Cubism/Core/Stencil.h:20-108)."""
+
+    def __init__(self, begin, end, tensorial=False):
+        b = [int(begin)] * 3 if np.isscalar(begin) else [int(v) for v in begin]
+        e = [int(end)] * 3 if np.isscalar(end) else [int(v) for v in end]
+        # Vector comparisons are all-component (Core/Vector.h:392-420)
+        if all(v > 0 for v in b):
+            raise RuntimeError("Stencil: begin_ must be <= 0")
+        if all(v < 1 for v in e):
+            raise RuntimeError("Stencil: end_ must be > 0")
+        self._b, self._e, self._t = b, e, bool(tensorial)
+
+    def getBegin(self):
+        return list(self._b)
+
+    def getEnd(self):
+        return list(self._e)
+
+    def isTensorial(self):
+        return self._t
+
+    def _c(self):
+        s = _abi.cnv_stencil()
+        for d in range(3):
+            s.begin[d] = self._b[d]
+            s.end[d] = self._e[d]
+        s.tensorial = 1 if self._t else 0
+        return s
+
+
+# ------------------------------------------------------------------ BCs
+
+
+class BoundaryInfo:
+    """BC::BoundaryInfo (include/Cubism/BC/Base.h:21-28)."""
+
+    def __init__(self, dir, side, is_periodic):
+        self.dir, self.side, self.is_periodic = dir, side, is_periodic
+
+
+class Base:
+    """BC::Base (include/Cubism/BC/Base.h:39-79): periodic, no-op apply."""
+    _kind = _abi.CNV_BC_BASE
+    _name = "Base"
+    _periodic = True
+
+    def __init__(self, dir, side):
+        self._info = BoundaryInfo(int(dir), int(side), self._periodic)
+        self._value = 0.0
+
+    def getBoundaryInfo(self):
+        return self._info
+
+    def name(self):
+        return self._name
+
+    def _c(self):
+        return _abi.cnv_bc(self._kind, self._info.dir, self._info.side, 0, float(self._value))
+
+
+class Absorbing(Base):
+    """BC::Absorbing, zeroth-order absorbing (include/Cubism/BC/Absorbing.h:24-105)."""
+    _kind = _abi.CNV_BC_ABSORBING
+    _name = "Zeroth-Order Absorbing"
+    _periodic = False
+
+
+class Dirichlet(Base):
+    """BC::Dirichlet, constant ghost value (include/Cubism/BC/Dirichlet.h:24-114)."""
+    _kind = _abi.CNV_BC_DIRICHLET
+    _name = "Dirichlet"
+    _periodic = False
+
+    def __init__(self, dir, side, value=0.0):
+        super().__init__(dir, side)
+        self._value = value
+
+    def getValue(self):
+        return self._value
+
+    def setValue(self, v):
+        self._value = v
+
+
+class Symmetry(Base):
+    """BC::Symmetry, ghost = sign * mirror (include/Cubism/BC/Symmetry.h:24-115)."""
+    _kind = _abi.CNV_BC_SYMMETRY
+    _name = "Symmetry"
+    _periodic = False
+
+    def __init__(self, dir, side, sign=1.0):
+        super().__init__(dir, side)
+        self._value = sign
+
+
+def _bc_array(bcs):
+    n = len(bcs)
+    arr = (_abi.cnv_bc * max(n, 1))()
+    for i, b in enumerate(bcs):
+        arr[i] = b._c()
+    return arr, n
+
+
+# ------------------------------------------------------------------ device memory
+
+
+class Stream:
+    """A HIP stream owned by the library (hipStreamNonBlocking)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        call("cnv_stream_create", ctypes.byref(h))
+        self.handle = h
+
+    def synchronize(self):
+        call("cnv_stream_synchronize", self.handle)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().cnv_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Graph:
+    """Executable HIP graph recorded from the calls issued on `stream`
+    between capture() and end()."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.handle = None
+
+    def capture(self):
+        call("cnv_graph_begin", self.stream.handle)
+        return self
+
+    def end(self):
+        h = ctypes.c_void_p()
+        call("cnv_graph_end", self.stream.handle, ctypes.byref(h))
+        self.handle = h
+        return self
+
+    def launch(self, stream=None):
+        call("cnv_graph_launch", self.handle, (stream or self.stream).handle)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().cnv_graph_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def _st(stream):
+    return stream.handle if stream is not None else None
+
+
+class DeviceBuffer:
+    """Raw device allocation for lab dumps, norms and other outputs."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        call("cnv_malloc", ctypes.byref(p), ctypes.c_size_t(self.nbytes))
+        self.ptr = p
+
+    def upload(self, arr, stream=None):
+        a = np.ascontiguousarray(arr)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than buffer")
+        call("cnv_memcpy_h2d", self.ptr, a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.nbytes),
+             _st(stream))
+
+    def download(self, dtype, count=None, stream=None):
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dt)
+        call("cnv_memcpy_d2h", out.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(out.nbytes),
+             _st(stream))
+        return out
+
+    def fill(self, dtype, value, count=None, stream=None):
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else int(count)
+        call("cnv_fill", self.ptr, _abi.CNV_F64 if dt == np.float64 else _abi.CNV_F32, float(value),
+             ctypes.c_size_t(n), _st(stream))
+
+    def at(self, byte_offset):
+        return ctypes.c_void_p(self.ptr.value + int(byte_offset))
+
+    def free(self):
+        if self.ptr:
+            lib().cnv_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ grid
+
+
+class _Mesh:
+    """Mesh::StructuredUniform queries used by the path (StructuredUniform.h:56-64,361-365)."""
+
+    def __init__(self, begin, end, cells):
+        self.begin = np.asarray(begin, dtype=np.float64)
+        self.end = np.asarray(end, dtype=np.float64)
+        self.cells = np.asarray(cells)
+        self.h = (self.end - self.begin) / self.cells
+
+    def getCellSize(self, i=0):
+        return self.h.copy()
+
+    def getCoordsCell(self, p):
+        return self.begin + (np.asarray(p, dtype=np.float64) + 0.5) * self.h
+
+
+class Cartesian:
+    """Grid::Cartesian for cell-centred scalar components on the GPU.
+
+    Mirrors include/Cubism/Grid/Cartesian.h:44-550: one allocation, component
+    slabs of block_bytes-aligned blocks, blocks numbered x-fastest.
+    """
+
+    def __init__(self, nblocks, block_cells, begin=0.0, end=1.0, dtype=np.float64, ncomp=1):
+        nb = [int(nblocks)] * 3 if np.isscalar(nblocks) else [int(v) for v in nblocks]
+        bc = [int(block_cells)] * 3 if np.isscalar(block_cells) else [int(v) for v in block_cells]
+        b = [float(begin)] * 3 if np.isscalar(begin) else [float(v) for v in begin]
+        e = [float(end)] * 3 if np.isscalar(end) else [float(v) for v in end]
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.float32, np.float64):
+            raise ValueError("dtype must be float32 or float64")
+        d = _abi.cnv_grid_desc()
+        for k in range(3):
+            d.nblocks[k], d.block_cells[k], d.begin[k], d.end[k] = nb[k], bc[k], b[k], e[k]
+        d.ncomp = int(ncomp)
+        d.dtype = _abi.CNV_F64 if self.dtype == np.float64 else _abi.CNV_F32
+        h = ctypes.c_void_p()
+        call("cnv_grid_create", ctypes.byref(d), ctypes.byref(h))
+        self.handle = h
+        bb, cb = ctypes.c_size_t(), ctypes.c_size_t()
+        call("cnv_grid_info", h, None, ctypes.byref(bb), ctypes.byref(cb))
+        self.block_bytes, self.component_bytes = bb.value, cb.value
+        self.block_stride = self.block_bytes // self.dtype.itemsize
+        self.nblocks, self.block_cells, self.ncomp = nb, bc, int(ncomp)
+        self.nblocks_total = nb[0] * nb[1] * nb[2]
+        self._mesh = _Mesh(b, e, [nb[k] * bc[k] for k in range(3)])
+        self._bcs = {}
+
+    # -- reference queries
+    def size(self):
+        return self.nblocks_total
+
+    def getSize(self):
+        return list(self.nblocks)
+
+    def getBlockCells(self):
+        return list(self.block_cells)
+
+    def getMesh(self):
+        return self._mesh
+
+    def block_flat(self, bi):
+        return bi[0] + self.nblocks[0] * (bi[1] + self.nblocks[1] * bi[2])
+
+    # -- data movement (reference byte layout)
+    def upload(self, comp, slab, stream=None):
+        a = np.ascontiguousarray(slab, dtype=self.dtype)
+        if a.nbytes != self.component_bytes:
+            raise ValueError("slab must hold component_bytes = %d bytes" % self.component_bytes)
+        call("cnv_grid_upload", self.handle, comp, a.ctypes.data_as(ctypes.c_void_p),
+             ctypes.c_size_t(a.nbytes), _st(stream))
+
+    def download(self, comp, stream=None):
+        out = np.empty(self.component_bytes // self.dtype.itemsize, dtype=self.dtype)
+        call("cnv_grid_download", self.handle, comp, out.ctypes.data_as(ctypes.c_void_p),
+             ctypes.c_size_t(out.nbytes), _st(stream))
+        return out
+
+    def upload_global(self, comp, u, stream=None):
+        """Upload a global (Z, Y, X) array into the block layout."""
+        self.upload(comp, global_to_blocks(np.asarray(u, dtype=self.dtype), self.nblocks, self.block_cells,
+                                           self.block_stride), stream)
+
+    def download_global(self, comp, stream=None):
+        return blocks_to_global(self.download(comp, stream), self.nblocks, self.block_cells, self.block_stride)
+
+    def data_ptr(self, comp=0):
+        p = ctypes.c_void_p()
+        call("cnv_grid_data", self.handle, comp, ctypes.byref(p))
+        return p
+
+    # -- BCs (Block::Field::getBC, Block/Field.h:437-442)
+    def setBC(self, block_index, bcs, comp=0):
+        arr, n = _bc_array(bcs)
+        bi = (ctypes.c_int * 3)(*block_index)
+        call("cnv_grid_set_block_bcs", self.handle, comp, bi, arr, n)
+        self._bcs[(comp, tuple(block_index))] = list(bcs)
+
+    def getBC(self, block_index, comp=0):
+        return list(self._bcs.get((comp, tuple(block_index)), []))
+
+    def addFaceBC(self, bc, comp=-1):
+        c = bc._c()
+        call("cnv_grid_add_face_bc", self.handle, comp, ctypes.byref(c))
+        d, s = bc.getBoundaryInfo().dir, bc.getBoundaryInfo().side
+        comps = range(self.ncomp) if comp == -1 else [comp]
+        for cc in comps:
+            for bz in range(self.nblocks[2]):
+                for by in range(self.nblocks[1]):
+                    for bx in range(self.nblocks[0]):
+                        bi = (bx, by, bz)
+                        if bi[d] == (0 if s == 0 else self.nblocks[d] - 1):
+                            self._bcs.setdefault((cc, bi), []).append(bc)
+
+    def clearBC(self, comp=-1):
+        call("cnv_grid_clear_bcs", self.handle, comp)
+        self._bcs = {k: v for k, v in self._bcs.items() if comp != -1 and k[0] != comp}
+
+    def loadLab(self, block_index, lab, comp=0):
+        """Grid::Cartesian::loadLab (Grid/Cartesian.h:411-427)."""
+        lab.loadData(block_index, self, comp=comp)
+
+    def lab_all(self, stencil, comp=0, out=None, stream=None):
+        """Labs of every block (device buffer, blocks consecutive); returns the buffer."""
+        ext = (ctypes.c_int * 3)()
+        sc = stencil._c()
+        call("cnv_lab_extent", self.handle, ctypes.byref(sc), ext)
+        n = ext[0] * ext[1] * ext[2] * self.nblocks_total
+        if out is None:
+            out = DeviceBuffer(n * self.dtype.itemsize)
+        call("cnv_lab_load_all", self.handle, comp, ctypes.byref(sc), out.ptr, _st(stream))
+        return out, list(ext)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().cnv_grid_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ lab
+
+
+class FieldLab:
+    """Block::FieldLab (include/Cubism/Block/FieldLab.h:30-492), assembled on the GPU.
+
+    loadData() runs the device lab kernel and keeps a host copy of the active
+    lab range so that lab[(ix, iy, iz)] (ghost indices allowed) reads it.
+    """
+
+    def __init__(self):
+        self._stencil = None
+        self._alloc = False
+        self._data = None
+        self._field = None
+
+    def allocate(self, stencil, max_extent):
+        self._stencil = stencil
+        self._max = [int(v) for v in max_extent]
+        self._alloc = True
+
+    def isAllocated(self):
+        return self._alloc
+
+    def loadData(self, block_index, grid, comp=0, apply_bc=True, extern_bc=None, poison=None):
+        if not self._alloc:
+            raise RuntimeError("FieldLab: can not load lab data when not allocated first")
+        if any(grid.block_cells[d] > self._max[d] + 2 for d in range(3)):
+            raise RuntimeError("FieldLab: block larger than the allocated maximum range")
+        s = self._stencil
+        sc = s._c()
+        ext = (ctypes.c_int * 3)()
+        call("cnv_lab_extent", grid.handle, ctypes.byref(sc), ext)
+        n = ext[0] * ext[1] * ext[2]
+        buf = DeviceBuffer(n * grid.dtype.itemsize)
+        buf.fill(grid.dtype, np.nan if poison is None else poison)
+        bi = (ctypes.c_int * 3)(*block_index)
+        if extern_bc is None:
+            call("cnv_lab_load", grid.handle, comp, bi, ctypes.byref(sc), 1 if apply_bc else 0, None, -1, buf.ptr,
+                 None)
+        else:
+            arr, nb = _bc_array(extern_bc)
+            call("cnv_lab_load", grid.handle, comp, bi, ctypes.byref(sc), 1 if apply_bc else 0, arr, nb, buf.ptr,
+                 None)
+        self._data = buf.download(grid.dtype).reshape(ext[2], ext[1], ext[0])
+        self._ext = list(ext)
+        self._N = list(grid.block_cells)
+        self._field = (grid, tuple(block_index), comp)
+        return self._data
+
+    def __getitem__(self, p):
+        b = self._stencil.getBegin()
+        return self._data[p[2] - b[2], p[1] - b[1], p[0] - b[0]]
+
+    def __call__(self, ix, iy=0, iz=0):
+        return self[(ix, iy, iz)]
+
+    def array(self):
+        """Active lab range as a (Z, Y, X) array starting at the stencil begin."""
+        return self._data
+
+    def getActiveStencil(self):
+        return self._stencil
+
+    def getActiveRange(self):
+        return [0, 0, 0], list(self._N)
+
+    def getActiveLabRange(self):
+        b = self._stencil.getBegin()
+        e = self._stencil.getEnd()
+        return list(b), [self._N[d] + e[d] - 1 for d in range(3)]
+
+    def getActiveField(self):
+        if self._field is None:
+            raise RuntimeError("FieldLab: no field loaded")
+        return self._field
+
+
+# ------------------------------------------------------------------ operators
+
+
+def laplacian(u, lap, comp_in=0, comp_out=0, stream=None):
+    """7-point Laplacian of u into lap (Laplacian.cpp:110-131 semantics)."""
+    call("cnv_laplacian", u.handle, comp_in, lap.handle, comp_out, _st(stream))
+
+
+def diffusion_step(u, unew, nu_dt, lap=None, sumsq=None, stream=None):
+    """unew = u + nu_dt * lap(u) for every component; optional FP64 sum of unew^2
+    into the device double at `sumsq` (a DeviceBuffer or ctypes.c_void_p)."""
+    p = sumsq.ptr if isinstance(sumsq, DeviceBuffer) else sumsq
+    call("cnv_diffusion_step", u.handle, unew.handle, ctypes.c_double(nu_dt), lap.handle if lap is not None else None,
+         p, _st(stream))
+
+
+def grid_sums(g, comp=0, stream=None):
+    """(sum, sum of squares) of one component, FP64, deterministic."""
+    buf = DeviceBuffer(16)
+    call("cnv_grid_sums", g.handle, comp, buf.ptr, _st(stream))
+    r = buf.download(np.float64, 2, stream)
+    return float(r[0]), float(r[1])

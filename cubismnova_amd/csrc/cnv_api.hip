@@ -1,0 +1,624 @@
+// cnv_api.hip -- host side of the C ABI: errors, device helpers, grid
+// objects, BC lists and argument checking in front of the kernels.
+#include "cnv_internal.h"
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+namespace cnv
+{
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int status, const std::string &msg)
+{
+    set_error(msg);
+    return status;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s failed: %s (%d)", what, hipGetErrorString(e), (int)e);
+    return fail(CNV_ERR_HIP, buf);
+}
+
+int check_stencil(const cnv_stencil *s)
+{
+    if (!s)
+        return fail(CNV_ERR_INVALID, "stencil is NULL");
+    // Core::Stencil::check_ (Core/Stencil.h:99-107)
+    for (int d = 0; d < 3; ++d) {
+        if (s->begin[d] > 0)
+            return fail(CNV_ERR_INVALID, "Stencil: begin_ must be <= 0");
+        if (s->end[d] < 1)
+            return fail(CNV_ERR_INVALID, "Stencil: end_ must be > 0");
+    }
+    return CNV_OK;
+}
+
+int ensure_bc_table(cnv_grid_s *g, int comp)
+{
+    BCTable &t = g->tables[comp];
+    if (t.version == g->bc_version)
+        return CNV_OK;
+    const long long nb = g->nblocks_total;
+    std::vector<DevBCHead> heads((size_t)nb);
+    std::vector<DevBC> entries;
+    bool any = false;
+    for (long long b = 0; b < nb; ++b) {
+        const auto &lst = g->bcs[comp][(size_t)b];
+        heads[(size_t)b].offset = (int)entries.size();
+        heads[(size_t)b].count = (int)lst.size();
+        for (const auto &bc : lst) {
+            entries.push_back({bc.kind, bc.dir, bc.side, 0, bc.value});
+            any = true;
+        }
+    }
+    if (!t.heads)
+        CNV_HIP(hipMalloc((void **)&t.heads, sizeof(DevBCHead) * (size_t)nb));
+    if ((int)entries.size() > t.nentries_cap) {
+        if (t.entries)
+            CNV_HIP(hipFree(t.entries));
+        t.entries = nullptr;
+        t.nentries_cap = (int)entries.size();
+        CNV_HIP(hipMalloc((void **)&t.entries, sizeof(DevBC) * (size_t)t.nentries_cap));
+    }
+    CNV_HIP(hipMemcpy(t.heads, heads.data(), sizeof(DevBCHead) * (size_t)nb, hipMemcpyHostToDevice));
+    if (!entries.empty())
+        CNV_HIP(hipMemcpy(t.entries, entries.data(), sizeof(DevBC) * entries.size(),
+                          hipMemcpyHostToDevice));
+    t.any = any;
+    t.version = g->bc_version;
+    return CNV_OK;
+}
+
+int ensure_partials(cnv_grid_s *g, size_t n)
+{
+    if (!g->ticket) {
+        CNV_HIP(hipMalloc((void **)&g->ticket, 64));
+        CNV_HIP(hipMemset(g->ticket, 0, 64));
+    }
+    if (n > g->partials_cap) {
+        if (g->partials)
+            CNV_HIP(hipFree(g->partials));
+        g->partials = nullptr;
+        CNV_HIP(hipMalloc((void **)&g->partials, sizeof(double) * n));
+        g->partials_cap = n;
+    }
+    return CNV_OK;
+}
+
+static int check_grid(const cnv_grid_s *g)
+{
+    if (!g)
+        return fail(CNV_ERR_INVALID, "grid handle is NULL");
+    return CNV_OK;
+}
+
+static int check_comp(const cnv_grid_s *g, int comp)
+{
+    if (comp < 0 || comp >= g->desc.ncomp)
+        return fail(CNV_ERR_INVALID, "component index out of range");
+    return CNV_OK;
+}
+
+} // namespace cnv
+
+using namespace cnv;
+
+#define CNV_CHECK(expr)                                                                      \
+    do {                                                                                     \
+        int rc_ = (expr);                                                                    \
+        if (rc_ != CNV_OK)                                                                   \
+            return rc_;                                                                      \
+    } while (0)
+
+#define CNV_GUARD_BEGIN try {
+#define CNV_GUARD_END                                                                        \
+    }                                                                                        \
+    catch (const std::bad_alloc &)                                                           \
+    {                                                                                        \
+        return fail(CNV_ERR_ALLOC, "host allocation failed");                                \
+    }                                                                                        \
+    catch (...)                                                                              \
+    {                                                                                        \
+        return fail(CNV_ERR_INVALID, "unexpected exception");                                \
+    }
+
+extern "C" {
+
+const char *cnv_strerror(int status)
+{
+    switch (status) {
+    case CNV_OK:
+        return "success";
+    case CNV_ERR_INVALID:
+        return "invalid argument";
+    case CNV_ERR_ALLOC:
+        return "allocation failed";
+    case CNV_ERR_HIP:
+        return "HIP runtime error";
+    case CNV_ERR_UNSUPPORTED:
+        return "unsupported configuration";
+    case CNV_ERR_NOT_ALLOCATED:
+        return "FieldLab: can not load lab data when not allocated first";
+    case CNV_ERR_SIZE:
+        return "size mismatch";
+    default:
+        return "unknown status";
+    }
+}
+
+const char *cnv_last_error(void) { return g_last_error.c_str(); }
+
+int cnv_device_count(int *count)
+{
+    if (!count)
+        return fail(CNV_ERR_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    *count = n;
+    return CNV_OK;
+}
+
+int cnv_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr)
+        return fail(CNV_ERR_INVALID, "ptr is NULL");
+    *ptr = nullptr;
+    if (bytes == 0)
+        return CNV_OK;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess)
+        return fail(CNV_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return CNV_OK;
+}
+
+int cnv_free(void *ptr)
+{
+    if (ptr)
+        CNV_HIP(hipFree(ptr));
+    return CNV_OK;
+}
+
+int cnv_memcpy_h2d(void *dst, const void *src, size_t bytes, cnv_stream s)
+{
+    CNV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)s));
+    CNV_HIP(hipStreamSynchronize((hipStream_t)s));
+    return CNV_OK;
+}
+
+int cnv_memcpy_d2h(void *dst, const void *src, size_t bytes, cnv_stream s)
+{
+    CNV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)s));
+    CNV_HIP(hipStreamSynchronize((hipStream_t)s));
+    return CNV_OK;
+}
+
+int cnv_memcpy_d2d(void *dst, const void *src, size_t bytes, cnv_stream s)
+{
+    CNV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)s));
+    return CNV_OK;
+}
+
+int cnv_fill(void *dst, int dtype, double value, size_t count, cnv_stream s)
+{
+    if (!dst && count)
+        return fail(CNV_ERR_INVALID, "dst is NULL");
+    if (dtype != CNV_F32 && dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    return launch_fill(dst, dtype, value, count, (hipStream_t)s);
+}
+
+int cnv_stream_create(cnv_stream *s)
+{
+    if (!s)
+        return fail(CNV_ERR_INVALID, "stream pointer is NULL");
+    hipStream_t st;
+    CNV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *s = (cnv_stream)st;
+    return CNV_OK;
+}
+
+int cnv_stream_destroy(cnv_stream s)
+{
+    if (s)
+        CNV_HIP(hipStreamDestroy((hipStream_t)s));
+    return CNV_OK;
+}
+
+int cnv_stream_synchronize(cnv_stream s)
+{
+    CNV_HIP(hipStreamSynchronize((hipStream_t)s));
+    return CNV_OK;
+}
+
+// ---------------------------------------------------------------- grids
+
+int cnv_grid_create(const cnv_grid_desc *desc, cnv_grid *out)
+{
+    CNV_GUARD_BEGIN
+    if (!desc || !out)
+        return fail(CNV_ERR_INVALID, "descriptor or output handle is NULL");
+    *out = nullptr;
+    for (int d = 0; d < 3; ++d) {
+        if (desc->nblocks[d] < 1 || desc->block_cells[d] < 1)
+            return fail(CNV_ERR_INVALID, "nblocks and block_cells must be >= 1");
+        if (!(desc->end[d] > desc->begin[d]))
+            return fail(CNV_ERR_INVALID, "mesh end must exceed begin");
+    }
+    if (desc->ncomp < 1)
+        return fail(CNV_ERR_INVALID, "ncomp must be >= 1");
+    if (desc->dtype != CNV_F32 && desc->dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    const long long cells = (long long)desc->block_cells[0] * desc->block_cells[1] * desc->block_cells[2];
+    const long long nbt = (long long)desc->nblocks[0] * desc->nblocks[1] * desc->nblocks[2];
+    if (cells > (1LL << 30) || nbt > (1LL << 30))
+        return fail(CNV_ERR_INVALID, "grid too large");
+
+    cnv_grid_s *g = new cnv_grid_s();
+    g->desc = *desc;
+    g->elem = desc->dtype == CNV_F64 ? 8 : 4;
+    // Cartesian::alloc_ (Grid/Cartesian.h:490-529): 32-byte aligned blocks
+    g->block_bytes = ((size_t)cells * g->elem + 31) / 32 * 32;
+    g->nblocks_total = nbt;
+    g->component_bytes = g->block_bytes * (size_t)nbt;
+    for (int d = 0; d < 3; ++d)
+        g->h[d] = (desc->end[d] - desc->begin[d]) / (double)(desc->nblocks[d] * desc->block_cells[d]);
+    g->bcs.assign(desc->ncomp, std::vector<std::vector<cnv_bc>>((size_t)nbt));
+    g->tables.resize(desc->ncomp);
+    const size_t all = g->component_bytes * (size_t)desc->ncomp;
+    hipError_t e = hipMalloc(&g->data, all);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(CNV_ERR_ALLOC, std::string("grid allocation: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return CNV_OK;
+    CNV_GUARD_END
+}
+
+int cnv_grid_destroy(cnv_grid g)
+{
+    if (!g)
+        return CNV_OK;
+    (void)hipFree(g->data);
+    for (auto &t : g->tables) {
+        if (t.heads)
+            (void)hipFree(t.heads);
+        if (t.entries)
+            (void)hipFree(t.entries);
+    }
+    if (g->partials)
+        (void)hipFree(g->partials);
+    if (g->ticket)
+        (void)hipFree(g->ticket);
+    delete g;
+    return CNV_OK;
+}
+
+int cnv_grid_info(cnv_grid g, cnv_grid_desc *desc, size_t *block_bytes, size_t *component_bytes)
+{
+    CNV_CHECK(check_grid(g));
+    if (desc)
+        *desc = g->desc;
+    if (block_bytes)
+        *block_bytes = g->block_bytes;
+    if (component_bytes)
+        *component_bytes = g->component_bytes;
+    return CNV_OK;
+}
+
+int cnv_grid_data(cnv_grid g, int comp, void **device_ptr)
+{
+    CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_comp(g, comp));
+    if (!device_ptr)
+        return fail(CNV_ERR_INVALID, "device_ptr is NULL");
+    *device_ptr = (char *)g->data + (size_t)comp * g->component_bytes;
+    return CNV_OK;
+}
+
+int cnv_grid_upload(cnv_grid g, int comp, const void *host, size_t bytes, cnv_stream s)
+{
+    CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_comp(g, comp));
+    if (bytes != g->component_bytes)
+        return fail(CNV_ERR_SIZE, "upload size must equal component_bytes");
+    if (!host)
+        return fail(CNV_ERR_INVALID, "host pointer is NULL");
+    CNV_HIP(hipMemcpyAsync((char *)g->data + (size_t)comp * g->component_bytes, host, bytes,
+                           hipMemcpyHostToDevice, (hipStream_t)s));
+    CNV_HIP(hipStreamSynchronize((hipStream_t)s));
+    return CNV_OK;
+}
+
+int cnv_grid_download(cnv_grid g, int comp, void *host, size_t bytes, cnv_stream s)
+{
+    CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_comp(g, comp));
+    if (bytes != g->component_bytes)
+        return fail(CNV_ERR_SIZE, "download size must equal component_bytes");
+    if (!host)
+        return fail(CNV_ERR_INVALID, "host pointer is NULL");
+    CNV_HIP(hipMemcpyAsync(host, (char *)g->data + (size_t)comp * g->component_bytes, bytes,
+                           hipMemcpyDeviceToHost, (hipStream_t)s));
+    CNV_HIP(hipStreamSynchronize((hipStream_t)s));
+    return CNV_OK;
+}
+
+// ---------------------------------------------------------------- BCs
+
+static int check_bc(const cnv_bc &b)
+{
+    if (b.kind < CNV_BC_BASE || b.kind > CNV_BC_SYMMETRY)
+        return fail(CNV_ERR_INVALID, "unknown BC kind");
+    if (b.dir < 0 || b.dir > 2)
+        return fail(CNV_ERR_INVALID, "BC dir must be 0, 1 or 2");
+    if (b.side != 0 && b.side != 1)
+        return fail(CNV_ERR_INVALID, "BC side must be 0 or 1");
+    return CNV_OK;
+}
+
+int cnv_grid_set_block_bcs(cnv_grid g, int comp, const int block[3], const cnv_bc *bcs, int n)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_comp(g, comp));
+    if (!block || n < 0 || (n > 0 && !bcs))
+        return fail(CNV_ERR_INVALID, "bad block index or BC list");
+    for (int d = 0; d < 3; ++d)
+        if (block[d] < 0 || block[d] >= g->desc.nblocks[d])
+            return fail(CNV_ERR_INVALID, "block index out of range");
+    for (int k = 0; k < n; ++k)
+        CNV_CHECK(check_bc(bcs[k]));
+    std::lock_guard<std::mutex> lk(g->mtx);
+    const long long b = block[0] + (long long)g->desc.nblocks[0] * (block[1] + (long long)g->desc.nblocks[1] * block[2]);
+    g->bcs[comp][(size_t)b].assign(bcs, bcs + n);
+    ++g->bc_version;
+    return CNV_OK;
+    CNV_GUARD_END
+}
+
+int cnv_grid_add_face_bc(cnv_grid g, int comp, const cnv_bc *bc)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(g));
+    if (!bc)
+        return fail(CNV_ERR_INVALID, "bc is NULL");
+    CNV_CHECK(check_bc(*bc));
+    if (comp != -1)
+        CNV_CHECK(check_comp(g, comp));
+    std::lock_guard<std::mutex> lk(g->mtx);
+    const int *nb = g->desc.nblocks;
+    for (int c = 0; c < g->desc.ncomp; ++c) {
+        if (comp != -1 && c != comp)
+            continue;
+        for (int bz = 0; bz < nb[2]; ++bz)
+            for (int by = 0; by < nb[1]; ++by)
+                for (int bx = 0; bx < nb[0]; ++bx) {
+                    const int bi[3] = {bx, by, bz};
+                    const int want = bc->side == 0 ? 0 : nb[bc->dir] - 1;
+                    if (bi[bc->dir] != want)
+                        continue;
+                    const long long b = bx + (long long)nb[0] * (by + (long long)nb[1] * bz);
+                    g->bcs[c][(size_t)b].push_back(*bc);
+                }
+    }
+    ++g->bc_version;
+    return CNV_OK;
+    CNV_GUARD_END
+}
+
+int cnv_grid_clear_bcs(cnv_grid g, int comp)
+{
+    CNV_CHECK(check_grid(g));
+    if (comp != -1)
+        CNV_CHECK(check_comp(g, comp));
+    std::lock_guard<std::mutex> lk(g->mtx);
+    for (int c = 0; c < g->desc.ncomp; ++c)
+        if (comp == -1 || c == comp)
+            for (auto &l : g->bcs[c])
+                l.clear();
+    ++g->bc_version;
+    return CNV_OK;
+}
+
+// ---------------------------------------------------------------- lab
+
+int cnv_lab_extent(cnv_grid g, const cnv_stencil *s, int extent[3])
+{
+    CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_stencil(s));
+    if (!extent)
+        return fail(CNV_ERR_INVALID, "extent is NULL");
+    for (int d = 0; d < 3; ++d)
+        extent[d] = g->desc.block_cells[d] + s->end[d] - 1 - s->begin[d];
+    return CNV_OK;
+}
+
+static int check_lab_widths(const cnv_grid_s *g, const cnv_stencil *s)
+{
+    for (int d = 0; d < 3; ++d)
+        if (-s->begin[d] > g->desc.block_cells[d] || s->end[d] - 1 > g->desc.block_cells[d])
+            return fail(CNV_ERR_UNSUPPORTED, "ghost width larger than the block extent");
+    return CNV_OK;
+}
+
+int cnv_lab_load(cnv_grid g, int comp, const int block[3], const cnv_stencil *s, int apply_bc,
+                 const cnv_bc *extern_bcs, int n_extern, void *out, cnv_stream st)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_comp(g, comp));
+    CNV_CHECK(check_stencil(s));
+    CNV_CHECK(check_lab_widths(g, s));
+    if (!block || !out)
+        return fail(CNV_ERR_INVALID, "block index or output is NULL");
+    for (int d = 0; d < 3; ++d)
+        if (block[d] < 0 || block[d] >= g->desc.nblocks[d])
+            return fail(CNV_ERR_INVALID, "block index out of range");
+    const std::vector<cnv_bc> *lst;
+    std::vector<cnv_bc> ext;
+    if (n_extern >= 0) {
+        if (n_extern > 0 && !extern_bcs)
+            return fail(CNV_ERR_INVALID, "extern BC list is NULL");
+        for (int k = 0; k < n_extern; ++k)
+            CNV_CHECK(check_bc(extern_bcs[k]));
+        ext.assign(extern_bcs, extern_bcs + n_extern);
+        lst = &ext;
+    } else {
+        const long long b = block[0] + (long long)g->desc.nblocks[0] * (block[1] + (long long)g->desc.nblocks[1] * block[2]);
+        lst = &g->bcs[comp][(size_t)b];
+    }
+    if (lst->size() > 16)
+        return fail(CNV_ERR_UNSUPPORTED, "more than 16 BCs on one block");
+    DevBC list[16];
+    for (size_t k = 0; k < lst->size(); ++k)
+        list[k] = {(*lst)[k].kind, (*lst)[k].dir, (*lst)[k].side, 0, (*lst)[k].value};
+    return launch_lab(g, comp, *s, apply_bc != 0, true, list, (int)lst->size(), block, out,
+                      (hipStream_t)st);
+    CNV_GUARD_END
+}
+
+int cnv_lab_load_all(cnv_grid g, int comp, const cnv_stencil *s, void *out, cnv_stream st)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_comp(g, comp));
+    CNV_CHECK(check_stencil(s));
+    CNV_CHECK(check_lab_widths(g, s));
+    if (!out)
+        return fail(CNV_ERR_INVALID, "output is NULL");
+    {
+        std::lock_guard<std::mutex> lk(g->mtx);
+        CNV_CHECK(ensure_bc_table(g, comp));
+    }
+    return launch_lab(g, comp, *s, true, false, nullptr, 0, nullptr, out, (hipStream_t)st);
+    CNV_GUARD_END
+}
+
+// ---------------------------------------------------------------- operators
+
+static int same_shape(const cnv_grid_s *a, const cnv_grid_s *b)
+{
+    for (int d = 0; d < 3; ++d)
+        if (a->desc.nblocks[d] != b->desc.nblocks[d] || a->desc.block_cells[d] != b->desc.block_cells[d])
+            return fail(CNV_ERR_SIZE, "grids differ in block count or block size");
+    if (a->desc.dtype != b->desc.dtype)
+        return fail(CNV_ERR_SIZE, "grids differ in dtype");
+    return CNV_OK;
+}
+
+int cnv_laplacian(cnv_grid in, int comp_in, cnv_grid out, int comp_out, cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(in));
+    CNV_CHECK(check_grid(out));
+    CNV_CHECK(check_comp(in, comp_in));
+    CNV_CHECK(check_comp(out, comp_out));
+    CNV_CHECK(same_shape(in, out));
+    if (in == out)
+        return fail(CNV_ERR_INVALID, "input and output grids must differ");
+    return launch_stencil7(in, out, nullptr, comp_in, comp_out, 1, 0, 0.0, nullptr, (hipStream_t)s);
+    CNV_GUARD_END
+}
+
+int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_out, double *sumsq,
+                       cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_grid(u_new));
+    CNV_CHECK(same_shape(u, u_new));
+    if (u == u_new)
+        return fail(CNV_ERR_INVALID, "u and u_new must be different grids");
+    if (u_new->desc.ncomp < u->desc.ncomp)
+        return fail(CNV_ERR_SIZE, "u_new has fewer components than u");
+    if (lap_out) {
+        CNV_CHECK(same_shape(u, lap_out));
+        if (lap_out->desc.ncomp < u->desc.ncomp || lap_out == u || lap_out == u_new)
+            return fail(CNV_ERR_INVALID, "lap_out must be a separate grid with enough components");
+    }
+    return launch_stencil7(u, u_new, lap_out, 0, 0, u->desc.ncomp, lap_out ? 2 : 1, nu_dt, sumsq,
+                           (hipStream_t)s);
+    CNV_GUARD_END
+}
+
+int cnv_grid_sums(cnv_grid g, int comp, double *out, cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_comp(g, comp));
+    if (!out)
+        return fail(CNV_ERR_INVALID, "out is NULL");
+    return launch_sums(g, comp, out, (hipStream_t)s);
+    CNV_GUARD_END
+}
+
+// ---------------------------------------------------------------- graphs
+
+struct cnv_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int cnv_graph_begin(cnv_stream s)
+{
+    if (!s)
+        return fail(CNV_ERR_INVALID, "graph capture needs a non-default stream");
+    CNV_HIP(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeThreadLocal));
+    return CNV_OK;
+}
+
+int cnv_graph_end(cnv_stream s, cnv_graph *out)
+{
+    CNV_GUARD_BEGIN
+    if (!s || !out)
+        return fail(CNV_ERR_INVALID, "stream or output is NULL");
+    *out = nullptr;
+    hipGraph_t gr = nullptr;
+    CNV_HIP(hipStreamEndCapture((hipStream_t)s, &gr));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(gr);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    cnv_graph_s *g = new cnv_graph_s();
+    g->graph = gr;
+    g->exec = ex;
+    *out = g;
+    return CNV_OK;
+    CNV_GUARD_END
+}
+
+int cnv_graph_launch(cnv_graph g, cnv_stream s)
+{
+    if (!g)
+        return fail(CNV_ERR_INVALID, "graph is NULL");
+    CNV_HIP(hipGraphLaunch(g->exec, (hipStream_t)s));
+    return CNV_OK;
+}
+
+int cnv_graph_destroy(cnv_graph g)
+{
+    if (!g)
+        return CNV_OK;
+    if (g->exec)
+        (void)hipGraphExecDestroy(g->exec);
+    if (g->graph)
+        (void)hipGraphDestroy(g->graph);
+    delete g;
+    return CNV_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,157 @@
+// cnv_internal.h -- shared host/device definitions of libcubismnova_amd.
+#pragma once
+
+#include "../../include/cubismnova_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace cnv
+{
+
+// ------------------------------------------------------------ errors
+
+void set_error(const std::string &msg);
+int fail(int status, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define CNV_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return ::cnv::hip_fail(e_, #call);                                               \
+    } while (0)
+
+// ------------------------------------------------------------ device-side tables
+
+// One BC entry as the kernels read it.
+struct DevBC {
+    int kind;
+    int dir;
+    int side;
+    int pad;
+    double value;
+};
+
+// Per-block header into the flattened BC entry array.
+struct DevBCHead {
+    int offset;
+    int count;
+};
+
+// Face of a block as the non-tensorial operator kernels see it after the
+// block's BC list is resolved for a given stencil.
+enum FaceKind { FACE_NBR = 0, FACE_ABSORB = 1, FACE_DIRICHLET = 2, FACE_SYMMETRY = 3, FACE_UNSET = 4 };
+
+// Geometry of a grid as passed to kernels.
+struct Geom {
+    int nb[3];        // blocks per dimension
+    int n[3];         // cells per block per dimension
+    long long bstride; // elements between blocks
+    long long cstride; // elements between component slabs
+};
+
+// ------------------------------------------------------------ host grid object
+
+struct BCTable {
+    uint64_t version = ~0ull; // BC version this table was built from
+    DevBCHead *heads = nullptr;
+    DevBC *entries = nullptr;
+    int nentries_cap = 0;
+    bool any = false; // false: no block of this component carries a BC
+};
+
+} // namespace cnv
+
+struct cnv_grid_s {
+    cnv_grid_desc desc;
+    size_t elem = 0;            // sizeof(T)
+    size_t block_bytes = 0;
+    size_t component_bytes = 0;
+    long long nblocks_total = 0;
+    void *data = nullptr;       // device allocation, ncomp component slabs
+    double h[3] = {0, 0, 0};    // mesh spacing (extent / cells)
+
+    // BC lists per component per block (host truth)
+    std::vector<std::vector<std::vector<cnv_bc>>> bcs;
+    uint64_t bc_version = 0;
+    std::vector<cnv::BCTable> tables; // device copies per component
+
+    // scratch for deterministic reductions: per-tile partial sums + ticket
+    double *partials = nullptr;
+    size_t partials_cap = 0;
+    unsigned *ticket = nullptr;
+
+    std::mutex mtx;
+};
+
+namespace cnv
+{
+
+inline cnv::Geom geom_of(const cnv_grid_s *g)
+{
+    Geom G;
+    for (int d = 0; d < 3; ++d) {
+        G.nb[d] = g->desc.nblocks[d];
+        G.n[d] = g->desc.block_cells[d];
+    }
+    G.bstride = (long long)(g->block_bytes / g->elem);
+    G.cstride = (long long)(g->component_bytes / g->elem);
+    return G;
+}
+
+// Upload (if stale) the BC table of component comp; returns status.
+int ensure_bc_table(cnv_grid_s *g, int comp);
+// Make sure the reduction scratch holds at least n partials.
+int ensure_partials(cnv_grid_s *g, size_t n);
+
+int check_stencil(const cnv_stencil *s);
+
+// Resolve one block's ordered BC list into six faces for a non-tensorial
+// stencil; shared by host and device.  Rule (FieldLab.h:206-223 with the
+// BC slabs of BC/*.h): the face (dir, side) starts as a neighbour copy unless
+// the last BC of `dir` is non-periodic with that side (then unwritten); every
+// non-Base BC on that face whose stencil has width in `dir`
+// (BC/Base.h:73-78) overwrites the whole slab, so the last one decides.
+__host__ __device__ inline void resolve_faces(const DevBC *list, int n, const int sb[3],
+                                              const int se[3], int kind[6], double value[6])
+{
+    bool periodic[3] = {true, true, true};
+    int skip[3] = {1, 1, 1};
+    for (int k = 0; k < n; ++k) {
+        periodic[list[k].dir] = (list[k].kind == CNV_BC_BASE);
+        skip[list[k].dir] = (list[k].side == 0) ? -1 : 1;
+    }
+    for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1, side = f & 1;
+        kind[f] = (periodic[dir] || skip[dir] != (side == 0 ? -1 : 1)) ? FACE_NBR : FACE_UNSET;
+        value[f] = 0.0;
+    }
+    for (int k = 0; k < n; ++k) {
+        const int dir = list[k].dir;
+        if (list[k].kind == CNV_BC_BASE || sb[dir] == 0 || se[dir] == 1)
+            continue;
+        const int f = 2 * dir + list[k].side;
+        kind[f] = list[k].kind; // ABSORBING=1, DIRICHLET=2, SYMMETRY=3 match FaceKind
+        value[f] = list[k].value;
+    }
+}
+
+// ------------------------------------------------------------ kernel launchers
+// (defined in the .hip files; all return a cnv_status)
+
+int launch_lab(const cnv_grid_s *g, int comp, const cnv_stencil &s, bool apply_bc, bool use_list,
+               const DevBC *list, int nlist, const int *block, void *out, hipStream_t st);
+
+int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
+                    int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st);
+
+int launch_sums(cnv_grid_s *g, int comp, double *out, hipStream_t st);
+
+int launch_fill(void *dst, int dtype, double value, size_t count, hipStream_t st);
+
+} // namespace cnv

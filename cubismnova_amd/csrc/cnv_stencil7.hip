@@ -1,0 +1,702 @@
+// cnv_stencil7.hip -- 7-point Laplacian / forward-Euler diffusion over the
+// ghost-extended blocks of a grid, with the lab assembled on the fly.
+//
+// Layout of the work: a workgroup owns one x-y plane of one block and marches
+// through a chunk of z-planes.  Each plane (plus its one-cell x/y halo, read
+// straight from the face-neighbour blocks or filled from the block's BCs) is
+// staged in LDS; the z neighbours come from a register queue of planes that is
+// loaded two planes ahead, so HBM reads of the next planes overlap the
+// arithmetic of the current one.  No lab buffer is ever written to HBM.
+//
+// Arithmetic follows test/sandbox/Benchmarks/Kernel/Laplacian/Laplacian.cpp
+// :125-129 operation for operation: ddx = ihx2*((u[+x] - 2u) + u[-x]), same
+// for y and z, lap = (ddx + ddy) + ddz; the Euler update is lap*nu_dt then
+// u + that (Block/Field.h:457-512, FieldOperator.h:43-114).  The library is
+// compiled with -ffp-contract=off so no step is fused into an FMA and FP64
+// results are bit-identical to the CPU evaluation of the same order.
+#include "cnv_internal.h"
+
+#include <cstdlib>
+#include <map>
+#include <tuple>
+
+namespace cnv
+{
+
+template <typename T, int N>
+struct alignas(16) VecT {
+    T v[N];
+};
+
+constexpr int kMaxComp = 8;
+
+struct S7Args {
+    const char *u;   // input grid base (component slabs)
+    char *out;       // output grid base
+    char *lap;       // optional Laplacian grid base
+    Geom G;
+    int comp_in, comp_out;
+    int ncomp;
+    int nzc;         // z-chunks per block
+    int zchunk;      // planes per chunk
+    int tiles_per_comp;
+    int has_bc[kMaxComp];
+    const DevBCHead *heads[kMaxComp];
+    const DevBC *entries[kMaxComp];
+    double ih2[3];
+    double nudt;
+    double *partials;
+    unsigned *ticket;
+    double *sumsq;
+};
+
+// Workgroup i is dispatched to XCD i % 8; give each XCD a contiguous range
+// of tiles so that the tiles sharing halo lines meet in the same L2.
+__device__ inline int xcd_remap(int i, int n)
+{
+    const int x = i & 7, local = i >> 3;
+    const int q = n >> 3, r = n & 7;
+    return x * q + (x < r ? x : r) + local;
+}
+
+// Per-face source description for one block (uniform in a workgroup).
+template <typename T>
+struct FaceSrc {
+    const T *p;  // base such that ghost(i) = p[i] (copy) or sign*p[i]
+    int mode;    // 0 copy, 1 scaled, 2 constant
+    T s;         // sign or constant
+};
+
+template <typename T>
+__device__ inline T face_val(const FaceSrc<T> &f, long long i)
+{
+    if (f.mode == 2)
+        return f.s;
+    const T v = f.p[i];
+    return f.mode == 1 ? f.s * v : v;
+}
+
+// Resolve the six faces of block `blk` of component `comp` for Stencil(-1,2).
+template <typename T>
+__device__ inline void block_faces(const S7Args &a, int comp, long long blk, const T *cbase,
+                                   FaceSrc<T> F[6])
+{
+    const int *nb = a.G.nb, *n = a.G.n;
+    const int bx = (int)(blk % nb[0]);
+    const int by = (int)((blk / nb[0]) % nb[1]);
+    const int bz = (int)(blk / ((long long)nb[0] * nb[1]));
+    int kind[6];
+    double val[6];
+    const int sb[3] = {-1, -1, -1}, se[3] = {2, 2, 2};
+    if (a.has_bc[comp]) {
+        const DevBCHead h = a.heads[comp][blk];
+        resolve_faces(a.entries[comp] + h.offset, h.count, sb, se, kind, val);
+    } else {
+        for (int f = 0; f < 6; ++f) {
+            kind[f] = FACE_NBR;
+            val[f] = 0.0;
+        }
+    }
+    const long long nx = n[0], nxy = (long long)n[0] * n[1];
+    const T *self = cbase + blk * a.G.bstride;
+    for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1, side = f & 1;
+        int b[3] = {bx, by, bz};
+        b[dir] = (b[dir] + (side ? 1 : -1) + nb[dir]) % nb[dir];
+        const long long nbr = b[0] + (long long)nb[0] * (b[1] + (long long)nb[1] * b[2]);
+        const T *src = kind[f] == FACE_NBR ? cbase + nbr * a.G.bstride : self;
+        // index of the source line/plane along dir
+        const int li = kind[f] == FACE_NBR ? (side ? 0 : n[dir] - 1) : (side ? n[dir] - 1 : 0);
+        const long long step = dir == 0 ? 1 : (dir == 1 ? nx : nxy);
+        F[f].p = src + li * step;
+        F[f].mode = kind[f] == FACE_DIRICHLET ? 2 : (kind[f] == FACE_SYMMETRY ? 1 : 0);
+        F[f].s = (T)val[f];
+    }
+}
+
+template <typename T, int NX, int NY, int WG, int MODE, bool SUMSQ>
+__global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
+{
+    constexpr int VEC = 16 / (int)sizeof(T);
+    constexpr int TX = NX / VEC;      // threads per row
+    constexpr int ROWS = WG / TX;     // rows per pass
+    constexpr int R = NY / ROWS;      // rows per thread
+    constexpr int P = NX + 2 * VEC;   // LDS row pitch (interior starts at VEC)
+    constexpr int PLANE = (NY + 2) * P;
+    static_assert(NX % VEC == 0 && WG % TX == 0 && NY % ROWS == 0, "tile shape");
+    static_assert(WG >= 2 * NY && WG >= 2 * TX, "halo roles need enough threads");
+    using V = VecT<T, VEC>;
+
+    __shared__ alignas(16) T lds[2][PLANE];
+    __shared__ double red[WG / 64];
+
+    const int tid = threadIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int comp = tile / a.tiles_per_comp;
+    const int tt = tile - comp * a.tiles_per_comp;
+    const long long blk = tt / a.nzc;
+    const int zc = tt - (int)blk * a.nzc;
+    const int NZ = a.G.n[2];
+    const int z0 = zc * a.zchunk;
+    const int z1 = min(NZ, z0 + a.zchunk);
+
+    const T *cbase = reinterpret_cast<const T *>(a.u) + (long long)(a.comp_in + comp) * a.G.cstride;
+    const T *ub = cbase + blk * a.G.bstride;
+    T *ob = reinterpret_cast<T *>(a.out) + (long long)(a.comp_out + comp) * a.G.cstride + blk * a.G.bstride;
+    T *lb = MODE == 2 ? reinterpret_cast<T *>(a.lap) + (long long)(a.comp_out + comp) * a.G.cstride +
+                            blk * a.G.bstride
+                      : nullptr;
+
+    FaceSrc<T> F[6];
+    block_faces<T>(a, comp, blk, cbase, F);
+
+    const int tx = tid % TX, ty = tid / TX;
+    const int x0 = tx * VEC;
+    constexpr long long SXY = (long long)NX * NY;
+
+    // halo roles: x-ghost scalar (tid < 2NY), y-ghost vector (tid < 2TX)
+    const bool hx_on = tid < 2 * NY;
+    const int hx_side = tid / NY, hx_y = tid % NY;
+    const bool hy_on = tid < 2 * TX;
+    const int hy_side = tid / TX, hy_x = (tid % TX) * VEC;
+
+    struct Plane {
+        V c[R];
+        T hx;
+        V hy;
+    };
+
+    auto load_plane = [&](int p, bool halo, Plane &P_) {
+        if (p >= 0 && p < NZ) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                P_.c[r] = *reinterpret_cast<const V *>(ub + x0 + NX * (ty + r * ROWS) + SXY * p);
+            if (halo) {
+                if (hx_on)
+                    P_.hx = face_val(F[hx_side], (long long)NX * hx_y + SXY * p);
+                if (hy_on) {
+                    const FaceSrc<T> &f = F[2 + hy_side];
+                    if (f.mode == 2) {
+#pragma unroll
+                        for (int v = 0; v < VEC; ++v)
+                            P_.hy.v[v] = f.s;
+                    } else {
+                        P_.hy = *reinterpret_cast<const V *>(f.p + hy_x + SXY * p);
+                        if (f.mode == 1)
+#pragma unroll
+                            for (int v = 0; v < VEC; ++v)
+                                P_.hy.v[v] = f.s * P_.hy.v[v];
+                    }
+                }
+            }
+        } else {
+            const FaceSrc<T> &f = F[p < 0 ? 4 : 5];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (f.mode == 2) {
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v)
+                        P_.c[r].v[v] = f.s;
+                } else {
+                    P_.c[r] = *reinterpret_cast<const V *>(f.p + x0 + NX * (ty + r * ROWS));
+                    if (f.mode == 1)
+#pragma unroll
+                        for (int v = 0; v < VEC; ++v)
+                            P_.c[r].v[v] = f.s * P_.c[r].v[v];
+                }
+            }
+        }
+    };
+
+    const T ihx2 = (T)a.ih2[0], ihy2 = (T)a.ih2[1], ihz2 = (T)a.ih2[2];
+    const T nudt = (T)a.nudt;
+    double acc = 0.0;
+
+    Plane pm, pc, pn, pnn;
+    load_plane(z0 - 1, false, pm);
+    load_plane(z0, true, pc);
+    load_plane(z0 + 1, z0 + 1 < z1, pn);
+
+    for (int z = z0; z < z1; ++z) {
+        if (z + 2 <= z1)
+            load_plane(z + 2, z + 2 < z1, pnn);
+        T *L = lds[(z - z0) & 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            *reinterpret_cast<V *>(L + (ty + r * ROWS + 1) * P + VEC + x0) = pc.c[r];
+        if (hx_on)
+            L[(hx_y + 1) * P + (hx_side ? VEC + NX : VEC - 1)] = pc.hx;
+        if (hy_on)
+            *reinterpret_cast<V *>(L + (hy_side ? NY + 1 : 0) * P + VEC + hy_x) = pc.hy;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int y = ty + r * ROWS;
+            const T *row = L + (y + 1) * P + VEC;
+            const V ym = *reinterpret_cast<const V *>(row - P + x0);
+            const V yp = *reinterpret_cast<const V *>(row + P + x0);
+            const T xl = row[x0 - 1];
+            const T xr = row[x0 + VEC];
+            V res, lapv;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+                const T c = pc.c[r].v[v];
+                const T xm = v == 0 ? xl : pc.c[r].v[v - 1];
+                const T xp = v == VEC - 1 ? xr : pc.c[r].v[v + 1];
+                const T ddx = ihx2 * (xp - 2 * c + xm);
+                const T ddy = ihy2 * (yp.v[v] - 2 * c + ym.v[v]);
+                const T ddz = ihz2 * (pn.c[r].v[v] - 2 * c + pm.c[r].v[v]);
+                const T lap = ddx + ddy + ddz;
+                lapv.v[v] = lap;
+                if (MODE == 0) {
+                    res.v[v] = lap;
+                } else {
+                    const T l = lap * nudt;
+                    const T un = c + l;
+                    res.v[v] = un;
+                    if (SUMSQ)
+                        acc += (double)un * (double)un;
+                }
+            }
+            const long long off = x0 + NX * y + SXY * z;
+            *reinterpret_cast<V *>(ob + off) = res;
+            if (MODE == 2)
+                *reinterpret_cast<V *>(lb + off) = lapv;
+        }
+        pm = pc;
+        pc = pn;
+        pn = pnn;
+    }
+
+    if (SUMSQ) {
+        // deterministic: fixed wave tree, fixed wave order, fixed tile order
+        for (int o = 32; o > 0; o >>= 1)
+            acc += __shfl_xor(acc, o, 64);
+        if ((tid & 63) == 0)
+            red[tid >> 6] = acc;
+        __syncthreads();
+        __shared__ bool last;
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < WG / 64; ++w)
+                s += red[w];
+            a.partials[tile] = s;
+            __threadfence();
+            const unsigned t = atomicAdd(a.ticket, 1u);
+            last = (t == gridDim.x - 1);
+        }
+        __syncthreads();
+        if (last) {
+            __threadfence();
+            double s = 0.0;
+            const int n = gridDim.x;
+            for (int i = tid; i < n; i += WG)
+                s += __builtin_nontemporal_load(a.partials + i);
+            for (int o = 32; o > 0; o >>= 1)
+                s += __shfl_xor(s, o, 64);
+            __syncthreads();
+            if ((tid & 63) == 0)
+                red[tid >> 6] = s;
+            __syncthreads();
+            if (tid == 0) {
+                double t = 0.0;
+                for (int w = 0; w < WG / 64; ++w)
+                    t += red[w];
+                *a.sumsq = t;
+                *a.ticket = 0u;
+            }
+        }
+    }
+}
+
+// Generic fallback (any block shape): one thread per cell, lab values read
+// straight from the block or through the resolved faces.
+template <typename T, int MODE, bool SUMSQ>
+__global__ __launch_bounds__(256) void k_stencil7_generic(const S7Args a)
+{
+    __shared__ double red[4];
+    __shared__ bool last;
+    const int tid = threadIdx.x;
+    const long long nb_tot = (long long)a.G.nb[0] * a.G.nb[1] * a.G.nb[2];
+    const long long slot = blockIdx.y; // comp * nblocks + blk
+    const int comp = (int)(slot / nb_tot);
+    const long long blk = slot - comp * nb_tot;
+    const T *cbase = reinterpret_cast<const T *>(a.u) + (long long)(a.comp_in + comp) * a.G.cstride;
+    const T *ub = cbase + blk * a.G.bstride;
+    T *ob = reinterpret_cast<T *>(a.out) + (long long)(a.comp_out + comp) * a.G.cstride + blk * a.G.bstride;
+    FaceSrc<T> F[6];
+    block_faces<T>(a, comp, blk, cbase, F);
+    const int *n = a.G.n;
+    const long long cells = (long long)n[0] * n[1] * n[2];
+    const long long i = (long long)blockIdx.x * 256 + tid;
+    double acc = 0.0;
+    if (i < cells) {
+        const int x = (int)(i % n[0]);
+        const int y = (int)((i / n[0]) % n[1]);
+        const int z = (int)(i / ((long long)n[0] * n[1]));
+        const long long nx = n[0], nxy = (long long)n[0] * n[1];
+        auto at = [&](int xx, int yy, int zz) -> T {
+            if (xx < 0)
+                return face_val(F[0], nx * yy + nxy * zz);
+            if (xx >= n[0])
+                return face_val(F[1], nx * yy + nxy * zz);
+            if (yy < 0)
+                return face_val(F[2], xx + nxy * zz);
+            if (yy >= n[1])
+                return face_val(F[3], xx + nxy * zz);
+            if (zz < 0)
+                return face_val(F[4], xx + nx * yy);
+            if (zz >= n[2])
+                return face_val(F[5], xx + nx * yy);
+            return ub[xx + nx * yy + nxy * zz];
+        };
+        const T ihx2 = (T)a.ih2[0], ihy2 = (T)a.ih2[1], ihz2 = (T)a.ih2[2];
+        const T c = ub[i];
+        const T ddx = ihx2 * (at(x + 1, y, z) - 2 * c + at(x - 1, y, z));
+        const T ddy = ihy2 * (at(x, y + 1, z) - 2 * c + at(x, y - 1, z));
+        const T ddz = ihz2 * (at(x, y, z + 1) - 2 * c + at(x, y, z - 1));
+        const T lap = ddx + ddy + ddz;
+        if (MODE == 0) {
+            ob[i] = lap;
+        } else {
+            const T l = lap * (T)a.nudt;
+            const T un = c + l;
+            ob[i] = un;
+            if (MODE == 2)
+                reinterpret_cast<T *>(a.lap)[(long long)(a.comp_out + comp) * a.G.cstride + blk * a.G.bstride + i] = lap;
+            acc = (double)un * (double)un;
+        }
+    }
+    if (SUMSQ) {
+        for (int o = 32; o > 0; o >>= 1)
+            acc += __shfl_xor(acc, o, 64);
+        if ((tid & 63) == 0)
+            red[tid >> 6] = acc;
+        __syncthreads();
+        const long long wg = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+        const long long nwg = (long long)gridDim.x * gridDim.y;
+        if (tid == 0) {
+            a.partials[wg] = ((red[0] + red[1]) + red[2]) + red[3];
+            __threadfence();
+            const unsigned t = atomicAdd(a.ticket, 1u);
+            last = (t == (unsigned)(nwg - 1));
+        }
+        __syncthreads();
+        if (last) {
+            __threadfence();
+            double s = 0.0;
+            for (long long j = tid; j < nwg; j += 256)
+                s += __builtin_nontemporal_load(a.partials + j);
+            for (int o = 32; o > 0; o >>= 1)
+                s += __shfl_xor(s, o, 64);
+            __syncthreads();
+            if ((tid & 63) == 0)
+                red[tid >> 6] = s;
+            __syncthreads();
+            if (tid == 0) {
+                *a.sumsq = ((red[0] + red[1]) + red[2]) + red[3];
+                *a.ticket = 0u;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+// Tile choice per (dtype, block edge): workgroup size and z-planes per tile.
+struct S7Tune {
+    int wg;
+    int zchunk;
+};
+
+static std::mutex g_tune_mtx;
+static std::map<std::tuple<int, int>, S7Tune> g_tune; // (dtype, edge) -> choice
+
+S7Tune s7_default_tune(int dtype, int edge)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    auto it = g_tune.find({dtype, edge});
+    if (it != g_tune.end())
+        return it->second;
+    if (edge >= 64)
+        return {512, 16};
+    if (edge >= 32)
+        return {256, 8};
+    if (dtype == CNV_F64)
+        return {128, 8};
+    return {64, 8};
+}
+
+void s7_set_tune(int dtype, int edge, int wg, int zchunk)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    g_tune[{dtype, edge}] = {wg, zchunk};
+}
+
+template <typename T, int NX, int WG>
+constexpr bool s7_valid()
+{
+    constexpr int VEC = 16 / (int)sizeof(T);
+    constexpr int TX = NX / VEC;
+    return NX % VEC == 0 && WG % TX == 0 && (WG / TX) <= NX && NX % (WG / TX) == 0 &&
+           WG >= 2 * NX && WG >= 2 * TX;
+}
+
+template <typename T, int NX, int WG, int MODE, bool SUMSQ>
+static bool launch_spec(const S7Args &a, int ntiles, hipStream_t st)
+{
+    if constexpr (s7_valid<T, NX, WG>()) {
+        hipLaunchKernelGGL((k_stencil7<T, NX, NX, WG, MODE, SUMSQ>), dim3(ntiles), dim3(WG), 0, st, a);
+        return true;
+    }
+    return false;
+}
+
+template <typename T, int NX, int MODE, bool SUMSQ>
+static bool dispatch_wg(const S7Args &a, int wg, int ntiles, hipStream_t st)
+{
+    switch (wg) {
+    case 64:
+        return launch_spec<T, NX, 64, MODE, SUMSQ>(a, ntiles, st);
+    case 128:
+        return launch_spec<T, NX, 128, MODE, SUMSQ>(a, ntiles, st);
+    case 256:
+        return launch_spec<T, NX, 256, MODE, SUMSQ>(a, ntiles, st);
+    case 512:
+        return launch_spec<T, NX, 512, MODE, SUMSQ>(a, ntiles, st);
+    default:
+        return false;
+    }
+}
+
+template <typename T, int MODE, bool SUMSQ>
+static bool dispatch_spec(const S7Args &a, int edge, int wg, int ntiles, hipStream_t st)
+{
+    switch (edge) {
+    case 16:
+        return dispatch_wg<T, 16, MODE, SUMSQ>(a, wg, ntiles, st);
+    case 32:
+        return dispatch_wg<T, 32, MODE, SUMSQ>(a, wg, ntiles, st);
+    case 64:
+        return dispatch_wg<T, 64, MODE, SUMSQ>(a, wg, ntiles, st);
+    default:
+        return false;
+    }
+}
+
+template <typename T, int MODE, bool SUMSQ>
+static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic)
+{
+    const int *n = a.G.n;
+    const long long nbt = (long long)a.G.nb[0] * a.G.nb[1] * a.G.nb[2];
+    const bool cubic_xy = n[0] == n[1];
+    if (!force_generic && cubic_xy && (n[0] == 16 || n[0] == 32 || n[0] == 64)) {
+        S7Tune tu = s7_default_tune(sizeof(T) == 8 ? CNV_F64 : CNV_F32, n[0]);
+        a.zchunk = tu.zchunk < n[2] ? tu.zchunk : n[2];
+        a.nzc = (n[2] + a.zchunk - 1) / a.zchunk;
+        a.tiles_per_comp = (int)(nbt * a.nzc);
+        const long long ntiles = (long long)a.tiles_per_comp * ncomp;
+        if (ntiles < (1LL << 31)) {
+            if (dispatch_spec<T, MODE, SUMSQ>(a, n[0], tu.wg, (int)ntiles, st)) {
+                CNV_HIP(hipGetLastError());
+                return CNV_OK;
+            }
+        }
+    }
+    const long long cells = (long long)n[0] * n[1] * n[2];
+    const long long gx = (cells + 255) / 256;
+    const long long gy = nbt * ncomp;
+    if (gy > 65535 || gx > 65535)
+        return fail(CNV_ERR_UNSUPPORTED, "grid too large for the generic stencil kernel");
+    hipLaunchKernelGGL((k_stencil7_generic<T, MODE, SUMSQ>), dim3((unsigned)gx, (unsigned)gy), dim3(256),
+                       0, st, a);
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
+
+static int g_force_generic = -1;
+
+static bool force_generic()
+{
+    if (g_force_generic < 0) {
+        const char *e = getenv("CNV_STENCIL7_GENERIC");
+        g_force_generic = (e && e[0] == '1') ? 1 : 0;
+    }
+    return g_force_generic == 1;
+}
+
+// Host check that every face the operator reads is written by the lab rules.
+static int check_faces(cnv_grid_s *g, int comp)
+{
+    const int sb[3] = {-1, -1, -1}, se[3] = {2, 2, 2};
+    DevBC tmp[64];
+    for (const auto &l : g->bcs[comp]) {
+        if (l.empty())
+            continue;
+        if (l.size() > 64)
+            return fail(CNV_ERR_UNSUPPORTED, "BC list too long");
+        for (size_t k = 0; k < l.size(); ++k)
+            tmp[k] = {l[k].kind, l[k].dir, l[k].side, 0, l[k].value};
+        int kind[6];
+        double val[6];
+        resolve_faces(tmp, (int)l.size(), sb, se, kind, val);
+        for (int f = 0; f < 6; ++f)
+            if (kind[f] == FACE_UNSET)
+                return fail(CNV_ERR_UNSUPPORTED,
+                            "BC list leaves ghost cells of a face unwritten for Stencil(-1,2)");
+    }
+    return CNV_OK;
+}
+
+int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
+                    int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st)
+{
+    if (ncomp > kMaxComp)
+        return fail(CNV_ERR_UNSUPPORTED, "more than 8 components in one stencil call");
+    for (int d = 0; d < 3; ++d)
+        if (u->desc.block_cells[d] < 1)
+            return fail(CNV_ERR_INVALID, "empty block");
+    S7Args a;
+    a.u = static_cast<const char *>(u->data);
+    a.out = static_cast<char *>(out->data);
+    a.lap = lap ? static_cast<char *>(lap->data) : nullptr;
+    a.G = geom_of(u);
+    a.comp_in = comp_in;
+    a.comp_out = comp_out;
+    a.ncomp = ncomp;
+    {
+        std::lock_guard<std::mutex> lk(u->mtx);
+        for (int c = 0; c < ncomp; ++c) {
+            int rc = ensure_bc_table(u, comp_in + c);
+            if (rc != CNV_OK)
+                return rc;
+            rc = check_faces(u, comp_in + c);
+            if (rc != CNV_OK)
+                return rc;
+            const BCTable &t = u->tables[comp_in + c];
+            a.has_bc[c] = t.any ? 1 : 0;
+            a.heads[c] = t.heads;
+            a.entries[c] = t.entries;
+        }
+    }
+    for (int c = ncomp; c < kMaxComp; ++c) {
+        a.has_bc[c] = 0;
+        a.heads[c] = nullptr;
+        a.entries[c] = nullptr;
+    }
+    for (int d = 0; d < 3; ++d)
+        a.ih2[d] = 1. / (u->h[d] * u->h[d]);
+    a.nudt = nu_dt;
+    a.sumsq = sumsq;
+    a.partials = nullptr;
+    a.ticket = nullptr;
+    a.zchunk = a.nzc = a.tiles_per_comp = 0;
+    if (sumsq) {
+        const long long nbt = u->nblocks_total;
+        const long long cells = (long long)u->desc.block_cells[0] * u->desc.block_cells[1] * u->desc.block_cells[2];
+        // upper bound on workgroups: generic kernel uses ceil(cells/256) per block
+        const long long need = nbt * ncomp * ((cells + 255) / 256 + u->desc.block_cells[2]);
+        int rc = ensure_partials(u, (size_t)need);
+        if (rc != CNV_OK)
+            return rc;
+        a.partials = u->partials;
+        a.ticket = u->ticket;
+    }
+    const bool gen = force_generic();
+    const bool dbl = u->desc.dtype == CNV_F64;
+    if (mode == 0)
+        return dbl ? run_t<double, 0, false>(a, ncomp, st, gen) : run_t<float, 0, false>(a, ncomp, st, gen);
+    if (mode == 1) {
+        if (sumsq)
+            return dbl ? run_t<double, 1, true>(a, ncomp, st, gen) : run_t<float, 1, true>(a, ncomp, st, gen);
+        return dbl ? run_t<double, 1, false>(a, ncomp, st, gen) : run_t<float, 1, false>(a, ncomp, st, gen);
+    }
+    if (sumsq)
+        return dbl ? run_t<double, 2, true>(a, ncomp, st, gen) : run_t<float, 2, true>(a, ncomp, st, gen);
+    return dbl ? run_t<double, 2, false>(a, ncomp, st, gen) : run_t<float, 2, false>(a, ncomp, st, gen);
+}
+
+// ---------------------------------------------------------------- sums
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sums(const T *p, long long bstride, long long cells, long long nbt,
+                                                double *partials, unsigned *ticket, double *out)
+{
+    __shared__ double red[2][4];
+    __shared__ bool last;
+    const int tid = threadIdx.x;
+    double s = 0.0, q = 0.0;
+    for (long long b = blockIdx.x; b < nbt; b += gridDim.x) {
+        const T *pb = p + b * bstride;
+        for (long long i = tid; i < cells; i += 256) {
+            const double v = (double)pb[i];
+            s += v;
+            q += v * v;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        q += __shfl_xor(q, o, 64);
+    }
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = s;
+        red[1][tid >> 6] = q;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        partials[2 * blockIdx.x] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+        partials[2 * blockIdx.x + 1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double a = 0.0, b = 0.0;
+        for (int i = tid; i < (int)gridDim.x; i += 256) {
+            a += __builtin_nontemporal_load(partials + 2 * i);
+            b += __builtin_nontemporal_load(partials + 2 * i + 1);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o, 64);
+            b += __shfl_xor(b, o, 64);
+        }
+        __syncthreads();
+        if ((tid & 63) == 0) {
+            red[0][tid >> 6] = a;
+            red[1][tid >> 6] = b;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            out[0] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+            out[1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+            *ticket = 0u;
+        }
+    }
+}
+
+int launch_sums(cnv_grid_s *g, int comp, double *out, hipStream_t st)
+{
+    const long long nbt = g->nblocks_total;
+    const long long cells = (long long)g->desc.block_cells[0] * g->desc.block_cells[1] * g->desc.block_cells[2];
+    const int nwg = (int)(nbt < 2048 ? nbt : 2048);
+    {
+        std::lock_guard<std::mutex> lk(g->mtx);
+        int rc = ensure_partials(g, 2 * (size_t)nwg);
+        if (rc != CNV_OK)
+            return rc;
+    }
+    const Geom G = geom_of(g);
+    if (g->desc.dtype == CNV_F64)
+        hipLaunchKernelGGL(k_sums<double>, dim3(nwg), dim3(256), 0, st,
+                           reinterpret_cast<const double *>((const char *)g->data + comp * g->component_bytes),
+                           G.bstride, cells, nbt, g->partials, g->ticket, out);
+    else
+        hipLaunchKernelGGL(k_sums<float>, dim3(nwg), dim3(256), 0, st,
+                           reinterpret_cast<const float *>((const char *)g->data + comp * g->component_bytes),
+                           G.bstride, cells, nbt, g->partials, g->ticket, out);
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
+
+} // namespace cnv

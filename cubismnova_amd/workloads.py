@@ -1,0 +1,66 @@
+"""Seeded synthetic inputs for the stencil workloads (numpy, host side).
+
+Every generator returns global (Z, Y, X) float64 arrays at cell centres
+x = begin + (i + 1/2) h (Mesh/StructuredUniform.h:361-365); callers cast to
+the grid dtype and upload with Cartesian.upload_global.  The seed of
+workload k is 0x5EED0000 + k.
+"""
+import math
+
+import numpy as np
+
+SEED0 = 0x5EED0000
+
+
+def rng(k, salt=0):
+    return np.random.Generator(np.random.PCG64(SEED0 + k + (salt << 16)))
+
+
+def cell_centres(ncells, begin=0.0, end=1.0):
+    h = (end - begin) / ncells
+    return begin + (np.arange(ncells) + 0.5) * h
+
+
+def diffusion_small(N=64):
+    """0.1*U[-1,1] noise + sin(2 pi x) cos(4 pi y) sin(6 pi z) on [0,1]^3."""
+    r = rng(1)
+    x = cell_centres(N)
+    sx, cy, sz = np.sin(2 * np.pi * x), np.cos(4 * np.pi * x), np.sin(6 * np.pi * x)
+    u = sz[:, None, None] * cy[None, :, None] * sx[None, None, :]
+    return u + 0.1 * (2.0 * r.random((N, N, N)) - 1.0)
+
+
+def fourier_modes(k, nmodes=8, kmax=16):
+    """nmodes integer wavevectors with 1 <= |k| <= kmax, amplitude 1/|k|,
+    uniform phases."""
+    r = rng(k)
+    modes = []
+    while len(modes) < nmodes:
+        kv = r.integers(-kmax, kmax + 1, size=3)
+        kn = float(np.sqrt((kv * kv).sum()))
+        if 1.0 <= kn <= kmax:
+            modes.append((kv, 1.0 / kn, r.random() * 2 * np.pi))
+    return modes
+
+
+def fourier_field(N, modes, dtype=np.float64):
+    """sum_m A_m cos(2 pi k_m . x + phi_m) on an N^3 grid, built plane by plane."""
+    x = cell_centres(N)
+    u = np.zeros((N, N, N), dtype=np.float64)
+    for kv, amp, ph in modes:
+        ax = np.exp(1j * 2 * np.pi * kv[0] * x)
+        ay = np.exp(1j * 2 * np.pi * kv[1] * x)
+        az = np.exp(1j * (2 * np.pi * kv[2] * x + ph))
+        plane = ay[:, None] * ax[None, :]
+        for iz in range(N):
+            u[iz] += amp * (az[iz] * plane).real
+    return u.astype(dtype, copy=False)
+
+
+def diffusion_large(N=256):
+    """Sum of 8 seeded Fourier modes, |k| in [1,16], amplitude 1/|k|."""
+    return fourier_field(N, fourier_modes(2))
+
+
+def diffusion_dt(h, nu=1.0):
+    return 0.1 * h * h, nu

@@ -1,0 +1,193 @@
+/* cubismnova_amd.h -- C ABI of the MI355X (gfx950) block-structured stencil
+ * library.
+ *
+ * The library keeps CubismNova's block storage on the GPU and exposes the
+ * path "FieldLab ghost assembly for every block of a Grid::Cartesian + the
+ * stencil operators applied over it" through plain C entry points: opaque
+ * grid handles, plain pointers and sizes, int status codes (0 = success, no
+ * exceptions cross this boundary) and calls ordered on a caller-supplied HIP
+ * stream (NULL selects the default stream).
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the CubismNova source tree).  Memory layout of a grid is the reference's
+ * (include/Cubism/Grid/Cartesian.h:490-529 with
+ * include/Cubism/Grid/BlockFieldAssembler.h:188-199): component c of block i
+ * starts at byte c*component_bytes + i*block_bytes, block_bytes =
+ * roundup32(cells*sizeof(T)), component_bytes = block_bytes*nblocks, blocks
+ * numbered x-fastest, cells inside a block x-fastest.
+ */
+#ifndef CUBISMNOVA_AMD_H
+#define CUBISMNOVA_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+typedef struct cnv_grid_s *cnv_grid; /* opaque grid handle */
+typedef void *cnv_stream;            /* a hipStream_t; NULL = default stream */
+
+enum cnv_status {
+    CNV_OK = 0,
+    CNV_ERR_INVALID = 1,       /* bad argument or descriptor */
+    CNV_ERR_ALLOC = 2,         /* device or host allocation failed */
+    CNV_ERR_HIP = 3,           /* a HIP runtime call failed */
+    CNV_ERR_UNSUPPORTED = 4,   /* valid request this library does not run */
+    CNV_ERR_NOT_ALLOCATED = 5, /* lab used before allocation */
+    CNV_ERR_SIZE = 6           /* size or shape mismatch */
+};
+
+enum cnv_dtype { CNV_F32 = 0, CNV_F64 = 1 };
+
+/* BC kinds: include/Cubism/BC/{Base,Absorbing,Dirichlet,Symmetry}.h */
+enum cnv_bc_kind {
+    CNV_BC_BASE = 0,      /* periodic, no-op apply (BC/Base.h:45-62) */
+    CNV_BC_ABSORBING = 1, /* zero-gradient copy (BC/Absorbing.h:147-189) */
+    CNV_BC_DIRICHLET = 2, /* ghost = value (BC/Dirichlet.h:75-113) */
+    CNV_BC_SYMMETRY = 3   /* ghost = value * mirror (BC/Symmetry.h:69-114) */
+};
+
+/* One boundary condition: BC::*<Lab>(dir, side[, value|sign]).
+ * `value` is the Dirichlet constant or the Symmetry sign. */
+typedef struct {
+    int kind;
+    int dir;
+    int side;
+    int reserved;
+    double value;
+} cnv_bc;
+
+/* Core::Stencil<3>(begin, end, tensorial) (include/Cubism/Core/Stencil.h:44-67):
+ * begin <= 0 inclusive, end >= 1 exclusive, per dimension. */
+typedef struct {
+    int begin[3];
+    int end[3];
+    int tensorial;
+} cnv_stencil;
+
+/* Grid::Cartesian(nblocks, block_cells, begin, end) for cell-centred scalar
+ * components (include/Cubism/Grid/Cartesian.h:132-143). ncomp independent
+ * scalar components stored as consecutive component slabs. */
+typedef struct {
+    int nblocks[3];
+    int block_cells[3];
+    int ncomp;
+    int dtype; /* enum cnv_dtype */
+    double begin[3];
+    double end[3];
+} cnv_grid_desc;
+
+/* --------------------------------------------------------------- errors */
+
+/* Static message for a status code. */
+const char *cnv_strerror(int status);
+/* Message of the last failing call on this thread ("" if none). */
+const char *cnv_last_error(void);
+
+/* ------------------------------------------------------- device helpers */
+/* Thin wrappers so that callers without their own HIP binding (ctypes,
+ * JNI, ...) can stage buffers; they call the HIP runtime directly. */
+int cnv_device_count(int *count);
+int cnv_malloc(void **ptr, size_t bytes);
+int cnv_free(void *ptr);
+int cnv_memcpy_h2d(void *dst, const void *src, size_t bytes, cnv_stream s);
+int cnv_memcpy_d2h(void *dst, const void *src, size_t bytes, cnv_stream s);
+int cnv_memcpy_d2d(void *dst, const void *src, size_t bytes, cnv_stream s);
+int cnv_fill(void *dst, int dtype, double value, size_t count, cnv_stream s);
+int cnv_stream_create(cnv_stream *s);
+int cnv_stream_destroy(cnv_stream s);
+int cnv_stream_synchronize(cnv_stream s);
+
+/* ------------------------------------------------------------------ grid */
+
+/* Replaces Grid::Cartesian::Cartesian + initTopology_/alloc_
+ * (Grid/Cartesian.h:132-143, 444-529): one device allocation holding every
+ * component slab, reference byte layout. Memory is not initialised. */
+int cnv_grid_create(const cnv_grid_desc *desc, cnv_grid *out);
+int cnv_grid_destroy(cnv_grid g);
+/* Layout query: block_bytes and component_bytes as in Cartesian.h:490-529. */
+int cnv_grid_info(cnv_grid g, cnv_grid_desc *desc, size_t *block_bytes, size_t *component_bytes);
+/* Device address of component `comp` (block 0). */
+int cnv_grid_data(cnv_grid g, int comp, void **device_ptr);
+/* Copy one component slab (component_bytes bytes, reference layout) to or
+ * from host memory. Replaces host writes through Block::Field accessors. */
+int cnv_grid_upload(cnv_grid g, int comp, const void *host, size_t bytes, cnv_stream s);
+int cnv_grid_download(cnv_grid g, int comp, void *host, size_t bytes, cnv_stream s);
+
+/* ------------------------------------------------------------------- BCs */
+
+/* Replaces Block::Field::getBC() (Block/Field.h:437-442): sets the ordered BC
+ * list of the scalar field (comp, block). Order matters exactly as in
+ * FieldLab::loadData (Block/FieldLab.h:206-223): the last BC per direction
+ * decides periodicity and the skipped side; BCs are applied in list order. */
+int cnv_grid_set_block_bcs(cnv_grid g, int comp, const int block[3], const cnv_bc *bcs, int n);
+/* Appends `bc` to the lists of every block that touches the grid face
+ * (bc->dir, bc->side), for one component or all (comp = -1).  This is the
+ * loop a CubismNova application writes over its boundary blocks. */
+int cnv_grid_add_face_bc(cnv_grid g, int comp, const cnv_bc *bc);
+int cnv_grid_clear_bcs(cnv_grid g, int comp);
+
+/* ------------------------------------------------------------------- lab */
+
+/* Extent of the active lab range of a block for stencil s:
+ * N + end - 1 - begin per dimension (Block/FieldLab.h:198-202). */
+int cnv_lab_extent(cnv_grid g, const cnv_stencil *s, int extent[3]);
+
+/* Replaces Grid::Cartesian::loadLab + FieldLab::loadData
+ * (Grid/Cartesian.h:411-427, Block/FieldLab.h:180-248, FieldLabLoader.h
+ * :129-287): assembles the lab of block `block` of component `comp` and
+ * writes it to device memory `out` in compact x-fastest layout over the
+ * active lab range [begin, N + end - 1). Cells the reference leaves unwritten
+ * (edges/corners of non-tensorial labs, skipped sides without BC) are left
+ * untouched in `out`. n_extern < 0 uses the block's own BC list, otherwise
+ * the `extern_bcs` list (FieldLab.h:184,206,241-248). */
+int cnv_lab_load(cnv_grid g, int comp, const int block[3], const cnv_stencil *s, int apply_bc,
+                 const cnv_bc *extern_bcs, int n_extern, void *out, cnv_stream st);
+/* The same for every block of the grid, blocks consecutive in block order
+ * (the reference's per-block loop over a grid, Laplacian.cpp:110-117). */
+int cnv_lab_load_all(cnv_grid g, int comp, const cnv_stencil *s, void *out, cnv_stream st);
+
+/* ------------------------------------------------------------- operators */
+
+/* 7-point Laplacian of `in` (component comp_in) into `out` (comp_out),
+ * res = ihx2*((u[+x] - 2u) + u[-x]) + ihy2*(...) + ihz2*(...), ih2 = 1/(h*h)
+ * from the grid mesh, ghosts from the lab with Stencil(-1,2,false) and the
+ * blocks' BC lists. Replaces the loop of
+ * test/sandbox/Benchmarks/Kernel/Laplacian/Laplacian.cpp:110-131. */
+int cnv_laplacian(cnv_grid in, int comp_in, cnv_grid out, int comp_out, cnv_stream s);
+
+/* One forward-Euler diffusion step fused with the Laplacian:
+ * lap *= nu_dt; u_new = u + lap (two roundings, Field.h:457-512 with
+ * FieldOperator.h:43-114), for every component of `u` into `u_new`.
+ * If lap_out is non-NULL the Laplacian is also stored there.
+ * If sumsq is non-NULL (device address of one double) the FP64 sum of
+ * u_new^2 over all cells and components is written to it (deterministic,
+ * fixed-order reduction). Calls on the same `u` must not overlap. */
+int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_out, double *sumsq,
+                       cnv_stream s);
+
+/* FP64 sum and sum of squares of one component into device doubles out[0],
+ * out[1] (deterministic). */
+int cnv_grid_sums(cnv_grid g, int comp, double *out, cnv_stream s);
+
+/* ------------------------------------------------------------ HIP graphs */
+/* No counterpart in the reference (its loop is host code per block): record
+ * the calls issued on stream `s` between begin and end into an executable
+ * graph, replayed with cnv_graph_launch.  Every entry point above is
+ * capturable after it has run once on the same grids (first calls upload
+ * BC tables synchronously); kernels clear their own counters, so no memset
+ * node is ever recorded. */
+typedef struct cnv_graph_s *cnv_graph;
+int cnv_graph_begin(cnv_stream s);
+int cnv_graph_end(cnv_stream s, cnv_graph *out);
+int cnv_graph_launch(cnv_graph g, cnv_stream s);
+int cnv_graph_destroy(cnv_graph g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUBISMNOVA_AMD_H */

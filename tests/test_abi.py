@@ -1,0 +1,70 @@
+"""CPU checks of the C ABI: the library loads, exports every entry point the
+header declares, and its host-side argument checks answer without a GPU."""
+import ctypes
+import subprocess
+
+import pytest
+
+from cubismnova_amd import _abi
+
+
+def test_library_exports_every_declared_symbol():
+    L = _abi.lib()
+    declared = _abi.declared_symbols()
+    assert len(declared) >= 20
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert set(declared) <= exported
+
+
+def test_every_declared_symbol_has_a_ctypes_signature():
+    assert set(_abi.declared_symbols()) <= set(_abi.SIGNATURES)
+
+
+def test_strerror_and_host_validation():
+    L = _abi.lib()
+    assert L.cnv_strerror(0) == b"success"
+    assert L.cnv_strerror(_abi.CNV_ERR_NOT_ALLOCATED).startswith(b"FieldLab")
+    d = _abi.cnv_grid_desc()
+    d.ncomp, d.dtype = 1, _abi.CNV_F64
+    for k in range(3):
+        d.nblocks[k], d.block_cells[k], d.begin[k], d.end[k] = 0, 8, 0.0, 1.0
+    h = ctypes.c_void_p()
+    rc = L.cnv_grid_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == _abi.CNV_ERR_INVALID and not h.value
+    assert b"nblocks" in L.cnv_last_error()
+    for k in range(3):
+        d.nblocks[k] = 2
+    d.dtype = 7
+    assert L.cnv_grid_create(ctypes.byref(d), ctypes.byref(h)) == _abi.CNV_ERR_INVALID
+    assert L.cnv_grid_create(None, ctypes.byref(h)) == _abi.CNV_ERR_INVALID
+    assert L.cnv_grid_destroy(None) == 0
+    assert L.cnv_laplacian(None, 0, None, 0, None) == _abi.CNV_ERR_INVALID
+
+
+def test_stencil_checks_mirror_reference():
+    import cubismnova_amd as cnv
+    with pytest.raises(RuntimeError, match="Stencil: begin_ must be <= 0"):
+        cnv.Stencil(1, 2)
+    with pytest.raises(RuntimeError, match="Stencil: end_ must be > 0"):
+        cnv.Stencil(-1, 0)
+    s = cnv.Stencil(-3, 4, True)
+    assert s.getBegin() == [-3] * 3 and s.getEnd() == [4] * 3 and s.isTensorial()
+    assert cnv.Absorbing(0, 1).name() == "Zeroth-Order Absorbing"
+    assert cnv.Dirichlet(0, 0, 3.0).getValue() == 3.0
+    assert cnv.Symmetry(2, 1).getBoundaryInfo().is_periodic is False
+    assert cnv.Base(1, 0).getBoundaryInfo().is_periodic is True
+
+
+def test_layout_roundtrip():
+    import numpy as np
+    from cubismnova_amd.layout import blocks_to_global, global_to_blocks
+    u = np.arange(6 * 8 * 10, dtype=np.float64).reshape(6, 8, 10)
+    nb, n = (2, 2, 3), (5, 4, 2)
+    s = global_to_blocks(u, nb, n, block_stride=44)
+    assert s.size == 12 * 44
+    assert s[0:5].tolist() == [0, 1, 2, 3, 4]
+    assert s[44] == 5  # block (1,0,0) starts at x = 5
+    assert np.array_equal(blocks_to_global(s, nb, n, 44), u)

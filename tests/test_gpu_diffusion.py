@@ -1,0 +1,200 @@
+"""GPU parity of the 7-point Laplacian and the fused diffusion step against
+the oracle.  Every comparison is bit-exact (same operation order, no FMA
+contraction) except FP64 reductions, which are order-dependent and checked to
+1e-13 relative against the oracle's long-double sum."""
+import numpy as np
+import pytest
+
+import cubismnova_amd as cnv
+import oracle
+from cubismnova_amd import workloads
+from tests.helpers import face_bc_lists, random_slab
+
+pytestmark = pytest.mark.gpu
+
+KIND = {0: cnv.Base, 1: cnv.Absorbing, 2: cnv.Dirichlet, 3: cnv.Symmetry}
+
+
+def mk_bc(t):
+    k, d, s, v = t
+    return KIND[k](d, s, v) if k in (2, 3) else KIND[k](d, s)
+
+
+def setup(nb, n, dtype, slab, bcl=None, ncomp=1):
+    g = cnv.Cartesian(nb, n, dtype=dtype, ncomp=ncomp)
+    g.upload(0, slab)
+    for b, lst in (bcl or {}).items():
+        bi = (b % nb[0], (b // nb[0]) % nb[1], b // (nb[0] * nb[1]))
+        g.setBC(bi, [mk_bc(t) for t in lst])
+    return g
+
+
+def test_small_config_laplacian_and_euler(gpu):
+    """4^3 blocks of 16^3, FP64, periodic: ghosts, Laplacian and one Euler step."""
+    nb, n, N = (4, 4, 4), (16, 16, 16), 64
+    u0 = workloads.diffusion_small(N)
+    g = cnv.Cartesian(nb, n)
+    g.upload_global(0, u0)
+    slab = g.download(0)
+    f = oracle.Field(slab, nb, n)
+    # ghosts of every block
+    buf, ext = g.lab_all(cnv.Stencil(-1, 2))
+    buf.fill(np.float64, np.nan)
+    g.lab_all(cnv.Stencil(-1, 2), out=buf)
+    labs = buf.download(np.float64).reshape(64, ext[2], ext[1], ext[0])
+    for b in range(64):
+        bi = (b % 4, (b // 4) % 4, b // 16)
+        want = oracle.lab_load(f, bi, -1, 2)
+        m = ~np.isnan(want)
+        assert np.array_equal(labs[b][m], want[m])
+    h = [1.0 / 64] * 3
+    lap = cnv.Cartesian(nb, n)
+    cnv.laplacian(g, lap)
+    assert np.array_equal(lap.download(0), oracle.laplacian(f, h))
+    dt = 0.1 * h[0] ** 2
+    u1 = cnv.Cartesian(nb, n)
+    lap2 = cnv.Cartesian(nb, n)
+    s = cnv.DeviceBuffer(8)
+    cnv.diffusion_step(g, u1, dt, lap=lap2, sumsq=s)
+    un, sref, lref = oracle.diffusion_step(f, h, dt, want_lap=True)
+    assert np.array_equal(lap2.download(0), lref)
+    got = u1.download(0)
+    assert np.array_equal(got, un)
+    rel = np.max(np.abs(got - un)) / np.max(np.abs(un))
+    assert rel <= 1e-14
+    ssum = s.download(np.float64, 1)[0]
+    assert abs(ssum - sref) <= 1e-13 * sref
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("edge", [16, 32, 64])
+@pytest.mark.parametrize("bcs", ["periodic", "mixed"])
+def test_diffusion_specialised_kernels(gpu, dtype, edge, bcs):
+    nb = (2, 2, 2) if edge < 64 else (2, 1, 1)
+    n = (edge,) * 3
+    slab = random_slab(31 + edge, nb, n, dtype)
+    bcl = None
+    if bcs == "mixed":
+        bcl = face_bc_lists(nb, [(2, 0, 0, 0.5), (1, 0, 1, 0.0), (3, 1, 0, -1.0), (1, 1, 1, 0.0),
+                                 (2, 2, 0, -0.25), (3, 2, 1, 1.0)])
+    g = setup(nb, n, dtype, slab, bcl)
+    f = oracle.Field(slab, nb, n, bcs=bcl)
+    h = [1.0 / (nb[d] * edge) for d in range(3)]
+    out = cnv.Cartesian(nb, n, dtype=dtype)
+    s = cnv.DeviceBuffer(8)
+    cnv.diffusion_step(g, out, 0.1 * h[0] ** 2, sumsq=s)
+    un, sref = oracle.diffusion_step(f, h, 0.1 * h[0] ** 2)
+    assert np.array_equal(out.download(0), un)
+    assert abs(s.download(np.float64, 1)[0] - sref) <= 1e-13 * sref
+    lap = cnv.Cartesian(nb, n, dtype=dtype)
+    cnv.laplacian(g, lap)
+    assert np.array_equal(lap.download(0), oracle.laplacian(f, h))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_diffusion_generic_shapes(gpu, dtype):
+    for nb, n in [((3, 2, 2), (12, 8, 10)), ((1, 1, 1), (5, 7, 3)), ((2, 2, 2), (8, 8, 8))]:
+        slab = random_slab(5, nb, n, dtype)
+        bcl = face_bc_lists(nb, [(2, 1, 0, 1.0), (1, 2, 1, 0.0)])
+        g = setup(nb, n, dtype, slab, bcl)
+        f = oracle.Field(slab, nb, n, bcs=bcl)
+        h = [1.0 / (nb[d] * n[d]) for d in range(3)]
+        out = cnv.Cartesian(nb, n, dtype=dtype)
+        s = cnv.DeviceBuffer(8)
+        cnv.diffusion_step(g, out, 1e-6, sumsq=s)
+        un, sref = oracle.diffusion_step(f, h, 1e-6)
+        assert np.array_equal(out.download(0), un)
+        assert abs(s.download(np.float64, 1)[0] - sref) <= 1e-13 * sref
+
+
+def test_diffusion_multicomponent(gpu):
+    nb, n = (2, 2, 2), (16, 16, 16)
+    g = cnv.Cartesian(nb, n, ncomp=3)
+    out = cnv.Cartesian(nb, n, ncomp=3)
+    h = [1.0 / 32] * 3
+    refs, tot = [], 0.0
+    for c in range(3):
+        slab = random_slab(40 + c, nb, n)
+        g.upload(c, slab)
+        un, s = oracle.diffusion_step(oracle.Field(slab, nb, n), h, 1e-5)
+        refs.append(un)
+        tot += s
+    sbuf = cnv.DeviceBuffer(8)
+    cnv.diffusion_step(g, out, 1e-5, sumsq=sbuf)
+    for c in range(3):
+        assert np.array_equal(out.download(c), refs[c])
+    assert abs(sbuf.download(np.float64, 1)[0] - tot) <= 1e-13 * tot
+
+
+def test_argument_errors(gpu):
+    g = cnv.Cartesian(1, 8)
+    out = cnv.Cartesian(1, 8)
+    g.setBC((0, 0, 0), [cnv.Absorbing(0, 0), cnv.Base(0, 1)])  # last of dir 0 periodic: x- absorbing
+    cnv.diffusion_step(g, out, 1e-3)
+    with pytest.raises(cnv.CnvError):
+        cnv.diffusion_step(g, g, 1e-3)             # in-place is rejected
+    with pytest.raises(cnv.CnvError):
+        cnv.diffusion_step(g, cnv.Cartesian(1, 16), 1e-3)   # shape mismatch
+    with pytest.raises(cnv.CnvError):
+        cnv.diffusion_step(g, cnv.Cartesian(1, 8, dtype=np.float32), 1e-3)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_full_size_100_steps(gpu, dtype):
+    """8^3 blocks of 32^3 (256^3), 100 Euler steps with an FP64 norm every step,
+    against the oracle run on the same bytes: final field bit-exact, norms 1e-13."""
+    nb, n, N = (8, 8, 8), (32, 32, 32), 256
+    u0 = workloads.diffusion_large(N).astype(dtype)
+    a = cnv.Cartesian(nb, n, dtype=dtype)
+    b = cnv.Cartesian(nb, n, dtype=dtype)
+    a.upload_global(0, u0)
+    slab = a.download(0)
+    h = [1.0 / N] * 3
+    dt = 0.1 * h[0] ** 2
+    norms = cnv.DeviceBuffer(8 * 100)
+    st = cnv.Stream()
+    src, dst = a, b
+    for k in range(100):
+        cnv.diffusion_step(src, dst, dt, sumsq=norms.at(8 * k), stream=st)
+        src, dst = dst, src
+    st.synchronize()
+    got = src.download(0)
+    gn = norms.download(np.float64, 100)
+    nt = max(1, min(16, oracle.max_threads()))
+    cur = slab
+    for k in range(100):
+        cur, s = oracle.diffusion_step(oracle.Field(cur, nb, n), h, dt, nthreads=nt)
+        assert abs(gn[k] - s) <= 1e-13 * s, k
+    assert np.array_equal(got, cur)
+    # conservation of the periodic mean
+    assert abs(got.astype(np.float64).sum() - slab.astype(np.float64).sum()) <= 1e-9 * np.abs(slab).sum()
+
+
+def test_fp32_vs_fp64_error(gpu):
+    nb, n, N = (8, 8, 8), (32, 32, 32), 256
+    u0 = workloads.diffusion_large(N)
+    h = 1.0 / N
+    dt = 0.1 * h * h
+    res = {}
+    for dtype in (np.float64, np.float32):
+        a = cnv.Cartesian(nb, n, dtype=dtype)
+        b = cnv.Cartesian(nb, n, dtype=dtype)
+        a.upload_global(0, u0.astype(dtype))
+        src, dst = a, b
+        for k in range(100):
+            cnv.diffusion_step(src, dst, dt)
+            src, dst = dst, src
+        res[dtype] = src.download(0).astype(np.float64)
+    rel = np.linalg.norm(res[np.float32] - res[np.float64]) / np.linalg.norm(res[np.float64])
+    assert rel <= 1e-6, rel
+
+
+def test_sums_deterministic(gpu):
+    nb, n = (4, 4, 4), (16, 16, 16)
+    slab = random_slab(77, nb, n)
+    g = setup(nb, n, np.float64, slab)
+    a = cnv.grid_sums(g)
+    b = cnv.grid_sums(g)
+    assert a == b
+    assert abs(a[0] - slab.sum()) <= 1e-12 * np.abs(slab).sum()
+    assert abs(a[1] - (slab * slab).sum()) <= 1e-13 * (slab * slab).sum()

@@ -1,0 +1,171 @@
+"""CPU pins of the oracle's stencil operators (no GPU).
+
+The 7-point Laplacian is pinned by the exact eigenvalue of a discrete
+Fourier mode (a dropped term, sign or wrong neighbour fails it) and by an
+independent whole-array evaluation with periodic rolls in the operation order
+of Laplacian.cpp:125-129; the Euler update by its two separate roundings
+(Field.h:457-512); the FP64 sum of squares by math.fsum.  WENO5 is pinned by
+polynomial exactness, mirror symmetry, the linear weights on smooth data,
+fifth-order convergence and ENO behaviour at a jump (Jiang & Shu 1996).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from cubismnova_amd.layout import blocks_to_global, global_to_blocks
+from tests.helpers import random_slab
+
+
+def periodic_grid(nb, n, u, dtype=np.float64):
+    return oracle.Field(global_to_blocks(u.astype(dtype), nb, n), nb, n)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 2e-3)])
+def test_laplacian_fourier_eigenvalue(dtype, tol):
+    nb, n = (2, 3, 2), (8, 4, 8)
+    N = [nb[d] * n[d] for d in range(3)]
+    h = [1.0 / N[d] for d in range(3)]
+    k = (1, 2, 3)
+    x = [(np.arange(N[d]) + 0.5) * h[d] for d in range(3)]
+    Z, Y, X = np.meshgrid(x[2], x[1], x[0], indexing="ij")
+    u = np.cos(2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z) + 0.3)
+    lam = sum((2 * math.cos(2 * math.pi * k[d] * h[d]) - 2) / h[d] ** 2 for d in range(3))
+    lap = blocks_to_global(oracle.laplacian(periodic_grid(nb, n, u, dtype), h), nb, n)
+    err = np.max(np.abs(lap.astype(np.float64) - lam * u)) / abs(lam)
+    assert err < tol
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_laplacian_roll_order_bitexact(dtype):
+    nb, n = (3, 2, 2), (6, 8, 4)
+    N = [nb[d] * n[d] for d in range(3)]
+    h = [1.0 / N[d] for d in range(3)]
+    u = random_slab(3, nb, n, dtype)
+    G = blocks_to_global(u, nb, n)
+    ih2 = [dtype(1.0 / (hh * hh)) for hh in h]
+    two = dtype(2)
+    ax = {0: 2, 1: 1, 2: 0}
+    dd = []
+    for d in range(3):
+        p = np.roll(G, -1, axis=ax[d])
+        m = np.roll(G, 1, axis=ax[d])
+        dd.append(ih2[d] * ((p - two * G) + m))
+    want = (dd[0] + dd[1]) + dd[2]
+    got = blocks_to_global(oracle.laplacian(oracle.Field(u, nb, n), h), nb, n)
+    assert got.dtype == dtype
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_euler_two_roundings_and_sumsq(dtype):
+    nb, n = (2, 2, 2), (8, 8, 8)
+    h = [1.0 / 16] * 3
+    u = random_slab(9, nb, n, dtype)
+    nu_dt = 0.1 * h[0] ** 2
+    f = oracle.Field(u, nb, n)
+    un, s, lap = oracle.diffusion_step(f, h, nu_dt, want_lap=True)
+    assert np.array_equal(lap, oracle.laplacian(f, h))
+    c = dtype(nu_dt)
+    want = u + (lap * c)
+    assert np.array_equal(un, want)
+    ref = math.fsum(float(v) * float(v) for v in un.astype(np.float64))
+    assert abs(s - ref) <= 1e-15 * ref
+
+
+def test_diffusion_threads_deterministic():
+    nb, n = (4, 4, 4), (8, 8, 8)
+    u = random_slab(4, nb, n)
+    f = oracle.Field(u, nb, n)
+    a, sa = oracle.diffusion_step(f, [1 / 32] * 3, 1e-5, nthreads=1)
+    b, sb = oracle.diffusion_step(f, [1 / 32] * 3, 1e-5, nthreads=4)
+    assert np.array_equal(a, b) and sa == sb
+
+
+def test_periodic_diffusion_conserves_sum():
+    nb, n = (2, 2, 2), (8, 8, 8)
+    u = random_slab(8, nb, n)
+    f = oracle.Field(u, nb, n)
+    un, _ = oracle.diffusion_step(f, [1 / 16] * 3, 0.1 / 256)
+    assert abs(un.sum() - u.sum()) < 1e-12 * np.abs(u).sum()
+
+
+# ------------------------------------------------------------------ WENO5
+
+
+def cell_avg_poly(c, j):
+    # average of c0 + c1 x + c2 x^2 over [j - 1/2, j + 1/2]
+    return c[0] + c[1] * j + c[2] * (j * j + 1.0 / 12.0)
+
+
+def poly(c, x):
+    return c[0] + c[1] * x + c[2] * x * x
+
+
+@pytest.mark.parametrize("coef", [(1.0, 0.0, 0.0), (0.3, -2.0, 0.0), (1.5, 0.7, -0.4), (-2.0, 3.0, 5.0)])
+def test_weno5_quadratic_exact(coef):
+    j = np.arange(-2, 3, dtype=np.float64)
+    v = cell_avg_poly(coef, j)
+    um, up = oracle.weno5_point(v)
+    assert abs(um - poly(coef, 0.5)) < 1e-12 * (1 + abs(poly(coef, 0.5)))
+    assert abs(up - poly(coef, -0.5)) < 1e-12 * (1 + abs(poly(coef, -0.5)))
+
+
+def test_weno5_mirror_symmetry():
+    rng = np.random.Generator(np.random.PCG64(1))
+    for _ in range(200):
+        v = rng.standard_normal(5)
+        um, up = oracle.weno5_point(v)
+        um2, up2 = oracle.weno5_point(v[::-1].copy())
+        assert abs(um - up2) <= 1e-14 * (1 + abs(um)) and abs(up - um2) <= 1e-14 * (1 + abs(up))
+
+
+def test_weno5_linear_weights_on_smooth_data():
+    # smooth data: the result approaches the optimal 5-point linear scheme
+    h = 1e-3
+    j = np.arange(-2, 3)
+    v = np.array([(np.sin((k + 0.5) * h) - np.sin((k - 0.5) * h)) / h for k in j])
+    a, b, c, d, e = v
+    lin = (2 * a - 13 * b + 47 * c + 27 * d - 3 * e) / 60.0
+    um, _ = oracle.weno5_point(v)
+    assert abs(um - lin) < 1e-10
+
+
+def test_weno5_fifth_order():
+    errs = []
+    for N in (32, 64, 128):
+        h = 2 * np.pi / N
+        xs = np.arange(N) * h
+        avg = (np.cos(xs - h / 2) - np.cos(xs + h / 2)) / h  # cell averages of sin
+        err = 0.0
+        for i in range(N):
+            v = avg[[(i + k) % N for k in range(-2, 3)]]
+            um, _ = oracle.weno5_point(v)
+            err = max(err, abs(um - np.sin(xs[i] + h / 2)))
+        errs.append(err)
+    orders = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert orders[-1] > 4.5, (errs, orders)
+
+
+def test_weno5_eno_at_jump():
+    um, up = oracle.weno5_point(np.array([0.0, 0.0, 0.0, 1.0, 1.0]))
+    assert abs(um) < 1e-6          # stencil {i-2,i-1,i} dominates: no overshoot
+    um, up = oracle.weno5_point(np.array([0.0, 0.0, 1.0, 1.0, 1.0]))
+    assert abs(up - 1.0) < 1e-6     # right state taken from the smooth side
+
+
+def test_weno5_field_matches_points():
+    nb, n = (2, 1, 1), (6, 4, 4)
+    u = random_slab(2, nb, n)
+    f = oracle.Field(u, nb, n)
+    outs = oracle.weno5(f, prec=1)
+    G = blocks_to_global(u, nb, n)
+    O = [blocks_to_global(o, nb, n) for o in outs]
+    z, y, x = 2, 1, 11
+    vx = np.array([G[z, y, (x + k) % 12] for k in range(-2, 3)])
+    vy = np.array([G[z, (y + k) % 4, x] for k in range(-2, 3)])
+    vz = np.array([G[(z + k) % 4, y, x] for k in range(-2, 3)])
+    for i, v in enumerate((vx, vy, vz)):
+        um, up = oracle.weno5_point(v)
+        assert O[2 * i][z, y, x] == um and O[2 * i + 1][z, y, x] == up
