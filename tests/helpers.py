@@ -60,3 +60,21 @@ def face_bc_lists(nblocks, face_bcs):
                     if bi[d] == (0 if s == 0 else nblocks[d] - 1):
                         lists.setdefault(bx + nbx * (by + nby * bz), []).append((k, d, s, v))
     return lists
+
+
+def cells_of(slab, nblocks, cells, block_stride):
+    """Drop the per-block padding of a slab (block_bytes rounded up to 32)."""
+    nbt = int(np.prod(nblocks))
+    n = int(np.prod(cells))
+    return np.asarray(slab).reshape(nbt, block_stride)[:, :n]
+
+
+def assert_same(got, want, what=""):
+    """Bit-exact comparison with a short message."""
+    got, want = np.asarray(got), np.asarray(want)
+    assert got.shape == want.shape, (what, got.shape, want.shape)
+    if not np.array_equal(got, want):
+        d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+        bad = int(np.count_nonzero(got != want))
+        raise AssertionError("%s: %d of %d values differ, max |diff| = %g at flat index %d"
+                             % (what, bad, got.size, float(np.nanmax(d)), int(np.nanargmax(d))))

@@ -8,7 +8,7 @@ import pytest
 import cubismnova_amd as cnv
 import oracle
 from cubismnova_amd import workloads
-from tests.helpers import face_bc_lists, random_slab
+from tests.helpers import assert_same, cells_of, face_bc_lists, random_slab
 
 pytestmark = pytest.mark.gpu
 
@@ -84,26 +84,27 @@ def test_diffusion_specialised_kernels(gpu, dtype, edge, bcs):
     s = cnv.DeviceBuffer(8)
     cnv.diffusion_step(g, out, 0.1 * h[0] ** 2, sumsq=s)
     un, sref = oracle.diffusion_step(f, h, 0.1 * h[0] ** 2)
-    assert np.array_equal(out.download(0), un)
+    assert_same(out.download(0), un, "euler")
     assert abs(s.download(np.float64, 1)[0] - sref) <= 1e-13 * sref
     lap = cnv.Cartesian(nb, n, dtype=dtype)
     cnv.laplacian(g, lap)
-    assert np.array_equal(lap.download(0), oracle.laplacian(f, h))
+    assert_same(lap.download(0), oracle.laplacian(f, h), "laplacian")
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_diffusion_generic_shapes(gpu, dtype):
     for nb, n in [((3, 2, 2), (12, 8, 10)), ((1, 1, 1), (5, 7, 3)), ((2, 2, 2), (8, 8, 8))]:
-        slab = random_slab(5, nb, n, dtype)
+        bs = cnv.Cartesian(nb, n, dtype=dtype).block_stride   # padded to 32 bytes per block
+        slab = random_slab(5, nb, n, dtype, block_stride=bs)
         bcl = face_bc_lists(nb, [(2, 1, 0, 1.0), (1, 2, 1, 0.0)])
         g = setup(nb, n, dtype, slab, bcl)
-        f = oracle.Field(slab, nb, n, bcs=bcl)
+        f = oracle.Field(slab, nb, n, bs, bcs=bcl)
         h = [1.0 / (nb[d] * n[d]) for d in range(3)]
         out = cnv.Cartesian(nb, n, dtype=dtype)
         s = cnv.DeviceBuffer(8)
         cnv.diffusion_step(g, out, 1e-6, sumsq=s)
         un, sref = oracle.diffusion_step(f, h, 1e-6)
-        assert np.array_equal(out.download(0), un)
+        assert_same(cells_of(out.download(0), nb, n, bs), cells_of(un, nb, n, bs), str(n))
         assert abs(s.download(np.float64, 1)[0] - sref) <= 1e-13 * sref
 
 
@@ -165,7 +166,7 @@ def test_full_size_100_steps(gpu, dtype):
     for k in range(100):
         cur, s = oracle.diffusion_step(oracle.Field(cur, nb, n), h, dt, nthreads=nt)
         assert abs(gn[k] - s) <= 1e-13 * s, k
-    assert np.array_equal(got, cur)
+    assert_same(got, cur, "256^3 after 100 steps")
     # conservation of the periodic mean
     assert abs(got.astype(np.float64).sum() - slab.astype(np.float64).sum()) <= 1e-9 * np.abs(slab).sum()
 
