@@ -77,13 +77,23 @@ def measured_traffic(name):
 # ------------------------------------------------------------------ diffusion
 
 
-def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup):
+def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup, autotune=True, tile=None):
     nb, n, N = (8, 8, 8), (32, 32, 32), 256
     h = 1.0 / N
     dt = 0.1 * h * h
     a = cnv.Cartesian(nb, n, dtype=dtype)
     b = cnv.Cartesian(nb, n, dtype=dtype)
     a.upload_global(0, u0.astype(dtype))
+    tune = None
+    if tile:
+        cnv.set_tune_stencil7(dtype, 32, tile[0], tile[1])
+        tune = {"wg": tile[0], "zchunk": tile[1]}
+    elif autotune:
+        wg, zc, table = cnv.autotune_stencil7(a, b, iters=10)
+        tune = {"wg": wg, "zchunk": zc}
+        print("autotune %s: best wg=%d zchunk=%d; %s" % (np.dtype(dtype).name, wg, zc,
+              " ".join("%d/%d:%.1fus" % t for t in table)), file=sys.stderr, flush=True)
+        a.upload_global(0, u0.astype(dtype))
     st = cnv.Stream()
     norms = cnv.DeviceBuffer(8 * max(1, steps + warmup))
     src, dst = a, b
@@ -115,7 +125,7 @@ def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup):
         "wall_s": wall, "event_ms": kern_ms, "final": final, "norms": hist,
         "gcups": cells * steps / wall / 1e9,
         "bytes_per_launch": cells * 2 * np.dtype(dtype).itemsize,
-        "avg_kernel_s": kern_ms / 1e3 / steps,
+        "avg_kernel_s": kern_ms / 1e3 / steps, "tile": tune,
     }
 
 
@@ -134,7 +144,7 @@ def cpu_baseline_diffusion(u0, budget_s=10.0):
         cur, _ = oracle.diffusion_step(oracle.Field(cur, nb, n), h, dt, nthreads=nt)
         done += 1
         el = time.perf_counter() - t0
-        if (el >= budget_s and done >= 2) or done >= 400:
+        if (el >= budget_s and done >= 2) or done >= 5000:
             break
     return {"value": round(N ** 3 * done / el / 1e9, 4), "unit": "GCell-updates/s", "cores": nt, "kind": "port",
             "sample": "%d FP64 diffusion steps on the full 256^3 grid (C++ oracle: per-block lab + Laplacian + "
@@ -146,8 +156,8 @@ def bench_diffusion(args):
     from cubismnova_amd import workloads
     ev = HipEvents()
     u0 = workloads.diffusion_large(256)
-    r32 = diffusion_gpu(cnv, ev, np.float32, u0, args.steps, args.warmup)
-    r64 = diffusion_gpu(cnv, ev, np.float64, u0, args.steps, args.warmup)
+    r32 = diffusion_gpu(cnv, ev, np.float32, u0, args.steps, args.warmup, not args.no_autotune, args.tile_f32)
+    r64 = diffusion_gpu(cnv, ev, np.float64, u0, args.steps, args.warmup, not args.no_autotune, args.tile_f64)
     f32, f64 = r32["final"].astype(np.float64), r64["final"]
     rel = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
     nrel = float(np.max(np.abs(r32["norms"] - r64["norms"]) / r64["norms"]))
@@ -158,7 +168,8 @@ def bench_diffusion(args):
         "data": "synthetic (seeded sum of 8 Fourier modes, |k| in [1,16], amplitude 1/|k|)",
         "config": {"workload": "diffusion: 8x8x8 blocks of 32^3 cells (256^3), periodic, ghost 1, fused "
                                "lab + 7-point Laplacian + forward Euler (dt=0.1h^2) + FP64 L2 norm per step",
-                   "blocks": [8, 8, 8], "block_cells": [32, 32, 32], "steps_per_graph": args.steps},
+                   "blocks": [8, 8, 8], "block_cells": [32, 32, 32], "steps_per_graph": args.steps,
+                   "tile_f32": r32["tile"], "tile_f64": r64["tile"]},
         "roofline": roofline(r32["bytes_per_launch"], r32["avg_kernel_s"],
                              measured_traffic("stencil7_diffusion_f32_256.json")),
         "fp64": {"value": round(r64["gcups"], 3), "unit": "GCell-updates/s",
@@ -182,6 +193,10 @@ def main():
     ap.add_argument("--workload", default="diffusion", choices=["diffusion"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-autotune", action="store_true", help="use the built-in tile table")
+    tile = lambda v: tuple(int(x) for x in v.split(","))
+    ap.add_argument("--tile-f32", type=tile, default=None, help="WG,ZCHUNK for the FP32 kernel (skips autotune)")
+    ap.add_argument("--tile-f64", type=tile, default=None, help="WG,ZCHUNK for the FP64 kernel (skips autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (got --gpus %d)" % args.gpus,

@@ -472,3 +472,26 @@ def grid_sums(g, comp=0, stream=None):
     call("cnv_grid_sums", g.handle, comp, buf.ptr, _st(stream))
     r = buf.download(np.float64, 2, stream)
     return float(r[0]), float(r[1])
+
+
+def autotune_stencil7(u, out, iters=20):
+    """Pick (workgroup size, z-planes per tile) of the diffusion kernel for u's
+    (dtype, block edge) by timing every candidate; returns (wg, zchunk, table)."""
+    wg, zc, nr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    res = np.zeros(3 * 64, dtype=np.float64)
+    call("cnv_autotune_stencil7", u.handle, out.handle, int(iters), ctypes.byref(wg), ctypes.byref(zc),
+         res.ctypes.data_as(ctypes.c_void_p), 64, ctypes.byref(nr))
+    table = [(int(res[3 * i]), int(res[3 * i + 1]), float(res[3 * i + 2])) for i in range(nr.value)]
+    return wg.value, zc.value, table
+
+
+def tune_stencil7(dtype, edge):
+    wg, zc = ctypes.c_int(), ctypes.c_int()
+    call("cnv_tune_get_stencil7", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
+         ctypes.byref(wg), ctypes.byref(zc))
+    return wg.value, zc.value
+
+
+def set_tune_stencil7(dtype, edge, wg, zchunk):
+    call("cnv_tune_set_stencil7", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
+         int(wg), int(zchunk))

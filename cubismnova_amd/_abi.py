@@ -88,6 +88,9 @@ SIGNATURES = {
     "cnv_laplacian": (i32, [vp, i32, vp, i32, vp]),
     "cnv_diffusion_step": (i32, [vp, vp, f64, vp, vp, vp]),
     "cnv_grid_sums": (i32, [vp, i32, vp, vp]),
+    "cnv_autotune_stencil7": (i32, [vp, vp, i32, c_int_p, c_int_p, vp, i32, c_int_p]),
+    "cnv_tune_set_stencil7": (i32, [i32, i32, i32, i32]),
+    "cnv_tune_get_stencil7": (i32, [i32, i32, c_int_p, c_int_p]),
     "cnv_graph_begin": (i32, [vp]),
     "cnv_graph_end": (i32, [vp, ctypes.POINTER(vp)]),
     "cnv_graph_launch": (i32, [vp, vp]),

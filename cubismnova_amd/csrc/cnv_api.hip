@@ -78,10 +78,6 @@ int ensure_bc_table(cnv_grid_s *g, int comp)
 
 int ensure_partials(cnv_grid_s *g, size_t n)
 {
-    if (!g->ticket) {
-        CNV_HIP(hipMalloc((void **)&g->ticket, 64));
-        CNV_HIP(hipMemset(g->ticket, 0, 64));
-    }
     if (n > g->partials_cap) {
         if (g->partials)
             CNV_HIP(hipFree(g->partials));
@@ -299,8 +295,6 @@ int cnv_grid_destroy(cnv_grid g)
     }
     if (g->partials)
         (void)hipFree(g->partials);
-    if (g->ticket)
-        (void)hipFree(g->ticket);
     delete g;
     return CNV_OK;
 }
@@ -562,6 +556,42 @@ int cnv_grid_sums(cnv_grid g, int comp, double *out, cnv_stream s)
         return fail(CNV_ERR_INVALID, "out is NULL");
     return launch_sums(g, comp, out, (hipStream_t)s);
     CNV_GUARD_END
+}
+
+// ---------------------------------------------------------------- tuning
+
+int cnv_autotune_stencil7(cnv_grid u, cnv_grid out, int iters, int *wg, int *zchunk, double *results,
+                          int max_results, int *nresults)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_grid(out));
+    CNV_CHECK(same_shape(u, out));
+    if (u == out)
+        return fail(CNV_ERR_INVALID, "autotune needs two different grids");
+    if (out->desc.ncomp < u->desc.ncomp)
+        return fail(CNV_ERR_SIZE, "out has fewer components than u");
+    return autotune_stencil7(u, out, iters, wg, zchunk, results, max_results, nresults);
+    CNV_GUARD_END
+}
+
+int cnv_tune_set_stencil7(int dtype, int edge, int wg, int zchunk)
+{
+    if (dtype != CNV_F32 && dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    if (wg != 64 && wg != 128 && wg != 256 && wg != 512)
+        return fail(CNV_ERR_INVALID, "workgroup size must be 64, 128, 256 or 512");
+    if (zchunk < 1 || edge < 1)
+        return fail(CNV_ERR_INVALID, "edge and zchunk must be >= 1");
+    s7_set_tune(dtype, edge, wg, zchunk);
+    return CNV_OK;
+}
+
+int cnv_tune_get_stencil7(int dtype, int edge, int *wg, int *zchunk)
+{
+    if (!wg || !zchunk)
+        return fail(CNV_ERR_INVALID, "output pointer is NULL");
+    return get_tune_stencil7(dtype, edge, wg, zchunk);
 }
 
 // ---------------------------------------------------------------- graphs

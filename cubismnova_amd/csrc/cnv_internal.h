@@ -81,10 +81,9 @@ struct cnv_grid_s {
     uint64_t bc_version = 0;
     std::vector<cnv::BCTable> tables; // device copies per component
 
-    // scratch for deterministic reductions: per-tile partial sums + ticket
+    // scratch for deterministic reductions: per-tile partial sums
     double *partials = nullptr;
     size_t partials_cap = 0;
-    unsigned *ticket = nullptr;
 
     std::mutex mtx;
 };
@@ -117,27 +116,34 @@ int check_stencil(const cnv_stencil *s);
 // the last BC of `dir` is non-periodic with that side (then unwritten); every
 // non-Base BC on that face whose stencil has width in `dir`
 // (BC/Base.h:73-78) overwrites the whole slab, so the last one decides.
+// Loops run over compile-time face indices so that device code keeps the
+// arrays in registers.
 __host__ __device__ inline void resolve_faces(const DevBC *list, int n, const int sb[3],
                                               const int se[3], int kind[6], double value[6])
 {
-    bool periodic[3] = {true, true, true};
-    int skip[3] = {1, 1, 1};
-    for (int k = 0; k < n; ++k) {
-        periodic[list[k].dir] = (list[k].kind == CNV_BC_BASE);
-        skip[list[k].dir] = (list[k].side == 0) ? -1 : 1;
-    }
+#pragma unroll
     for (int f = 0; f < 6; ++f) {
         const int dir = f >> 1, side = f & 1;
-        kind[f] = (periodic[dir] || skip[dir] != (side == 0 ? -1 : 1)) ? FACE_NBR : FACE_UNSET;
-        value[f] = 0.0;
-    }
-    for (int k = 0; k < n; ++k) {
-        const int dir = list[k].dir;
-        if (list[k].kind == CNV_BC_BASE || sb[dir] == 0 || se[dir] == 1)
-            continue;
-        const int f = 2 * dir + list[k].side;
-        kind[f] = list[k].kind; // ABSORBING=1, DIRICHLET=2, SYMMETRY=3 match FaceKind
-        value[f] = list[k].value;
+        bool periodic = true;
+        int skip = 1;
+        int k_f = FACE_NBR;
+        double v_f = 0.0;
+        const bool valid = !(sb[dir] == 0 || se[dir] == 1);
+        for (int k = 0; k < n; ++k) {
+            const DevBC b = list[k];
+            if (b.dir != dir)
+                continue;
+            periodic = (b.kind == CNV_BC_BASE);
+            skip = (b.side == 0) ? -1 : 1;
+            if (valid && b.kind != CNV_BC_BASE && b.side == side) {
+                k_f = b.kind; // ABSORBING=1, DIRICHLET=2, SYMMETRY=3 match FaceKind
+                v_f = b.value;
+            }
+        }
+        if (k_f == FACE_NBR && !periodic && skip == (side == 0 ? -1 : 1))
+            k_f = FACE_UNSET;
+        kind[f] = k_f;
+        value[f] = v_f;
     }
 }
 
@@ -151,6 +157,11 @@ int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in
                     int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st);
 
 int launch_sums(cnv_grid_s *g, int comp, double *out, hipStream_t st);
+
+int autotune_stencil7(cnv_grid_s *u, cnv_grid_s *out, int iters, int *best_wg, int *best_z, double *results,
+                      int max_results, int *nresults);
+void s7_set_tune(int dtype, int edge, int wg, int zchunk);
+int get_tune_stencil7(int dtype, int edge, int *wg, int *zchunk);
 
 int launch_fill(void *dst, int dtype, double value, size_t count, hipStream_t st);
 

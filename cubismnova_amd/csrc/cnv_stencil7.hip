@@ -46,7 +46,6 @@ struct S7Args {
     double ih2[3];
     double nudt;
     double *partials;
-    unsigned *ticket;
     double *sumsq;
 };
 
@@ -82,9 +81,8 @@ __device__ inline void block_faces(const S7Args &a, int comp, long long blk, con
                                    FaceSrc<T> F[6])
 {
     const int *nb = a.G.nb, *n = a.G.n;
-    const int bx = (int)(blk % nb[0]);
-    const int by = (int)((blk / nb[0]) % nb[1]);
-    const int bz = (int)(blk / ((long long)nb[0] * nb[1]));
+    const int bidx[3] = {(int)(blk % nb[0]), (int)((blk / nb[0]) % nb[1]),
+                         (int)(blk / ((long long)nb[0] * nb[1]))};
     int kind[6];
     double val[6];
     const int sb[3] = {-1, -1, -1}, se[3] = {2, 2, 2};
@@ -92,6 +90,7 @@ __device__ inline void block_faces(const S7Args &a, int comp, long long blk, con
         const DevBCHead h = a.heads[comp][blk];
         resolve_faces(a.entries[comp] + h.offset, h.count, sb, se, kind, val);
     } else {
+#pragma unroll
         for (int f = 0; f < 6; ++f) {
             kind[f] = FACE_NBR;
             val[f] = 0.0;
@@ -99,9 +98,10 @@ __device__ inline void block_faces(const S7Args &a, int comp, long long blk, con
     }
     const long long nx = n[0], nxy = (long long)n[0] * n[1];
     const T *self = cbase + blk * a.G.bstride;
+#pragma unroll
     for (int f = 0; f < 6; ++f) {
         const int dir = f >> 1, side = f & 1;
-        int b[3] = {bx, by, bz};
+        int b[3] = {bidx[0], bidx[1], bidx[2]};
         b[dir] = (b[dir] + (side ? 1 : -1) + nb[dir]) % nb[dir];
         const long long nbr = b[0] + (long long)nb[0] * (b[1] + (long long)nb[1] * b[2]);
         const T *src = kind[f] == FACE_NBR ? cbase + nbr * a.G.bstride : self;
@@ -154,11 +154,15 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
     const int x0 = tx * VEC;
     constexpr long long SXY = (long long)NX * NY;
 
-    // halo roles: x-ghost scalar (tid < 2NY), y-ghost vector (tid < 2TX)
+    // halo roles: x-ghost scalar (tid < 2NY), y-ghost vector (tid < 2TX);
+    // faces are picked by value so no array is indexed at run time
     const bool hx_on = tid < 2 * NY;
     const int hx_side = tid / NY, hx_y = tid % NY;
     const bool hy_on = tid < 2 * TX;
     const int hy_side = tid / TX, hy_x = (tid % TX) * VEC;
+    const FaceSrc<T> FX = hx_side ? F[1] : F[0];
+    const FaceSrc<T> FY = hy_side ? F[3] : F[2];
+    const FaceSrc<T> FZ0 = F[4], FZ1 = F[5];
 
     struct Plane {
         V c[R];
@@ -173,9 +177,9 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
                 P_.c[r] = *reinterpret_cast<const V *>(ub + x0 + NX * (ty + r * ROWS) + SXY * p);
             if (halo) {
                 if (hx_on)
-                    P_.hx = face_val(F[hx_side], (long long)NX * hx_y + SXY * p);
+                    P_.hx = face_val(FX, (long long)NX * hx_y + SXY * p);
                 if (hy_on) {
-                    const FaceSrc<T> &f = F[2 + hy_side];
+                    const FaceSrc<T> f = FY;
                     if (f.mode == 2) {
 #pragma unroll
                         for (int v = 0; v < VEC; ++v)
@@ -190,7 +194,7 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
                 }
             }
         } else {
-            const FaceSrc<T> &f = F[p < 0 ? 4 : 5];
+            const FaceSrc<T> f = p < 0 ? FZ0 : FZ1;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if (f.mode == 2) {
@@ -269,44 +273,39 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
     }
 
     if (SUMSQ) {
-        // deterministic: fixed wave tree, fixed wave order, fixed tile order
+        // per-tile partial in a fixed wave tree and wave order; the tiles are
+        // summed in tile order by k_finalize_sum after this kernel
         for (int o = 32; o > 0; o >>= 1)
             acc += __shfl_xor(acc, o, 64);
         if ((tid & 63) == 0)
             red[tid >> 6] = acc;
         __syncthreads();
-        __shared__ bool last;
         if (tid == 0) {
             double s = 0.0;
             for (int w = 0; w < WG / 64; ++w)
                 s += red[w];
             a.partials[tile] = s;
-            __threadfence();
-            const unsigned t = atomicAdd(a.ticket, 1u);
-            last = (t == gridDim.x - 1);
-        }
-        __syncthreads();
-        if (last) {
-            __threadfence();
-            double s = 0.0;
-            const int n = gridDim.x;
-            for (int i = tid; i < n; i += WG)
-                s += __builtin_nontemporal_load(a.partials + i);
-            for (int o = 32; o > 0; o >>= 1)
-                s += __shfl_xor(s, o, 64);
-            __syncthreads();
-            if ((tid & 63) == 0)
-                red[tid >> 6] = s;
-            __syncthreads();
-            if (tid == 0) {
-                double t = 0.0;
-                for (int w = 0; w < WG / 64; ++w)
-                    t += red[w];
-                *a.sumsq = t;
-                *a.ticket = 0u;
-            }
         }
     }
+}
+
+// Sum of n partials in a fixed order (one workgroup): strided per-thread sums,
+// a fixed wave tree, then the waves in order.  Deterministic run to run.
+__global__ __launch_bounds__(256) void k_finalize_sum(const double *__restrict__ partials, int n,
+                                                      double *__restrict__ out)
+{
+    __shared__ double red[4];
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    for (int i = tid; i < n; i += 256)
+        s += partials[i];
+    for (int o = 32; o > 0; o >>= 1)
+        s += __shfl_xor(s, o, 64);
+    if ((tid & 63) == 0)
+        red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0)
+        *out = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 // Generic fallback (any block shape): one thread per cell, lab values read
@@ -315,7 +314,6 @@ template <typename T, int MODE, bool SUMSQ>
 __global__ __launch_bounds__(256) void k_stencil7_generic(const S7Args a)
 {
     __shared__ double red[4];
-    __shared__ bool last;
     const int tid = threadIdx.x;
     const long long nb_tot = (long long)a.G.nb[0] * a.G.nb[1] * a.G.nb[2];
     const long long slot = blockIdx.y; // comp * nblocks + blk
@@ -373,31 +371,8 @@ __global__ __launch_bounds__(256) void k_stencil7_generic(const S7Args a)
         if ((tid & 63) == 0)
             red[tid >> 6] = acc;
         __syncthreads();
-        const long long wg = (long long)blockIdx.y * gridDim.x + blockIdx.x;
-        const long long nwg = (long long)gridDim.x * gridDim.y;
-        if (tid == 0) {
-            a.partials[wg] = ((red[0] + red[1]) + red[2]) + red[3];
-            __threadfence();
-            const unsigned t = atomicAdd(a.ticket, 1u);
-            last = (t == (unsigned)(nwg - 1));
-        }
-        __syncthreads();
-        if (last) {
-            __threadfence();
-            double s = 0.0;
-            for (long long j = tid; j < nwg; j += 256)
-                s += __builtin_nontemporal_load(a.partials + j);
-            for (int o = 32; o > 0; o >>= 1)
-                s += __shfl_xor(s, o, 64);
-            __syncthreads();
-            if ((tid & 63) == 0)
-                red[tid >> 6] = s;
-            __syncthreads();
-            if (tid == 0) {
-                *a.sumsq = ((red[0] + red[1]) + red[2]) + red[3];
-                *a.ticket = 0u;
-            }
-        }
+        if (tid == 0)
+            a.partials[(long long)blockIdx.y * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
     }
 }
 
@@ -485,13 +460,13 @@ static bool dispatch_spec(const S7Args &a, int edge, int wg, int ntiles, hipStre
 }
 
 template <typename T, int MODE, bool SUMSQ>
-static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic)
+static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic, const S7Tune *over)
 {
     const int *n = a.G.n;
     const long long nbt = (long long)a.G.nb[0] * a.G.nb[1] * a.G.nb[2];
     const bool cubic_xy = n[0] == n[1];
     if (!force_generic && cubic_xy && (n[0] == 16 || n[0] == 32 || n[0] == 64)) {
-        S7Tune tu = s7_default_tune(sizeof(T) == 8 ? CNV_F64 : CNV_F32, n[0]);
+        S7Tune tu = over ? *over : s7_default_tune(sizeof(T) == 8 ? CNV_F64 : CNV_F32, n[0]);
         a.zchunk = tu.zchunk < n[2] ? tu.zchunk : n[2];
         a.nzc = (n[2] + a.zchunk - 1) / a.zchunk;
         a.tiles_per_comp = (int)(nbt * a.nzc);
@@ -499,8 +474,14 @@ static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic)
         if (ntiles < (1LL << 31)) {
             if (dispatch_spec<T, MODE, SUMSQ>(a, n[0], tu.wg, (int)ntiles, st)) {
                 CNV_HIP(hipGetLastError());
+                if (SUMSQ) {
+                    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(256), 0, st, a.partials, (int)ntiles, a.sumsq);
+                    CNV_HIP(hipGetLastError());
+                }
                 return CNV_OK;
             }
+            if (over)
+                return fail(CNV_ERR_UNSUPPORTED, "tile shape not available for this block edge and dtype");
         }
     }
     const long long cells = (long long)n[0] * n[1] * n[2];
@@ -511,6 +492,10 @@ static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic)
     hipLaunchKernelGGL((k_stencil7_generic<T, MODE, SUMSQ>), dim3((unsigned)gx, (unsigned)gy), dim3(256),
                        0, st, a);
     CNV_HIP(hipGetLastError());
+    if (SUMSQ) {
+        hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(256), 0, st, a.partials, (int)(gx * gy), a.sumsq);
+        CNV_HIP(hipGetLastError());
+    }
     return CNV_OK;
 }
 
@@ -548,8 +533,9 @@ static int check_faces(cnv_grid_s *g, int comp)
     return CNV_OK;
 }
 
-int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
-                    int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st)
+static int launch_stencil7_t(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
+                             int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st,
+                             const S7Tune *over)
 {
     if (ncomp > kMaxComp)
         return fail(CNV_ERR_UNSUPPORTED, "more than 8 components in one stencil call");
@@ -589,7 +575,6 @@ int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in
     a.nudt = nu_dt;
     a.sumsq = sumsq;
     a.partials = nullptr;
-    a.ticket = nullptr;
     a.zchunk = a.nzc = a.tiles_per_comp = 0;
     if (sumsq) {
         const long long nbt = u->nblocks_total;
@@ -600,30 +585,126 @@ int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in
         if (rc != CNV_OK)
             return rc;
         a.partials = u->partials;
-        a.ticket = u->ticket;
     }
-    const bool gen = force_generic();
+    const bool gen = over ? false : force_generic();
     const bool dbl = u->desc.dtype == CNV_F64;
     if (mode == 0)
-        return dbl ? run_t<double, 0, false>(a, ncomp, st, gen) : run_t<float, 0, false>(a, ncomp, st, gen);
+        return dbl ? run_t<double, 0, false>(a, ncomp, st, gen, over)
+                   : run_t<float, 0, false>(a, ncomp, st, gen, over);
     if (mode == 1) {
         if (sumsq)
-            return dbl ? run_t<double, 1, true>(a, ncomp, st, gen) : run_t<float, 1, true>(a, ncomp, st, gen);
-        return dbl ? run_t<double, 1, false>(a, ncomp, st, gen) : run_t<float, 1, false>(a, ncomp, st, gen);
+            return dbl ? run_t<double, 1, true>(a, ncomp, st, gen, over)
+                       : run_t<float, 1, true>(a, ncomp, st, gen, over);
+        return dbl ? run_t<double, 1, false>(a, ncomp, st, gen, over)
+                   : run_t<float, 1, false>(a, ncomp, st, gen, over);
     }
     if (sumsq)
-        return dbl ? run_t<double, 2, true>(a, ncomp, st, gen) : run_t<float, 2, true>(a, ncomp, st, gen);
-    return dbl ? run_t<double, 2, false>(a, ncomp, st, gen) : run_t<float, 2, false>(a, ncomp, st, gen);
+        return dbl ? run_t<double, 2, true>(a, ncomp, st, gen, over)
+                   : run_t<float, 2, true>(a, ncomp, st, gen, over);
+    return dbl ? run_t<double, 2, false>(a, ncomp, st, gen, over)
+               : run_t<float, 2, false>(a, ncomp, st, gen, over);
+}
+
+int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
+                    int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st)
+{
+    return launch_stencil7_t(u, out, lap, comp_in, comp_out, ncomp, mode, nu_dt, sumsq, st, nullptr);
+}
+
+// ---------------------------------------------------------------- autotuner
+
+// Times every (workgroup size, z-planes per tile) candidate of the fused
+// diffusion step (with the FP64 norm) on u -> out and keeps the fastest for
+// (dtype, block edge).  Candidates that the block shape cannot take are
+// skipped by the dispatcher's validity check.
+int autotune_stencil7(cnv_grid_s *u, cnv_grid_s *out, int iters, int *best_wg, int *best_z, double *results,
+                      int max_results, int *nresults)
+{
+    const int *n = u->desc.block_cells;
+    if (!(n[0] == n[1] && (n[0] == 16 || n[0] == 32 || n[0] == 64)))
+        return fail(CNV_ERR_UNSUPPORTED, "autotuning needs x/y block edges of 16, 32 or 64");
+    if (iters < 1)
+        return fail(CNV_ERR_INVALID, "iters must be >= 1");
+    hipStream_t st;
+    CNV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    double *norm = nullptr;
+    hipEvent_t e0, e1;
+    CNV_HIP(hipMalloc((void **)&norm, sizeof(double)));
+    CNV_HIP(hipEventCreate(&e0));
+    CNV_HIP(hipEventCreate(&e1));
+    const int wgs[4] = {64, 128, 256, 512};
+    const int zcs[6] = {2, 4, 8, 16, 32, 64};
+    float best = 1e30f;
+    int bw = 0, bz = 0, cnt = 0;
+    int rc = CNV_OK;
+    for (int wi = 0; wi < 4 && rc == CNV_OK; ++wi)
+        for (int zi = 0; zi < 6 && rc == CNV_OK; ++zi) {
+            if (zcs[zi] > n[2])
+                continue;
+            S7Tune tu{wgs[wi], zcs[zi]};
+            int r = launch_stencil7_t(u, out, nullptr, 0, 0, u->desc.ncomp, 1, 1e-9, norm, st, &tu);
+            if (r == CNV_ERR_UNSUPPORTED)
+                continue;
+            if (r != CNV_OK) {
+                rc = r;
+                break;
+            }
+            (void)hipEventRecord(e0, st);
+            for (int i = 0; i < iters; ++i)
+                (void)launch_stencil7_t(u, out, nullptr, 0, 0, u->desc.ncomp, 1, 1e-9, norm, st, &tu);
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                rc = fail(CNV_ERR_HIP, "autotune: kernel failed");
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const float us = ms * 1000.f / iters;
+            if (results && cnt < max_results) {
+                results[3 * cnt] = tu.wg;
+                results[3 * cnt + 1] = tu.zchunk;
+                results[3 * cnt + 2] = us;
+            }
+            ++cnt;
+            if (us < best) {
+                best = us;
+                bw = tu.wg;
+                bz = tu.zchunk;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(norm);
+    (void)hipStreamDestroy(st);
+    if (rc != CNV_OK)
+        return rc;
+    if (bw == 0)
+        return fail(CNV_ERR_UNSUPPORTED, "no tile candidate ran");
+    s7_set_tune(u->desc.dtype, n[0], bw, bz);
+    if (best_wg)
+        *best_wg = bw;
+    if (best_z)
+        *best_z = bz;
+    if (nresults)
+        *nresults = cnt < max_results ? cnt : max_results;
+    return CNV_OK;
+}
+
+int get_tune_stencil7(int dtype, int edge, int *wg, int *zchunk)
+{
+    S7Tune t = s7_default_tune(dtype, edge);
+    *wg = t.wg;
+    *zchunk = t.zchunk;
+    return CNV_OK;
 }
 
 // ---------------------------------------------------------------- sums
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_sums(const T *p, long long bstride, long long cells, long long nbt,
-                                                double *partials, unsigned *ticket, double *out)
+                                                double *partials)
 {
     __shared__ double red[2][4];
-    __shared__ bool last;
     const int tid = threadIdx.x;
     double s = 0.0, q = 0.0;
     for (long long b = blockIdx.x; b < nbt; b += gridDim.x) {
@@ -644,34 +725,8 @@ __global__ __launch_bounds__(256) void k_sums(const T *p, long long bstride, lon
     }
     __syncthreads();
     if (tid == 0) {
-        partials[2 * blockIdx.x] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
-        partials[2 * blockIdx.x + 1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        double a = 0.0, b = 0.0;
-        for (int i = tid; i < (int)gridDim.x; i += 256) {
-            a += __builtin_nontemporal_load(partials + 2 * i);
-            b += __builtin_nontemporal_load(partials + 2 * i + 1);
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            a += __shfl_xor(a, o, 64);
-            b += __shfl_xor(b, o, 64);
-        }
-        __syncthreads();
-        if ((tid & 63) == 0) {
-            red[0][tid >> 6] = a;
-            red[1][tid >> 6] = b;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            out[0] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
-            out[1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
-            *ticket = 0u;
-        }
+        partials[blockIdx.x] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+        partials[gridDim.x + blockIdx.x] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
     }
 }
 
@@ -690,11 +745,14 @@ int launch_sums(cnv_grid_s *g, int comp, double *out, hipStream_t st)
     if (g->desc.dtype == CNV_F64)
         hipLaunchKernelGGL(k_sums<double>, dim3(nwg), dim3(256), 0, st,
                            reinterpret_cast<const double *>((const char *)g->data + comp * g->component_bytes),
-                           G.bstride, cells, nbt, g->partials, g->ticket, out);
+                           G.bstride, cells, nbt, g->partials);
     else
         hipLaunchKernelGGL(k_sums<float>, dim3(nwg), dim3(256), 0, st,
                            reinterpret_cast<const float *>((const char *)g->data + comp * g->component_bytes),
-                           G.bstride, cells, nbt, g->partials, g->ticket, out);
+                           G.bstride, cells, nbt, g->partials);
+    CNV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(256), 0, st, g->partials, nwg, out);
+    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(256), 0, st, g->partials + nwg, nwg, out + 1);
     CNV_HIP(hipGetLastError());
     return CNV_OK;
 }

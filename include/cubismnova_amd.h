@@ -173,13 +173,25 @@ int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_ou
  * out[1] (deterministic). */
 int cnv_grid_sums(cnv_grid g, int comp, double *out, cnv_stream s);
 
+/* ------------------------------------------------------------ autotuner */
+/* No counterpart in the reference.  Times every valid (workgroup size,
+ * z-planes per tile) pair of the fused diffusion kernel on u -> out (`iters`
+ * launches each; `out` is overwritten), keeps the fastest in the
+ * process-wide table used by cnv_laplacian / cnv_diffusion_step for
+ * (dtype, block edge) and returns it.  results (may be NULL) receives up to
+ * max_results triples {wg, zchunk, microseconds per launch}. */
+int cnv_autotune_stencil7(cnv_grid u, cnv_grid out, int iters, int *wg, int *zchunk, double *results,
+                          int max_results, int *nresults);
+int cnv_tune_set_stencil7(int dtype, int edge, int wg, int zchunk);
+int cnv_tune_get_stencil7(int dtype, int edge, int *wg, int *zchunk);
+
 /* ------------------------------------------------------------ HIP graphs */
 /* No counterpart in the reference (its loop is host code per block): record
  * the calls issued on stream `s` between begin and end into an executable
  * graph, replayed with cnv_graph_launch.  Every entry point above is
  * capturable after it has run once on the same grids (first calls upload
- * BC tables synchronously); kernels clear their own counters, so no memset
- * node is ever recorded. */
+ * BC tables synchronously); reductions use plain stores and a fixed-order
+ * finalize kernel, so no memset or counter node is ever recorded. */
 typedef struct cnv_graph_s *cnv_graph;
 int cnv_graph_begin(cnv_stream s);
 int cnv_graph_end(cnv_stream s, cnv_graph *out);
