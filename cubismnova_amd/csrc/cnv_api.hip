@@ -45,6 +45,17 @@ int ensure_bc_table(cnv_grid_s *g, int comp)
     BCTable &t = g->tables[comp];
     if (t.version == g->bc_version)
         return CNV_OK;
+    bool none = true;
+    for (const auto &l : g->bcs[comp])
+        if (!l.empty()) {
+            none = false;
+            break;
+        }
+    if (none) { // kernels read no table when no block carries a BC
+        t.any = false;
+        t.version = g->bc_version;
+        return CNV_OK;
+    }
     const long long nb = g->nblocks_total;
     std::vector<DevBCHead> heads((size_t)nb);
     std::vector<DevBC> entries;
@@ -277,6 +288,16 @@ int cnv_grid_create(const cnv_grid_desc *desc, cnv_grid *out)
         delete g;
         return fail(CNV_ERR_ALLOC, std::string("grid allocation: ") + hipGetErrorString(e));
     }
+    // reduction scratch sized for the largest launch of any operator, so that
+    // no call allocates later (calls may be recorded into a HIP graph)
+    const size_t nparts = (size_t)nbt * desc->ncomp * ((size_t)(cells + 255) / 256 + desc->block_cells[2]) + 4096;
+    e = hipMalloc((void **)&g->partials, sizeof(double) * nparts);
+    if (e != hipSuccess) {
+        (void)hipFree(g->data);
+        delete g;
+        return fail(CNV_ERR_ALLOC, std::string("scratch allocation: ") + hipGetErrorString(e));
+    }
+    g->partials_cap = nparts;
     *out = g;
     return CNV_OK;
     CNV_GUARD_END

@@ -289,23 +289,37 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
     }
 }
 
-// Sum of n partials in a fixed order (one workgroup): strided per-thread sums,
-// a fixed wave tree, then the waves in order.  Deterministic run to run.
-__global__ __launch_bounds__(256) void k_finalize_sum(const double *__restrict__ partials, int n,
-                                                      double *__restrict__ out)
+// Sum of n partials in a fixed order (one workgroup of 1024 threads): every
+// thread issues its loads for a round of 8 before adding them in index order,
+// then a fixed wave tree and the waves in order.  Deterministic run to run.
+__global__ __launch_bounds__(1024) void k_finalize_sum(const double *__restrict__ partials, int n,
+                                                       double *__restrict__ out)
 {
-    __shared__ double red[4];
+    __shared__ double red[16];
     const int tid = threadIdx.x;
     double s = 0.0;
-    for (int i = tid; i < n; i += 256)
-        s += partials[i];
+    for (int base = 0; base < n; base += 8 * 1024) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int i = base + j * 1024 + tid;
+            v[j] = i < n ? partials[i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            s += v[j];
+    }
     for (int o = 32; o > 0; o >>= 1)
         s += __shfl_xor(s, o, 64);
     if ((tid & 63) == 0)
         red[tid >> 6] = s;
     __syncthreads();
-    if (tid == 0)
-        *out = ((red[0] + red[1]) + red[2]) + red[3];
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; ++w)
+            t += red[w];
+        *out = t;
+    }
 }
 
 // Generic fallback (any block shape): one thread per cell, lab values read
@@ -475,7 +489,7 @@ static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic, const 
             if (dispatch_spec<T, MODE, SUMSQ>(a, n[0], tu.wg, (int)ntiles, st)) {
                 CNV_HIP(hipGetLastError());
                 if (SUMSQ) {
-                    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(256), 0, st, a.partials, (int)ntiles, a.sumsq);
+                    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, a.partials, (int)ntiles, a.sumsq);
                     CNV_HIP(hipGetLastError());
                 }
                 return CNV_OK;
@@ -493,7 +507,7 @@ static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic, const 
                        0, st, a);
     CNV_HIP(hipGetLastError());
     if (SUMSQ) {
-        hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(256), 0, st, a.partials, (int)(gx * gy), a.sumsq);
+        hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, a.partials, (int)(gx * gy), a.sumsq);
         CNV_HIP(hipGetLastError());
     }
     return CNV_OK;
@@ -751,8 +765,8 @@ int launch_sums(cnv_grid_s *g, int comp, double *out, hipStream_t st)
                            reinterpret_cast<const float *>((const char *)g->data + comp * g->component_bytes),
                            G.bstride, cells, nbt, g->partials);
     CNV_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(256), 0, st, g->partials, nwg, out);
-    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(256), 0, st, g->partials + nwg, nwg, out + 1);
+    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, g->partials, nwg, out);
+    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, g->partials + nwg, nwg, out + 1);
     CNV_HIP(hipGetLastError());
     return CNV_OK;
 }
