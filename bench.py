@@ -185,12 +185,122 @@ def bench_diffusion(args):
     return out
 
 
+# ------------------------------------------------------------------ WENO5
+
+
+def weno_gpu(cnv, ev, dtype, steps, warmup, autotune, comps):
+    N, nb, n = 512, (16, 16, 16), (32, 32, 32)
+    g = cnv.Cartesian(nb, n, dtype=dtype, ncomp=5)
+    for c in range(5):
+        g.upload_global(c, comps[c].astype(dtype))
+    for bc in (cnv.Absorbing(1, 0), cnv.Absorbing(1, 1), cnv.Absorbing(2, 0), cnv.Absorbing(2, 1)):
+        g.addFaceBC(bc)              # x periodic, y/z zero-gradient
+    out = cnv.Cartesian(nb, n, dtype=dtype, ncomp=30)
+    tune = None
+    if autotune:
+        wg, zc, table = cnv.autotune_weno5(g, out, iters=3)
+        tune = {"wg": wg, "zchunk": zc}
+        print("autotune weno5 %s: best wg=%d zchunk=%d; %s" % (np.dtype(dtype).name, wg, zc,
+              " ".join("%d/%d:%.0fus" % t for t in table)), file=sys.stderr, flush=True)
+    st = cnv.Stream()
+    for _ in range(warmup):
+        cnv.weno5(g, out, stream=st)
+    st.synchronize()
+    gr = cnv.Graph(st).capture()
+    for _ in range(steps):
+        cnv.weno5(g, out, stream=st)
+    gr.end()
+    e0, e1 = ev.new(), ev.new()
+    ev.device_sync()
+    t0 = time.perf_counter()
+    ev.record(e0, st)
+    gr.launch()
+    ev.record(e1, st)
+    st.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = ev.elapsed_ms(e0, e1)
+    cells = N ** 3
+    return {"grid": g, "out": out, "wall_s": wall, "gcups": cells * steps / wall / 1e9,
+            "bytes_per_launch": cells * 35 * np.dtype(dtype).itemsize, "avg_kernel_s": kern_ms / 1e3 / steps,
+            "tile": tune}
+
+
+def cpu_baseline_weno(grid64, budget_s=10.0):
+    import oracle
+    nt = max(1, min(16, os.cpu_count() or 1))
+    nb, n = grid64.nblocks, grid64.block_cells
+    cells_blk = int(np.prod(n))
+    slab = grid64.download(0)
+    bcl = {}
+    for b in range(grid64.nblocks_total):
+        by, bz = (b // nb[0]) % nb[1], b // (nb[0] * nb[1])
+        lst = []
+        if by == 0:
+            lst.append((1, 1, 0, 0.0))
+        if by == nb[1] - 1:
+            lst.append((1, 1, 1, 0.0))
+        if bz == 0:
+            lst.append((1, 2, 0, 0.0))
+        if bz == nb[2] - 1:
+            lst.append((1, 2, 1, 0.0))
+        bcl[b] = lst
+    f = oracle.Field(slab, nb, n, grid64.block_stride, bcl)
+    done, t0 = 0, time.perf_counter()
+    chunk = 2 * nt
+    while True:
+        sel = [(done + j) % grid64.nblocks_total for j in range(chunk)]
+        oracle.weno5(f, prec=1, nthreads=nt, blocks=sel)
+        done += chunk
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    # one cell-update = all 5 components: the sample ran one component
+    return {"value": round(done * cells_blk / 5 / el / 1e9, 5), "unit": "GCell-updates/s", "cores": nt,
+            "kind": "port",
+            "sample": "%d blocks of 32^3 of one component (x/y/z WENO5 states, FP64 C++ oracle with per-block "
+                      "labs, OpenMP over blocks) in %.1f s, scaled to 5 components" % (done, el)}
+
+
+def bench_weno5(args):
+    import cubismnova_amd as cnv
+    from cubismnova_amd import workloads
+    ev = HipEvents()
+    comps = [workloads.weno_component(512, c) for c in range(5)]
+    r32 = weno_gpu(cnv, ev, np.float32, args.steps, args.warmup, not args.no_autotune, comps)
+    r64 = weno_gpu(cnv, ev, np.float64, args.steps, args.warmup, not args.no_autotune, comps)
+    del comps
+    worst = 0.0
+    for k in range(30):
+        d2, b2, _ = cnv.grid_compare(r32["out"], r64["out"], k, k)
+        worst = max(worst, float(np.sqrt(d2 / b2)))
+    out = {
+        "metric": METRIC, "value": round(r32["gcups"], 3), "unit": "GCell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (smooth compressible state + tanh pressure front, ratio 10, seeded)",
+        "config": {"workload": "WENO5 face states in x, y, z: 16x16x16 blocks of 32^3 (512^3), 5 components SoA, "
+                               "ghost 3, x periodic, y/z absorbing", "blocks": [16, 16, 16],
+                   "block_cells": [32, 32, 32], "components": 5, "tile_f32": r32["tile"], "tile_f64": r64["tile"]},
+        "roofline": roofline(r32["bytes_per_launch"], r32["avg_kernel_s"],
+                             measured_traffic("weno5_f32_512.json")),
+        "fp64": {"value": round(r64["gcups"], 3), "unit": "GCell-updates/s",
+                 "ms_per_step": round(r64["wall_s"] * 1e3 / args.steps, 4),
+                 "roofline": roofline(r64["bytes_per_launch"], r64["avg_kernel_s"],
+                                      measured_traffic("weno5_f64_512.json"))},
+        "fp32_over_fp64": round(r32["gcups"] / r64["gcups"], 3),
+        "fp32_rel_l2_vs_fp64_worst_state": worst,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_weno(r64["grid"], args.cpu_seconds)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="diffusion", choices=["diffusion"])
+    ap.add_argument("--workload", default="diffusion", choices=["diffusion", "weno5"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in tile table")
@@ -204,7 +314,7 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
-    out = bench_diffusion(args)
+    out = bench_weno5(args) if args.workload == "weno5" else bench_diffusion(args)
     print(json.dumps(out), flush=True)
 
 

@@ -308,6 +308,16 @@ class Cartesian:
              ctypes.c_size_t(out.nbytes), _st(stream))
         return out
 
+    def download_blocks(self, comp, blocks):
+        """Host copy of selected blocks (flat indices) of one component, one
+        block_stride each, in the order given."""
+        base = self.data_ptr(comp).value
+        out = np.empty((len(blocks), self.block_stride), dtype=self.dtype)
+        for j, b in enumerate(blocks):
+            call("cnv_memcpy_d2h", out[j].ctypes.data_as(ctypes.c_void_p),
+                 ctypes.c_void_p(base + int(b) * self.block_bytes), ctypes.c_size_t(self.block_bytes), None)
+        return out.reshape(-1)
+
     def upload_global(self, comp, u, stream=None):
         """Upload a global (Z, Y, X) array into the block layout."""
         self.upload(comp, global_to_blocks(np.asarray(u, dtype=self.dtype), self.nblocks, self.block_cells,
@@ -495,3 +505,31 @@ def tune_stencil7(dtype, edge):
 def set_tune_stencil7(dtype, edge, wg, zchunk):
     call("cnv_tune_set_stencil7", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
          int(wg), int(zchunk))
+
+
+def weno5(u, out, stream=None):
+    """WENO5 face states of every component of u into out (6 components each:
+    6c+2d = u^-_{i+1/2}, 6c+2d+1 = u^+_{i-1/2} in direction d)."""
+    call("cnv_weno5", u.handle, out.handle, _st(stream))
+
+
+def autotune_weno5(u, out, iters=5):
+    wg, zc, nr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    res = np.zeros(3 * 64, dtype=np.float64)
+    call("cnv_autotune_weno5", u.handle, out.handle, int(iters), ctypes.byref(wg), ctypes.byref(zc),
+         res.ctypes.data_as(ctypes.c_void_p), 64, ctypes.byref(nr))
+    table = [(int(res[3 * i]), int(res[3 * i + 1]), float(res[3 * i + 2])) for i in range(nr.value)]
+    return wg.value, zc.value, table
+
+
+def grid_compare(a, b, comp_a=0, comp_b=0, stream=None):
+    """(sum (a-b)^2, sum b^2, max|a-b|) over one component, FP64 on the device."""
+    buf = DeviceBuffer(24)
+    call("cnv_grid_compare", a.handle, comp_a, b.handle, comp_b, buf.ptr, _st(stream))
+    r = buf.download(np.float64, 3, stream)
+    return float(r[0]), float(r[1]), float(r[2])
+
+
+def set_tune_weno5(dtype, wg, zchunk):
+    call("cnv_tune_set_weno5", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(wg),
+         int(zchunk))

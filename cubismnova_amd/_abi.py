@@ -91,6 +91,11 @@ SIGNATURES = {
     "cnv_autotune_stencil7": (i32, [vp, vp, i32, c_int_p, c_int_p, vp, i32, c_int_p]),
     "cnv_tune_set_stencil7": (i32, [i32, i32, i32, i32]),
     "cnv_tune_get_stencil7": (i32, [i32, i32, c_int_p, c_int_p]),
+    "cnv_grid_compare": (i32, [vp, i32, vp, i32, vp, vp]),
+    "cnv_weno5": (i32, [vp, vp, vp]),
+    "cnv_autotune_weno5": (i32, [vp, vp, i32, c_int_p, c_int_p, vp, i32, c_int_p]),
+    "cnv_tune_set_weno5": (i32, [i32, i32, i32]),
+    "cnv_tune_get_weno5": (i32, [i32, c_int_p, c_int_p]),
     "cnv_graph_begin": (i32, [vp]),
     "cnv_graph_end": (i32, [vp, ctypes.POINTER(vp)]),
     "cnv_graph_launch": (i32, [vp, vp]),

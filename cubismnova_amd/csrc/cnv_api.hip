@@ -579,6 +579,67 @@ int cnv_grid_sums(cnv_grid g, int comp, double *out, cnv_stream s)
     CNV_GUARD_END
 }
 
+int cnv_grid_compare(cnv_grid a, int comp_a, cnv_grid b, int comp_b, double *out, cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(a));
+    CNV_CHECK(check_grid(b));
+    CNV_CHECK(check_comp(a, comp_a));
+    CNV_CHECK(check_comp(b, comp_b));
+    if (!out)
+        return fail(CNV_ERR_INVALID, "out is NULL");
+    for (int d = 0; d < 3; ++d)
+        if (a->desc.nblocks[d] != b->desc.nblocks[d] || a->desc.block_cells[d] != b->desc.block_cells[d])
+            return fail(CNV_ERR_SIZE, "grids differ in block count or block size");
+    return launch_compare(a, comp_a, b, comp_b, out, (hipStream_t)s);
+    CNV_GUARD_END
+}
+
+int cnv_weno5(cnv_grid in, cnv_grid out, cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(in));
+    CNV_CHECK(check_grid(out));
+    CNV_CHECK(same_shape(in, out));
+    if (in == out)
+        return fail(CNV_ERR_INVALID, "input and output grids must differ");
+    if (out->desc.ncomp < 6 * in->desc.ncomp)
+        return fail(CNV_ERR_SIZE, "out needs 6 components per input component");
+    return launch_weno5(in, out, (hipStream_t)s, nullptr);
+    CNV_GUARD_END
+}
+
+int cnv_autotune_weno5(cnv_grid in, cnv_grid out, int iters, int *wg, int *zchunk, double *results,
+                       int max_results, int *nresults)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(in));
+    CNV_CHECK(check_grid(out));
+    CNV_CHECK(same_shape(in, out));
+    if (in == out || out->desc.ncomp < 6 * in->desc.ncomp)
+        return fail(CNV_ERR_INVALID, "out must be a separate grid with 6 components per input component");
+    return autotune_weno5(in, out, iters, wg, zchunk, results, max_results, nresults);
+    CNV_GUARD_END
+}
+
+int cnv_tune_set_weno5(int dtype, int wg, int zchunk)
+{
+    if (dtype != CNV_F32 && dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    if (wg != 64 && wg != 128 && wg != 256)
+        return fail(CNV_ERR_INVALID, "workgroup size must be 64, 128 or 256");
+    if (zchunk < 1)
+        return fail(CNV_ERR_INVALID, "zchunk must be >= 1");
+    return weno5_set_tune(dtype, wg, zchunk);
+}
+
+int cnv_tune_get_weno5(int dtype, int *wg, int *zchunk)
+{
+    if (!wg || !zchunk)
+        return fail(CNV_ERR_INVALID, "output pointer is NULL");
+    return weno5_get_tune(dtype, wg, zchunk);
+}
+
 // ---------------------------------------------------------------- tuning
 
 int cnv_autotune_stencil7(cnv_grid u, cnv_grid out, int iters, int *wg, int *zchunk, double *results,

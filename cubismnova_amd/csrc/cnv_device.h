@@ -1,0 +1,118 @@
+// cnv_device.h -- device helpers shared by the z-marching stencil kernels.
+#pragma once
+
+#include "cnv_internal.h"
+
+namespace cnv
+{
+
+template <typename T, int N>
+struct alignas(16) VecT {
+    T v[N];
+};
+
+// Streaming (non-temporal) 16-byte store of a VecT.
+template <typename T, int N>
+__device__ inline void store_nt(T *p, const VecT<T, N> &v)
+{
+    typedef T vt __attribute__((ext_vector_type(N)));
+    vt x;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        x[i] = v.v[i];
+    __builtin_nontemporal_store(x, reinterpret_cast<vt *>(p));
+}
+
+// Workgroup i is dispatched to XCD i % 8; give each XCD a contiguous range
+// of tiles so that the tiles sharing halo lines meet in the same L2.
+__device__ inline int xcd_remap(int i, int n)
+{
+    const int x = i & 7, local = i >> 3;
+    const int q = n >> 3, r = n & 7;
+    return x * q + (x < r ? x : r) + local;
+}
+
+// Source of the ghost layers of one block face.  Ghost layer l (1 = next to
+// the face) at in-face offset i reads p[(li0 + lstep*l) * dstride + i], then
+// copies it (mode 0), scales it by s (mode 1, Symmetry), or is the constant s
+// (mode 2, Dirichlet).  Derivation from the lab rules:
+//   neighbour, side 0: layer l is the neighbour's index N-l  (li0 = N,  lstep = -1)
+//   neighbour, side 1: layer l is the neighbour's index l-1  (li0 = -1, lstep = +1)
+//   absorbing, side 0/1: index 0 / N-1 for every layer       (lstep = 0)
+//   symmetry,  side 0: mirror -1-(-l) = l-1                  (li0 = -1, lstep = +1)
+//   symmetry,  side 1: mirror 2N-1-(N-1+l) = N-l             (li0 = N,  lstep = -1)
+template <typename T>
+struct FaceL {
+    const T *p;
+    int li0;
+    int lstep;
+    int mode;
+    T s;
+};
+
+template <typename T>
+__device__ inline T face_layer(const FaceL<T> &f, int l, long long dstride, long long i)
+{
+    if (f.mode == 2)
+        return f.s;
+    const T v = f.p[(long long)(f.li0 + f.lstep * l) * dstride + i];
+    return f.mode == 1 ? f.s * v : v;
+}
+
+// Six faces of block `blk` for a non-tensorial stencil (sb, se) from the
+// block's BC list (heads/entries may be null when no block carries a BC).
+template <typename T>
+__device__ inline void make_faces(const Geom &G, const DevBCHead *heads, const DevBC *entries, int has_bc,
+                                  long long blk, const T *cbase, const int sb[3], const int se[3], FaceL<T> F[6])
+{
+    const int *nb = G.nb, *n = G.n;
+    const int bidx[3] = {(int)(blk % nb[0]), (int)((blk / nb[0]) % nb[1]),
+                         (int)(blk / ((long long)nb[0] * nb[1]))};
+    int kind[6];
+    double val[6];
+    if (has_bc) {
+        const DevBCHead h = heads[blk];
+        resolve_faces(entries + h.offset, h.count, sb, se, kind, val);
+    } else {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            kind[f] = FACE_NBR;
+            val[f] = 0.0;
+        }
+    }
+    const T *self = cbase + blk * G.bstride;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1, side = f & 1;
+        int b[3] = {bidx[0], bidx[1], bidx[2]};
+        b[dir] = (b[dir] + (side ? 1 : -1) + nb[dir]) % nb[dir];
+        const long long nbr = b[0] + (long long)nb[0] * (b[1] + (long long)nb[1] * b[2]);
+        const int N = n[dir];
+        FaceL<T> F_;
+        F_.s = (T)val[f];
+        if (kind[f] == FACE_NBR) {
+            F_.p = cbase + nbr * G.bstride;
+            F_.li0 = side ? -1 : N;
+            F_.lstep = side ? 1 : -1;
+            F_.mode = 0;
+        } else if (kind[f] == FACE_ABSORB) {
+            F_.p = self;
+            F_.li0 = side ? N - 1 : 0;
+            F_.lstep = 0;
+            F_.mode = 0;
+        } else if (kind[f] == FACE_SYMMETRY) {
+            F_.p = self;
+            F_.li0 = side ? N : -1;
+            F_.lstep = side ? -1 : 1;
+            F_.mode = 1;
+        } else { // Dirichlet (unwritten faces are rejected on the host)
+            F_.p = self;
+            F_.li0 = 0;
+            F_.lstep = 0;
+            F_.mode = 2;
+        }
+        F[f] = F_;
+    }
+}
+
+} // namespace cnv

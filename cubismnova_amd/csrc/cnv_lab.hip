@@ -32,7 +32,8 @@ struct LabArgs {
     int block[3];
     int nlist;
     DevBC list[kMaxBC];
-    // all-blocks mode: per-block tables
+    // all-blocks mode: per-block tables (null when no block carries a BC)
+    int has_tab;
     const DevBCHead *heads;
     const DevBC *entries;
 };
@@ -143,10 +144,13 @@ __global__ __launch_bounds__(256) void k_lab(const LabArgs<T> a)
         B[0] = (int)(slot % a.G.nb[0]);
         B[1] = (int)((slot / a.G.nb[0]) % a.G.nb[1]);
         B[2] = (int)(slot / ((long long)a.G.nb[0] * a.G.nb[1]));
-        const DevBCHead h = a.heads[slot];
-        const int n = h.count < kMaxBC ? h.count : kMaxBC;
-        if ((int)threadIdx.x < n)
-            s_list[threadIdx.x] = a.entries[h.offset + threadIdx.x];
+        int n = 0;
+        if (a.has_tab) {
+            const DevBCHead h = a.heads[slot];
+            n = h.count < kMaxBC ? h.count : kMaxBC;
+            if ((int)threadIdx.x < n)
+                s_list[threadIdx.x] = a.entries[h.offset + threadIdx.x];
+        }
         if (threadIdx.x == 0)
             s_n = n;
     }
@@ -181,6 +185,7 @@ static int launch_lab_t(const cnv_grid_s *g, int comp, const cnv_stencil &s, boo
     a.out = static_cast<T *>(out);
     a.single = single ? 1 : 0;
     a.nlist = 0;
+    a.has_tab = 0;
     a.heads = nullptr;
     a.entries = nullptr;
     long long nslots = 1;
@@ -192,6 +197,7 @@ static int launch_lab_t(const cnv_grid_s *g, int comp, const cnv_stencil &s, boo
             a.list[k] = list[k];
     } else {
         const BCTable &t = g->tables[comp];
+        a.has_tab = (t.any && t.heads && t.entries) ? 1 : 0;
         a.heads = t.heads;
         a.entries = t.entries;
         nslots = g->nblocks_total;

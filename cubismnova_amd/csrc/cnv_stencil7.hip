@@ -574,7 +574,7 @@ static int launch_stencil7_t(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, in
             if (rc != CNV_OK)
                 return rc;
             const BCTable &t = u->tables[comp_in + c];
-            a.has_bc[c] = t.any ? 1 : 0;
+            a.has_bc[c] = (t.any && t.heads && t.entries) ? 1 : 0;
             a.heads[c] = t.heads;
             a.entries[c] = t.entries;
         }
@@ -767,6 +767,108 @@ int launch_sums(cnv_grid_s *g, int comp, double *out, hipStream_t st)
     CNV_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, g->partials, nwg, out);
     hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, g->partials + nwg, nwg, out + 1);
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
+
+} // namespace cnv
+
+namespace cnv
+{
+
+// ---------------------------------------------------------------- compare
+
+// Per-workgroup partials of sum (a-b)^2, sum b^2 and max |a-b| over one
+// component of two grids of equal shape (any dtypes), evaluated in FP64.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_compare(const TA *pa, const TB *pb, long long bstride_a, long long bstride_b,
+                                                   long long cells, long long nbt, double *partials)
+{
+    __shared__ double red[3][4];
+    const int tid = threadIdx.x;
+    double d2 = 0.0, b2 = 0.0, mx = 0.0;
+    for (long long b = blockIdx.x; b < nbt; b += gridDim.x) {
+        const TA *qa = pa + b * bstride_a;
+        const TB *qb = pb + b * bstride_b;
+        for (long long i = tid; i < cells; i += 256) {
+            const double x = (double)qa[i], y = (double)qb[i];
+            const double d = x - y;
+            d2 += d * d;
+            b2 += y * y;
+            mx = fmax(mx, fabs(d));
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        d2 += __shfl_xor(d2, o, 64);
+        b2 += __shfl_xor(b2, o, 64);
+        mx = fmax(mx, __shfl_xor(mx, o, 64));
+    }
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = d2;
+        red[1][tid >> 6] = b2;
+        red[2][tid >> 6] = mx;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int n = gridDim.x;
+        partials[blockIdx.x] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+        partials[n + blockIdx.x] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+        partials[2 * n + blockIdx.x] = fmax(fmax(red[2][0], red[2][1]), fmax(red[2][2], red[2][3]));
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_finalize_max(const double *__restrict__ partials, int n,
+                                                       double *__restrict__ out)
+{
+    __shared__ double red[16];
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    for (int i = tid; i < n; i += 1024)
+        s = fmax(s, partials[i]);
+    for (int o = 32; o > 0; o >>= 1)
+        s = fmax(s, __shfl_xor(s, o, 64));
+    if ((tid & 63) == 0)
+        red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; ++w)
+            t = fmax(t, red[w]);
+        *out = t;
+    }
+}
+
+template <typename TA, typename TB>
+static void compare_t(const cnv_grid_s *a, int ca, const cnv_grid_s *b, int cb, double *partials, int nwg,
+                      double *out, hipStream_t st)
+{
+    const long long nbt = a->nblocks_total;
+    const long long cells = (long long)a->desc.block_cells[0] * a->desc.block_cells[1] * a->desc.block_cells[2];
+    const TA *pa = reinterpret_cast<const TA *>((const char *)a->data + (size_t)ca * a->component_bytes);
+    const TB *pb = reinterpret_cast<const TB *>((const char *)b->data + (size_t)cb * b->component_bytes);
+    hipLaunchKernelGGL((k_compare<TA, TB>), dim3(nwg), dim3(256), 0, st, pa, pb,
+                       (long long)(a->block_bytes / a->elem), (long long)(b->block_bytes / b->elem), cells, nbt,
+                       partials);
+    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, partials, nwg, out);
+    hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, partials + nwg, nwg, out + 1);
+    hipLaunchKernelGGL(k_finalize_max, dim3(1), dim3(1024), 0, st, partials + 2 * nwg, nwg, out + 2);
+}
+
+int launch_compare(const cnv_grid_s *a, int ca, const cnv_grid_s *b, int cb, double *out, hipStream_t st)
+{
+    const long long nbt = a->nblocks_total;
+    const int nwg = (int)(nbt < 1024 ? nbt : 1024);
+    // the reduction scratch of `a` holds at least 4096 doubles
+    double *partials = a->partials;
+    const bool da = a->desc.dtype == CNV_F64, db = b->desc.dtype == CNV_F64;
+    if (da && db)
+        compare_t<double, double>(a, ca, b, cb, partials, nwg, out, st);
+    else if (da)
+        compare_t<double, float>(a, ca, b, cb, partials, nwg, out, st);
+    else if (db)
+        compare_t<float, double>(a, ca, b, cb, partials, nwg, out, st);
+    else
+        compare_t<float, float>(a, ca, b, cb, partials, nwg, out, st);
     CNV_HIP(hipGetLastError());
     return CNV_OK;
 }

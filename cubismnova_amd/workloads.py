@@ -64,3 +64,65 @@ def diffusion_large(N=256):
 
 def diffusion_dt(h, nu=1.0):
     return 0.1 * h * h, nu
+
+
+# ------------------------------------------------------------------ WENO5 state
+
+GAMMA = 1.4
+
+
+def _sin_modes(r, nmodes, kmax):
+    modes = []
+    while len(modes) < nmodes:
+        kv = r.integers(-kmax, kmax + 1, size=3)
+        if (kv != 0).any():
+            modes.append((kv, r.random() * 2 * np.pi))
+    return modes
+
+
+def weno_setup(k=3):
+    """Seeded parameters of the compressible state: density modes, velocity
+    modes per direction, and the unit normal of the pressure front."""
+    r = rng(k)
+    rho = _sin_modes(r, 4, 4)
+    vel = [_sin_modes(r, 4, 4) for _ in range(3)]
+    nrm = r.standard_normal(3)
+    nrm /= np.linalg.norm(nrm)
+    return rho, vel, nrm
+
+
+def _modes_plane(modes, x, zc, amp):
+    # amp * mean_m sin(2 pi k.x + phi) on the x-y plane at height zc, as
+    # sin(a x) cos(b y + c) + cos(a x) sin(b y + c)
+    out = np.zeros((x.size, x.size))
+    for kv, ph in modes:
+        ax = 2 * np.pi * kv[0] * x
+        byc = 2 * np.pi * kv[1] * x + (2 * np.pi * kv[2] * zc + ph)
+        out += np.outer(np.cos(byc), np.sin(ax))
+        out += np.outer(np.sin(byc), np.cos(ax))
+    return amp * out / len(modes)
+
+
+def weno_component(N, comp, k=3):
+    """Component comp of (rho, rho u, rho v, rho w, E) on an N^3 grid:
+    rho = 1 + 0.2 sin-modes, velocity 0.1 random-phase modes, p = 1 + 9 s(x)
+    with s = (1 + tanh(n.(x - 1/2) / (2h))) / 2 a planar front of width 2h
+    (pressure ratio 10), E = p/(gamma-1) + rho |u|^2 / 2."""
+    rho_m, vel_m, nrm = weno_setup(k)
+    x = cell_centres(N)
+    h = 1.0 / N
+    out = np.empty((N, N, N), dtype=np.float64)
+    for iz in range(N):
+        zc = x[iz]
+        rho = 1.0 + _modes_plane(rho_m, x, zc, 0.2)
+        if comp == 0:
+            out[iz] = rho
+            continue
+        vel = [_modes_plane(vel_m[d], x, zc, 0.1) for d in range(3)]
+        if comp in (1, 2, 3):
+            out[iz] = rho * vel[comp - 1]
+            continue
+        s = nrm[0] * (x[None, :] - 0.5) + nrm[1] * (x[:, None] - 0.5) + nrm[2] * (zc - 0.5)
+        p = 1.0 + 9.0 * 0.5 * (1.0 + np.tanh(s / (2.0 * h)))
+        out[iz] = p / (GAMMA - 1.0) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+    return out

@@ -169,9 +169,28 @@ int cnv_laplacian(cnv_grid in, int comp_in, cnv_grid out, int comp_out, cnv_stre
 int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_out, double *sumsq,
                        cnv_stream s);
 
+/* Fifth-order WENO face states (Jiang & Shu, J. Comput. Phys. 126:202-228,
+ * 1996).  CubismNova advertises a WENO toolbox (README.rst:28-31) but ships
+ * none; the lab read is the FieldLab of Stencil(-3,4,false) with the blocks'
+ * BC lists (Block/FieldLab.h:180-224).  For every cell i, direction d and
+ * component c of `in`: out component 6c+2d   = u^-_{i+1/2} (linear weights
+ * (1,6,3)/10), out component 6c+2d+1 = u^+_{i-1/2} (weights (3,6,1)/10), from
+ * cells i-2..i+2.  eps = 1e-6 * m2 with m2 the mean squared value of the five
+ * cells (+1e-30), applied as alpha_k = d_k / (1e-6 + beta_k/m2)^2.  FP64
+ * grids are evaluated in IEEE order without contraction; FP32 grids use fused
+ * multiply-adds and hardware reciprocals.  `out` has the same shape and
+ * dtype and at least 6*ncomp(in) components. */
+int cnv_weno5(cnv_grid in, cnv_grid out, cnv_stream s);
+
 /* FP64 sum and sum of squares of one component into device doubles out[0],
  * out[1] (deterministic). */
 int cnv_grid_sums(cnv_grid g, int comp, double *out, cnv_stream s);
+
+/* No counterpart in the reference: error of one grid component against
+ * another of the same shape (any dtypes), in FP64 on the device:
+ * out[0] = sum (a-b)^2, out[1] = sum b^2, out[2] = max |a-b| (device doubles).
+ * rel-L2 = sqrt(out[0]/out[1]).  Deterministic. */
+int cnv_grid_compare(cnv_grid a, int comp_a, cnv_grid b, int comp_b, double *out, cnv_stream s);
 
 /* ------------------------------------------------------------ autotuner */
 /* No counterpart in the reference.  Times every valid (workgroup size,
@@ -184,6 +203,12 @@ int cnv_autotune_stencil7(cnv_grid u, cnv_grid out, int iters, int *wg, int *zch
                           int max_results, int *nresults);
 int cnv_tune_set_stencil7(int dtype, int edge, int wg, int zchunk);
 int cnv_tune_get_stencil7(int dtype, int edge, int *wg, int *zchunk);
+/* The same for the WENO5 kernel (candidates: wg 64/128/256, z-chunks 4..64),
+ * keyed by dtype. */
+int cnv_autotune_weno5(cnv_grid in, cnv_grid out, int iters, int *wg, int *zchunk, double *results,
+                       int max_results, int *nresults);
+int cnv_tune_set_weno5(int dtype, int wg, int zchunk);
+int cnv_tune_get_weno5(int dtype, int *wg, int *zchunk);
 
 /* ------------------------------------------------------------ HIP graphs */
 /* No counterpart in the reference (its loop is host code per block): record

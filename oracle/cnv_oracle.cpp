@@ -480,16 +480,20 @@ inline void weno5_cell(R a, R b, R c, R d, R e, R &um, R &up)
 // in: field of one component (input type T); out: 6 slabs (xm,xp,ym,yp,zm,zp)
 // in grid layout, computed in precision R and stored as R.
 template <typename T, typename R>
-void weno5_field(const orc_field &f, R *const *out, long long out_stride, int nthreads)
+void weno5_field(const orc_field &f, R *const *out, long long out_stride, int nthreads,
+                 const long long *sel, int nsel)
 {
-    const long long nbt = prod3(f.nblocks);
+    // sel == NULL: every block, output at its block slot; otherwise only the
+    // listed blocks, the j-th one written at slot j
+    const long long nbt = sel ? nsel : prod3(f.nblocks);
     const orc_stencil st = {{-3, -3, -3}, {4, 4, 4}, 0};
 #pragma omp parallel num_threads(nthreads)
     {
         OLab<T> lab;
         lab.allocate(st, f.cells, (T)0);
 #pragma omp for schedule(dynamic, 1)
-        for (long long b = 0; b < nbt; ++b) {
+        for (long long j = 0; j < nbt; ++j) {
+            const long long b = sel ? sel[j] : j;
             int bidx[3];
             block_index(b, f.nblocks, bidx);
             lab.loadData(f, bidx, blockBCs<T>(f, b), true);
@@ -497,7 +501,7 @@ void weno5_field(const orc_field &f, R *const *out, long long out_stride, int nt
             for (int iz = 0; iz < N[2]; ++iz)
                 for (int iy = 0; iy < N[1]; ++iy)
                     for (int ix = 0; ix < N[0]; ++ix) {
-                        const size_t o = b * out_stride + ix + (size_t)N[0] * (iy + (size_t)N[1] * iz);
+                        const size_t o = j * out_stride + ix + (size_t)N[0] * (iy + (size_t)N[1] * iz);
                         R um, up;
                         weno5_cell<R>(lab.at(ix - 2, iy, iz), lab.at(ix - 1, iy, iz), lab.at(ix, iy, iz),
                                       lab.at(ix + 1, iy, iz), lab.at(ix + 2, iy, iz), um, up);
@@ -622,20 +626,21 @@ double orc_diffusion(const orc_field *in, void *out, long long out_stride, void 
 
 // WENO5 states of one component. prec: 0 = float arithmetic/outputs,
 // 1 = double arithmetic/outputs (input dtype from the field).
-void orc_weno5(const orc_field *f, void *const *out6, long long out_stride, int prec, int nthreads)
+void orc_weno5(const orc_field *f, void *const *out6, long long out_stride, int prec, int nthreads,
+               const long long *sel, int nsel)
 {
     if (prec == 1) {
         double *const *o = reinterpret_cast<double *const *>(out6);
         if (f->dtype == 1)
-            weno5_field<double, double>(*f, o, out_stride, nthreads);
+            weno5_field<double, double>(*f, o, out_stride, nthreads, sel, nsel);
         else
-            weno5_field<float, double>(*f, o, out_stride, nthreads);
+            weno5_field<float, double>(*f, o, out_stride, nthreads, sel, nsel);
     } else {
         float *const *o = reinterpret_cast<float *const *>(out6);
         if (f->dtype == 1)
-            weno5_field<double, float>(*f, o, out_stride, nthreads);
+            weno5_field<double, float>(*f, o, out_stride, nthreads, sel, nsel);
         else
-            weno5_field<float, float>(*f, o, out_stride, nthreads);
+            weno5_field<float, float>(*f, o, out_stride, nthreads, sel, nsel);
     }
 }
 
