@@ -522,6 +522,37 @@ def autotune_weno5(u, out, iters=5):
     return wg.value, zc.value, table
 
 
+def set_zfaces(g, zfaces):
+    """Stretched z mesh of grid g: nz_total + 1 increasing face positions."""
+    zf = np.ascontiguousarray(zfaces, dtype=np.float64)
+    call("cnv_grid_set_zfaces", g.handle, zf.ctypes.data_as(ctypes.c_void_p), int(zf.size))
+
+
+def jacobi_sweep(u, f, out, omega=2.0 / 3.0, cu=0, cf=0, co=0, stream=None):
+    call("cnv_jacobi_sweep", u.handle, cu, f.handle, cf, out.handle, co, ctypes.c_double(omega), _st(stream))
+
+
+def jacobi_defect(u, f, r, sumsq=None, cu=0, cf=0, cr=0, stream=None):
+    p = sumsq.ptr if isinstance(sumsq, DeviceBuffer) else sumsq
+    call("cnv_jacobi_defect", u.handle, cu, f.handle, cf, r.handle, cr, p, _st(stream))
+
+
+def add_correction(u, e, cu=0, ce=0, stream=None):
+    call("cnv_add_correction", u.handle, cu, e.handle, ce, _st(stream))
+
+
+def jacobi_solve(u, f, work, refine=True, sweeps_per_cycle=50, max_cycles=1000, omega=2.0 / 3.0, tol=1e-12,
+                 history=True, stream=None):
+    """Weighted-Jacobi solve of A u = f in place; returns (cycles, sweeps,
+    relative residual, residual history per cycle)."""
+    o = _abi.cnv_solve_opts(1 if refine else 0, int(sweeps_per_cycle), int(max_cycles), float(omega), float(tol))
+    info = _abi.cnv_solve_info()
+    hist = np.zeros(max_cycles + 1, dtype=np.float64)
+    call("cnv_jacobi_solve", u.handle, f.handle, work.handle, ctypes.byref(o), ctypes.byref(info),
+         hist.ctypes.data_as(ctypes.c_void_p) if history else None, int(hist.size), _st(stream))
+    return info.cycles, info.sweeps, info.rel_residual, hist[:info.cycles + 1]
+
+
 def grid_compare(a, b, comp_a=0, comp_b=0, stream=None):
     """(sum (a-b)^2, sum b^2, max|a-b|) over one component, FP64 on the device."""
     buf = DeviceBuffer(24)

@@ -43,8 +43,13 @@ class cnv_grid_desc(ctypes.Structure):
                 ("begin", ctypes.c_double * 3), ("end", ctypes.c_double * 3)]
 
 
-class cnv_jacobi_desc(ctypes.Structure):
-    _fields_ = [("czm", ctypes.c_void_p), ("czp", ctypes.c_void_p), ("omega", ctypes.c_double)]
+class cnv_solve_opts(ctypes.Structure):
+    _fields_ = [("refine", ctypes.c_int), ("sweeps_per_cycle", ctypes.c_int), ("max_cycles", ctypes.c_int),
+                ("omega", ctypes.c_double), ("tol", ctypes.c_double)]
+
+
+class cnv_solve_info(ctypes.Structure):
+    _fields_ = [("cycles", ctypes.c_int), ("sweeps", ctypes.c_longlong), ("rel_residual", ctypes.c_double)]
 
 
 class cnv_batch_item(ctypes.Structure):
@@ -92,6 +97,11 @@ SIGNATURES = {
     "cnv_tune_set_stencil7": (i32, [i32, i32, i32, i32]),
     "cnv_tune_get_stencil7": (i32, [i32, i32, c_int_p, c_int_p]),
     "cnv_grid_compare": (i32, [vp, i32, vp, i32, vp, vp]),
+    "cnv_grid_set_zfaces": (i32, [vp, vp, i32]),
+    "cnv_jacobi_sweep": (i32, [vp, i32, vp, i32, vp, i32, f64, vp]),
+    "cnv_jacobi_defect": (i32, [vp, i32, vp, i32, vp, i32, vp, vp]),
+    "cnv_add_correction": (i32, [vp, i32, vp, i32, vp]),
+    "cnv_jacobi_solve": (i32, [vp, vp, vp, vp, vp, vp, i32, vp]),
     "cnv_weno5": (i32, [vp, vp, vp]),
     "cnv_autotune_weno5": (i32, [vp, vp, i32, c_int_p, c_int_p, vp, i32, c_int_p]),
     "cnv_tune_set_weno5": (i32, [i32, i32, i32]),

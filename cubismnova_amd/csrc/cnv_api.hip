@@ -316,6 +316,10 @@ int cnv_grid_destroy(cnv_grid g)
     }
     if (g->partials)
         (void)hipFree(g->partials);
+    if (g->czm_dev)
+        (void)hipFree(g->czm_dev);
+    if (g->czp_dev)
+        (void)hipFree(g->czp_dev);
     delete g;
     return CNV_OK;
 }
@@ -638,6 +642,221 @@ int cnv_tune_get_weno5(int dtype, int *wg, int *zchunk)
     if (!wg || !zchunk)
         return fail(CNV_ERR_INVALID, "output pointer is NULL");
     return weno5_get_tune(dtype, wg, zchunk);
+}
+
+// ---------------------------------------------------------------- Jacobi
+
+int cnv_grid_set_zfaces(cnv_grid g, const double *zfaces, int n)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(g));
+    if (!zfaces)
+        return fail(CNV_ERR_INVALID, "zfaces is NULL");
+    std::lock_guard<std::mutex> lk(g->mtx);
+    return set_zfaces(g, zfaces, n);
+    CNV_GUARD_END
+}
+
+static int check_jacobi_grids(const cnv_grid_s *u, int cu, const cnv_grid_s *f, int cf, const cnv_grid_s *o, int co)
+{
+    CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_grid(f));
+    CNV_CHECK(check_grid(o));
+    CNV_CHECK(check_comp(u, cu));
+    CNV_CHECK(check_comp(f, cf));
+    CNV_CHECK(check_comp(o, co));
+    for (int d = 0; d < 3; ++d)
+        if (u->desc.nblocks[d] != f->desc.nblocks[d] || u->desc.block_cells[d] != f->desc.block_cells[d] ||
+            u->desc.nblocks[d] != o->desc.nblocks[d] || u->desc.block_cells[d] != o->desc.block_cells[d])
+            return fail(CNV_ERR_SIZE, "grids differ in block count or block size");
+    if (u == o && cu == co)
+        return fail(CNV_ERR_INVALID, "output must not alias the input component");
+    return CNV_OK;
+}
+
+int cnv_jacobi_sweep(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int co, double omega, cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_jacobi_grids(u, cu, f, cf, out, co));
+    return launch_jacobi(u, cu, f, cf, out, co, 0, omega, nullptr, (hipStream_t)s);
+    CNV_GUARD_END
+}
+
+int cnv_jacobi_defect(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid r, int cr, double *sumsq, cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_jacobi_grids(u, cu, f, cf, r, cr));
+    return launch_jacobi(u, cu, f, cf, r, cr, 1, 0.0, sumsq, (hipStream_t)s);
+    CNV_GUARD_END
+}
+
+int cnv_add_correction(cnv_grid u, int cu, cnv_grid e, int ce, cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_grid(e));
+    CNV_CHECK(check_comp(u, cu));
+    CNV_CHECK(check_comp(e, ce));
+    if (u->desc.dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "the corrected grid must be FP64");
+    for (int d = 0; d < 3; ++d)
+        if (u->desc.nblocks[d] != e->desc.nblocks[d] || u->desc.block_cells[d] != e->desc.block_cells[d])
+            return fail(CNV_ERR_SIZE, "grids differ in block count or block size");
+    return launch_correct(u, cu, e, ce, (hipStream_t)s);
+    CNV_GUARD_END
+}
+
+// BC lists of `src` component 0 copied to `dst` component c; homogeneous
+// copies zero the Dirichlet values (the correction equation A e = r).
+static void copy_bcs(cnv_grid_s *src, cnv_grid_s *dst, int c, bool homogeneous)
+{
+    std::lock_guard<std::mutex> lk(dst->mtx);
+    for (long long b = 0; b < dst->nblocks_total; ++b) {
+        auto l = src->bcs[0][(size_t)b];
+        if (homogeneous)
+            for (auto &x : l)
+                if (x.kind == CNV_BC_DIRICHLET)
+                    x.value = 0.0;
+        dst->bcs[c][(size_t)b] = l;
+    }
+    ++dst->bc_version;
+}
+
+int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts *o, cnv_solve_info *info,
+                     double *history, int history_cap, cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_grid(f));
+    CNV_CHECK(check_grid(work));
+    if (!o || !info)
+        return fail(CNV_ERR_INVALID, "options or info is NULL");
+    if (o->sweeps_per_cycle < 2 || o->sweeps_per_cycle % 2 != 0)
+        return fail(CNV_ERR_INVALID, "sweeps_per_cycle must be even and >= 2");
+    if (o->max_cycles < 0)
+        return fail(CNV_ERR_INVALID, "max_cycles must be >= 0");
+    if (u->desc.dtype != f->desc.dtype)
+        return fail(CNV_ERR_SIZE, "u and f must share a dtype");
+    const bool refine = o->refine != 0;
+    if (refine && (u->desc.dtype != CNV_F64 || work->desc.dtype != CNV_F32 || work->desc.ncomp < 3))
+        return fail(CNV_ERR_INVALID, "refinement needs FP64 u and f and an FP32 work grid with 3 components");
+    if (!refine && (work->desc.dtype != u->desc.dtype || work->desc.ncomp < 2))
+        return fail(CNV_ERR_INVALID, "plain Jacobi needs a work grid of u's dtype with 2 components");
+    CNV_CHECK(check_jacobi_grids(u, 0, f, 0, work, 0));
+    // the correction e lives in work components 0/1 with homogeneous BCs;
+    // plain sweeps ping-pong between u and work component 0 with u's BCs
+    copy_bcs(u, work, 0, refine);
+    if (refine)
+        copy_bcs(u, work, 1, true);
+    // set up metric and BC tables now: nothing may allocate inside the capture
+    CNV_CHECK(jacobi_prepare(u, 0));
+    {
+        std::lock_guard<std::mutex> lk(work->mtx);
+        const int nz = u->desc.nblocks[2] * u->desc.block_cells[2];
+        if (!work->czm_dev) {
+            CNV_HIP(hipMalloc((void **)&work->czm_dev, sizeof(double) * nz));
+            CNV_HIP(hipMalloc((void **)&work->czp_dev, sizeof(double) * nz));
+        }
+        CNV_HIP(hipMemcpy(work->czm_dev, u->czm_dev, sizeof(double) * nz, hipMemcpyDeviceToDevice));
+        CNV_HIP(hipMemcpy(work->czp_dev, u->czp_dev, sizeof(double) * nz, hipMemcpyDeviceToDevice));
+    }
+    CNV_CHECK(jacobi_prepare(work, 0));
+    CNV_CHECK(jacobi_prepare(work, 1));
+    hipStream_t st = (hipStream_t)s;
+    hipStream_t cs;
+    CNV_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    double *dev = nullptr; // [0] = |f|^2 pieces, [2] = |r|^2
+    int rc = CNV_OK;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double hst[3] = {0, 0, 0};
+    const int S = o->sweeps_per_cycle;
+    long long sweeps = 0;
+    int cycles = 0;
+    double rel = 0.0;
+    auto defect = [&](hipStream_t q) -> int {
+        // refine: r (FP32, work comp 2) = f - A u in FP64; plain: work comp 1
+        return launch_jacobi(u, 0, f, 0, work, refine ? 2 : 1, 1, 0.0, dev + 2, q);
+    };
+    do {
+        if (hipMalloc((void **)&dev, 3 * sizeof(double)) != hipSuccess) {
+            rc = fail(CNV_ERR_ALLOC, "solver scratch");
+            break;
+        }
+        if (st)
+            CNV_HIP(hipStreamSynchronize(st));
+        rc = launch_sums(f, 0, dev, cs);
+        if (rc != CNV_OK)
+            break;
+        rc = defect(cs);
+        if (rc != CNV_OK)
+            break;
+        CNV_HIP(hipMemcpyAsync(hst, dev, 3 * sizeof(double), hipMemcpyDeviceToHost, cs));
+        CNV_HIP(hipStreamSynchronize(cs));
+        const double fn = std::sqrt(hst[1]);
+        if (!(fn > 0.0)) {
+            rel = 0.0;
+            break;
+        }
+        rel = std::sqrt(hst[2]) / fn;
+        if (history && history_cap > 0)
+            history[0] = rel;
+        // one cycle, recorded once and replayed
+        CNV_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        int crc = CNV_OK;
+        if (refine) {
+            crc = launch_fill((char *)work->data, CNV_F32, 0.0, work->component_bytes / 4, cs);
+            for (int k = 0; k < S && crc == CNV_OK; ++k)
+                crc = launch_jacobi(work, k % 2, work, 2, work, 1 - k % 2, 0, o->omega, nullptr, cs);
+            if (crc == CNV_OK)
+                crc = launch_correct(u, 0, work, S % 2, cs);
+        } else {
+            for (int k = 0; k < S && crc == CNV_OK; ++k)
+                crc = (k % 2 == 0) ? launch_jacobi(u, 0, f, 0, work, 0, 0, o->omega, nullptr, cs)
+                                   : launch_jacobi(work, 0, f, 0, u, 0, 0, o->omega, nullptr, cs);
+        }
+        if (crc == CNV_OK)
+            crc = defect(cs);
+        hipError_t ce = hipStreamEndCapture(cs, &graph);
+        if (crc != CNV_OK) {
+            rc = crc;
+            break;
+        }
+        if (ce != hipSuccess) {
+            rc = hip_fail(ce, "hipStreamEndCapture");
+            break;
+        }
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            rc = fail(CNV_ERR_HIP, "hipGraphInstantiate failed");
+            break;
+        }
+        while (cycles < o->max_cycles && rel > o->tol) {
+            CNV_HIP(hipGraphLaunch(exec, cs));
+            CNV_HIP(hipMemcpyAsync(hst + 2, dev + 2, sizeof(double), hipMemcpyDeviceToHost, cs));
+            CNV_HIP(hipStreamSynchronize(cs));
+            ++cycles;
+            sweeps += S;
+            rel = std::sqrt(hst[2]) / fn;
+            if (history && cycles < history_cap)
+                history[cycles] = rel;
+            if (!std::isfinite(rel)) {
+                rc = fail(CNV_ERR_INVALID, "solver diverged (non-finite residual)");
+                break;
+            }
+        }
+    } while (false);
+    if (exec)
+        (void)hipGraphExecDestroy(exec);
+    if (graph)
+        (void)hipGraphDestroy(graph);
+    if (dev)
+        (void)hipFree(dev);
+    (void)hipStreamDestroy(cs);
+    info->cycles = cycles;
+    info->sweeps = sweeps;
+    info->rel_residual = rel;
+    return rc;
+    CNV_GUARD_END
 }
 
 // ---------------------------------------------------------------- tuning

@@ -83,6 +83,10 @@ struct cnv_grid_s {
 
     // scratch for deterministic reductions: per-tile partial sums
     double *partials = nullptr;
+
+    // z metric of a stretched mesh (per global z cell), set on first use
+    double *czm_dev = nullptr;
+    double *czp_dev = nullptr;
     size_t partials_cap = 0;
 
     std::mutex mtx;
@@ -164,6 +168,12 @@ void s7_set_tune(int dtype, int edge, int wg, int zchunk);
 int get_tune_stencil7(int dtype, int edge, int *wg, int *zchunk);
 
 int launch_compare(const cnv_grid_s *a, int ca, const cnv_grid_s *b, int cb, double *out, hipStream_t st);
+
+int set_zfaces(cnv_grid_s *g, const double *zf, int n);
+int launch_jacobi(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out, int co, int mode, double omega,
+                  double *sumsq, hipStream_t st);
+int launch_correct(cnv_grid_s *u, int cu, cnv_grid_s *e, int ce, hipStream_t st);
+int jacobi_prepare(cnv_grid_s *g, int comp);
 
 int launch_weno5(cnv_grid_s *in, cnv_grid_s *out, hipStream_t st, const int *tile);
 int autotune_weno5(cnv_grid_s *in, cnv_grid_s *out, int iters, int *best_wg, int *best_z, double *results,

@@ -1,0 +1,472 @@
+// cnv_jacobi.hip -- weighted Jacobi on the variable-coefficient 7-point
+// operator of a z-stretched grid, and its mixed-precision iterative
+// refinement (FP32 sweeps, FP64 defect and correction).
+//
+// CubismNova has no solver and only names a stretched mesh class
+// (include/Cubism/Common.h:86); the operator here is the flux form
+//   (A u) = ihx2 ((u[+x] - 2u) + u[-x]) + ihy2 ((u[+y] - 2u) + u[-y])
+//         + czp_k (u[+z] - u) - czm_k (u - u[-z]),
+// summed left to right, with ghosts from the lab of Stencil(-1,2) and the
+// blocks' BC lists (Dirichlet: constant ghost, BC/Dirichlet.h:75-113), and
+//   sweep:  u' = u + omega ((f - A u) / diag),  diag = -(((2 ihx2 + 2 ihy2) + czm_k) + czp_k)
+//   defect: r  = f - A u  (+ FP64 sum of r^2 before any rounding of r).
+// The z-march, LDS plane and register queue are those of the 7-point kernel.
+#include "cnv_device.h"
+
+#include <cmath>
+#include <map>
+
+namespace cnv
+{
+
+struct JArgs {
+    const char *u;  // grid base of u (component cu)
+    const char *f;  // grid base of f (component cf)
+    char *out;      // grid base of the output (component co)
+    Geom G;         // geometry of u
+    long long cs_u, cs_f, cs_o; // component strides (elements) of the three grids
+    long long bs_f, bs_o;       // block strides of f and out
+    int cu, cf, co;
+    int nzc, zchunk;
+    int has_bc;
+    const DevBCHead *heads;
+    const DevBC *entries;
+    const double *czm, *czp; // per global z cell
+    double ihx2, ihy2, omega;
+    double *partials;
+};
+
+template <typename T, typename TF, typename TO, int NX, int WG, int MODE>
+__global__ __launch_bounds__(WG) void k_jacobi(const JArgs a)
+{
+    constexpr int NY = NX;
+    constexpr int VEC = 16 / (int)sizeof(T);
+    constexpr int TX = NX / VEC;
+    constexpr int ROWS = WG / TX;
+    constexpr int R = NY / ROWS;
+    constexpr int P = NX + 2 * VEC;
+    constexpr int PLANE = (NY + 2) * P;
+    static_assert(NX % VEC == 0 && WG % TX == 0 && NY % ROWS == 0, "tile shape");
+    static_assert(WG >= 2 * NY && WG >= 2 * TX, "halo roles need enough threads");
+    using V = VecT<T, VEC>;
+
+    __shared__ alignas(16) T lds[2][PLANE];
+    __shared__ double red[WG / 64];
+
+    const int tid = threadIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const long long blk = tile / a.nzc;
+    const int zc = tile - (int)blk * a.nzc;
+    const int NZ = a.G.n[2];
+    const int z0 = zc * a.zchunk;
+    const int z1 = min(NZ, z0 + a.zchunk);
+    const int bz = (int)(blk / ((long long)a.G.nb[0] * a.G.nb[1]));
+
+    const T *cbase = reinterpret_cast<const T *>(a.u) + (long long)a.cu * a.cs_u;
+    const T *ub = cbase + blk * a.G.bstride;
+    const TF *fb = reinterpret_cast<const TF *>(a.f) + (long long)a.cf * a.cs_f + blk * a.bs_f;
+    TO *ob = reinterpret_cast<TO *>(a.out) + (long long)a.co * a.cs_o + blk * a.bs_o;
+
+    FaceL<T> F[6];
+    const int sb[3] = {-1, -1, -1}, se[3] = {2, 2, 2};
+    make_faces<T>(a.G, a.heads, a.entries, a.has_bc, blk, cbase, sb, se, F);
+
+    const int tx = tid % TX, ty = tid / TX;
+    const int x0 = tx * VEC;
+    constexpr long long SXY = (long long)NX * NY;
+    const bool hx_on = tid < 2 * NY;
+    const int hx_side = tid / NY, hx_y = tid % NY;
+    const bool hy_on = tid < 2 * TX;
+    const int hy_side = tid / TX, hy_x = (tid % TX) * VEC;
+    const FaceL<T> FX = hx_side ? F[1] : F[0];
+    const FaceL<T> FY = hy_side ? F[3] : F[2];
+    const FaceL<T> FZ0 = F[4], FZ1 = F[5];
+
+    struct Plane {
+        V c[R];
+        T hx;
+        V hy;
+    };
+    auto load_plane = [&](int p, bool halo, Plane &Q) {
+        if (p >= 0 && p < NZ) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                Q.c[r] = *reinterpret_cast<const V *>(ub + x0 + NX * (ty + r * ROWS) + SXY * p);
+            if (halo) {
+                if (hx_on)
+                    Q.hx = face_layer(FX, 1, 1, (long long)NX * hx_y + SXY * p);
+                if (hy_on) {
+                    if (FY.mode == 2) {
+#pragma unroll
+                        for (int v = 0; v < VEC; ++v)
+                            Q.hy.v[v] = FY.s;
+                    } else {
+                        Q.hy = *reinterpret_cast<const V *>(FY.p + (long long)(FY.li0 + FY.lstep) * NX + hy_x + SXY * p);
+                        if (FY.mode == 1)
+#pragma unroll
+                            for (int v = 0; v < VEC; ++v)
+                                Q.hy.v[v] = FY.s * Q.hy.v[v];
+                    }
+                }
+            }
+        } else {
+            const FaceL<T> f = p < 0 ? FZ0 : FZ1;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (f.mode == 2) {
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v)
+                        Q.c[r].v[v] = f.s;
+                } else {
+                    Q.c[r] = *reinterpret_cast<const V *>(f.p + (long long)(f.li0 + f.lstep) * SXY + x0 +
+                                                          NX * (ty + r * ROWS));
+                    if (f.mode == 1)
+#pragma unroll
+                        for (int v = 0; v < VEC; ++v)
+                            Q.c[r].v[v] = f.s * Q.c[r].v[v];
+                }
+            }
+        }
+    };
+
+    const T ihx2 = (T)a.ihx2, ihy2 = (T)a.ihy2, w = (T)a.omega;
+    double acc = 0.0;
+    Plane pm, pc, pn, pnn;
+    load_plane(z0 - 1, false, pm);
+    load_plane(z0, true, pc);
+    load_plane(z0 + 1, z0 + 1 < z1, pn);
+    for (int z = z0; z < z1; ++z) {
+        if (z + 2 <= z1)
+            load_plane(z + 2, z + 2 < z1, pnn);
+        // rhs of this plane (no halo)
+        using VF = VecT<TF, VEC>;
+        VF fr[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int v = 0; v < VEC; ++v)
+                fr[r].v[v] = fb[x0 + v + NX * (ty + r * ROWS) + SXY * z];
+        T *L = lds[(z - z0) & 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            *reinterpret_cast<V *>(L + (ty + r * ROWS + 1) * P + VEC + x0) = pc.c[r];
+        if (hx_on)
+            L[(hx_y + 1) * P + (hx_side ? VEC + NX : VEC - 1)] = pc.hx;
+        if (hy_on)
+            *reinterpret_cast<V *>(L + (hy_side ? NY + 1 : 0) * P + VEC + hy_x) = pc.hy;
+        __syncthreads();
+        const int kz = bz * NZ + z;
+        const T cm = (T)a.czm[kz], cp = (T)a.czp[kz];
+        const T diag = -(2 * ihx2 + 2 * ihy2 + cm + cp);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int y = ty + r * ROWS;
+            const T *row = L + (y + 1) * P + VEC;
+            const V ym = *reinterpret_cast<const V *>(row - P + x0);
+            const V yp = *reinterpret_cast<const V *>(row + P + x0);
+            const T xl = row[x0 - 1];
+            const T xr = row[x0 + VEC];
+            VecT<TO, VEC> res;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+                const T c = pc.c[r].v[v];
+                const T xm = v == 0 ? xl : pc.c[r].v[v - 1];
+                const T xp = v == VEC - 1 ? xr : pc.c[r].v[v + 1];
+                const T ax = ihx2 * (xp - 2 * c + xm);
+                const T ay = ihy2 * (yp.v[v] - 2 * c + ym.v[v]);
+                const T az = cp * (pn.c[r].v[v] - c) - cm * (c - pm.c[r].v[v]);
+                const T au = ax + ay + az;
+                const T rr = (T)fr[r].v[v] - au;
+                if (MODE == 0) {
+                    res.v[v] = (TO)(c + w * (rr / diag));
+                } else {
+                    res.v[v] = (TO)rr;
+                    acc += (double)rr * (double)rr;
+                }
+            }
+            TO *po = ob + x0 + NX * y + SXY * z;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v)
+                po[v] = res.v[v];
+        }
+        pm = pc;
+        pc = pn;
+        pn = pnn;
+    }
+    if (MODE == 1) {
+        for (int o = 32; o > 0; o >>= 1)
+            acc += __shfl_xor(acc, o, 64);
+        if ((tid & 63) == 0)
+            red[tid >> 6] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int wv = 0; wv < WG / 64; ++wv)
+                s += red[wv];
+            a.partials[tile] = s;
+        }
+    }
+}
+
+// Sum of n partials in a fixed order (one workgroup of 1024 threads).
+__global__ __launch_bounds__(1024) void k_jfinalize(const double *__restrict__ partials, int n, double *__restrict__ out)
+{
+    __shared__ double red[16];
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    for (int base = 0; base < n; base += 8 * 1024) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int i = base + j * 1024 + tid;
+            v[j] = i < n ? partials[i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            s += v[j];
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        s += __shfl_xor(s, o, 64);
+    if ((tid & 63) == 0)
+        red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; ++w)
+            t += red[w];
+        *out = t;
+    }
+}
+
+// u64 += e (any dtype of e), and e-fill with zero
+template <typename TE>
+__global__ __launch_bounds__(256) void k_correct(double *u, const TE *e, long long bs_u, long long bs_e, long long cells,
+                                                   long long nbt)
+{
+    const long long n = cells * nbt;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const long long b = i / cells, c = i - b * cells;
+        u[b * bs_u + c] += (double)e[b * bs_e + c];
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+struct JTune {
+    int wg, zchunk;
+};
+
+template <typename T, int NX, int WG>
+constexpr bool j_valid()
+{
+    constexpr int VEC = 16 / (int)sizeof(T);
+    constexpr int TX = NX / VEC;
+    return NX % VEC == 0 && WG % TX == 0 && (WG / TX) <= NX && NX % (WG / TX) == 0 && WG >= 2 * NX &&
+           WG >= 2 * TX;
+}
+
+template <typename T, typename TF, typename TO, int NX, int WG, int MODE>
+static bool j_launch(const JArgs &a, int ntiles, hipStream_t st)
+{
+    if constexpr (j_valid<T, NX, WG>()) {
+        hipLaunchKernelGGL((k_jacobi<T, TF, TO, NX, WG, MODE>), dim3(ntiles), dim3(WG), 0, st, a);
+        return true;
+    }
+    return false;
+}
+
+template <typename T, typename TF, typename TO, int MODE>
+static bool j_dispatch(const JArgs &a, int edge, int wg, int ntiles, hipStream_t st)
+{
+#define J_WG(NX)                                                                                        \
+    switch (wg) {                                                                                       \
+    case 128:                                                                                           \
+        return j_launch<T, TF, TO, NX, 128, MODE>(a, ntiles, st);                                       \
+    case 256:                                                                                           \
+        return j_launch<T, TF, TO, NX, 256, MODE>(a, ntiles, st);                                       \
+    case 512:                                                                                           \
+        return j_launch<T, TF, TO, NX, 512, MODE>(a, ntiles, st);                                       \
+    default:                                                                                            \
+        return false;                                                                                   \
+    }
+    switch (edge) {
+    case 16:
+        J_WG(16)
+    case 32:
+        J_WG(32)
+    case 64:
+        J_WG(64)
+    default:
+        return false;
+    }
+#undef J_WG
+}
+
+static JTune j_tune(int dtype, int edge)
+{
+    if (edge >= 64)
+        return {512, 16};
+    if (edge >= 32)
+        return {256, 16};
+    return {128, 16};
+}
+
+// metric of the grid's z direction: uniform (1/h^2) unless faces were set
+static int ensure_zmetric(cnv_grid_s *g)
+{
+    if (g->czm_dev)
+        return CNV_OK;
+    const int nz = g->desc.nblocks[2] * g->desc.block_cells[2];
+    std::vector<double> zf(nz + 1);
+    for (int j = 0; j <= nz; ++j)
+        zf[j] = g->desc.begin[2] + j * g->h[2];
+    return set_zfaces(g, zf.data(), nz + 1);
+}
+
+int set_zfaces(cnv_grid_s *g, const double *zf, int n)
+{
+    const int nz = g->desc.nblocks[2] * g->desc.block_cells[2];
+    if (n != nz + 1)
+        return fail(CNV_ERR_SIZE, "need nz + 1 face positions");
+    for (int j = 0; j < nz; ++j)
+        if (!(zf[j + 1] > zf[j]))
+            return fail(CNV_ERR_INVALID, "face positions must increase");
+    std::vector<double> dz(nz), zc(nz + 2), czm(nz), czp(nz);
+    for (int k = 0; k < nz; ++k) {
+        dz[k] = zf[k + 1] - zf[k];
+        zc[k + 1] = 0.5 * (zf[k + 1] + zf[k]);
+    }
+    zc[0] = 2 * zf[0] - zc[1];
+    zc[nz + 1] = 2 * zf[nz] - zc[nz];
+    for (int k = 0; k < nz; ++k) {
+        czm[k] = 1.0 / (dz[k] * (zc[k + 1] - zc[k]));
+        czp[k] = 1.0 / (dz[k] * (zc[k + 2] - zc[k + 1]));
+    }
+    if (!g->czm_dev) {
+        CNV_HIP(hipMalloc((void **)&g->czm_dev, sizeof(double) * nz));
+        CNV_HIP(hipMalloc((void **)&g->czp_dev, sizeof(double) * nz));
+    }
+    CNV_HIP(hipMemcpy(g->czm_dev, czm.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    CNV_HIP(hipMemcpy(g->czp_dev, czp.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    return CNV_OK;
+}
+
+// mode 0: sweep out = u + omega (f - Au)/diag (u, f, out of one dtype)
+// mode 1: defect out = f - A u (u, f FP64 or both FP32; out any dtype) + sumsq
+int launch_jacobi(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out, int co, int mode, double omega,
+                  double *sumsq, hipStream_t st)
+{
+    const int *n = u->desc.block_cells;
+    if (!(n[0] == n[1] && (n[0] == 16 || n[0] == 32 || n[0] == 64)))
+        return fail(CNV_ERR_UNSUPPORTED, "Jacobi kernels need x/y block edges of 16, 32 or 64");
+    if (f->desc.dtype != u->desc.dtype)
+        return fail(CNV_ERR_SIZE, "u and f must share a dtype");
+    if (mode == 0 && out->desc.dtype != u->desc.dtype)
+        return fail(CNV_ERR_SIZE, "sweep output must share the dtype of u");
+    JArgs a;
+    {
+        std::lock_guard<std::mutex> lk(u->mtx);
+        int rc = ensure_zmetric(u);
+        if (rc != CNV_OK)
+            return rc;
+        rc = ensure_bc_table(u, cu);
+        if (rc != CNV_OK)
+            return rc;
+        const int sb[3] = {-1, -1, -1}, se[3] = {2, 2, 2};
+        for (const auto &l : u->bcs[cu]) {
+            if (l.empty())
+                continue;
+            std::vector<DevBC> tmp;
+            for (const auto &b : l)
+                tmp.push_back({b.kind, b.dir, b.side, 0, b.value});
+            int kind[6];
+            double val[6];
+            resolve_faces(tmp.data(), (int)tmp.size(), sb, se, kind, val);
+            for (int k = 0; k < 6; ++k)
+                if (kind[k] == FACE_UNSET)
+                    return fail(CNV_ERR_UNSUPPORTED, "BC list leaves ghost cells unwritten");
+        }
+        const BCTable &t = u->tables[cu];
+        a.has_bc = (t.any && t.heads && t.entries) ? 1 : 0;
+        a.heads = t.heads;
+        a.entries = t.entries;
+    }
+    a.u = static_cast<const char *>(u->data);
+    a.f = static_cast<const char *>(f->data);
+    a.out = static_cast<char *>(out->data);
+    a.G = geom_of(u);
+    a.cs_u = (long long)(u->component_bytes / u->elem);
+    a.cs_f = (long long)(f->component_bytes / f->elem);
+    a.cs_o = (long long)(out->component_bytes / out->elem);
+    a.bs_f = (long long)(f->block_bytes / f->elem);
+    a.bs_o = (long long)(out->block_bytes / out->elem);
+    a.cu = cu;
+    a.cf = cf;
+    a.co = co;
+    a.czm = u->czm_dev;
+    a.czp = u->czp_dev;
+    a.ihx2 = 1. / (u->h[0] * u->h[0]);
+    a.ihy2 = 1. / (u->h[1] * u->h[1]);
+    a.omega = omega;
+    a.partials = u->partials;
+    const JTune tu = j_tune(u->desc.dtype, n[0]);
+    a.zchunk = tu.zchunk < n[2] ? tu.zchunk : n[2];
+    a.nzc = (n[2] + a.zchunk - 1) / a.zchunk;
+    const long long ntiles = u->nblocks_total * a.nzc;
+    if (ntiles > (long long)u->partials_cap || ntiles >= (1LL << 31))
+        return fail(CNV_ERR_UNSUPPORTED, "grid too large for the Jacobi kernels");
+    bool ok;
+    const bool dbl = u->desc.dtype == CNV_F64;
+    if (mode == 0)
+        ok = dbl ? j_dispatch<double, double, double, 0>(a, n[0], tu.wg, (int)ntiles, st)
+                 : j_dispatch<float, float, float, 0>(a, n[0], tu.wg, (int)ntiles, st);
+    else if (dbl)
+        ok = out->desc.dtype == CNV_F64 ? j_dispatch<double, double, double, 1>(a, n[0], tu.wg, (int)ntiles, st)
+                                        : j_dispatch<double, double, float, 1>(a, n[0], tu.wg, (int)ntiles, st);
+    else
+        ok = out->desc.dtype == CNV_F64 ? j_dispatch<float, float, double, 1>(a, n[0], tu.wg, (int)ntiles, st)
+                                        : j_dispatch<float, float, float, 1>(a, n[0], tu.wg, (int)ntiles, st);
+    if (!ok)
+        return fail(CNV_ERR_UNSUPPORTED, "no Jacobi tile for this block edge");
+    CNV_HIP(hipGetLastError());
+    if (mode == 1 && sumsq) {
+        hipLaunchKernelGGL(k_jfinalize, dim3(1), dim3(1024), 0, st, (const double *)u->partials, (int)ntiles, sumsq);
+        CNV_HIP(hipGetLastError());
+    }
+    return CNV_OK;
+}
+
+int launch_correct(cnv_grid_s *u, int cu, cnv_grid_s *e, int ce, hipStream_t st)
+{
+    const long long cells = (long long)u->desc.block_cells[0] * u->desc.block_cells[1] * u->desc.block_cells[2];
+    const long long nbt = u->nblocks_total;
+    const long long nwg = std::min<long long>((cells * nbt + 255) / 256, 8192);
+    double *ub = reinterpret_cast<double *>((char *)u->data + (size_t)cu * u->component_bytes);
+    const long long bs_u = (long long)(u->block_bytes / 8);
+    if (e->desc.dtype == CNV_F64)
+        hipLaunchKernelGGL(k_correct<double>, dim3((unsigned)nwg), dim3(256), 0, st, ub,
+                           reinterpret_cast<const double *>((char *)e->data + (size_t)ce * e->component_bytes), bs_u,
+                           (long long)(e->block_bytes / 8), cells, nbt);
+    else
+        hipLaunchKernelGGL(k_correct<float>, dim3((unsigned)nwg), dim3(256), 0, st, ub,
+                           reinterpret_cast<const float *>((char *)e->data + (size_t)ce * e->component_bytes), bs_u,
+                           (long long)(e->block_bytes / 4), cells, nbt);
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
+
+} // namespace cnv
+
+namespace cnv
+{
+// Everything a Jacobi launch on (g, comp) would set up on first use (z
+// metric, BC table), done ahead of a graph capture.
+int jacobi_prepare(cnv_grid_s *g, int comp)
+{
+    std::lock_guard<std::mutex> lk(g->mtx);
+    int rc = ensure_zmetric(g);
+    if (rc != CNV_OK)
+        return rc;
+    return ensure_bc_table(g, comp);
+}
+} // namespace cnv

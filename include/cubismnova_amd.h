@@ -192,6 +192,52 @@ int cnv_grid_sums(cnv_grid g, int comp, double *out, cnv_stream s);
  * rel-L2 = sqrt(out[0]/out[1]).  Deterministic. */
 int cnv_grid_compare(cnv_grid a, int comp_a, cnv_grid b, int comp_b, double *out, cnv_stream s);
 
+/* ---------------------------------------------------- stretched-z Jacobi */
+/* No counterpart in the reference: CubismNova names MeshClass::Stretched
+ * (include/Cubism/Common.h:86) but ships neither a stretched mesh nor a
+ * solver.  Operator, per global z cell k with faces zf[k], zf[k+1]:
+ *   A u = ihx2 ((u[+x] - 2u) + u[-x]) + ihy2 ((u[+y] - 2u) + u[-y])
+ *       + czp_k (u[+z] - u) - czm_k (u - u[-z]),
+ *   czp_k = 1/(dz_k (zc_{k+1} - zc_k)), czm_k = 1/(dz_k (zc_k - zc_{k-1})),
+ * ghost centres mirrored about the boundary faces; ghosts from the lab of
+ * Stencil(-1,2) with the BC lists of u (Dirichlet = constant ghost,
+ * BC/Dirichlet.h:75-113).  Without cnv_grid_set_zfaces the z faces are
+ * uniform. */
+int cnv_grid_set_zfaces(cnv_grid g, const double *zfaces, int n /* nz_total + 1 */);
+/* weighted Jacobi sweep: out = u + omega ((f - A u) / diag),
+ * diag = -(((2 ihx2 + 2 ihy2) + czm_k) + czp_k); u, f, out of one dtype */
+int cnv_jacobi_sweep(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int co, double omega, cnv_stream s);
+/* defect r = f - A u in u's dtype, stored in r's dtype; if sumsq is
+ * non-NULL the FP64 sum of r^2 (before r is rounded) is written there */
+int cnv_jacobi_defect(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid r, int cr, double *sumsq, cnv_stream s);
+/* u (FP64) += e (FP32 or FP64), elementwise */
+int cnv_add_correction(cnv_grid u, int cu, cnv_grid e, int ce, cnv_stream s);
+
+typedef struct {
+    int refine;           /* 1: FP32 sweeps on A e = r, FP64 defect and correction */
+    int sweeps_per_cycle; /* even; 50 in the refined Poisson workload */
+    int max_cycles;
+    double omega;         /* 2/3 */
+    double tol;           /* stop when ||f - A u||_2 / ||f||_2 <= tol */
+} cnv_solve_opts;
+
+typedef struct {
+    int cycles;
+    long long sweeps;
+    double rel_residual;
+} cnv_solve_info;
+
+/* Solve A u = f for component 0 of u (in place) by weighted Jacobi.
+ * refine = 1: u and f FP64, work FP32 with 3 components (correction pair and
+ * FP32 defect); each cycle zeroes e, runs sweeps_per_cycle FP32 sweeps on
+ * A e = r with u's BCs made homogeneous, adds e to u in FP64 and recomputes
+ * the FP64 defect.  refine = 0: plain sweeps in u's dtype, work of u's dtype
+ * with 2 components.  One cycle is recorded once into a HIP graph and
+ * replayed; the relative residual after cycle c goes to history[c]
+ * (history[0] = initial).  Calls on the same grids must not overlap. */
+int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts *opts, cnv_solve_info *info,
+                     double *history, int history_cap, cnv_stream s);
+
 /* ------------------------------------------------------------ autotuner */
 /* No counterpart in the reference.  Times every valid (workgroup size,
  * z-planes per tile) pair of the fused diffusion kernel on u -> out (`iters`
