@@ -295,12 +295,100 @@ def bench_weno5(args):
     return out
 
 
+# ------------------------------------------------------------------ batch
+
+
+def bench_batch(args):
+    import cubismnova_amd as cnv
+    import oracle
+    from cubismnova_amd import workloads
+    ev = HipEvents()
+    specs = workloads.batch_specs(96, k=4)
+    grids, items = [], []
+    cells_total, bytes_total = 0, 0
+    for s in specs:
+        nb, e = s["nblocks"], s["edge"]
+        g = cnv.Cartesian(nb, e, dtype=np.float32, ncomp=s["ncomp"])
+        for c in range(s["ncomp"]):
+            g.upload_global(c, workloads.batch_field(s, c).astype(np.float32))
+        for kind, d, side, v in s["bcs"]:
+            g.addFaceBC(cnv.Dirichlet(d, side, v) if kind == "dirichlet" else cnv.Absorbing(d, side))
+        nout = 6 * s["ncomp"] if s["op"] == "weno5" else s["ncomp"]
+        o = cnv.Cartesian(nb, e, dtype=np.float32, ncomp=nout)
+        h = 1.0 / (nb[0] * e)
+        items.append((g, o, s["op"], s["ghost"], 0.1 * h * h))
+        grids.append((g, o, s))
+        cells = nb[0] * nb[1] * nb[2] * e ** 3
+        cells_total += cells
+        bytes_total += cells * s["ncomp"] * (8 if s["op"] == "diffusion" else 28)
+    b = cnv.Batch(items)
+    table = [] if args.no_autotune else b.autotune(iters=3)
+    if table:
+        print("autotune batch: " + " ".join("%d/%d/%d:%.0fus" % t for t in table), file=sys.stderr, flush=True)
+    st = cnv.Stream()
+    for _ in range(args.warmup):
+        b.run(stream=st)
+    st.synchronize()
+    gr = cnv.Graph(st).capture()
+    for _ in range(args.steps):
+        b.run(stream=st)
+    gr.end()
+    e0, e1 = ev.new(), ev.new()
+    ev.device_sync()
+    t0 = time.perf_counter()
+    ev.record(e0, st)
+    gr.launch()
+    ev.record(e1, st)
+    st.synchronize()
+    wall = time.perf_counter() - t0
+    kern_s = ev.elapsed_ms(e0, e1) / 1e3 / args.steps
+    info = b.info()
+    out = {
+        "metric": METRIC, "value": round(cells_total * args.steps / wall / 1e9, 3), "unit": "GCell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Fourier-mode fields per grid)",
+        "config": {"workload": "96 independent grids (blocks 1x1x1..8x8x4, edges {8,16,32}, 1-3 components, "
+                               "ghost 1 or 3, per-face periodic/Dirichlet/absorbing), each a diffusion step or "
+                               "WENO5, one persistent launch per step, HIP-graph replayed",
+                   "grids": len(specs), "cells": cells_total, "work_items": info["items"], "wg": info["wg"],
+                   "lds_bytes": info["lds_bytes"]},
+        "roofline": roofline(bytes_total, kern_s, measured_traffic("batch_96.json")),
+    }
+    if not args.no_cpu:
+        nt = max(1, min(16, os.cpu_count() or 1))
+        done_cells, t0 = 0, time.perf_counter()
+        for g, o, s in grids:
+            nb, e = s["nblocks"], s["edge"]
+            lists = {}
+            for kind, d, side, v in s["bcs"]:
+                for bb in range(g.nblocks_total):
+                    bi = (bb % nb[0], (bb // nb[0]) % nb[1], bb // (nb[0] * nb[1]))
+                    if bi[d] == (0 if side == 0 else nb[d] - 1):
+                        lists.setdefault(bb, []).append((2 if kind == "dirichlet" else 1, d, side, v))
+            for c in range(s["ncomp"]):
+                f = oracle.Field(g.download(c), nb, e, g.block_stride, lists)
+                if s["op"] == "diffusion":
+                    oracle.diffusion_step(f, [1.0 / (nb[d] * e) for d in range(3)], items[0][4], nthreads=nt)
+                else:
+                    oracle.weno5(f, prec=1, nthreads=nt)
+            done_cells += g.nblocks_total * e ** 3
+            if time.perf_counter() - t0 >= args.cpu_seconds:
+                break
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(done_cells / el / 1e9, 5), "unit": "GCell-updates/s", "cores": nt,
+                               "kind": "port",
+                               "sample": "the first grids of the batch in order (%d of %d cells), FP64 C++ oracle, "
+                                         "%.1f s" % (done_cells, cells_total, el)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="diffusion", choices=["diffusion", "weno5"])
+    ap.add_argument("--workload", default="diffusion", choices=["diffusion", "weno5", "batch"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in tile table")
@@ -314,7 +402,7 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
-    out = bench_weno5(args) if args.workload == "weno5" else bench_diffusion(args)
+    out = {"weno5": bench_weno5, "batch": bench_batch}.get(args.workload, bench_diffusion)(args)
     print(json.dumps(out), flush=True)
 
 

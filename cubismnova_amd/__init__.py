@@ -522,6 +522,46 @@ def autotune_weno5(u, out, iters=5):
     return wg.value, zc.value, table
 
 
+class Batch:
+    """Independent FP32 grids processed by one persistent launch.
+
+    items: list of (in_grid, out_grid, op, ghost, nu_dt) with op "diffusion"
+    (out = in + nu_dt lap(in)) or "weno5" (6 states per component)."""
+
+    OPS = {"diffusion": _abi.CNV_OP_DIFFUSION, "weno5": _abi.CNV_OP_WENO5}
+
+    def __init__(self, items):
+        arr = (_abi.cnv_batch_item * len(items))()
+        for i, (gi, go, op, ghost, nudt) in enumerate(items):
+            arr[i] = _abi.cnv_batch_item(gi.handle.value, go.handle.value, self.OPS[op], int(ghost), float(nudt))
+        self._keep = [(it[0], it[1]) for it in items]
+        h = ctypes.c_void_p()
+        call("cnv_batch_create", arr, len(items), ctypes.byref(h))
+        self.handle = h
+
+    def autotune(self, iters=5):
+        res = np.zeros(4 * 256, dtype=np.float64)
+        nr = ctypes.c_int()
+        call("cnv_batch_autotune", self.handle, int(iters), res.ctypes.data_as(ctypes.c_void_p), 256,
+             ctypes.byref(nr))
+        return [tuple(res[4 * i:4 * i + 4]) for i in range(nr.value)]
+
+    def run(self, stream=None):
+        call("cnv_batch_run", self.handle, _st(stream))
+
+    def info(self):
+        n, wg, lds = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        call("cnv_batch_info", self.handle, ctypes.byref(n), ctypes.byref(wg), ctypes.byref(lds))
+        return {"items": n.value, "wg": wg.value, "lds_bytes": lds.value}
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().cnv_batch_destroy(self.handle)
+        except Exception:
+            pass
+
+
 def set_zfaces(g, zfaces):
     """Stretched z mesh of grid g: nz_total + 1 increasing face positions."""
     zf = np.ascontiguousarray(zfaces, dtype=np.float64)

@@ -54,7 +54,11 @@ class cnv_solve_info(ctypes.Structure):
 
 class cnv_batch_item(ctypes.Structure):
     _fields_ = [("grid_in", ctypes.c_void_p), ("grid_out", ctypes.c_void_p), ("op", ctypes.c_int),
-                ("reserved", ctypes.c_int), ("nu_dt", ctypes.c_double)]
+                ("ghost", ctypes.c_int), ("nu_dt", ctypes.c_double)]
+
+
+CNV_OP_DIFFUSION = 0
+CNV_OP_WENO5 = 1
 
 
 c_int_p = ctypes.POINTER(ctypes.c_int)
@@ -106,6 +110,11 @@ SIGNATURES = {
     "cnv_autotune_weno5": (i32, [vp, vp, i32, c_int_p, c_int_p, vp, i32, c_int_p]),
     "cnv_tune_set_weno5": (i32, [i32, i32, i32]),
     "cnv_tune_get_weno5": (i32, [i32, c_int_p, c_int_p]),
+    "cnv_batch_create": (i32, [vp, i32, ctypes.POINTER(vp)]),
+    "cnv_batch_autotune": (i32, [vp, i32, vp, i32, c_int_p]),
+    "cnv_batch_run": (i32, [vp, vp]),
+    "cnv_batch_info": (i32, [vp, c_int_p, c_int_p, ctypes.POINTER(sz)]),
+    "cnv_batch_destroy": (i32, [vp]),
     "cnv_graph_begin": (i32, [vp]),
     "cnv_graph_end": (i32, [vp, ctypes.POINTER(vp)]),
     "cnv_graph_launch": (i32, [vp, vp]),

@@ -1,0 +1,498 @@
+// cnv_batch.hip -- many independent grids of uneven block counts, block
+// sizes, component counts, ghost widths and BCs processed by one persistent
+// launch that walks a device work list.
+//
+// A work item is (grid, component, block, z-range).  The workgroup stages the
+// item's cells plus the face halo its operator reads (one layer for the
+// 7-point Euler step, two for WENO5) in LDS, evaluating every ghost with the
+// same lab rules as cnv_lab_load (neighbour copy or the block's BC list,
+// resolved for the grid's Stencil(-g, g+1)), then applies the operator:
+//   op 0: u_new = u + nu_dt * lap(u) with the Laplacian.cpp:125-129 order
+//   op 1: WENO5 u^-_{i+1/2}, u^+_{i-1/2} in x, y, z (outputs 6c+2d, 6c+2d+1)
+// Items are sorted by cost (largest first) and dealt round-robin to a grid of
+// CU-count x occupancy workgroups, so one launch balances uneven grids.  The
+// z depth of the items of each (block edge, ghost width) class and the
+// workgroup size are chosen by cnv_batch_autotune.  FP32 grids only.
+#include "cnv_device.h"
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+
+namespace cnv
+{
+
+constexpr int kBMaxComp = 3;
+
+// device descriptor of one grid of the batch
+struct BGrid {
+    const float *in; // component 0
+    float *out;      // component 0 of the output grid
+    Geom G;
+    long long ocstride;
+    int op, g, ncomp, pad;
+    int has_bc[kBMaxComp];
+    const DevBCHead *heads[kBMaxComp];
+    const DevBC *entries[kBMaxComp];
+    float ih2[3];
+    float nudt;
+};
+
+struct BItem {
+    int grid;
+    int comp;
+    int blk;
+    short z0, nz;
+};
+
+__device__ inline void weno5f(float a, float b, float c, float d, float e, float &um, float &up)
+{
+#pragma clang fp contract(fast)
+    const float m2 = (a * a + b * b + c * c + d * d + e * e) * 0.2f + 1e-30f;
+    const float im2 = __builtin_amdgcn_rcpf(m2);
+    const float t0 = a - 2.f * b + c, s0 = a - 4.f * b + 3.f * c;
+    const float t1 = b - 2.f * c + d, s1 = b - d;
+    const float t2 = c - 2.f * d + e, s2 = 3.f * c - 4.f * d + e;
+    const float k13 = 13.f / 12.f;
+    const float b0 = (k13 * t0 * t0 + 0.25f * s0 * s0) * im2 + 1e-6f;
+    const float b1 = (k13 * t1 * t1 + 0.25f * s1 * s1) * im2 + 1e-6f;
+    const float b2 = (k13 * t2 * t2 + 0.25f * s2 * s2) * im2 + 1e-6f;
+    const float r0 = __builtin_amdgcn_rcpf(b0 * b0);
+    const float r1 = __builtin_amdgcn_rcpf(b1 * b1);
+    const float r2 = __builtin_amdgcn_rcpf(b2 * b2);
+    {
+        const float a0 = 0.1f * r0, a1 = 0.6f * r1, a2 = 0.3f * r2;
+        const float q0 = 2.f * a - 7.f * b + 11.f * c;
+        const float q1 = -b + 5.f * c + 2.f * d;
+        const float q2 = 2.f * c + 5.f * d - e;
+        um = (a0 * q0 + a1 * q1 + a2 * q2) * __builtin_amdgcn_rcpf(6.f * (a0 + a1 + a2));
+    }
+    {
+        const float a0 = 0.3f * r0, a1 = 0.6f * r1, a2 = 0.1f * r2;
+        const float q0 = -a + 5.f * b + 2.f * c;
+        const float q1 = 2.f * b + 5.f * c - d;
+        const float q2 = 11.f * c - 7.f * d + 2.f * e;
+        up = (a0 * q0 + a1 * q1 + a2 * q2) * __builtin_amdgcn_rcpf(6.f * (a0 + a1 + a2));
+    }
+}
+
+template <int WG>
+__global__ __launch_bounds__(WG) void k_batch(const BGrid *__restrict__ grids, const BItem *__restrict__ items,
+                                              int nitems)
+{
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const BItem I = items[it];
+        const BGrid &Gd = grids[I.grid];
+        const int c = I.comp;
+        const int nx = Gd.G.n[0], ny = Gd.G.n[1], nzb = Gd.G.n[2];
+        const int gl = Gd.op == 0 ? 1 : 2; // layers read by the operator
+        const int Lx = nx + 2 * gl, Ly = ny + 2 * gl, Lz = I.nz + 2 * gl;
+        const float *cbase = Gd.in + (long long)c * Gd.G.cstride;
+        const float *ub = cbase + (long long)I.blk * Gd.G.bstride;
+        FaceL<float> F[6];
+        const int sb[3] = {-Gd.g, -Gd.g, -Gd.g}, se[3] = {Gd.g + 1, Gd.g + 1, Gd.g + 1};
+        make_faces<float>(Gd.G, Gd.heads[c], Gd.entries[c], Gd.has_bc[c], I.blk, cbase, sb, se, F);
+        const long long sx = nx, sxy = (long long)nx * ny;
+        // stage the tile and its face halo
+        const int ltot = Lx * Ly * Lz;
+        for (int l = tid; l < ltot; l += WG) {
+            const int ix = l % Lx, iy = (l / Lx) % Ly, iz = l / (Lx * Ly);
+            const int x = ix - gl, y = iy - gl, z = I.z0 + iz - gl;
+            const bool ox = x < 0 || x >= nx, oy = y < 0 || y >= ny, oz = z < 0 || z >= nzb;
+            if ((int)ox + (int)oy + (int)oz > 1)
+                continue; // edges and corners are never read
+            float v;
+            if (ox)
+                v = x < 0 ? face_layer(F[0], -x, 1, sx * y + sxy * z) : face_layer(F[1], x - nx + 1, 1, sx * y + sxy * z);
+            else if (oy)
+                v = y < 0 ? face_layer(F[2], -y, sx, x + sxy * z) : face_layer(F[3], y - ny + 1, sx, x + sxy * z);
+            else if (oz)
+                v = z < 0 ? face_layer(F[4], -z, sxy, x + sx * y) : face_layer(F[5], z - nzb + 1, sxy, x + sx * y);
+            else
+                v = ub[x + sx * y + sxy * z];
+            lds[l] = v;
+        }
+        __syncthreads();
+        const int ctot = nx * ny * I.nz;
+        float *ob = Gd.out + (long long)I.blk * Gd.G.bstride;
+        const int LXY = Lx * Ly;
+        for (int l = tid; l < ctot; l += WG) {
+            const int x = l % nx, y = (l / nx) % ny, zt = l / (nx * ny);
+            const int z = I.z0 + zt;
+            const int p = (x + gl) + Lx * (y + gl) + LXY * (zt + gl);
+            const long long o = x + sx * y + sxy * z;
+            const float cc = lds[p];
+            if (Gd.op == 0) {
+                const float ddx = Gd.ih2[0] * (lds[p + 1] - 2 * cc + lds[p - 1]);
+                const float ddy = Gd.ih2[1] * (lds[p + Lx] - 2 * cc + lds[p - Lx]);
+                const float ddz = Gd.ih2[2] * (lds[p + LXY] - 2 * cc + lds[p - LXY]);
+                const float lap = ddx + ddy + ddz;
+                const float lt = lap * Gd.nudt;
+                ob[(long long)c * Gd.ocstride + o] = cc + lt;
+            } else {
+                float um, up;
+                float *oc = ob + (long long)(6 * c) * Gd.ocstride + o;
+                weno5f(lds[p - 2], lds[p - 1], cc, lds[p + 1], lds[p + 2], um, up);
+                oc[0] = um;
+                oc[Gd.ocstride] = up;
+                weno5f(lds[p - 2 * Lx], lds[p - Lx], cc, lds[p + Lx], lds[p + 2 * Lx], um, up);
+                oc[2 * Gd.ocstride] = um;
+                oc[3 * Gd.ocstride] = up;
+                weno5f(lds[p - 2 * LXY], lds[p - LXY], cc, lds[p + LXY], lds[p + 2 * LXY], um, up);
+                oc[4 * Gd.ocstride] = um;
+                oc[5 * Gd.ocstride] = up;
+            }
+        }
+        __syncthreads(); // the next item overwrites the tile
+    }
+}
+
+} // namespace cnv
+
+// ---------------------------------------------------------------- host side
+
+using namespace cnv;
+
+struct cnv_batch_s {
+    std::vector<cnv_batch_item> src;
+    std::vector<BGrid> grids;
+    std::vector<std::vector<uint64_t>> versions; // BC versions the descriptors were built from
+    BGrid *d_grids = nullptr;
+    BItem *d_items = nullptr;
+    int nitems = 0;
+    int wg = 256;
+    int nwg = 0;
+    size_t lds_bytes = 0;
+    std::map<std::pair<int, int>, int> zdepth; // (edge, ghost) -> planes per item
+    std::vector<BItem> items;
+};
+
+static int class_key_edge(const cnv_grid_s *g) { return g->desc.block_cells[0]; }
+
+static size_t tile_bytes(const cnv_grid_s *g, int op, int nz)
+{
+    const int gl = op == 0 ? 1 : 2;
+    return sizeof(float) * (size_t)(g->desc.block_cells[0] + 2 * gl) * (g->desc.block_cells[1] + 2 * gl) *
+           (nz + 2 * gl);
+}
+
+static constexpr size_t kMaxLds = 64 * 1024;
+
+// Items for the current depth table, sorted by cost (cells x op weight).
+static int build_items(cnv_batch_s *b, const std::map<std::pair<int, int>, int> *only)
+{
+    b->items.clear();
+    b->lds_bytes = 0;
+    for (size_t gi = 0; gi < b->src.size(); ++gi) {
+        const cnv_batch_item &it = b->src[gi];
+        const cnv_grid_s *g = it.in;
+        const std::pair<int, int> key(class_key_edge(g), it.ghost);
+        if (only && !only->count(key))
+            continue;
+        int nz = b->zdepth.count(key) ? b->zdepth.at(key) : g->desc.block_cells[2];
+        nz = std::max(1, std::min(nz, g->desc.block_cells[2]));
+        while (nz > 1 && tile_bytes(g, it.op, nz) > kMaxLds)
+            nz /= 2;
+        if (tile_bytes(g, it.op, nz) > kMaxLds)
+            return fail(CNV_ERR_UNSUPPORTED, "block too large for the batched kernel");
+        b->lds_bytes = std::max(b->lds_bytes, tile_bytes(g, it.op, nz));
+        for (int c = 0; c < g->desc.ncomp; ++c)
+            for (long long blk = 0; blk < g->nblocks_total; ++blk)
+                for (int z0 = 0; z0 < g->desc.block_cells[2]; z0 += nz) {
+                    BItem I;
+                    I.grid = (int)gi;
+                    I.comp = c;
+                    I.blk = (int)blk;
+                    I.z0 = (short)z0;
+                    I.nz = (short)std::min(nz, g->desc.block_cells[2] - z0);
+                    b->items.push_back(I);
+                }
+    }
+    auto cost = [&](const BItem &I) {
+        const cnv_grid_s *g = b->src[I.grid].in;
+        return (long long)g->desc.block_cells[0] * g->desc.block_cells[1] * I.nz * (b->src[I.grid].op == 1 ? 8 : 1);
+    };
+    std::stable_sort(b->items.begin(), b->items.end(),
+                     [&](const BItem &x, const BItem &y) { return cost(x) > cost(y); });
+    b->nitems = (int)b->items.size();
+    if (b->d_items)
+        CNV_HIP(hipFree(b->d_items));
+    b->d_items = nullptr;
+    if (b->nitems > 0) {
+        CNV_HIP(hipMalloc((void **)&b->d_items, sizeof(BItem) * b->items.size()));
+        CNV_HIP(hipMemcpy(b->d_items, b->items.data(), sizeof(BItem) * b->items.size(), hipMemcpyHostToDevice));
+    }
+    return CNV_OK;
+}
+
+static int build_grids(cnv_batch_s *b)
+{
+    b->grids.clear();
+    b->versions.clear();
+    for (const auto &it : b->src) {
+        cnv_grid_s *g = it.in;
+        BGrid d{};
+        d.in = static_cast<const float *>(g->data);
+        d.out = static_cast<float *>(it.out->data);
+        d.G = geom_of(g);
+        d.ocstride = (long long)(it.out->component_bytes / 4);
+        d.op = it.op;
+        d.g = it.ghost;
+        d.ncomp = g->desc.ncomp;
+        std::vector<uint64_t> ver;
+        std::lock_guard<std::mutex> lk(g->mtx);
+        for (int c = 0; c < g->desc.ncomp; ++c) {
+            int rc = ensure_bc_table(g, c);
+            if (rc != CNV_OK)
+                return rc;
+            const BCTable &t = g->tables[c];
+            d.has_bc[c] = (t.any && t.heads && t.entries) ? 1 : 0;
+            d.heads[c] = t.heads;
+            d.entries[c] = t.entries;
+            // every face the operator reads must be written by the lab rules
+            const int sb[3] = {-it.ghost, -it.ghost, -it.ghost}, se[3] = {it.ghost + 1, it.ghost + 1, it.ghost + 1};
+            for (const auto &l : g->bcs[c]) {
+                if (l.empty())
+                    continue;
+                std::vector<DevBC> tmp;
+                for (const auto &x : l)
+                    tmp.push_back({x.kind, x.dir, x.side, 0, x.value});
+                int kind[6];
+                double val[6];
+                resolve_faces(tmp.data(), (int)tmp.size(), sb, se, kind, val);
+                for (int f = 0; f < 6; ++f)
+                    if (kind[f] == FACE_UNSET)
+                        return fail(CNV_ERR_UNSUPPORTED, "a BC list leaves ghost cells unwritten");
+            }
+        }
+        ver.push_back(g->bc_version);
+        b->versions.push_back(ver);
+        for (int d2 = 0; d2 < 3; ++d2)
+            d.ih2[d2] = (float)(1. / (g->h[d2] * g->h[d2]));
+        d.nudt = (float)it.nu_dt;
+        b->grids.push_back(d);
+    }
+    if (!b->d_grids)
+        CNV_HIP(hipMalloc((void **)&b->d_grids, sizeof(BGrid) * b->grids.size()));
+    CNV_HIP(hipMemcpy(b->d_grids, b->grids.data(), sizeof(BGrid) * b->grids.size(), hipMemcpyHostToDevice));
+    return CNV_OK;
+}
+
+static int launch_batch(cnv_batch_s *b, hipStream_t st)
+{
+    if (b->nitems == 0)
+        return CNV_OK;
+    int dev = 0, ncu = 0;
+    CNV_HIP(hipGetDevice(&dev));
+    CNV_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int per_cu = b->wg >= 256 ? 4 : 8;
+    const int nwg = std::min(b->nitems, ncu * per_cu);
+    b->nwg = nwg;
+    if (b->wg == 128)
+        hipLaunchKernelGGL(k_batch<128>, dim3(nwg), dim3(128), b->lds_bytes, st, b->d_grids, b->d_items, b->nitems);
+    else if (b->wg == 512)
+        hipLaunchKernelGGL(k_batch<512>, dim3(nwg), dim3(512), b->lds_bytes, st, b->d_grids, b->d_items, b->nitems);
+    else
+        hipLaunchKernelGGL(k_batch<256>, dim3(nwg), dim3(256), b->lds_bytes, st, b->d_grids, b->d_items, b->nitems);
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
+
+static bool stale(const cnv_batch_s *b)
+{
+    for (size_t i = 0; i < b->src.size(); ++i)
+        if (b->versions[i][0] != b->src[i].in->bc_version)
+            return true;
+    return false;
+}
+
+extern "C" {
+
+int cnv_batch_create(const cnv_batch_item *items, int n, cnv_batch *out)
+{
+    try {
+        if (!items || n < 1 || !out)
+            return fail(CNV_ERR_INVALID, "need at least one batch item and an output handle");
+        *out = nullptr;
+        for (int i = 0; i < n; ++i) {
+            const cnv_batch_item &it = items[i];
+            if (!it.in || !it.out || it.in == it.out)
+                return fail(CNV_ERR_INVALID, "batch item needs distinct input and output grids");
+            if (it.in->desc.dtype != CNV_F32 || it.out->desc.dtype != CNV_F32)
+                return fail(CNV_ERR_UNSUPPORTED, "the batched kernel runs FP32 grids");
+            if (it.op != CNV_OP_DIFFUSION && it.op != CNV_OP_WENO5)
+                return fail(CNV_ERR_INVALID, "unknown batch operator");
+            if (it.ghost < (it.op == CNV_OP_WENO5 ? 2 : 1) || it.ghost > 8)
+                return fail(CNV_ERR_INVALID, "ghost width too small for the operator");
+            if (it.in->desc.ncomp > kBMaxComp)
+                return fail(CNV_ERR_UNSUPPORTED, "at most 3 components per batched grid");
+            for (int d = 0; d < 3; ++d) {
+                if (it.in->desc.nblocks[d] != it.out->desc.nblocks[d] ||
+                    it.in->desc.block_cells[d] != it.out->desc.block_cells[d])
+                    return fail(CNV_ERR_SIZE, "input and output grids differ in shape");
+                if (it.in->desc.block_cells[d] < it.ghost && it.op == CNV_OP_WENO5)
+                    return fail(CNV_ERR_UNSUPPORTED, "block edge smaller than the ghost width");
+            }
+            const int need = it.op == CNV_OP_WENO5 ? 6 * it.in->desc.ncomp : it.in->desc.ncomp;
+            if (it.out->desc.ncomp < need)
+                return fail(CNV_ERR_SIZE, "output grid has too few components");
+            if (it.in->nblocks_total > 0x7fffffff)
+                return fail(CNV_ERR_UNSUPPORTED, "grid too large");
+        }
+        cnv_batch_s *b = new cnv_batch_s();
+        b->src.assign(items, items + n);
+        int rc = build_grids(b);
+        if (rc == CNV_OK)
+            rc = build_items(b, nullptr);
+        if (rc != CNV_OK) {
+            cnv_batch_destroy(b);
+            return rc;
+        }
+        *out = b;
+        return CNV_OK;
+    } catch (...) {
+        return fail(CNV_ERR_ALLOC, "batch creation failed");
+    }
+}
+
+int cnv_batch_destroy(cnv_batch b)
+{
+    if (!b)
+        return CNV_OK;
+    if (b->d_grids)
+        (void)hipFree(b->d_grids);
+    if (b->d_items)
+        (void)hipFree(b->d_items);
+    delete b;
+    return CNV_OK;
+}
+
+int cnv_batch_run(cnv_batch b, cnv_stream s)
+{
+    if (!b)
+        return fail(CNV_ERR_INVALID, "batch is NULL");
+    if (stale(b)) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (s)
+            (void)hipStreamIsCapturing((hipStream_t)s, &cs);
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(CNV_ERR_INVALID, "BC lists changed since the batch was built; run it once before capture");
+        int rc = build_grids(b);
+        if (rc != CNV_OK)
+            return rc;
+    }
+    return launch_batch(b, (hipStream_t)s);
+}
+
+int cnv_batch_info(cnv_batch b, int *nitems, int *wg, size_t *lds_bytes)
+{
+    if (!b)
+        return fail(CNV_ERR_INVALID, "batch is NULL");
+    if (nitems)
+        *nitems = b->nitems;
+    if (wg)
+        *wg = b->wg;
+    if (lds_bytes)
+        *lds_bytes = b->lds_bytes;
+    return CNV_OK;
+}
+
+// For every (block edge, ghost width) class: time the class's items alone for
+// each z depth, keep the fastest; then time the whole batch for each
+// workgroup size.  results: {edge, ghost, depth, microseconds} per trial
+// (workgroup trials are reported with edge = 0, ghost = wg).
+int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results, int *nresults)
+{
+    if (!b)
+        return fail(CNV_ERR_INVALID, "batch is NULL");
+    if (iters < 1)
+        return fail(CNV_ERR_INVALID, "iters must be >= 1");
+    hipStream_t st;
+    CNV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    CNV_HIP(hipEventCreate(&e0));
+    CNV_HIP(hipEventCreate(&e1));
+    int cnt = 0, rc = CNV_OK;
+    auto time_it = [&](float &us) -> int {
+        int r = launch_batch(b, st); // warm
+        if (r != CNV_OK)
+            return r;
+        (void)hipEventRecord(e0, st);
+        for (int i = 0; i < iters && r == CNV_OK; ++i)
+            r = launch_batch(b, st);
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess)
+            return fail(CNV_ERR_HIP, "autotune: batch kernel failed");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        us = ms * 1000.f / iters;
+        return r;
+    };
+    std::map<std::pair<int, int>, int> classes;
+    for (const auto &it : b->src)
+        classes[{class_key_edge(it.in), it.ghost}] = 1;
+    b->wg = 256;
+    for (const auto &kv : classes) {
+        const auto key = kv.first;
+        const int edge = key.first;
+        float best = 1e30f;
+        int bestz = edge;
+        std::map<std::pair<int, int>, int> only{{key, 1}};
+        for (int z = 1; z <= edge && rc == CNV_OK; z *= 2) {
+            b->zdepth[key] = z;
+            rc = build_items(b, &only);
+            if (rc != CNV_OK)
+                break;
+            float us = 0.f;
+            rc = time_it(us);
+            if (rc != CNV_OK)
+                break;
+            if (results && cnt < max_results) {
+                results[4 * cnt] = edge;
+                results[4 * cnt + 1] = key.second;
+                results[4 * cnt + 2] = z;
+                results[4 * cnt + 3] = us;
+            }
+            ++cnt;
+            if (us < best) {
+                best = us;
+                bestz = z;
+            }
+        }
+        b->zdepth[key] = bestz;
+    }
+    if (rc == CNV_OK)
+        rc = build_items(b, nullptr);
+    float best = 1e30f;
+    int bestwg = 256;
+    const int wgs[3] = {128, 256, 512};
+    for (int i = 0; i < 3 && rc == CNV_OK; ++i) {
+        b->wg = wgs[i];
+        float us = 0.f;
+        rc = time_it(us);
+        if (rc != CNV_OK)
+            break;
+        if (results && cnt < max_results) {
+            results[4 * cnt] = 0;
+            results[4 * cnt + 1] = wgs[i];
+            results[4 * cnt + 2] = 0;
+            results[4 * cnt + 3] = us;
+        }
+        ++cnt;
+        if (us < best) {
+            best = us;
+            bestwg = wgs[i];
+        }
+    }
+    b->wg = bestwg;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    if (nresults)
+        *nresults = std::min(cnt, max_results);
+    return rc;
+}
+
+} // extern "C"

@@ -280,6 +280,8 @@ static bool j_dispatch(const JArgs &a, int edge, int wg, int ntiles, hipStream_t
 {
 #define J_WG(NX)                                                                                        \
     switch (wg) {                                                                                       \
+    case 64:                                                                                            \
+        return j_launch<T, TF, TO, NX, 64, MODE>(a, ntiles, st);                                        \
     case 128:                                                                                           \
         return j_launch<T, TF, TO, NX, 128, MODE>(a, ntiles, st);                                       \
     case 256:                                                                                           \
@@ -308,7 +310,7 @@ static JTune j_tune(int dtype, int edge)
         return {512, 16};
     if (edge >= 32)
         return {256, 16};
-    return {128, 16};
+    return {dtype == CNV_F64 ? 128 : 64, 16};
 }
 
 // metric of the grid's z direction: uniform (1/h^2) unless faces were set

@@ -66,6 +66,53 @@ def diffusion_dt(h, nu=1.0):
     return 0.1 * h * h, nu
 
 
+# ------------------------------------------------------------------ batch of grids
+
+
+def batch_specs(ngrids=96, k=4, max_blocks=(8, 8, 4), edges=(8, 16, 32)):
+    """Seeded descriptions of independent grids: block counts in
+    [1, max_blocks], block edge from `edges`, 1-3 components, operator
+    (diffusion step or WENO5; WENO5 uses ghost 3, diffusion ghost 1 or 3), and
+    per axis either periodic or, per side, Dirichlet U[-1,1] / absorbing."""
+    r = rng(k)
+    specs = []
+    for gi in range(ngrids):
+        nb = tuple(int(r.integers(1, m + 1)) for m in max_blocks)
+        edge = int(edges[r.integers(0, len(edges))])
+        ncomp = int(r.integers(1, 4))
+        op = "weno5" if r.random() < 0.5 else "diffusion"
+        ghost = 3 if op == "weno5" else int((1, 3)[r.integers(0, 2)])
+        bcs = []
+        for d in range(3):
+            if r.random() < 0.4:
+                continue  # periodic axis
+            for s in (0, 1):
+                if r.random() < 0.5:
+                    bcs.append(("dirichlet", d, s, float(r.uniform(-1, 1))))
+                else:
+                    bcs.append(("absorbing", d, s, 0.0))
+        specs.append({"nblocks": nb, "edge": edge, "ncomp": ncomp, "op": op, "ghost": ghost, "bcs": bcs,
+                      "seed": int(r.integers(0, 2 ** 31))})
+    return specs
+
+
+def batch_field(spec, comp):
+    """Seeded Fourier-mode field (4 modes, |k| <= 4, amplitude 1/|k|) of one
+    component of a batch grid, global (Z, Y, X) float64."""
+    nb, e = spec["nblocks"], spec["edge"]
+    shape = (nb[2] * e, nb[1] * e, nb[0] * e)
+    r = np.random.Generator(np.random.PCG64(spec["seed"] + comp))
+    xs = [cell_centres(n) for n in shape[::-1]]
+    u = np.zeros(shape)
+    for _ in range(4):
+        kv = r.integers(-4, 5, size=3)
+        kn = max(1.0, float(np.sqrt((kv * kv).sum())))
+        ph = r.random() * 2 * np.pi
+        u += np.cos(2 * np.pi * (kv[0] * xs[0][None, None, :] + kv[1] * xs[1][None, :, None]
+                                 + kv[2] * xs[2][:, None, None]) + ph) / kn
+    return u
+
+
 # ------------------------------------------------------------------ WENO5 state
 
 GAMMA = 1.4

@@ -238,6 +238,39 @@ typedef struct {
 int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts *opts, cnv_solve_info *info,
                      double *history, int history_cap, cnv_stream s);
 
+/* ------------------------------------------------------------ batches */
+/* No counterpart in the reference (its applications loop over one grid's
+ * blocks on the host, Laplacian.cpp:110-131).  Many independent FP32 grids
+ * of uneven block counts, block sizes, component counts, ghost widths and
+ * BCs, each with its own operator, processed by ONE persistent launch over a
+ * device work list of (grid, component, block, z-range) items, largest first.
+ * Ghosts follow the lab rules of cnv_lab_load for the grid's
+ * Stencil(-ghost, ghost+1) and BC lists.  The launch is capturable in a HIP
+ * graph (cnv_graph_begin/end) after one plain run. */
+enum cnv_op {
+    CNV_OP_DIFFUSION = 0, /* out = in + nu_dt * laplacian(in), as cnv_diffusion_step */
+    CNV_OP_WENO5 = 1      /* out = WENO5 states of in, as cnv_weno5 */
+};
+
+typedef struct {
+    cnv_grid in;
+    cnv_grid out;
+    int op;       /* enum cnv_op */
+    int ghost;    /* g of the grid's Stencil(-g, g+1); >= 2 for CNV_OP_WENO5 */
+    double nu_dt; /* Euler step (CNV_OP_DIFFUSION) */
+} cnv_batch_item;
+
+typedef struct cnv_batch_s *cnv_batch;
+int cnv_batch_create(const cnv_batch_item *items, int n, cnv_batch *out);
+/* Autotuner: per (block edge, ghost width) class the z depth of the work
+ * items, then the workgroup size of the launch.  results: up to max_results
+ * records {edge, ghost, depth, microseconds} (workgroup trials: {0, wg, 0,
+ * microseconds}). */
+int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results, int *nresults);
+int cnv_batch_run(cnv_batch b, cnv_stream s);
+int cnv_batch_info(cnv_batch b, int *nitems, int *wg, size_t *lds_bytes);
+int cnv_batch_destroy(cnv_batch b);
+
 /* ------------------------------------------------------------ autotuner */
 /* No counterpart in the reference.  Times every valid (workgroup size,
  * z-planes per tile) pair of the fused diffusion kernel on u -> out (`iters`

@@ -1,0 +1,112 @@
+"""GPU parity of the batched persistent launch: every grid of a batch of
+uneven grids gives the result of its single-grid entry point (diffusion
+bit-exact, also against the oracle; WENO5 within FP32 rounding of the
+single-grid kernel and 1e-6 rel-L2 of the oracle's FP64 evaluation), and a
+graph replay of the batch reproduces the eager run every time."""
+import numpy as np
+import pytest
+
+import cubismnova_amd as cnv
+import oracle
+from cubismnova_amd import workloads
+from tests.helpers import cells_of
+
+pytestmark = pytest.mark.gpu
+
+
+def make_grids(spec):
+    nb, e = spec["nblocks"], spec["edge"]
+    g = cnv.Cartesian(nb, e, dtype=np.float32, ncomp=spec["ncomp"])
+    for c in range(spec["ncomp"]):
+        g.upload_global(c, workloads.batch_field(spec, c).astype(np.float32))
+    for kind, d, s, v in spec["bcs"]:
+        g.addFaceBC(cnv.Dirichlet(d, s, v) if kind == "dirichlet" else cnv.Absorbing(d, s))
+    nout = 6 * spec["ncomp"] if spec["op"] == "weno5" else spec["ncomp"]
+    return g, cnv.Cartesian(nb, e, dtype=np.float32, ncomp=nout)
+
+
+def oracle_bcs(spec):
+    nb = spec["nblocks"]
+    lists = {}
+    for kind, d, s, v in spec["bcs"]:
+        for b in range(nb[0] * nb[1] * nb[2]):
+            bi = (b % nb[0], (b // nb[0]) % nb[1], b // (nb[0] * nb[1]))
+            if bi[d] == (0 if s == 0 else nb[d] - 1):
+                lists.setdefault(b, []).append((2 if kind == "dirichlet" else 1, d, s, v))
+    return lists
+
+
+@pytest.fixture(scope="module")
+def batch_setup():
+    specs = workloads.batch_specs(ngrids=14, k=40, max_blocks=(3, 3, 2), edges=(8, 16, 32))
+    grids = [make_grids(s) for s in specs]
+    items = []
+    for s, (gi, go) in zip(specs, grids):
+        h = 1.0 / (s["nblocks"][0] * s["edge"])
+        items.append((gi, go, s["op"], s["ghost"], 0.1 * h * h))
+    return specs, grids, items
+
+
+def test_batch_matches_single_grid_and_oracle(gpu, batch_setup):
+    specs, grids, items = batch_setup
+    b = cnv.Batch(items)
+    b.run()
+    for s, (gi, go), it in zip(specs, grids, items):
+        nb, e = s["nblocks"], s["edge"]
+        ref = cnv.Cartesian(nb, e, dtype=np.float32, ncomp=go.ncomp)
+        if s["op"] == "diffusion":
+            cnv.diffusion_step(gi, ref, it[4])
+        else:
+            cnv.weno5(gi, ref)
+        bcs = oracle_bcs(s)
+        for c in range(s["ncomp"]):
+            slab = gi.download(c)
+            f = oracle.Field(slab, nb, e, gi.block_stride, bcs)
+            if s["op"] == "diffusion":
+                got = cells_of(go.download(c), nb, (e,) * 3, gi.block_stride)
+                assert np.array_equal(got, cells_of(ref.download(c), nb, (e,) * 3, gi.block_stride))
+                hh = [1.0 / (nb[d] * e) for d in range(3)]
+                un, _ = oracle.diffusion_step(f, hh, it[4])
+                assert np.array_equal(got, cells_of(un, nb, (e,) * 3, gi.block_stride))
+            else:
+                o64 = oracle.weno5(f, prec=1)
+                for k in range(6):
+                    got = cells_of(go.download(6 * c + k), nb, (e,) * 3, gi.block_stride).astype(np.float64)
+                    one = cells_of(ref.download(6 * c + k), nb, (e,) * 3, gi.block_stride).astype(np.float64)
+                    want = cells_of(o64[k], nb, (e,) * 3, gi.block_stride)
+                    scale = np.max(np.abs(want))
+                    assert np.max(np.abs(got - one)) <= 1e-5 * scale
+                    assert np.linalg.norm(got - want) <= 1e-6 * np.linalg.norm(want)
+
+
+def test_batch_autotune_and_graph_replays(gpu, batch_setup):
+    specs, grids, items = batch_setup
+    b = cnv.Batch(items)
+    table = b.autotune(iters=2)
+    assert len(table) > 0
+    b.run()
+    eager = [[go.download(c) for c in range(go.ncomp)] for _, go in grids]
+    st = cnv.Stream()
+    gr = cnv.Graph(st).capture()
+    b.run(stream=st)
+    gr.end()
+    for rep in range(3):
+        for _, go in grids:            # clobber the outputs between replays
+            for c in range(go.ncomp):
+                cnv.call("cnv_fill", go.data_ptr(c), 0, -7.0, go.component_bytes // 4, None)
+        gr.launch()
+        st.synchronize()
+        for (_, go), ref in zip(grids, eager):
+            for c in range(go.ncomp):
+                assert np.array_equal(go.download(c), ref[c]), (rep, c)
+
+
+def test_batch_rejects_fp64_and_small_ghost(gpu):
+    g = cnv.Cartesian(1, 8, dtype=np.float64)
+    o = cnv.Cartesian(1, 8, dtype=np.float64)
+    with pytest.raises(cnv.CnvError):
+        cnv.Batch([(g, o, "diffusion", 1, 1e-3)])
+    g32 = cnv.Cartesian(1, 8, dtype=np.float32)
+    o32 = cnv.Cartesian(1, 8, dtype=np.float32, ncomp=6)
+    with pytest.raises(cnv.CnvError):
+        cnv.Batch([(g32, o32, "weno5", 1, 0.0)])
