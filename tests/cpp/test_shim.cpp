@@ -1,0 +1,257 @@
+// test_shim.cpp -- the reference's own unit tests for the stencil path,
+// written against the C++ drop-in (Cubism/AMD.h) and run on the GPU.
+// Mirrors test/UnitTests/Block/FieldLabTest.cpp, test/UnitTests/BC/*Test.cpp
+// and test/UnitTests/Grid/CartesianTest.cpp:172-329, plus the Laplacian loop
+// of test/sandbox/Benchmarks/Kernel/Laplacian/Laplacian.cpp:110-131 against
+// the device operator.
+#include "Cubism/AMD.h"
+
+#include <cmath>
+#include <cstdio>
+#include <limits>
+
+using namespace Cubism;
+
+static int g_fail = 0, g_checks = 0;
+#define EXPECT(cond)                                                                                         \
+    do {                                                                                                     \
+        ++g_checks;                                                                                          \
+        if (!(cond)) {                                                                                       \
+            ++g_fail;                                                                                        \
+            if (g_fail < 20)                                                                                 \
+                std::printf("FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);                                \
+        }                                                                                                    \
+    } while (0)
+
+using Grid64 = Grid::Cartesian<double>;
+using Field64 = Grid64::BaseType;
+using Lab64 = Grid64::FieldLabType;
+using Stencil = Core::Stencil<3>;
+
+// FieldLabTest.cpp:16-115 (3-D cells): iota field, self-periodic one-block grid
+static void fieldlab_ghosts(bool tensorial)
+{
+    Grid64 grid({1, 1, 1}, {16, 16, 16});
+    Field64 &f = grid[0];
+    double k = 0;
+    for (Index z = 0; z < 16; ++z)
+        for (Index y = 0; y < 16; ++y)
+            for (Index x = 0; x < 16; ++x)
+                f(x, y, z) = k++;
+    Lab64 lab;
+    const Stencil s(-2, 3, tensorial);
+    lab.allocate(s, {16, 16, 16});
+    grid.loadLab(f, lab);
+    for (Index z = 0; z < 16; ++z)
+        for (Index y = 0; y < 16; ++y)
+            for (Index x = 0; x < 16; ++x)
+                EXPECT(lab(x, y, z) == f(x, y, z));
+    const Index n = 16;
+    for (Index z = -2; z < n + 2; ++z)
+        for (Index y = -2; y < n + 2; ++y)
+            for (Index x = -2; x < n + 2; ++x) {
+                const MIndex q{x, y, z};
+                int out = 0;
+                for (int d = 0; d < 3; ++d)
+                    out += (q[d] < 0 || q[d] >= n);
+                if (out == 0 || (!tensorial && out > 1))
+                    continue;
+                MIndex p;
+                for (int d = 0; d < 3; ++d)
+                    p[d] = (q[d] + n) % n;
+                EXPECT(lab[q] == lab[p]);
+            }
+}
+
+// BC/AbsorbingTest.cpp, DirichletTest.cpp, SymmetryTest.cpp, BaseTest.cpp
+static void bc_tests()
+{
+    Grid64 grid({1, 1, 1}, {16, 16, 16});
+    Field64 &f = grid[0];
+    double k = 0;
+    for (Index z = 0; z < 16; ++z)
+        for (Index y = 0; y < 16; ++y)
+            for (Index x = 0; x < 16; ++x)
+                f(x, y, z) = k++;
+    auto fields = grid.getIndexFunctor();
+    const Index n = 16;
+    {
+        Field64::BCVector bcv;
+        for (size_t d = 0; d < 3; ++d)
+            for (size_t side = 0; side < 2; ++side)
+                bcv.push_back(new BC::Absorbing<Lab64>(d, side));
+        EXPECT(bcv[0]->name() == "Zeroth-Order Absorbing");
+        Lab64 lab;
+        lab.allocate(Stencil(-3, 4), {n, n, n});
+        lab.loadData(MIndex{0, 0, 0}, fields, bcv);
+        for (int d = 0; d < 3; ++d)
+            for (Index a = 0; a < n; ++a)
+                for (Index b = 0; b < n; ++b)
+                    for (Index l = 1; l <= 3; ++l) {
+                        MIndex q0{a, b, 0}, q1{a, b, 0}, r0{a, b, 0}, r1{a, b, 0};
+                        // put the slab index in dimension d and (a, b) in the others
+                        const int e1 = (d + 1) % 3, e2 = (d + 2) % 3;
+                        q0[e1] = q1[e1] = r0[e1] = r1[e1] = a;
+                        q0[e2] = q1[e2] = r0[e2] = r1[e2] = b;
+                        q0[d] = -l;
+                        r0[d] = 0;
+                        q1[d] = n - 1 + l;
+                        r1[d] = n - 1;
+                        EXPECT(lab[q0] == lab[r0]);
+                        EXPECT(lab[q1] == lab[r1]);
+                    }
+        for (auto *bc : bcv)
+            delete bc;
+    }
+    {
+        Field64::BCVector bcv;
+        for (size_t d = 0; d < 3; ++d)
+            for (size_t side = 0; side < 2; ++side)
+                bcv.push_back(new BC::Dirichlet<Lab64>(d, side, 10.0 + 2 * d + side));
+        EXPECT(bcv[0]->name() == "Dirichlet");
+        Lab64 lab;
+        lab.allocate(Stencil(-3, 4), {n, n, n});
+        lab.loadData(MIndex{0, 0, 0}, fields, bcv);
+        for (int d = 0; d < 3; ++d)
+            for (Index a = 0; a < n; ++a)
+                for (Index l = 1; l <= 3; ++l) {
+                    MIndex q0{a, a, a}, q1{a, a, a};
+                    q0[d] = -l;
+                    q1[d] = n - 1 + l;
+                    EXPECT(lab[q0] == 10.0 + 2 * d);
+                    EXPECT(lab[q1] == 11.0 + 2 * d);
+                }
+        for (auto *bc : bcv)
+            delete bc;
+    }
+    {
+        Field64::BCVector bcv;
+        for (size_t d = 0; d < 3; ++d) {
+            bcv.push_back(new BC::Symmetry<Lab64>(d, 0, 1.0));
+            bcv.push_back(new BC::Symmetry<Lab64>(d, 1, -1.0));
+        }
+        EXPECT(bcv[0]->name() == "Symmetry");
+        Lab64 lab;
+        lab.allocate(Stencil(-3, 4), {n, n, n});
+        lab.loadData(MIndex{0, 0, 0}, fields, bcv);
+        for (int d = 0; d < 3; ++d)
+            for (Index a = 0; a < n; ++a)
+                for (Index l = 1; l <= 3; ++l) {
+                    MIndex q0{a, a, a}, r0{a, a, a}, q1{a, a, a}, r1{a, a, a};
+                    q0[d] = -l;
+                    r0[d] = l - 1;
+                    q1[d] = n - 1 + l;
+                    r1[d] = n - l;
+                    EXPECT(lab[q0] == 1.0 * lab[r0]);
+                    EXPECT(lab[q1] == -1.0 * lab[r1]);
+                }
+        for (auto *bc : bcv)
+            delete bc;
+    }
+    {
+        Lab64 lab;
+        bool threw = false;
+        try {
+            lab.getActiveField();
+        } catch (const std::runtime_error &e) {
+            threw = std::string(e.what()) == "FieldLab: no field loaded";
+        }
+        EXPECT(threw);
+        threw = false;
+        try {
+            lab.loadData(MIndex{0, 0, 0}, fields);
+        } catch (const std::runtime_error &e) {
+            threw = std::string(e.what()) == "FieldLab: can not load lab data when not allocated first";
+        }
+        EXPECT(threw);
+        threw = false;
+        try {
+            Stencil bad(1, 2);
+        } catch (const std::runtime_error &e) {
+            threw = std::string(e.what()) == "Stencil: begin_ must be <= 0";
+        }
+        EXPECT(threw);
+    }
+}
+
+// CartesianTest.cpp:172-329 (3-D cells): every lab cell of every block equals
+// the analytic field at its coordinate to 8 eps
+static void cartesian_lab_analytic()
+{
+    Grid64 grid({3, 3, 3}, {8, 8, 8});
+    const double h = 1.0 / 24;
+    auto fexact = [](double x, double y, double z) {
+        const double fac = 2 * M_PI;
+        return std::sin(fac * x) * std::cos(fac * y) * std::sin(fac * z);
+    };
+    for (size_t i = 0; i < grid.size(); ++i) {
+        Field64 &f = grid[i];
+        const MIndex bi = f.getState().block_index;
+        for (Index z = 0; z < 8; ++z)
+            for (Index y = 0; y < 8; ++y)
+                for (Index x = 0; x < 8; ++x)
+                    f(x, y, z) = fexact((bi[0] * 8 + x + 0.5) * h, (bi[1] * 8 + y + 0.5) * h, (bi[2] * 8 + z + 0.5) * h);
+    }
+    Lab64 lab;
+    lab.allocate(Stencil(-2, 3, true), {8, 8, 8});
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (size_t i = 0; i < grid.size(); ++i) {
+        Field64 &f = grid[i];
+        grid.loadLab(f, lab);
+        const MIndex bi = f.getState().block_index;
+        for (Index z = -2; z < 10; ++z)
+            for (Index y = -2; y < 10; ++y)
+                for (Index x = -2; x < 10; ++x) {
+                    const double v =
+                        fexact((bi[0] * 8 + x + 0.5) * h, (bi[1] * 8 + y + 0.5) * h, (bi[2] * 8 + z + 0.5) * h);
+                    EXPECT(std::fabs(lab(x, y, z) - v) <= 8 * eps);
+                }
+    }
+}
+
+// Laplacian.cpp:110-131 on the host through the lab vs the device operator
+static void laplacian_vs_host_loop()
+{
+    Grid64 u({2, 2, 2}, {16, 16, 16});
+    Grid64 lap_u({2, 2, 2}, {16, 16, 16});
+    for (size_t i = 0; i < u.size(); ++i) {
+        Field64 &f = u[i];
+        const MIndex bi = f.getState().block_index;
+        for (Index z = 0; z < 16; ++z)
+            for (Index y = 0; y < 16; ++y)
+                for (Index x = 0; x < 16; ++x)
+                    f(x, y, z) = std::sin(0.3 * (bi[0] * 16 + x)) * std::cos(0.2 * (bi[1] * 16 + y) + 0.1 * z) +
+                                 1e-3 * ((x * 7 + y * 13 + z * 29) % 17);
+    }
+    u.upload(0);
+    Device::laplacian(u, lap_u);
+    lap_u.download(0);
+    const std::array<double, 3> h = u.getCellSize();
+    const double ihx2 = 1. / (h[0] * h[0]), ihy2 = 1. / (h[1] * h[1]), ihz2 = 1. / (h[2] * h[2]);
+    Lab64 lab;
+    lab.allocate(Stencil(-1, 2, false), {16, 16, 16});
+    for (size_t i = 0; i < u.size(); ++i) {
+        Field64 &f = u[i];
+        u.loadLab(f, lab);
+        Field64 &res = lap_u[f.getState().block_index];
+        for (Index z = 0; z < 16; ++z)
+            for (Index y = 0; y < 16; ++y)
+                for (Index x = 0; x < 16; ++x) {
+                    const double ddx = ihx2 * (lab(x + 1, y, z) - 2 * lab(x, y, z) + lab(x - 1, y, z));
+                    const double ddy = ihy2 * (lab(x, y + 1, z) - 2 * lab(x, y, z) + lab(x, y - 1, z));
+                    const double ddz = ihz2 * (lab(x, y, z + 1) - 2 * lab(x, y, z) + lab(x, y, z - 1));
+                    EXPECT(res(x, y, z) == ddx + ddy + ddz);
+                }
+    }
+}
+
+int main()
+{
+    fieldlab_ghosts(false);
+    fieldlab_ghosts(true);
+    bc_tests();
+    cartesian_lab_analytic();
+    laplacian_vs_host_loop();
+    std::printf("%d checks, %d failed\n", g_checks, g_fail);
+    return g_fail == 0 ? 0 : 1;
+}
