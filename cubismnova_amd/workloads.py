@@ -113,6 +113,25 @@ def batch_field(spec, comp):
     return u
 
 
+# ------------------------------------------------------------------ Poisson source
+
+
+def poisson_source(N, zc, k=5, nblobs=3, sigma=0.05):
+    """nblobs seeded Gaussians (sigma, amplitudes U[-1,1], centres U[0.2,0.8]^3)
+    on cell centres x, y uniform and z = zc (stretched), mean removed."""
+    r = rng(k)
+    x = cell_centres(N)
+    f = np.zeros((N, N, N))
+    for _ in range(nblobs):
+        c = r.uniform(0.2, 0.8, 3)
+        amp = r.uniform(-1, 1)
+        gx = np.exp(-(x - c[0]) ** 2 / (2 * sigma ** 2))
+        gy = np.exp(-(x - c[1]) ** 2 / (2 * sigma ** 2))
+        gz = np.exp(-(np.asarray(zc) - c[2]) ** 2 / (2 * sigma ** 2))
+        f += amp * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+    return f - f.mean()
+
+
 # ------------------------------------------------------------------ WENO5 state
 
 GAMMA = 1.4
