@@ -349,6 +349,8 @@ def bench_jacobi(args):
     u2 = jacobi_grids(cnv, N, nb, n, np.float64, zf)
     w2 = jacobi_grids(cnv, N, nb, n, np.float64, zf)
     cnv.call("cnv_fill", u2.data_ptr(0), 1, 0.0, u2.component_bytes // 8, None)
+    u2.prepare()
+    w2.prepare()
     g64 = cnv.Graph(st).capture()
     for k in range(S):
         a, b = (u2, w2) if k % 2 == 0 else (w2, u2)

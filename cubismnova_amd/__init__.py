@@ -358,6 +358,10 @@ class Cartesian:
         call("cnv_grid_clear_bcs", self.handle, comp)
         self._bcs = {k: v for k, v in self._bcs.items() if comp != -1 and k[0] != comp}
 
+    def prepare(self):
+        """Build device BC tables (and z metric) now, e.g. before graph capture."""
+        call("cnv_grid_prepare", self.handle)
+
     def loadLab(self, block_index, lab, comp=0):
         """Grid::Cartesian::loadLab (Grid/Cartesian.h:411-427)."""
         lab.loadData(block_index, self, comp=comp)

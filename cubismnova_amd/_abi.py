@@ -115,6 +115,7 @@ SIGNATURES = {
     "cnv_batch_run": (i32, [vp, vp]),
     "cnv_batch_info": (i32, [vp, c_int_p, c_int_p, ctypes.POINTER(sz)]),
     "cnv_batch_destroy": (i32, [vp]),
+    "cnv_grid_prepare": (i32, [vp]),
     "cnv_graph_begin": (i32, [vp]),
     "cnv_graph_end": (i32, [vp, ctypes.POINTER(vp)]),
     "cnv_graph_launch": (i32, [vp, vp]),

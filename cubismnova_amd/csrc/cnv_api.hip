@@ -859,6 +859,16 @@ int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts
     CNV_GUARD_END
 }
 
+int cnv_grid_prepare(cnv_grid g)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(g));
+    for (int c = 0; c < g->desc.ncomp; ++c)
+        CNV_CHECK(jacobi_prepare(g, c));
+    return CNV_OK;
+    CNV_GUARD_END
+}
+
 // ---------------------------------------------------------------- tuning
 
 int cnv_autotune_stencil7(cnv_grid u, cnv_grid out, int iters, int *wg, int *zchunk, double *results,

@@ -296,6 +296,11 @@ int cnv_tune_get_weno5(int dtype, int *wg, int *zchunk);
  * capturable after it has run once on the same grids (first calls upload
  * BC tables synchronously); reductions use plain stores and a fixed-order
  * finalize kernel, so no memset or counter node is ever recorded. */
+/* Does now what the first operator call on g would do synchronously (device
+ * BC tables of every component, the z metric): call it before recording
+ * calls on g into a graph. */
+int cnv_grid_prepare(cnv_grid g);
+
 typedef struct cnv_graph_s *cnv_graph;
 int cnv_graph_begin(cnv_stream s);
 int cnv_graph_end(cnv_stream s, cnv_graph *out);
