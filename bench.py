@@ -9,7 +9,7 @@ GCell-updates/s; the FP64 path, the FP32-vs-FP64 rel-L2 error, the roofline
 of the dominant kernel (HIP events on the launch stream) and the CPU baseline
 (the C++ oracle on the host cores) are reported in the same JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--workload diffusion|weno5|jacobi|batch]
+  python bench.py [--steps K] [--warmup W] [--workload diffusion|weno5|jacobi|batch|copy]
 """
 import argparse
 import ctypes
@@ -96,17 +96,18 @@ def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup, autotune=True, tile=None):
         a.upload_global(0, u0.astype(dtype))
     st = cnv.Stream()
     norms = cnv.DeviceBuffer(8 * max(1, steps + warmup))
+    a.prepare()
+    b.prepare()
     src, dst = a, b
-    for k in range(warmup):
-        cnv.diffusion_step(src, dst, dt, sumsq=norms.at(8 * k), stream=st)
-        src, dst = dst, src
+    if warmup:
+        res = cnv.diffusion_run(src, dst, warmup, dt, norms=norms, stream=st)
+        if res is dst:
+            src, dst = dst, src
     st.synchronize()
-    # record exactly `steps` steps into one graph
+    # record exactly `steps` steps (one multi-step run, each step's norm
+    # reduced inside the next step's launch) into one graph
     g = cnv.Graph(st).capture()
-    s2, d2 = src, dst
-    for k in range(steps):
-        cnv.diffusion_step(s2, d2, dt, sumsq=norms.at(8 * (warmup + k)), stream=st)
-        s2, d2 = d2, s2
+    s2 = cnv.diffusion_run(src, dst, steps, dt, norms=norms.at(8 * warmup), stream=st)
     g.end()
     e0, e1 = ev.new(), ev.new()
     ev.device_sync()
@@ -514,12 +515,42 @@ def bench_batch(args):
     return out
 
 
+# ------------------------------------------------------------------ calibration
+
+
+def bench_copy(args):
+    """Device-to-device copies of a known size: the HBM counters (FETCH_SIZE,
+    WRITE_SIZE) read for this kernel calibrate the counters of the others."""
+    import cubismnova_amd as cnv
+    from cubismnova_amd import _abi
+    ev = HipEvents()
+    nbytes = 256 << 20
+    a, b = cnv.DeviceBuffer(nbytes), cnv.DeviceBuffer(nbytes)
+    a.fill(np.float32, 1.0)
+    st = cnv.Stream()
+    for _ in range(args.warmup):
+        _abi.call("cnv_memcpy_d2d", b.ptr, a.ptr, nbytes, st.handle)
+    st.synchronize()
+    e0, e1 = ev.new(), ev.new()
+    ev.record(e0, st)
+    for _ in range(args.steps):
+        _abi.call("cnv_memcpy_d2d", b.ptr, a.ptr, nbytes, st.handle)
+    ev.record(e1, st)
+    avg = ev.elapsed_ms(e0, e1) / 1e3 / args.steps
+    return {"metric": "device copy GB/s (counter calibration)", "value": round(2 * nbytes / avg / 1e9, 1),
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(avg * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u8", "data": "synthetic", "config": {"workload": "copy 256 MiB device to device",
+                                                           "bytes_read": nbytes, "bytes_written": nbytes},
+            "roofline": roofline(2 * nbytes, avg)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="diffusion", choices=["diffusion", "weno5", "batch", "jacobi"])
+    ap.add_argument("--workload", default="diffusion", choices=["diffusion", "weno5", "batch", "jacobi", "copy"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in tile table")
@@ -533,8 +564,8 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
-    out = {"weno5": bench_weno5, "batch": bench_batch, "jacobi": bench_jacobi}.get(args.workload,
-                                                                                 bench_diffusion)(args)
+    out = {"weno5": bench_weno5, "batch": bench_batch, "jacobi": bench_jacobi,
+           "copy": bench_copy}.get(args.workload, bench_diffusion)(args)
     print(json.dumps(out), flush=True)
 
 

@@ -16,7 +16,7 @@ from .layout import blocks_to_global, global_to_blocks
 
 __all__ = [
     "Stencil", "Base", "Absorbing", "Dirichlet", "Symmetry", "BoundaryInfo", "Cartesian", "FieldLab",
-    "DeviceBuffer", "Stream", "Graph", "laplacian", "diffusion_step", "grid_sums", "CnvError", "lib",
+    "DeviceBuffer", "Stream", "Graph", "laplacian", "diffusion_step", "diffusion_run", "grid_sums", "CnvError", "lib",
     "global_to_blocks", "blocks_to_global",
 ]
 
@@ -478,6 +478,16 @@ def diffusion_step(u, unew, nu_dt, lap=None, sumsq=None, stream=None):
     p = sumsq.ptr if isinstance(sumsq, DeviceBuffer) else sumsq
     call("cnv_diffusion_step", u.handle, unew.handle, ctypes.c_double(nu_dt), lap.handle if lap is not None else None,
          p, _st(stream))
+
+
+def diffusion_run(u, tmp, nsteps, nu_dt, norms=None, stream=None):
+    """nsteps Euler steps alternating u -> tmp -> u ...; returns the grid that
+    holds the result (u for even nsteps, tmp for odd).  norms: optional
+    DeviceBuffer of nsteps doubles receiving the FP64 sum of squares after
+    every step."""
+    p = norms.ptr if isinstance(norms, DeviceBuffer) else norms
+    call("cnv_diffusion_run", u.handle, tmp.handle, int(nsteps), ctypes.c_double(nu_dt), p, _st(stream))
+    return u if nsteps % 2 == 0 else tmp
 
 
 def grid_sums(g, comp=0, stream=None):

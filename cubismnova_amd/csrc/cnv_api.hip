@@ -93,7 +93,7 @@ int ensure_partials(cnv_grid_s *g, size_t n)
         if (g->partials)
             CNV_HIP(hipFree(g->partials));
         g->partials = nullptr;
-        CNV_HIP(hipMalloc((void **)&g->partials, sizeof(double) * n));
+        CNV_HIP(hipMalloc((void **)&g->partials, 2 * sizeof(double) * n));
         g->partials_cap = n;
     }
     return CNV_OK;
@@ -291,7 +291,7 @@ int cnv_grid_create(const cnv_grid_desc *desc, cnv_grid *out)
     // reduction scratch sized for the largest launch of any operator, so that
     // no call allocates later (calls may be recorded into a HIP graph)
     const size_t nparts = (size_t)nbt * desc->ncomp * ((size_t)(cells + 255) / 256 + desc->block_cells[2]) + 4096;
-    e = hipMalloc((void **)&g->partials, sizeof(double) * nparts);
+    e = hipMalloc((void **)&g->partials, 2 * sizeof(double) * nparts); // two halves: multi-step runs
     if (e != hipSuccess) {
         (void)hipFree(g->data);
         delete g;
@@ -569,6 +569,37 @@ int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_ou
     }
     return launch_stencil7(u, u_new, lap_out, 0, 0, u->desc.ncomp, lap_out ? 2 : 1, nu_dt, sumsq,
                            (hipStream_t)s);
+    CNV_GUARD_END
+}
+
+int cnv_diffusion_run(cnv_grid u, cnv_grid tmp, int nsteps, double nu_dt, double *norms, cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_grid(tmp));
+    CNV_CHECK(same_shape(u, tmp));
+    if (u == tmp)
+        return fail(CNV_ERR_INVALID, "u and tmp must be different grids");
+    if (tmp->desc.ncomp < u->desc.ncomp)
+        return fail(CNV_ERR_SIZE, "tmp has fewer components than u");
+    if (nsteps < 0)
+        return fail(CNV_ERR_INVALID, "nsteps must be >= 0");
+    // tmp is read as the source of every other step: same BCs as u
+    if (nsteps > 1) {
+        std::lock_guard<std::mutex> lk(tmp->mtx);
+        bool same = true;
+        for (int c = 0; c < u->desc.ncomp && same; ++c)
+            for (long long b = 0; b < u->nblocks_total && same; ++b) {
+                const auto &x = u->bcs[c][(size_t)b], &y = tmp->bcs[c][(size_t)b];
+                same = x.size() == y.size();
+                for (size_t i = 0; same && i < x.size(); ++i)
+                    same = x[i].kind == y[i].kind && x[i].dir == y[i].dir && x[i].side == y[i].side &&
+                           x[i].value == y[i].value;
+            }
+        if (!same)
+            return fail(CNV_ERR_INVALID, "u and tmp must carry the same BC lists");
+    }
+    return launch_diffusion_run(u, tmp, nsteps, nu_dt, norms, (hipStream_t)s);
     CNV_GUARD_END
 }
 

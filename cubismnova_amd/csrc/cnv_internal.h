@@ -81,7 +81,8 @@ struct cnv_grid_s {
     uint64_t bc_version = 0;
     std::vector<cnv::BCTable> tables; // device copies per component
 
-    // scratch for deterministic reductions: per-tile partial sums
+    // scratch for deterministic reductions: per-tile partial sums, two halves
+    // of partials_cap doubles each
     double *partials = nullptr;
 
     // z metric of a stretched mesh (per global z cell), set on first use
@@ -161,6 +162,7 @@ int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in
                     int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st);
 
 int launch_sums(cnv_grid_s *g, int comp, double *out, hipStream_t st);
+int launch_diffusion_run(cnv_grid_s *u, cnv_grid_s *tmp, int nsteps, double nu_dt, double *norms, hipStream_t st);
 
 int autotune_stencil7(cnv_grid_s *u, cnv_grid_s *out, int iters, int *best_wg, int *best_z, double *results,
                       int max_results, int *nresults);

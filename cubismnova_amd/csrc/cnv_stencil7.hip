@@ -47,7 +47,44 @@ struct S7Args {
     double nudt;
     double *partials;
     double *sumsq;
+    // deferred norm of the previous step of a multi-step run: workgroup 0
+    // sums prev_n partials (written by the previous launch) into *prev_out
+    const double *prev_partials;
+    int prev_n;
+    double *prev_out;
 };
+
+// Fixed-order sum of n partials by one workgroup: strided per-thread sums
+// issued eight loads at a time, a fixed wave tree, the waves in order.
+template <int WG>
+__device__ inline void wg_sum_partials(const double *__restrict__ p, int n, double *out, double *red)
+{
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    for (int base = 0; base < n; base += 8 * WG) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int i = base + j * WG + tid;
+            v[j] = i < n ? p[i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            s += v[j];
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        s += __shfl_xor(s, o, 64);
+    if ((tid & 63) == 0)
+        red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < WG / 64; ++w)
+            t += red[w];
+        *out = t;
+    }
+    __syncthreads();
+}
 
 // Workgroup i is dispatched to XCD i % 8; give each XCD a contiguous range
 // of tiles so that the tiles sharing halo lines meet in the same L2.
@@ -131,6 +168,8 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
     __shared__ double red[WG / 64];
 
     const int tid = threadIdx.x;
+    if (a.prev_partials && blockIdx.x == 0)
+        wg_sum_partials<WG>(a.prev_partials, a.prev_n, a.prev_out, red);
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int comp = tile / a.tiles_per_comp;
     const int tt = tile - comp * a.tiles_per_comp;
@@ -329,6 +368,8 @@ __global__ __launch_bounds__(256) void k_stencil7_generic(const S7Args a)
 {
     __shared__ double red[4];
     const int tid = threadIdx.x;
+    if (a.prev_partials && blockIdx.x == 0 && blockIdx.y == 0)
+        wg_sum_partials<256>(a.prev_partials, a.prev_n, a.prev_out, red);
     const long long nb_tot = (long long)a.G.nb[0] * a.G.nb[1] * a.G.nb[2];
     const long long slot = blockIdx.y; // comp * nblocks + blk
     const int comp = (int)(slot / nb_tot);
@@ -474,7 +515,8 @@ static bool dispatch_spec(const S7Args &a, int edge, int wg, int ntiles, hipStre
 }
 
 template <typename T, int MODE, bool SUMSQ>
-static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic, const S7Tune *over)
+static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic, const S7Tune *over, bool finalize,
+                 int *nparts)
 {
     const int *n = a.G.n;
     const long long nbt = (long long)a.G.nb[0] * a.G.nb[1] * a.G.nb[2];
@@ -488,7 +530,9 @@ static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic, const 
         if (ntiles < (1LL << 31)) {
             if (dispatch_spec<T, MODE, SUMSQ>(a, n[0], tu.wg, (int)ntiles, st)) {
                 CNV_HIP(hipGetLastError());
-                if (SUMSQ) {
+                if (nparts)
+                    *nparts = (int)ntiles;
+                if (SUMSQ && finalize) {
                     hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, a.partials, (int)ntiles, a.sumsq);
                     CNV_HIP(hipGetLastError());
                 }
@@ -506,7 +550,9 @@ static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic, const 
     hipLaunchKernelGGL((k_stencil7_generic<T, MODE, SUMSQ>), dim3((unsigned)gx, (unsigned)gy), dim3(256),
                        0, st, a);
     CNV_HIP(hipGetLastError());
-    if (SUMSQ) {
+    if (nparts)
+        *nparts = (int)(gx * gy);
+    if (SUMSQ && finalize) {
         hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, a.partials, (int)(gx * gy), a.sumsq);
         CNV_HIP(hipGetLastError());
     }
@@ -547,9 +593,19 @@ static int check_faces(cnv_grid_s *g, int comp)
     return CNV_OK;
 }
 
+// State of one launch inside a multi-step run (cnv_diffusion_run).
+struct S7Run {
+    double *partials;     // this launch's partial buffer
+    const double *prev;   // the previous launch's partials, summed by workgroup 0
+    int prev_n;
+    double *prev_out;
+    int *nparts;          // out: partials written by this launch
+    size_t cap;           // capacity of each half
+};
+
 static int launch_stencil7_t(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
                              int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st,
-                             const S7Tune *over)
+                             const S7Tune *over, const S7Run *run = nullptr)
 {
     if (ncomp > kMaxComp)
         return fail(CNV_ERR_UNSUPPORTED, "more than 8 components in one stencil call");
@@ -589,40 +645,81 @@ static int launch_stencil7_t(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, in
     a.nudt = nu_dt;
     a.sumsq = sumsq;
     a.partials = nullptr;
+    a.prev_partials = run ? run->prev : nullptr;
+    a.prev_n = run ? run->prev_n : 0;
+    a.prev_out = run ? run->prev_out : nullptr;
     a.zchunk = a.nzc = a.tiles_per_comp = 0;
     if (sumsq) {
         const long long nbt = u->nblocks_total;
         const long long cells = (long long)u->desc.block_cells[0] * u->desc.block_cells[1] * u->desc.block_cells[2];
         // upper bound on workgroups: generic kernel uses ceil(cells/256) per block
         const long long need = nbt * ncomp * ((cells + 255) / 256 + u->desc.block_cells[2]);
-        int rc = ensure_partials(u, (size_t)need);
-        if (rc != CNV_OK)
-            return rc;
-        a.partials = u->partials;
+        if (run) { // fixed buffer halves: never reallocated inside a run
+            if (need > (long long)run->cap)
+                return fail(CNV_ERR_SIZE, "partial buffer too small for this run");
+            a.partials = run->partials;
+        } else {
+            int rc = ensure_partials(u, (size_t)need);
+            if (rc != CNV_OK)
+                return rc;
+            a.partials = u->partials;
+        }
     }
     const bool gen = over ? false : force_generic();
+    const bool fin = run == nullptr;
+    int *np = run ? run->nparts : nullptr;
     const bool dbl = u->desc.dtype == CNV_F64;
     if (mode == 0)
-        return dbl ? run_t<double, 0, false>(a, ncomp, st, gen, over)
-                   : run_t<float, 0, false>(a, ncomp, st, gen, over);
+        return dbl ? run_t<double, 0, false>(a, ncomp, st, gen, over, fin, np)
+                   : run_t<float, 0, false>(a, ncomp, st, gen, over, fin, np);
     if (mode == 1) {
         if (sumsq)
-            return dbl ? run_t<double, 1, true>(a, ncomp, st, gen, over)
-                       : run_t<float, 1, true>(a, ncomp, st, gen, over);
-        return dbl ? run_t<double, 1, false>(a, ncomp, st, gen, over)
-                   : run_t<float, 1, false>(a, ncomp, st, gen, over);
+            return dbl ? run_t<double, 1, true>(a, ncomp, st, gen, over, fin, np)
+                       : run_t<float, 1, true>(a, ncomp, st, gen, over, fin, np);
+        return dbl ? run_t<double, 1, false>(a, ncomp, st, gen, over, fin, np)
+                   : run_t<float, 1, false>(a, ncomp, st, gen, over, fin, np);
     }
     if (sumsq)
-        return dbl ? run_t<double, 2, true>(a, ncomp, st, gen, over)
-                   : run_t<float, 2, true>(a, ncomp, st, gen, over);
-    return dbl ? run_t<double, 2, false>(a, ncomp, st, gen, over)
-               : run_t<float, 2, false>(a, ncomp, st, gen, over);
+        return dbl ? run_t<double, 2, true>(a, ncomp, st, gen, over, fin, np)
+                   : run_t<float, 2, true>(a, ncomp, st, gen, over, fin, np);
+    return dbl ? run_t<double, 2, false>(a, ncomp, st, gen, over, fin, np)
+               : run_t<float, 2, false>(a, ncomp, st, gen, over, fin, np);
 }
 
 int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
                     int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st)
 {
     return launch_stencil7_t(u, out, lap, comp_in, comp_out, ncomp, mode, nu_dt, sumsq, st, nullptr);
+}
+
+// nsteps Euler steps ping-ponging u -> tmp -> u ...; the norm of step k is
+// reduced by workgroup 0 of step k+1 from the other half of u's partial
+// buffer, the last one by k_finalize_sum.
+int launch_diffusion_run(cnv_grid_s *u, cnv_grid_s *tmp, int nsteps, double nu_dt, double *norms, hipStream_t st)
+{
+    double *P[2] = {u->partials, u->partials + u->partials_cap};
+    int prev_n = 0;
+    for (int k = 0; k < nsteps; ++k) {
+        cnv_grid_s *src = (k % 2 == 0) ? u : tmp;
+        cnv_grid_s *dst = (k % 2 == 0) ? tmp : u;
+        S7Run run;
+        run.partials = P[k % 2];
+        run.prev = (norms && k > 0) ? P[(k + 1) % 2] : nullptr;
+        run.prev_n = prev_n;
+        run.prev_out = (norms && k > 0) ? norms + (k - 1) : nullptr;
+        run.nparts = &prev_n;
+        run.cap = u->partials_cap;
+        const int rc = launch_stencil7_t(src, dst, nullptr, 0, 0, u->desc.ncomp, 1, nu_dt, norms ? norms + k : nullptr,
+                                         st, nullptr, &run);
+        if (rc != CNV_OK)
+            return rc;
+    }
+    if (norms && nsteps > 0) {
+        hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, (const double *)P[(nsteps - 1) % 2], prev_n,
+                           norms + (nsteps - 1));
+        CNV_HIP(hipGetLastError());
+    }
+    return CNV_OK;
 }
 
 // ---------------------------------------------------------------- autotuner

@@ -169,6 +169,15 @@ int cnv_laplacian(cnv_grid in, int comp_in, cnv_grid out, int comp_out, cnv_stre
 int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_out, double *sumsq,
                        cnv_stream s);
 
+/* nsteps forward-Euler steps alternating u -> tmp -> u -> ... (the same
+ * arithmetic as nsteps calls of cnv_diffusion_step; the field ends in u when
+ * nsteps is even, in tmp when odd).  tmp must carry the same BC lists as u.
+ * If norms is non-NULL (device array of nsteps doubles) norms[k] is the FP64
+ * sum of squares after step k; the reduction of step k runs inside the
+ * launch of step k+1 (one finalize launch after the last step).  The loop of
+ * the reference's time stepping over Field.h:457-512 / FieldOperator.h. */
+int cnv_diffusion_run(cnv_grid u, cnv_grid tmp, int nsteps, double nu_dt, double *norms, cnv_stream s);
+
 /* Fifth-order WENO face states (Jiang & Shu, J. Comput. Phys. 126:202-228,
  * 1996).  CubismNova advertises a WENO toolbox (README.rst:28-31) but ships
  * none; the lab read is the FieldLab of Stencil(-3,4,false) with the blocks'

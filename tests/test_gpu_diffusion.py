@@ -199,3 +199,75 @@ def test_sums_deterministic(gpu):
     assert a == b
     assert abs(a[0] - slab.sum()) <= 1e-12 * np.abs(slab).sum()
     assert abs(a[1] - (slab * slab).sum()) <= 1e-13 * (slab * slab).sum()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nsteps", [1, 2, 7])
+def test_run_matches_steps(gpu, dtype, nsteps):
+    """cnv_diffusion_run: the same fields as nsteps single steps (bit-exact) and
+    norms (reduced inside the next step's launch) against the oracle to 1e-13,
+    on a specialised shape with mixed BCs and on a generic shape."""
+    for nb, n in [((2, 2, 2), (16, 16, 16)), ((3, 2, 1), (12, 8, 10))]:
+        bs = cnv.Cartesian(nb, n, dtype=dtype).block_stride
+        slab = random_slab(90 + nsteps, nb, n, dtype, block_stride=bs)
+        bcl = face_bc_lists(nb, [(2, 0, 0, 0.5), (1, 0, 1, 0.0), (3, 2, 1, -1.0)])
+        u = setup(nb, n, dtype, slab, bcl)
+        tmp = setup(nb, n, dtype, slab, bcl)
+        h = [1.0 / (nb[d] * n[d]) for d in range(3)]
+        dt = 0.1 * min(h) ** 2
+        norms = cnv.DeviceBuffer(8 * nsteps)
+        norms.fill(np.float64, np.nan)
+        res = cnv.diffusion_run(u, tmp, nsteps, dt, norms=norms)
+        assert res is (u if nsteps % 2 == 0 else tmp)
+        got = res.download(0)
+        gn = norms.download(np.float64, nsteps)
+        cur = slab
+        for k in range(nsteps):
+            cur, s = oracle.diffusion_step(oracle.Field(cur, nb, n, bs, bcs=bcl), h, dt)
+            assert abs(gn[k] - s) <= 1e-13 * s, (n, k, gn[k], s)
+        assert_same(cells_of(got, nb, n, bs), cells_of(cur, nb, n, bs), "run %s" % (n,))
+        # without norms
+        u.upload(0, slab)
+        res = cnv.diffusion_run(u, tmp, nsteps, dt)
+        assert np.array_equal(cells_of(res.download(0), nb, n, bs), cells_of(got, nb, n, bs))
+
+
+def test_run_full_size_norm_order(gpu):
+    """256^3 FP64 run of 100 steps against 100 single steps: identical fields,
+    norms equal to 1e-14 (the two paths sum the same partials in different
+    fixed trees) and the same on a repeat (deterministic)."""
+    nb, n, N = (8, 8, 8), (32, 32, 32), 256
+    u0 = workloads.diffusion_large(N)
+    h = 1.0 / N
+    dt = 0.1 * h * h
+    a, b = cnv.Cartesian(nb, n), cnv.Cartesian(nb, n)
+    a.upload_global(0, u0)
+    ns = cnv.DeviceBuffer(8 * 100)
+    src, dst = a, b
+    for k in range(100):
+        cnv.diffusion_step(src, dst, dt, sumsq=ns.at(8 * k))
+        src, dst = dst, src
+    want = src.download(0)
+    wn = ns.download(np.float64, 100)
+    runs = []
+    for rep in range(2):
+        a.upload_global(0, u0)
+        nr = cnv.DeviceBuffer(8 * 100)
+        res = cnv.diffusion_run(a, b, 100, dt, norms=nr)
+        assert np.array_equal(res.download(0), want)
+        runs.append(nr.download(np.float64, 100))
+    assert np.array_equal(runs[0], runs[1])
+    assert np.max(np.abs(runs[0] - wn) / wn) <= 1e-14
+
+
+def test_run_argument_errors(gpu):
+    u, t = cnv.Cartesian(2, 8), cnv.Cartesian(2, 8)
+    u.setBC((0, 0, 0), [cnv.Absorbing(0, 0)])
+    with pytest.raises(cnv.CnvError):
+        cnv.diffusion_run(u, t, 2, 1e-3)            # BC lists differ
+    cnv.diffusion_run(u, t, 1, 1e-3)                # one step reads only u
+    with pytest.raises(cnv.CnvError):
+        cnv.diffusion_run(u, u, 2, 1e-3)
+    with pytest.raises(cnv.CnvError):
+        cnv.diffusion_run(u, t, -1, 1e-3)
+    cnv.diffusion_run(t, cnv.Cartesian(2, 8), 0, 1e-3)
