@@ -62,16 +62,16 @@ def roofline(bytes_per_launch, avg_s, traffic=None):
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
 
 
-def measured_traffic(name):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+def measured_traffic(name, key):
+    """HBM bytes per launch of one kernel from a committed rocprofv3 --pmc
+    summary: the line `hbm_bytes_per_launch[key] = N` of profiles/<name>
+    (FETCH_SIZE and WRITE_SIZE scaled by their copy-kernel calibration)."""
+    import re
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
-    try:
-        with open(p) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
-    except Exception:
-        return None
+    m = re.search(r"^hbm_bytes_per_launch\[%s\] = (\d+)" % re.escape(key), open(p).read(), re.M)
+    return int(m.group(1)) if m else None
 
 
 # ------------------------------------------------------------------ diffusion
@@ -172,11 +172,11 @@ def bench_diffusion(args):
                    "blocks": [8, 8, 8], "block_cells": [32, 32, 32], "steps_per_graph": args.steps,
                    "tile_f32": r32["tile"], "tile_f64": r64["tile"]},
         "roofline": roofline(r32["bytes_per_launch"], r32["avg_kernel_s"],
-                             measured_traffic("stencil7_diffusion_f32_256.json")),
+                             measured_traffic("stencil7_diffusion_256.md", "f32")),
         "fp64": {"value": round(r64["gcups"], 3), "unit": "GCell-updates/s",
                  "ms_per_step": round(r64["wall_s"] * 1e3 / args.steps, 5),
                  "roofline": roofline(r64["bytes_per_launch"], r64["avg_kernel_s"],
-                                      measured_traffic("stencil7_diffusion_f64_256.json"))},
+                                      measured_traffic("stencil7_diffusion_256.md", "f64"))},
         "fp32_over_fp64": round(r32["gcups"] / r64["gcups"], 3),
         "fp32_rel_l2_vs_fp64": rel,
         "fp32_norm_history_max_rel_dev": nrel,
@@ -283,11 +283,11 @@ def bench_weno5(args):
                                "ghost 3, x periodic, y/z absorbing", "blocks": [16, 16, 16],
                    "block_cells": [32, 32, 32], "components": 5, "tile_f32": r32["tile"], "tile_f64": r64["tile"]},
         "roofline": roofline(r32["bytes_per_launch"], r32["avg_kernel_s"],
-                             measured_traffic("weno5_f32_512.json")),
+                             measured_traffic("weno5_512.md", "f32")),
         "fp64": {"value": round(r64["gcups"], 3), "unit": "GCell-updates/s",
                  "ms_per_step": round(r64["wall_s"] * 1e3 / args.steps, 4),
                  "roofline": roofline(r64["bytes_per_launch"], r64["avg_kernel_s"],
-                                      measured_traffic("weno5_f64_512.json"))},
+                                      measured_traffic("weno5_512.md", "f64"))},
         "fp32_over_fp64": round(r32["gcups"] / r64["gcups"], 3),
         "fp32_rel_l2_vs_fp64_worst_state": worst,
     }
@@ -388,9 +388,9 @@ def bench_jacobi(args):
                                "+ FP64 correction + FP64 defect", "sweeps_per_cycle": S,
                    "value_is": "FP32 sweep kernel GCell-updates/s (cells per sweep / sweep time)"},
         "sweeps_per_s": round(sw / wall, 1), "cycle_gcups": round(cells * sw / wall / 1e9, 3),
-        "roofline": roofline(cells * 12, sweep_s, measured_traffic("jacobi_sweep_f32_512.json")),
+        "roofline": roofline(cells * 12, sweep_s, measured_traffic("jacobi_sweep_512.md", "f32")),
         "fp64": {"value": round(cells / sweep64_s / 1e9, 3), "unit": "GCell-updates/s",
-                 "roofline": roofline(cells * 24, sweep64_s, None)},
+                 "roofline": roofline(cells * 24, sweep64_s, measured_traffic("jacobi_sweep_512.md", "f64"))},
         "fp32_over_fp64": round(sweep64_s / sweep_s, 3),
         "convergence_32cube": {"refined_sweeps": int(s1), "refined_rel_residual": r1, "fp64_sweeps": int(s2),
                                "fp64_rel_residual": r2,
@@ -480,7 +480,7 @@ def bench_batch(args):
                                "WENO5, one persistent launch per step, HIP-graph replayed",
                    "grids": len(specs), "cells": cells_total, "work_items": info["items"], "wg": info["wg"],
                    "lds_bytes": info["lds_bytes"]},
-        "roofline": roofline(bytes_total, kern_s, measured_traffic("batch_96.json")),
+        "roofline": roofline(bytes_total, kern_s, measured_traffic("batch_96.md", "batch")),
     }
     if not args.no_cpu:
         nt = max(1, min(16, os.cpu_count() or 1))
