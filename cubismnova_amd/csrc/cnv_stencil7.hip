@@ -151,6 +151,10 @@ __device__ inline void block_faces(const S7Args &a, int comp, long long blk, con
     }
 }
 
+#ifndef S7_PF
+#define S7_PF 2 // planes loaded ahead of the plane being computed
+#endif
+
 template <typename T, int NX, int NY, int WG, int MODE, bool SUMSQ>
 __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
 {
@@ -255,60 +259,71 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
     const T nudt = (T)a.nudt;
     double acc = 0.0;
 
-    Plane pm, pc, pn, pnn;
-    load_plane(z0 - 1, false, pm);
-    load_plane(z0, true, pc);
-    load_plane(z0 + 1, z0 + 1 < z1, pn);
+    // Ring of NQ planes in registers: plane p lives in slot (p - z0 + 1) % NQ.
+    // The z loop is unrolled by NQ so every slot index is a constant (no
+    // register moves); plane z + S7_PF is loaded while plane z is computed.
+    constexpr int NQ = S7_PF + 2;
+    Plane Q[NQ];
+#pragma unroll
+    for (int k = 0; k <= S7_PF; ++k)
+        if (z0 - 1 + k <= z1)
+            load_plane(z0 - 1 + k, k >= 1 && z0 - 1 + k < z1, Q[k]);
 
-    for (int z = z0; z < z1; ++z) {
-        if (z + 2 <= z1)
-            load_plane(z + 2, z + 2 < z1, pnn);
-        T *L = lds[(z - z0) & 1];
+    for (int zb = z0; zb < z1; zb += NQ) {
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            *reinterpret_cast<V *>(L + (ty + r * ROWS + 1) * P + VEC + x0) = pc.c[r];
-        if (hx_on)
-            L[(hx_y + 1) * P + (hx_side ? VEC + NX : VEC - 1)] = pc.hx;
-        if (hy_on)
-            *reinterpret_cast<V *>(L + (hy_side ? NY + 1 : 0) * P + VEC + hy_x) = pc.hy;
-        __syncthreads();
+        for (int u = 0; u < NQ; ++u) {
+            const int z = zb + u;
+            if (z >= z1)
+                break;
+            const Plane &pm = Q[u % NQ];
+            const Plane &pc = Q[(u + 1) % NQ];
+            const Plane &pn = Q[(u + 2) % NQ];
+            if (z + S7_PF <= z1)
+                load_plane(z + S7_PF, z + S7_PF < z1, Q[(u + NQ - 1) % NQ]);
+            T *L = lds[(z - z0) & 1];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int y = ty + r * ROWS;
-            const T *row = L + (y + 1) * P + VEC;
-            const V ym = *reinterpret_cast<const V *>(row - P + x0);
-            const V yp = *reinterpret_cast<const V *>(row + P + x0);
-            const T xl = row[x0 - 1];
-            const T xr = row[x0 + VEC];
-            V res, lapv;
+            for (int r = 0; r < R; ++r)
+                *reinterpret_cast<V *>(L + (ty + r * ROWS + 1) * P + VEC + x0) = pc.c[r];
+            if (hx_on)
+                L[(hx_y + 1) * P + (hx_side ? VEC + NX : VEC - 1)] = pc.hx;
+            if (hy_on)
+                *reinterpret_cast<V *>(L + (hy_side ? NY + 1 : 0) * P + VEC + hy_x) = pc.hy;
+            __syncthreads();
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) {
-                const T c = pc.c[r].v[v];
-                const T xm = v == 0 ? xl : pc.c[r].v[v - 1];
-                const T xp = v == VEC - 1 ? xr : pc.c[r].v[v + 1];
-                const T ddx = ihx2 * (xp - 2 * c + xm);
-                const T ddy = ihy2 * (yp.v[v] - 2 * c + ym.v[v]);
-                const T ddz = ihz2 * (pn.c[r].v[v] - 2 * c + pm.c[r].v[v]);
-                const T lap = ddx + ddy + ddz;
-                lapv.v[v] = lap;
-                if (MODE == 0) {
-                    res.v[v] = lap;
-                } else {
-                    const T l = lap * nudt;
-                    const T un = c + l;
-                    res.v[v] = un;
-                    if (SUMSQ)
-                        acc += (double)un * (double)un;
+            for (int r = 0; r < R; ++r) {
+                const int y = ty + r * ROWS;
+                const T *row = L + (y + 1) * P + VEC;
+                const V ym = *reinterpret_cast<const V *>(row - P + x0);
+                const V yp = *reinterpret_cast<const V *>(row + P + x0);
+                const T xl = row[x0 - 1];
+                const T xr = row[x0 + VEC];
+                V res, lapv;
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    const T c = pc.c[r].v[v];
+                    const T xm = v == 0 ? xl : pc.c[r].v[v - 1];
+                    const T xp = v == VEC - 1 ? xr : pc.c[r].v[v + 1];
+                    const T ddx = ihx2 * (xp - 2 * c + xm);
+                    const T ddy = ihy2 * (yp.v[v] - 2 * c + ym.v[v]);
+                    const T ddz = ihz2 * (pn.c[r].v[v] - 2 * c + pm.c[r].v[v]);
+                    const T lap = ddx + ddy + ddz;
+                    lapv.v[v] = lap;
+                    if (MODE == 0) {
+                        res.v[v] = lap;
+                    } else {
+                        const T l = lap * nudt;
+                        const T un = c + l;
+                        res.v[v] = un;
+                        if (SUMSQ) // one rounding of un^2 + acc
+                            acc = __builtin_fma((double)un, (double)un, acc);
+                    }
                 }
+                const long long off = x0 + NX * y + SXY * z;
+                *reinterpret_cast<V *>(ob + off) = res;
+                if (MODE == 2)
+                    *reinterpret_cast<V *>(lb + off) = lapv;
             }
-            const long long off = x0 + NX * y + SXY * z;
-            *reinterpret_cast<V *>(ob + off) = res;
-            if (MODE == 2)
-                *reinterpret_cast<V *>(lb + off) = lapv;
         }
-        pm = pc;
-        pc = pn;
-        pn = pnn;
     }
 
     if (SUMSQ) {
