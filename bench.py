@@ -310,7 +310,6 @@ def jacobi_grids(cnv, N, nb, n, dtype, zf, ncomp=1):
 
 def bench_jacobi(args):
     import cubismnova_amd as cnv
-    import oracle
     from cubismnova_amd import stretch, workloads
     ev = HipEvents()
     N, nb, n = 512, (8, 8, 8), (64, 64, 64)
@@ -399,6 +398,7 @@ def bench_jacobi(args):
         "residual_after_timed_cycles": rel,
     }
     if not args.no_cpu:
+        import oracle   # the CPU baseline leg only
         nt = max(1, min(16, os.cpu_count() or 1))
         lists = {}
         for b in range(512):
@@ -427,7 +427,6 @@ def bench_jacobi(args):
 
 def bench_batch(args):
     import cubismnova_amd as cnv
-    import oracle
     from cubismnova_amd import workloads
     ev = HipEvents()
     specs = workloads.batch_specs(96, k=4)
@@ -483,6 +482,7 @@ def bench_batch(args):
         "roofline": roofline(bytes_total, kern_s, measured_traffic("batch_96.md", "batch")),
     }
     if not args.no_cpu:
+        import oracle   # the CPU baseline leg only
         nt = max(1, min(16, os.cpu_count() or 1))
         done_cells, t0 = 0, time.perf_counter()
         host = []   # the oracle's inputs, staged before the clock starts

@@ -11,6 +11,31 @@ struct alignas(16) VecT {
     T v[N];
 };
 
+// Plain vector load / store of exactly N elements (N * sizeof(T) bytes,
+// naturally aligned), whatever the padding of VecT.
+template <typename T, int N>
+__device__ inline VecT<T, N> load_vec(const T *p)
+{
+    typedef T vt __attribute__((ext_vector_type(N)));
+    const vt x = *reinterpret_cast<const vt *>(p);
+    VecT<T, N> r;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        r.v[i] = x[i];
+    return r;
+}
+
+template <typename T, int N>
+__device__ inline void store_vec(T *p, const VecT<T, N> &v)
+{
+    typedef T vt __attribute__((ext_vector_type(N)));
+    vt x;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        x[i] = v.v[i];
+    *reinterpret_cast<vt *>(p) = x;
+}
+
 // Streaming (non-temporal) 16-byte store of a VecT.
 template <typename T, int N>
 __device__ inline void store_nt(T *p, const VecT<T, N> &v)
@@ -21,6 +46,63 @@ __device__ inline void store_nt(T *p, const VecT<T, N> &v)
     for (int i = 0; i < N; ++i)
         x[i] = v.v[i];
     __builtin_nontemporal_store(x, reinterpret_cast<vt *>(p));
+}
+
+// Per-face source description for one block (uniform in a workgroup).
+template <typename T>
+struct FaceSrc {
+    const T *p;  // base such that ghost(i) = p[i] (copy) or sign*p[i]
+    int mode;    // 0 copy, 1 scaled, 2 constant
+    T s;         // sign or constant
+};
+
+template <typename T>
+__device__ inline T face_val(const FaceSrc<T> &f, long long i)
+{
+    if (f.mode == 2)
+        return f.s;
+    const T v = f.p[i];
+    return f.mode == 1 ? f.s * v : v;
+}
+
+// Resolve the six one-layer faces of block `blk` (ghost layer 1 of the lab,
+// the only one the 7-point operators read; the same for any ghost width).
+template <typename T>
+__device__ inline void block_faces(const Geom &G, const DevBCHead *heads, const DevBC *entries, int has_bc,
+                                   long long blk, const T *cbase, FaceSrc<T> F[6])
+{
+    const int *nb = G.nb, *n = G.n;
+    const int bidx[3] = {(int)(blk % nb[0]), (int)((blk / nb[0]) % nb[1]),
+                         (int)(blk / ((long long)nb[0] * nb[1]))};
+    int kind[6];
+    double val[6];
+    const int sb[3] = {-1, -1, -1}, se[3] = {2, 2, 2};
+    if (has_bc) {
+        const DevBCHead h = heads[blk];
+        resolve_faces(entries + h.offset, h.count, sb, se, kind, val);
+    } else {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            kind[f] = FACE_NBR;
+            val[f] = 0.0;
+        }
+    }
+    const long long nx = n[0], nxy = (long long)n[0] * n[1];
+    const T *self = cbase + blk * G.bstride;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1, side = f & 1;
+        int b[3] = {bidx[0], bidx[1], bidx[2]};
+        b[dir] = (b[dir] + (side ? 1 : -1) + nb[dir]) % nb[dir];
+        const long long nbr = b[0] + (long long)nb[0] * (b[1] + (long long)nb[1] * b[2]);
+        const T *src = kind[f] == FACE_NBR ? cbase + nbr * G.bstride : self;
+        // index of the source line/plane along dir
+        const int li = kind[f] == FACE_NBR ? (side ? 0 : n[dir] - 1) : (side ? n[dir] - 1 : 0);
+        const long long step = dir == 0 ? 1 : (dir == 1 ? nx : nxy);
+        F[f].p = src + li * step;
+        F[f].mode = kind[f] == FACE_DIRICHLET ? 2 : (kind[f] == FACE_SYMMETRY ? 1 : 0);
+        F[f].s = (T)val[f];
+    }
 }
 
 // Workgroup i is dispatched to XCD i % 8; give each XCD a contiguous range

@@ -8,7 +8,8 @@
 //         + czp_k (u[+z] - u) - czm_k (u - u[-z]),
 // summed left to right, with ghosts from the lab of Stencil(-1,2) and the
 // blocks' BC lists (Dirichlet: constant ghost, BC/Dirichlet.h:75-113), and
-//   sweep:  u' = u + omega ((f - A u) / diag),  diag = -(((2 ihx2 + 2 ihy2) + czm_k) + czp_k)
+//   sweep:  u' = u + (f - A u) * wd_k,  wd_k = omega / diag_k,
+//           diag_k = -(((2 ihx2 + 2 ihy2) + czm_k) + czp_k)
 //   defect: r  = f - A u  (+ FP64 sum of r^2 before any rounding of r).
 // The z-march, LDS plane and register queue are those of the 7-point kernel.
 #include "cnv_device.h"
@@ -82,11 +83,13 @@ __global__ __launch_bounds__(WG) void k_jacobi(const JArgs a)
     const FaceL<T> FY = hy_side ? F[3] : F[2];
     const FaceL<T> FZ0 = F[4], FZ1 = F[5];
 
+    using VF = VecT<TF, VEC>;
     struct Plane {
         V c[R];
         T hx;
         V hy;
     };
+    // halo: plane p is computed by this workgroup (x/y ghosts needed)
     auto load_plane = [&](int p, bool halo, Plane &Q) {
         if (p >= 0 && p < NZ) {
 #pragma unroll
@@ -138,14 +141,10 @@ __global__ __launch_bounds__(WG) void k_jacobi(const JArgs a)
     for (int z = z0; z < z1; ++z) {
         if (z + 2 <= z1)
             load_plane(z + 2, z + 2 < z1, pnn);
-        // rhs of this plane (no halo)
-        using VF = VecT<TF, VEC>;
         VF fr[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
-#pragma unroll
-            for (int v = 0; v < VEC; ++v)
-                fr[r].v[v] = fb[x0 + v + NX * (ty + r * ROWS) + SXY * z];
+            fr[r] = load_vec<TF, VEC>(fb + x0 + NX * (ty + r * ROWS) + SXY * z);
         T *L = lds[(z - z0) & 1];
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -158,6 +157,7 @@ __global__ __launch_bounds__(WG) void k_jacobi(const JArgs a)
         const int kz = bz * NZ + z;
         const T cm = (T)a.czm[kz], cp = (T)a.czp[kz];
         const T diag = -(2 * ihx2 + 2 * ihy2 + cm + cp);
+        const T wd = w / diag;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int y = ty + r * ROWS;
@@ -178,16 +178,13 @@ __global__ __launch_bounds__(WG) void k_jacobi(const JArgs a)
                 const T au = ax + ay + az;
                 const T rr = (T)fr[r].v[v] - au;
                 if (MODE == 0) {
-                    res.v[v] = (TO)(c + w * (rr / diag));
+                    res.v[v] = (TO)(c + rr * wd);
                 } else {
                     res.v[v] = (TO)rr;
-                    acc += (double)rr * (double)rr;
+                    acc = __builtin_fma((double)rr, (double)rr, acc);
                 }
             }
-            TO *po = ob + x0 + NX * y + SXY * z;
-#pragma unroll
-            for (int v = 0; v < VEC; ++v)
-                po[v] = res.v[v];
+            store_vec<TO, VEC>(ob + x0 + NX * y + SXY * z, res);
         }
         pm = pc;
         pc = pn;

@@ -14,7 +14,7 @@
 // u + that (Block/Field.h:457-512, FieldOperator.h:43-114).  The library is
 // compiled with -ffp-contract=off so no step is fused into an FMA and FP64
 // results are bit-identical to the CPU evaluation of the same order.
-#include "cnv_internal.h"
+#include "cnv_device.h"
 
 #include <cstdlib>
 #include <map>
@@ -22,11 +22,6 @@
 
 namespace cnv
 {
-
-template <typename T, int N>
-struct alignas(16) VecT {
-    T v[N];
-};
 
 constexpr int kMaxComp = 8;
 
@@ -86,113 +81,38 @@ __device__ inline void wg_sum_partials(const double *__restrict__ p, int n, doub
     __syncthreads();
 }
 
-// Workgroup i is dispatched to XCD i % 8; give each XCD a contiguous range
-// of tiles so that the tiles sharing halo lines meet in the same L2.
-__device__ inline int xcd_remap(int i, int n)
-{
-    const int x = i & 7, local = i >> 3;
-    const int q = n >> 3, r = n & 7;
-    return x * q + (x < r ? x : r) + local;
-}
-
-// Per-face source description for one block (uniform in a workgroup).
-template <typename T>
-struct FaceSrc {
-    const T *p;  // base such that ghost(i) = p[i] (copy) or sign*p[i]
-    int mode;    // 0 copy, 1 scaled, 2 constant
-    T s;         // sign or constant
-};
-
-template <typename T>
-__device__ inline T face_val(const FaceSrc<T> &f, long long i)
-{
-    if (f.mode == 2)
-        return f.s;
-    const T v = f.p[i];
-    return f.mode == 1 ? f.s * v : v;
-}
-
-// Resolve the six faces of block `blk` of component `comp` for Stencil(-1,2).
-template <typename T>
-__device__ inline void block_faces(const S7Args &a, int comp, long long blk, const T *cbase,
-                                   FaceSrc<T> F[6])
-{
-    const int *nb = a.G.nb, *n = a.G.n;
-    const int bidx[3] = {(int)(blk % nb[0]), (int)((blk / nb[0]) % nb[1]),
-                         (int)(blk / ((long long)nb[0] * nb[1]))};
-    int kind[6];
-    double val[6];
-    const int sb[3] = {-1, -1, -1}, se[3] = {2, 2, 2};
-    if (a.has_bc[comp]) {
-        const DevBCHead h = a.heads[comp][blk];
-        resolve_faces(a.entries[comp] + h.offset, h.count, sb, se, kind, val);
-    } else {
-#pragma unroll
-        for (int f = 0; f < 6; ++f) {
-            kind[f] = FACE_NBR;
-            val[f] = 0.0;
-        }
-    }
-    const long long nx = n[0], nxy = (long long)n[0] * n[1];
-    const T *self = cbase + blk * a.G.bstride;
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        const int dir = f >> 1, side = f & 1;
-        int b[3] = {bidx[0], bidx[1], bidx[2]};
-        b[dir] = (b[dir] + (side ? 1 : -1) + nb[dir]) % nb[dir];
-        const long long nbr = b[0] + (long long)nb[0] * (b[1] + (long long)nb[1] * b[2]);
-        const T *src = kind[f] == FACE_NBR ? cbase + nbr * a.G.bstride : self;
-        // index of the source line/plane along dir
-        const int li = kind[f] == FACE_NBR ? (side ? 0 : n[dir] - 1) : (side ? n[dir] - 1 : 0);
-        const long long step = dir == 0 ? 1 : (dir == 1 ? nx : nxy);
-        F[f].p = src + li * step;
-        F[f].mode = kind[f] == FACE_DIRICHLET ? 2 : (kind[f] == FACE_SYMMETRY ? 1 : 0);
-        F[f].s = (T)val[f];
-    }
-}
-
 #ifndef S7_PF
 #define S7_PF 2 // planes loaded ahead of the plane being computed
 #endif
 
+// Tile geometry of the z-marching 7-point kernels for an NX x NY plane and
+// WG threads: VEC-wide rows, R rows per thread, LDS planes of pitch P.
+template <typename T, int NX, int NY, int WG>
+struct S7Shape {
+    static constexpr int VEC = 16 / (int)sizeof(T);
+    static constexpr int TX = NX / VEC;  // threads per row
+    static constexpr int ROWS = WG / TX; // rows per pass
+    static constexpr int R = NY / ROWS;  // rows per thread
+    static constexpr int P = NX + 2 * VEC; // LDS row pitch (interior starts at VEC)
+    static constexpr int PLANE = (NY + 2) * P;
+    static constexpr bool valid = NX % VEC == 0 && WG % TX == 0 && ROWS <= NY && NY % ROWS == 0 &&
+                                  WG >= 2 * NY && WG >= 2 * TX;
+};
+
+// One tile of the fused 7-point kernel: planes [z0, z1) of the block at `ub`
+// (NZ planes), one-layer faces F, output at `ob` (and the Laplacian at `lb`
+// for MODE 2).  `lds` holds two planes (2 * PLANE elements).  Returns this
+// thread's FP64 sum of u_new^2 (SUMSQ).
 template <typename T, int NX, int NY, int WG, int MODE, bool SUMSQ>
-__global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
+__device__ inline double s7_march(const T *__restrict__ ub, T *__restrict__ ob, T *__restrict__ lb,
+                                  const FaceSrc<T> F[6], int NZ, int z0, int z1, T ihx2, T ihy2, T ihz2, T nudt,
+                                  T *lds)
 {
-    constexpr int VEC = 16 / (int)sizeof(T);
-    constexpr int TX = NX / VEC;      // threads per row
-    constexpr int ROWS = WG / TX;     // rows per pass
-    constexpr int R = NY / ROWS;      // rows per thread
-    constexpr int P = NX + 2 * VEC;   // LDS row pitch (interior starts at VEC)
-    constexpr int PLANE = (NY + 2) * P;
-    static_assert(NX % VEC == 0 && WG % TX == 0 && NY % ROWS == 0, "tile shape");
-    static_assert(WG >= 2 * NY && WG >= 2 * TX, "halo roles need enough threads");
+    using S = S7Shape<T, NX, NY, WG>;
+    constexpr int VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, P = S::P, PLANE = S::PLANE;
+    static_assert(S::valid, "tile shape");
     using V = VecT<T, VEC>;
-
-    __shared__ alignas(16) T lds[2][PLANE];
-    __shared__ double red[WG / 64];
-
     const int tid = threadIdx.x;
-    if (a.prev_partials && blockIdx.x == 0)
-        wg_sum_partials<WG>(a.prev_partials, a.prev_n, a.prev_out, red);
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int comp = tile / a.tiles_per_comp;
-    const int tt = tile - comp * a.tiles_per_comp;
-    const long long blk = tt / a.nzc;
-    const int zc = tt - (int)blk * a.nzc;
-    const int NZ = a.G.n[2];
-    const int z0 = zc * a.zchunk;
-    const int z1 = min(NZ, z0 + a.zchunk);
-
-    const T *cbase = reinterpret_cast<const T *>(a.u) + (long long)(a.comp_in + comp) * a.G.cstride;
-    const T *ub = cbase + blk * a.G.bstride;
-    T *ob = reinterpret_cast<T *>(a.out) + (long long)(a.comp_out + comp) * a.G.cstride + blk * a.G.bstride;
-    T *lb = MODE == 2 ? reinterpret_cast<T *>(a.lap) + (long long)(a.comp_out + comp) * a.G.cstride +
-                            blk * a.G.bstride
-                      : nullptr;
-
-    FaceSrc<T> F[6];
-    block_faces<T>(a, comp, blk, cbase, F);
-
     const int tx = tid % TX, ty = tid / TX;
     const int x0 = tx * VEC;
     constexpr long long SXY = (long long)NX * NY;
@@ -255,8 +175,6 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
         }
     };
 
-    const T ihx2 = (T)a.ih2[0], ihy2 = (T)a.ih2[1], ihz2 = (T)a.ih2[2];
-    const T nudt = (T)a.nudt;
     double acc = 0.0;
 
     // Ring of NQ planes in registers: plane p lives in slot (p - z0 + 1) % NQ.
@@ -280,7 +198,7 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
             const Plane &pn = Q[(u + 2) % NQ];
             if (z + S7_PF <= z1)
                 load_plane(z + S7_PF, z + S7_PF < z1, Q[(u + NQ - 1) % NQ]);
-            T *L = lds[(z - z0) & 1];
+            T *L = lds + ((z - z0) & 1) * PLANE;
 #pragma unroll
             for (int r = 0; r < R; ++r)
                 *reinterpret_cast<V *>(L + (ty + r * ROWS + 1) * P + VEC + x0) = pc.c[r];
@@ -325,6 +243,40 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
             }
         }
     }
+
+    return acc;
+}
+
+template <typename T, int NX, int NY, int WG, int MODE, bool SUMSQ>
+__global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
+{
+    using S = S7Shape<T, NX, NY, WG>;
+    __shared__ alignas(16) T lds[2 * S::PLANE];
+    __shared__ double red[WG / 64];
+
+    const int tid = threadIdx.x;
+    if (a.prev_partials && blockIdx.x == 0)
+        wg_sum_partials<WG>(a.prev_partials, a.prev_n, a.prev_out, red);
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int comp = tile / a.tiles_per_comp;
+    const int tt = tile - comp * a.tiles_per_comp;
+    const long long blk = tt / a.nzc;
+    const int zc = tt - (int)blk * a.nzc;
+    const int NZ = a.G.n[2];
+    const int z0 = zc * a.zchunk;
+    const int z1 = min(NZ, z0 + a.zchunk);
+
+    const T *cbase = reinterpret_cast<const T *>(a.u) + (long long)(a.comp_in + comp) * a.G.cstride;
+    const T *ub = cbase + blk * a.G.bstride;
+    T *ob = reinterpret_cast<T *>(a.out) + (long long)(a.comp_out + comp) * a.G.cstride + blk * a.G.bstride;
+    T *lb = MODE == 2 ? reinterpret_cast<T *>(a.lap) + (long long)(a.comp_out + comp) * a.G.cstride +
+                            blk * a.G.bstride
+                      : nullptr;
+
+    FaceSrc<T> F[6];
+    block_faces<T>(a.G, a.heads[comp], a.entries[comp], a.has_bc[comp], blk, cbase, F);
+    double acc = s7_march<T, NX, NY, WG, MODE, SUMSQ>(ub, ob, lb, F, NZ, z0, z1, (T)a.ih2[0], (T)a.ih2[1],
+                                                      (T)a.ih2[2], (T)a.nudt, lds);
 
     if (SUMSQ) {
         // per-tile partial in a fixed wave tree and wave order; the tiles are
@@ -393,7 +345,7 @@ __global__ __launch_bounds__(256) void k_stencil7_generic(const S7Args a)
     const T *ub = cbase + blk * a.G.bstride;
     T *ob = reinterpret_cast<T *>(a.out) + (long long)(a.comp_out + comp) * a.G.cstride + blk * a.G.bstride;
     FaceSrc<T> F[6];
-    block_faces<T>(a, comp, blk, cbase, F);
+    block_faces<T>(a.G, a.heads[comp], a.entries[comp], a.has_bc[comp], blk, cbase, F);
     const int *n = a.G.n;
     const long long cells = (long long)n[0] * n[1] * n[2];
     const long long i = (long long)blockIdx.x * 256 + tid;
@@ -481,10 +433,7 @@ void s7_set_tune(int dtype, int edge, int wg, int zchunk)
 template <typename T, int NX, int WG>
 constexpr bool s7_valid()
 {
-    constexpr int VEC = 16 / (int)sizeof(T);
-    constexpr int TX = NX / VEC;
-    return NX % VEC == 0 && WG % TX == 0 && (WG / TX) <= NX && NX % (WG / TX) == 0 &&
-           WG >= 2 * NX && WG >= 2 * TX;
+    return S7Shape<T, NX, NX, WG>::valid;
 }
 
 template <typename T, int NX, int WG, int MODE, bool SUMSQ>

@@ -10,7 +10,9 @@
 // sharing the three smoothness indicators between the two states.
 // eps = 1e-6 * m2, m2 = mean of the five squared values + 1e-30, applied in
 // the scale-free form  alpha_k = d_k / (1e-6 + beta_k / m2)^2, which keeps
-// FP32 free of underflow for any data magnitude.
+// FP32 free of underflow for any data magnitude; the weights are evaluated
+// multiplied by prod_j (1e-6 + beta_j / m2)^2 (three divisions per state
+// pair instead of twelve).
 //
 // Work layout as in the 7-point kernel: a workgroup owns an x-y plane of one
 // block of one component and marches through a z-chunk; the plane plus a
@@ -37,7 +39,11 @@ struct W5Args {
     const DevBC *entries[kWMaxComp];
 };
 
-// IEEE evaluation in the operation order of the CPU restatement.
+// IEEE evaluation in the operation order of the CPU restatement.  The
+// weights alpha_k = d_k / (eps' + beta_k / m2)^2 are scaled by the common
+// factor prod_j (eps' + beta_j / m2)^2, so only 1/m2 and the two
+// normalisations divide; the 1/6 of the candidate polynomials goes into the
+// normalisation.
 template <typename R>
 __device__ inline void weno5_exact(R a, R b, R c, R d, R e, R &um, R &up)
 {
@@ -47,33 +53,33 @@ __device__ inline void weno5_exact(R a, R b, R c, R d, R e, R &um, R &up)
     const R t1 = b - 2 * c + d, s1 = b - d;
     const R t2 = c - 2 * d + e, s2 = 3 * c - 4 * d + e;
     const R k13 = (R)13 / (R)12;
-    const R b0 = (k13 * t0 * t0 + (R)0.25 * s0 * s0) * im2;
-    const R b1 = (k13 * t1 * t1 + (R)0.25 * s1 * s1) * im2;
-    const R b2 = (k13 * t2 * t2 + (R)0.25 * s2 * s2) * im2;
-    const R eps = (R)1e-6;
-    const R r0 = (R)1 / ((eps + b0) * (eps + b0));
-    const R r1 = (R)1 / ((eps + b1) * (eps + b1));
-    const R r2 = (R)1 / ((eps + b2) * (eps + b2));
+    const R b0 = (k13 * t0 * t0 + (R)0.25 * s0 * s0) * im2 + (R)1e-6;
+    const R b1 = (k13 * t1 * t1 + (R)0.25 * s1 * s1) * im2 + (R)1e-6;
+    const R b2 = (k13 * t2 * t2 + (R)0.25 * s2 * s2) * im2 + (R)1e-6;
+    const R g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
+    const R p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
     {
-        const R a0 = (R)0.1 * r0, a1 = (R)0.6 * r1, a2 = (R)0.3 * r2;
-        const R q0 = (2 * a - 7 * b + 11 * c) / (R)6;
-        const R q1 = (-b + 5 * c + 2 * d) / (R)6;
-        const R q2 = (2 * c + 5 * d - e) / (R)6;
-        um = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+        const R w0 = (R)0.1 * p12, w1 = (R)0.6 * p02, w2 = (R)0.3 * p01;
+        const R q0 = 2 * a - 7 * b + 11 * c;
+        const R q1 = -b + 5 * c + 2 * d;
+        const R q2 = 2 * c + 5 * d - e;
+        um = (w0 * q0 + w1 * q1 + w2 * q2) / ((R)6 * (w0 + w1 + w2));
     }
     {
-        const R a0 = (R)0.3 * r0, a1 = (R)0.6 * r1, a2 = (R)0.1 * r2;
-        const R q0 = (-a + 5 * b + 2 * c) / (R)6;
-        const R q1 = (2 * b + 5 * c - d) / (R)6;
-        const R q2 = (11 * c - 7 * d + 2 * e) / (R)6;
-        up = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+        const R w0 = (R)0.3 * p12, w1 = (R)0.6 * p02, w2 = (R)0.1 * p01;
+        const R q0 = -a + 5 * b + 2 * c;
+        const R q1 = 2 * b + 5 * c - d;
+        const R q2 = 11 * c - 7 * d + 2 * e;
+        up = (w0 * q0 + w1 * q1 + w2 * q2) / ((R)6 * (w0 + w1 + w2));
     }
 }
 
 __device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// FP32 evaluation: same formulas, contracted, divisions by reciprocals, the
-// 1/6 of the candidate polynomials folded into the final normalisation.
+// FP32 evaluation: same formulas, contracted, divisions by the hardware
+// reciprocal.  Every scaled weight stays within [1e-25, 1e10] for any data
+// (beta_k / m2 <= 170 since max u^2 <= 5 m2), so no product over- or
+// underflows in FP32.
 __device__ inline void weno5_fast(float a, float b, float c, float d, float e, float &um, float &up)
 {
 #pragma clang fp contract(fast)
@@ -86,22 +92,21 @@ __device__ inline void weno5_fast(float a, float b, float c, float d, float e, f
     const float b0 = (k13 * t0 * t0 + 0.25f * s0 * s0) * im2 + 1e-6f;
     const float b1 = (k13 * t1 * t1 + 0.25f * s1 * s1) * im2 + 1e-6f;
     const float b2 = (k13 * t2 * t2 + 0.25f * s2 * s2) * im2 + 1e-6f;
-    const float r0 = rcpf(b0 * b0);
-    const float r1 = rcpf(b1 * b1);
-    const float r2 = rcpf(b2 * b2);
+    const float g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
+    const float p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
     {
-        const float a0 = 0.1f * r0, a1 = 0.6f * r1, a2 = 0.3f * r2;
+        const float w0 = 0.1f * p12, w1 = 0.6f * p02, w2 = 0.3f * p01;
         const float q0 = 2.f * a - 7.f * b + 11.f * c;
         const float q1 = -b + 5.f * c + 2.f * d;
         const float q2 = 2.f * c + 5.f * d - e;
-        um = (a0 * q0 + a1 * q1 + a2 * q2) * rcpf(6.f * (a0 + a1 + a2));
+        um = (w0 * q0 + w1 * q1 + w2 * q2) * rcpf(6.f * (w0 + w1 + w2));
     }
     {
-        const float a0 = 0.3f * r0, a1 = 0.6f * r1, a2 = 0.1f * r2;
+        const float w0 = 0.3f * p12, w1 = 0.6f * p02, w2 = 0.1f * p01;
         const float q0 = -a + 5.f * b + 2.f * c;
         const float q1 = 2.f * b + 5.f * c - d;
         const float q2 = 11.f * c - 7.f * d + 2.f * e;
-        up = (a0 * q0 + a1 * q1 + a2 * q2) * rcpf(6.f * (a0 + a1 + a2));
+        up = (w0 * q0 + w1 * q1 + w2 * q2) * rcpf(6.f * (w0 + w1 + w2));
     }
 }
 
@@ -114,42 +119,36 @@ __device__ inline void weno5(T a, T b, T c, T d, T e, T &um, T &up)
         weno5_exact<T>(a, b, c, d, e, um, up);
 }
 
+// Tile geometry of the z-marching WENO5 kernels (two ghost layers read).
 template <typename T, int NX, int NY, int WG>
-__global__ __launch_bounds__(WG) void k_weno5(const W5Args a)
+struct W5Shape {
+    static constexpr int G = 2;          // ghost layers read (i-2..i+2)
+    static constexpr int VEC = 16 / (int)sizeof(T);
+    static constexpr int TX = NX / VEC;
+    static constexpr int ROWS = WG / TX;
+    static constexpr int R = NY / ROWS;
+    static constexpr int P = NX + 2 * VEC; // x halo of 2 fits in the VEC padding
+    static constexpr int PLANE = (NY + 2 * G) * P;
+    static constexpr int NHX = 2 * G * NY; // x-ghost scalars per plane
+    static constexpr int NHY = 2 * G * TX; // y-ghost vectors per plane
+    static constexpr int HX = (NHX + WG - 1) / WG;
+    static constexpr int HY = (NHY + WG - 1) / WG;
+    static constexpr bool valid = NX % VEC == 0 && WG % TX == 0 && ROWS <= NY && NY % ROWS == 0 && VEC >= G;
+};
+
+// One tile of WENO5: planes [z0, z1) of the block at `ub` (NZ planes), faces
+// F resolved for the lab's stencil, the six state components at ob + k *
+// ocstride.  `lds` holds two planes (2 * PLANE elements).
+template <typename T, int NX, int NY, int WG>
+__device__ inline void w5_march(const T *__restrict__ ub, T *__restrict__ ob, long long ocstride,
+                                const FaceL<T> F[6], int NZ, int z0, int z1, T *lds)
 {
-    constexpr int G = 2;                 // ghost layers read (i-2..i+2)
-    constexpr int VEC = 16 / (int)sizeof(T);
-    constexpr int TX = NX / VEC;
-    constexpr int ROWS = WG / TX;
-    constexpr int R = NY / ROWS;
-    constexpr int P = NX + 2 * VEC;      // x halo of 2 fits in the VEC padding
-    constexpr int PLANE = (NY + 2 * G) * P;
-    constexpr int NHX = 2 * G * NY;      // x-ghost scalars per plane
-    constexpr int NHY = 2 * G * TX;      // y-ghost vectors per plane
-    constexpr int HX = (NHX + WG - 1) / WG;
-    constexpr int HY = (NHY + WG - 1) / WG;
-    static_assert(NX % VEC == 0 && WG % TX == 0 && NY % ROWS == 0 && VEC >= G, "tile shape");
+    using S = W5Shape<T, NX, NY, WG>;
+    constexpr int G = S::G, VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, P = S::P, PLANE = S::PLANE;
+    constexpr int NHX = S::NHX, NHY = S::NHY, HX = S::HX, HY = S::HY;
+    static_assert(S::valid, "tile shape");
     using V = VecT<T, VEC>;
-
-    __shared__ alignas(16) T lds[2][PLANE];
-
     const int tid = threadIdx.x;
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int comp = tile / a.tiles_per_comp;
-    const int tt = tile - comp * a.tiles_per_comp;
-    const long long blk = tt / a.nzc;
-    const int zc = tt - (int)blk * a.nzc;
-    const int NZ = a.G.n[2];
-    const int z0 = zc * a.zchunk;
-    const int z1 = min(NZ, z0 + a.zchunk);
-
-    const T *cbase = reinterpret_cast<const T *>(a.in) + (long long)comp * a.G.cstride;
-    const T *ub = cbase + blk * a.G.bstride;
-    T *ob = reinterpret_cast<T *>(a.out) + (long long)(6 * comp) * a.ocstride + blk * a.G.bstride;
-
-    FaceL<T> F[6];
-    const int sb[3] = {-3, -3, -3}, se[3] = {4, 4, 4};
-    make_faces<T>(a.G, a.heads[comp], a.entries[comp], a.has_bc[comp], blk, cbase, sb, se, F);
     const FaceL<T> FX0 = F[0], FX1 = F[1], FY0 = F[2], FY1 = F[3], FZ0 = F[4], FZ1 = F[5];
 
     const int tx = tid % TX, ty = tid / TX;
@@ -232,7 +231,7 @@ __global__ __launch_bounds__(WG) void k_weno5(const W5Args a)
     for (int z = z0; z < z1; ++z) {
         if (z + 3 <= z1 + 1)
             load_plane(z + 3, z + 3 < z1, q5);
-        T *L = lds[(z - z0) & 1];
+        T *L = lds + ((z - z0) & 1) * PLANE;
 #pragma unroll
         for (int r = 0; r < R; ++r)
             *reinterpret_cast<V *>(L + (ty + r * ROWS + G) * P + VEC + x0) = q2.c[r];
@@ -281,7 +280,7 @@ __global__ __launch_bounds__(WG) void k_weno5(const W5Args a)
             const long long off = x0 + NX * y + SXY * z;
 #pragma unroll
             for (int k = 0; k < 6; ++k)
-                store_nt<T, VEC>(ob + k * a.ocstride + off, o[k]);
+                store_nt<T, VEC>(ob + k * ocstride + off, o[k]);
         }
         q0 = q1;
         q1 = q2;
@@ -289,6 +288,31 @@ __global__ __launch_bounds__(WG) void k_weno5(const W5Args a)
         q3 = q4;
         q4 = q5;
     }
+}
+
+template <typename T, int NX, int NY, int WG>
+__global__ __launch_bounds__(WG) void k_weno5(const W5Args a)
+{
+    using S = W5Shape<T, NX, NY, WG>;
+    __shared__ alignas(16) T lds[2 * S::PLANE];
+
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int comp = tile / a.tiles_per_comp;
+    const int tt = tile - comp * a.tiles_per_comp;
+    const long long blk = tt / a.nzc;
+    const int zc = tt - (int)blk * a.nzc;
+    const int NZ = a.G.n[2];
+    const int z0 = zc * a.zchunk;
+    const int z1 = min(NZ, z0 + a.zchunk);
+
+    const T *cbase = reinterpret_cast<const T *>(a.in) + (long long)comp * a.G.cstride;
+    const T *ub = cbase + blk * a.G.bstride;
+    T *ob = reinterpret_cast<T *>(a.out) + (long long)(6 * comp) * a.ocstride + blk * a.G.bstride;
+
+    FaceL<T> F[6];
+    const int sb[3] = {-3, -3, -3}, se[3] = {4, 4, 4};
+    make_faces<T>(a.G, a.heads[comp], a.entries[comp], a.has_bc[comp], blk, cbase, sb, se, F);
+    w5_march<T, NX, NY, WG>(ub, ob, a.ocstride, F, NZ, z0, z1, lds);
 }
 
 // Generic fallback for any block shape: one thread per cell, values through

@@ -213,8 +213,8 @@ int cnv_grid_compare(cnv_grid a, int comp_a, cnv_grid b, int comp_b, double *out
  * BC/Dirichlet.h:75-113).  Without cnv_grid_set_zfaces the z faces are
  * uniform. */
 int cnv_grid_set_zfaces(cnv_grid g, const double *zfaces, int n /* nz_total + 1 */);
-/* weighted Jacobi sweep: out = u + omega ((f - A u) / diag),
- * diag = -(((2 ihx2 + 2 ihy2) + czm_k) + czp_k); u, f, out of one dtype */
+/* weighted Jacobi sweep: out = u + (f - A u) * wd_k, wd_k = omega / diag_k,
+ * diag_k = -(((2 ihx2 + 2 ihy2) + czm_k) + czp_k); u, f, out of one dtype */
 int cnv_jacobi_sweep(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int co, double omega, cnv_stream s);
 /* defect r = f - A u in u's dtype, stored in r's dtype; if sumsq is
  * non-NULL the FP64 sum of r^2 (before r is rounded) is written there */

@@ -446,34 +446,36 @@ double diffusion(const orc_field &in, void *out_v, long long out_stride, void *l
 template <typename R>
 inline void weno5_cell(R a, R b, R c, R d, R e, R &um, R &up)
 {
+    // Jiang & Shu weights alpha_k = d_k / (eps' + beta_k / m2)^2, written
+    // with the common factor prod_j (eps' + beta_j / m2)^2 multiplied out so
+    // that only 1/m2 and the two normalisations divide; the 1/6 of the
+    // candidate polynomials goes into the normalisation.
     const R m2 = (a * a + b * b + c * c + d * d + e * e) * (R)0.2 + (R)1e-30;
     const R im2 = (R)1 / m2;
     const R t0 = a - 2 * b + c, s0 = a - 4 * b + 3 * c;
     const R t1 = b - 2 * c + d, s1 = b - d;
     const R t2 = c - 2 * d + e, s2 = 3 * c - 4 * d + e;
     const R k13 = (R)13 / (R)12;
-    const R b0 = (k13 * t0 * t0 + (R)0.25 * s0 * s0) * im2;
-    const R b1 = (k13 * t1 * t1 + (R)0.25 * s1 * s1) * im2;
-    const R b2 = (k13 * t2 * t2 + (R)0.25 * s2 * s2) * im2;
-    const R eps = (R)1e-6;
-    const R r0 = (R)1 / ((eps + b0) * (eps + b0));
-    const R r1 = (R)1 / ((eps + b1) * (eps + b1));
-    const R r2 = (R)1 / ((eps + b2) * (eps + b2));
+    const R b0 = (k13 * t0 * t0 + (R)0.25 * s0 * s0) * im2 + (R)1e-6;
+    const R b1 = (k13 * t1 * t1 + (R)0.25 * s1 * s1) * im2 + (R)1e-6;
+    const R b2 = (k13 * t2 * t2 + (R)0.25 * s2 * s2) * im2 + (R)1e-6;
+    const R g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
+    const R p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
     // u^- at i+1/2: linear weights (1,6,3)/10 on stencils 0,1,2
     {
-        const R a0 = (R)0.1 * r0, a1 = (R)0.6 * r1, a2 = (R)0.3 * r2;
-        const R q0 = (2 * a - 7 * b + 11 * c) / (R)6;
-        const R q1 = (-b + 5 * c + 2 * d) / (R)6;
-        const R q2 = (2 * c + 5 * d - e) / (R)6;
-        um = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+        const R w0 = (R)0.1 * p12, w1 = (R)0.6 * p02, w2 = (R)0.3 * p01;
+        const R q0 = 2 * a - 7 * b + 11 * c;
+        const R q1 = -b + 5 * c + 2 * d;
+        const R q2 = 2 * c + 5 * d - e;
+        um = (w0 * q0 + w1 * q1 + w2 * q2) / ((R)6 * (w0 + w1 + w2));
     }
     // u^+ at i-1/2: mirrored stencils, weights (3,6,1)/10
     {
-        const R a0 = (R)0.3 * r0, a1 = (R)0.6 * r1, a2 = (R)0.1 * r2;
-        const R q0 = (-a + 5 * b + 2 * c) / (R)6;
-        const R q1 = (2 * b + 5 * c - d) / (R)6;
-        const R q2 = (11 * c - 7 * d + 2 * e) / (R)6;
-        up = (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+        const R w0 = (R)0.3 * p12, w1 = (R)0.6 * p02, w2 = (R)0.1 * p01;
+        const R q0 = -a + 5 * b + 2 * c;
+        const R q1 = 2 * b + 5 * c - d;
+        const R q2 = 11 * c - 7 * d + 2 * e;
+        up = (w0 * q0 + w1 * q1 + w2 * q2) / ((R)6 * (w0 + w1 + w2));
     }
 }
 
@@ -539,7 +541,7 @@ double jacobi(const orc_field &u, const void *f_v, long long fstride, void *out_
               const double *czm, const double *czp, const double h[3], double omega, int mode,
               int nthreads)
 {
-    // mode 0: weighted Jacobi sweep out = u + omega (f - A u) / diag(A)
+    // mode 0: weighted Jacobi sweep out = u + (f - A u) * (omega / diag(A))
     // mode 1: defect out = f - A u; returns sum of squared defect (long double)
     const T ihx2 = (T)(1. / (h[0] * h[0]));
     const T ihy2 = (T)(1. / (h[1] * h[1]));
@@ -564,13 +566,14 @@ double jacobi(const orc_field &u, const void *f_v, long long fstride, void *out_
                 const int kz = bidx[2] * N[2] + iz;
                 const T cm = (T)czm[kz], cp = (T)czp[kz];
                 const T diag = -(2 * ihx2 + 2 * ihy2 + cm + cp);
+                const T wd = w / diag; // one division per z plane
                 for (int iy = 0; iy < N[1]; ++iy)
                     for (int ix = 0; ix < N[0]; ++ix) {
                         const size_t li = ix + (size_t)N[0] * (iy + (size_t)N[1] * iz);
                         const T au = applyA(lab, ix, iy, iz, ihx2, ihy2, cm, cp);
                         const T r = f[b * fstride + li] - au;
                         if (mode == 0) {
-                            out[b * ostride + li] = lab.at(ix, iy, iz) + w * (r / diag);
+                            out[b * ostride + li] = lab.at(ix, iy, iz) + r * wd;
                         } else {
                             out[b * ostride + li] = r;
                             acc += (long double)((double)r * (double)r);
