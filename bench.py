@@ -1,13 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark of the block-structured stencil path on one MI355X.
 
-Default workload (one "step" = one fused lab + 7-point Laplacian + forward
-Euler + FP64 L2-norm pass over the grid): 8x8x8 blocks of 32^3 cells (256^3),
-periodic, ghost width 1, seeded sum of 8 Fourier modes, dt = 0.1 h^2, run in
-FP32 and in FP64 from the same initial field.  `value` is the FP32 path in
-GCell-updates/s; the FP64 path, the FP32-vs-FP64 rel-L2 error, the roofline
-of the dominant kernel (HIP events on the launch stream) and the CPU baseline
-(the C++ oracle on the host cores) are reported in the same JSON line.
+Default workload (one "step" = lab + 7-point Laplacian + forward Euler +
+FP64 L2 norm of the new field, over the whole grid): 8x8x8 blocks of 32^3
+cells (256^3), periodic, ghost width 1, seeded sum of 8 Fourier modes,
+dt = 0.1 h^2, 100 steps (one diffusion_run: two steps per launch on this
+periodic grid) run in FP32 and in FP64 from the same initial field.  `value`
+is the FP32 path in GCell-updates/s; the FP64 path, the FP32-vs-FP64 rel-L2
+error, the roofline of the dominant kernel (HIP events on the launch stream;
+algorithmic bytes = one read and one write of the field per launch) and the
+CPU baseline (the C++ oracle on the host cores) are reported in the same JSON
+line.
 
   python bench.py [--steps K] [--warmup W] [--workload diffusion|weno5|jacobi|batch|copy]
 """
@@ -77,7 +80,14 @@ def measured_traffic(name, key):
 # ------------------------------------------------------------------ diffusion
 
 
-def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup, autotune=True, tile=None):
+def run_launches(steps):
+    """Launches of a diffusion_run of `steps` steps on a periodic grid of
+    32^3 blocks: two-step launches up to a multiple of four, then one-step."""
+    fused = 2 * (steps // 4)
+    return fused + (steps - 2 * fused)
+
+
+def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup, autotune=True, tile=None, tile2=None):
     nb, n, N = (8, 8, 8), (32, 32, 32), 256
     h = 1.0 / N
     dt = 0.1 * h * h
@@ -94,6 +104,20 @@ def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup, autotune=True, tile=None):
         print("autotune %s: best wg=%d zchunk=%d; %s" % (np.dtype(dtype).name, wg, zc,
               " ".join("%d/%d:%.1fus" % t for t in table)), file=sys.stderr, flush=True)
         a.upload_global(0, u0.astype(dtype))
+    # the two-steps-per-launch kernel that diffusion_run uses on this periodic grid
+    tune2 = None
+    if tile2:
+        cnv.set_tune_diffusion_run(dtype, 32, tile2[0], tile2[1])
+        tune2 = {"wg": tile2[0], "zchunk": tile2[1]}
+    elif autotune:
+        wg, zc, table = cnv.autotune_diffusion_run(a, b, iters=10)
+        tune2 = {"wg": wg, "zchunk": zc}
+        print("autotune two-step %s: best wg=%d zchunk=%d; %s" % (np.dtype(dtype).name, wg, zc,
+              " ".join("%d/%d:%.1fus" % t for t in table)), file=sys.stderr, flush=True)
+        a.upload_global(0, u0.astype(dtype))
+    else:
+        wg, zc = cnv.tune_diffusion_run(dtype, 32)
+        tune2 = {"wg": wg, "zchunk": zc}
     st = cnv.Stream()
     norms = cnv.DeviceBuffer(8 * max(1, steps + warmup))
     a.prepare()
@@ -125,8 +149,10 @@ def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup, autotune=True, tile=None):
     return {
         "wall_s": wall, "event_ms": kern_ms, "final": final, "norms": hist,
         "gcups": cells * steps / wall / 1e9,
+        # every launch (two-step or one-step) reads the field once and writes it once
         "bytes_per_launch": cells * 2 * np.dtype(dtype).itemsize,
-        "avg_kernel_s": kern_ms / 1e3 / steps, "tile": tune,
+        "launches": run_launches(steps), "avg_kernel_s": kern_ms / 1e3 / run_launches(steps),
+        "tile": tune, "tile2": tune2,
     }
 
 
@@ -157,8 +183,10 @@ def bench_diffusion(args):
     from cubismnova_amd import workloads
     ev = HipEvents()
     u0 = workloads.diffusion_large(256)
-    r32 = diffusion_gpu(cnv, ev, np.float32, u0, args.steps, args.warmup, not args.no_autotune, args.tile_f32)
-    r64 = diffusion_gpu(cnv, ev, np.float64, u0, args.steps, args.warmup, not args.no_autotune, args.tile_f64)
+    r32 = diffusion_gpu(cnv, ev, np.float32, u0, args.steps, args.warmup, not args.no_autotune, args.tile_f32,
+                        args.tile2_f32)
+    r64 = diffusion_gpu(cnv, ev, np.float64, u0, args.steps, args.warmup, not args.no_autotune, args.tile_f64,
+                        args.tile2_f64)
     f32, f64 = r32["final"].astype(np.float64), r64["final"]
     rel = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
     nrel = float(np.max(np.abs(r32["norms"] - r64["norms"]) / r64["norms"]))
@@ -168,9 +196,11 @@ def bench_diffusion(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded sum of 8 Fourier modes, |k| in [1,16], amplitude 1/|k|)",
         "config": {"workload": "diffusion: 8x8x8 blocks of 32^3 cells (256^3), periodic, ghost 1, fused "
-                               "lab + 7-point Laplacian + forward Euler (dt=0.1h^2) + FP64 L2 norm per step",
+                               "lab + 7-point Laplacian + forward Euler (dt=0.1h^2) + FP64 L2 norm per step, "
+                               "two steps per launch",
                    "blocks": [8, 8, 8], "block_cells": [32, 32, 32], "steps_per_graph": args.steps,
-                   "tile_f32": r32["tile"], "tile_f64": r64["tile"]},
+                   "launches_per_graph": r32["launches"], "tile_f32": r32["tile"], "tile_f64": r64["tile"],
+                   "tile2_f32": r32["tile2"], "tile2_f64": r64["tile2"]},
         "roofline": roofline(r32["bytes_per_launch"], r32["avg_kernel_s"],
                              measured_traffic("stencil7_diffusion_256.md", "f32")),
         "fp64": {"value": round(r64["gcups"], 3), "unit": "GCell-updates/s",
@@ -189,7 +219,7 @@ def bench_diffusion(args):
 # ------------------------------------------------------------------ WENO5
 
 
-def weno_gpu(cnv, ev, dtype, steps, warmup, autotune, comps):
+def weno_gpu(cnv, ev, dtype, steps, warmup, autotune, comps, tile=None):
     N, nb, n = 512, (16, 16, 16), (32, 32, 32)
     g = cnv.Cartesian(nb, n, dtype=dtype, ncomp=5)
     for c in range(5):
@@ -198,7 +228,10 @@ def weno_gpu(cnv, ev, dtype, steps, warmup, autotune, comps):
         g.addFaceBC(bc)              # x periodic, y/z zero-gradient
     out = cnv.Cartesian(nb, n, dtype=dtype, ncomp=30)
     tune = None
-    if autotune:
+    if tile:
+        cnv.set_tune_weno5(dtype, tile[0], tile[1])
+        tune = {"wg": tile[0], "zchunk": tile[1]}
+    elif autotune:
         wg, zc, table = cnv.autotune_weno5(g, out, iters=3)
         tune = {"wg": wg, "zchunk": zc}
         print("autotune weno5 %s: best wg=%d zchunk=%d; %s" % (np.dtype(dtype).name, wg, zc,
@@ -267,8 +300,8 @@ def bench_weno5(args):
     from cubismnova_amd import workloads
     ev = HipEvents()
     comps = [workloads.weno_component(512, c) for c in range(5)]
-    r32 = weno_gpu(cnv, ev, np.float32, args.steps, args.warmup, not args.no_autotune, comps)
-    r64 = weno_gpu(cnv, ev, np.float64, args.steps, args.warmup, not args.no_autotune, comps)
+    r32 = weno_gpu(cnv, ev, np.float32, args.steps, args.warmup, not args.no_autotune, comps, args.tile_f32)
+    r64 = weno_gpu(cnv, ev, np.float64, args.steps, args.warmup, not args.no_autotune, comps, args.tile_f64)
     del comps
     worst = 0.0
     for k in range(30):
@@ -450,7 +483,8 @@ def bench_batch(args):
     b = cnv.Batch(items)
     table = [] if args.no_autotune else b.autotune(iters=3)
     if table:
-        print("autotune batch: " + " ".join("%d/%d/%d:%.0fus" % t for t in table), file=sys.stderr, flush=True)
+        print("autotune batch (op/edge/wg/depth): " + " ".join("%d/%d/%d/%d:%.0fus" % t for t in table),
+              file=sys.stderr, flush=True)
     st = cnv.Stream()
     for _ in range(args.warmup):
         b.run(stream=st)
@@ -476,9 +510,11 @@ def bench_batch(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Fourier-mode fields per grid)",
         "config": {"workload": "96 independent grids (blocks 1x1x1..8x8x4, edges {8,16,32}, 1-3 components, "
                                "ghost 1 or 3, per-face periodic/Dirichlet/absorbing), each a diffusion step or "
-                               "WENO5, one persistent launch per step, HIP-graph replayed",
-                   "grids": len(specs), "cells": cells_total, "work_items": info["items"], "wg": info["wg"],
-                   "lds_bytes": info["lds_bytes"]},
+                               "WENO5; per step one launch per (operator, edge) class of 16^3/32^3 blocks and one "
+                               "persistent launch for the 8^3 blocks, HIP-graph replayed",
+                   "grids": len(specs), "cells": cells_total, "work_items": info["items"],
+                   "classes": [dict(zip(("op", "edge", "wg", "zdepth", "items"), c)) for c in b.classes()],
+                   "persistent_wg": info["wg"], "persistent_lds_bytes": info["lds_bytes"]},
         "roofline": roofline(bytes_total, kern_s, measured_traffic("batch_96.md", "batch")),
     }
     if not args.no_cpu:
@@ -555,8 +591,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in tile table")
     tile = lambda v: tuple(int(x) for x in v.split(","))
-    ap.add_argument("--tile-f32", type=tile, default=None, help="WG,ZCHUNK for the FP32 kernel (skips autotune)")
-    ap.add_argument("--tile-f64", type=tile, default=None, help="WG,ZCHUNK for the FP64 kernel (skips autotune)")
+    ap.add_argument("--tile-f32", type=tile, default=None, help="WG,ZCHUNK for the FP32 kernel of the diffusion or WENO5 workload (skips autotune)")
+    ap.add_argument("--tile2-f32", type=tile, default=None,
+                    help="WG,ZCHUNK for the FP32 two-step diffusion kernel (skips its autotune)")
+    ap.add_argument("--tile2-f64", type=tile, default=None,
+                    help="WG,ZCHUNK for the FP64 two-step diffusion kernel (skips its autotune)")
+    ap.add_argument("--tile-f64", type=tile, default=None, help="WG,ZCHUNK for the FP64 kernel of the diffusion or WENO5 workload (skips autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (got --gpus %d)" % args.gpus,

@@ -509,6 +509,30 @@ def autotune_stencil7(u, out, iters=20):
     return wg.value, zc.value, table
 
 
+def autotune_diffusion_run(u, tmp, iters=20):
+    """Pick (workgroup size, z-planes per tile) of the two-step kernel that
+    diffusion_run uses on periodic grids of 16^3 / 32^3 blocks; returns
+    (wg, zchunk, table).  tmp is overwritten."""
+    wg, zc, nr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    res = np.zeros(3 * 64, dtype=np.float64)
+    call("cnv_autotune_diffusion_run", u.handle, tmp.handle, int(iters), ctypes.byref(wg), ctypes.byref(zc),
+         res.ctypes.data_as(ctypes.c_void_p), 64, ctypes.byref(nr))
+    table = [(int(res[3 * i]), int(res[3 * i + 1]), float(res[3 * i + 2])) for i in range(nr.value)]
+    return wg.value, zc.value, table
+
+
+def set_tune_diffusion_run(dtype, edge, wg, zchunk):
+    call("cnv_tune_set_diffusion_run", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
+         int(wg), int(zchunk))
+
+
+def tune_diffusion_run(dtype, edge):
+    wg, zc = ctypes.c_int(), ctypes.c_int()
+    call("cnv_tune_get_diffusion_run", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
+         ctypes.byref(wg), ctypes.byref(zc))
+    return wg.value, zc.value
+
+
 def tune_stencil7(dtype, edge):
     wg, zc = ctypes.c_int(), ctypes.c_int()
     call("cnv_tune_get_stencil7", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
@@ -537,7 +561,9 @@ def autotune_weno5(u, out, iters=5):
 
 
 class Batch:
-    """Independent FP32 grids processed by one persistent launch.
+    """Independent FP32 grids processed together: one launch per (operator,
+    edge) class of cubic 16^3 / 32^3 blocks, one persistent launch for the
+    rest.
 
     items: list of (in_grid, out_grid, op, ghost, nu_dt) with op "diffusion"
     (out = in + nu_dt lap(in)) or "weno5" (6 states per component)."""
@@ -554,11 +580,20 @@ class Batch:
         self.handle = h
 
     def autotune(self, iters=5):
-        res = np.zeros(4 * 256, dtype=np.float64)
+        """Returns the trials as (op, edge, wg, depth, microseconds); op -1 is
+        the persistent launch."""
+        res = np.zeros(5 * 512, dtype=np.float64)
         nr = ctypes.c_int()
-        call("cnv_batch_autotune", self.handle, int(iters), res.ctypes.data_as(ctypes.c_void_p), 256,
+        call("cnv_batch_autotune", self.handle, int(iters), res.ctypes.data_as(ctypes.c_void_p), 512,
              ctypes.byref(nr))
-        return [tuple(res[4 * i:4 * i + 4]) for i in range(nr.value)]
+        return [tuple(res[5 * i:5 * i + 5]) for i in range(nr.value)]
+
+    def classes(self):
+        """[(op, edge, wg, z depth, items)] of the per-class launches."""
+        out = np.zeros(5 * 16, dtype=np.int32)
+        n = ctypes.c_int()
+        call("cnv_batch_class_info", self.handle, 16, out.ctypes.data_as(_abi.c_int_p), ctypes.byref(n))
+        return [tuple(int(x) for x in out[5 * i:5 * i + 5]) for i in range(min(n.value, 16))]
 
     def run(self, stream=None):
         call("cnv_batch_run", self.handle, _st(stream))

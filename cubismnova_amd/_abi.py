@@ -101,6 +101,9 @@ SIGNATURES = {
     "cnv_autotune_stencil7": (i32, [vp, vp, i32, c_int_p, c_int_p, vp, i32, c_int_p]),
     "cnv_tune_set_stencil7": (i32, [i32, i32, i32, i32]),
     "cnv_tune_get_stencil7": (i32, [i32, i32, c_int_p, c_int_p]),
+    "cnv_autotune_diffusion_run": (i32, [vp, vp, i32, c_int_p, c_int_p, vp, i32, c_int_p]),
+    "cnv_tune_set_diffusion_run": (i32, [i32, i32, i32, i32]),
+    "cnv_tune_get_diffusion_run": (i32, [i32, i32, c_int_p, c_int_p]),
     "cnv_grid_compare": (i32, [vp, i32, vp, i32, vp, vp]),
     "cnv_grid_set_zfaces": (i32, [vp, vp, i32]),
     "cnv_jacobi_sweep": (i32, [vp, i32, vp, i32, vp, i32, f64, vp]),
@@ -115,6 +118,7 @@ SIGNATURES = {
     "cnv_batch_autotune": (i32, [vp, i32, vp, i32, c_int_p]),
     "cnv_batch_run": (i32, [vp, vp]),
     "cnv_batch_info": (i32, [vp, c_int_p, c_int_p, ctypes.POINTER(sz)]),
+    "cnv_batch_class_info": (i32, [vp, i32, c_int_p, c_int_p]),
     "cnv_batch_destroy": (i32, [vp]),
     "cnv_grid_prepare": (i32, [vp]),
     "cnv_graph_begin": (i32, [vp]),

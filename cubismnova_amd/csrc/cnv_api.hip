@@ -917,6 +917,43 @@ int cnv_autotune_stencil7(cnv_grid u, cnv_grid out, int iters, int *wg, int *zch
     CNV_GUARD_END
 }
 
+int cnv_autotune_diffusion_run(cnv_grid u, cnv_grid tmp, int iters, int *wg, int *zchunk, double *results,
+                               int max_results, int *nresults)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_grid(tmp));
+    CNV_CHECK(same_shape(u, tmp));
+    if (u == tmp)
+        return fail(CNV_ERR_INVALID, "autotune needs two different grids");
+    if (tmp->desc.ncomp < u->desc.ncomp)
+        return fail(CNV_ERR_SIZE, "tmp has fewer components than u");
+    if (iters < 1)
+        return fail(CNV_ERR_INVALID, "iters must be >= 1");
+    if (!x2_applicable(u))
+        return fail(CNV_ERR_UNSUPPORTED, "the two-step kernel needs periodic grids of 16^3 or 32^3 blocks");
+    return autotune_x2(u, tmp, iters, wg, zchunk, results, max_results, nresults);
+    CNV_GUARD_END
+}
+
+int cnv_tune_set_diffusion_run(int dtype, int edge, int wg, int zchunk)
+{
+    if (dtype != CNV_F32 && dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    return x2_set_tune(dtype, edge, wg, zchunk);
+}
+
+int cnv_tune_get_diffusion_run(int dtype, int edge, int *wg, int *zchunk)
+{
+    if (!wg || !zchunk)
+        return fail(CNV_ERR_INVALID, "output pointer is NULL");
+    if (dtype != CNV_F32 && dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    if (edge != 16 && edge != 32)
+        return fail(CNV_ERR_INVALID, "two-step kernel: block edge must be 16 or 32");
+    return x2_get_tune(dtype, edge, wg, zchunk);
+}
+
 int cnv_tune_set_stencil7(int dtype, int edge, int wg, int zchunk)
 {
     if (dtype != CNV_F32 && dtype != CNV_F64)

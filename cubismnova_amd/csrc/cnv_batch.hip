@@ -1,19 +1,23 @@
 // cnv_batch.hip -- many independent grids of uneven block counts, block
-// sizes, component counts, ghost widths and BCs processed by one persistent
-// launch that walks a device work list.
+// sizes, component counts, ghost widths and BCs processed together.
 //
-// A work item is (grid, component, block, z-range).  The workgroup stages the
-// item's cells plus the face halo its operator reads (one layer for the
-// 7-point Euler step, two for WENO5) in LDS, evaluating every ghost with the
-// same lab rules as cnv_lab_load (neighbour copy or the block's BC list,
-// resolved for the grid's Stencil(-g, g+1)), then applies the operator:
-//   op 0: u_new = u + nu_dt * lap(u) with the Laplacian.cpp:125-129 order
-//   op 1: WENO5 u^-_{i+1/2}, u^+_{i-1/2} in x, y, z (outputs 6c+2d, 6c+2d+1)
-// Items are sorted by cost (largest first) and dealt round-robin to a grid of
-// CU-count x occupancy workgroups, so one launch balances uneven grids.  The
-// z depth of the items of each (block edge, ghost width) class and the
-// workgroup size are chosen by cnv_batch_autotune.  FP32 grids only.
-#include "cnv_device.h"
+// A work item is (grid, component, block, z-range).  Items are grouped by
+// class:
+//  * (operator, block edge) for cubic FP32 blocks of edge 16 or 32: one
+//    launch per class runs the same z-marching tile bodies as the single-grid
+//    kernels (s7_march, w5_march), one workgroup per item, with the
+//    workgroup size and z depth of that class chosen by cnv_batch_autotune;
+//  * everything else (other block shapes): one persistent launch that walks
+//    a device work list sorted by cost (largest first).  The workgroup stages
+//    the item's cells plus the face halo its operator reads (one layer for
+//    the 7-point Euler step, two for WENO5) in LDS, evaluating every ghost
+//    with the same lab rules as cnv_lab_load, then applies the operator.
+// Operators: op 0: u_new = u + nu_dt * lap(u) in the Laplacian.cpp:125-129
+// order; op 1: WENO5 u^-_{i+1/2}, u^+_{i-1/2} in x, y, z (outputs 6c+2d,
+// 6c+2d+1).  Ghosts follow the grid's Stencil(-g, g+1) and BC lists.
+// FP32 grids only.  All launches of a batch go to one stream, so a batch run
+// is capturable as one unit in a HIP graph.
+#include "cnv_march.h"
 
 #include <algorithm>
 #include <chrono>
@@ -44,37 +48,6 @@ struct BItem {
     int blk;
     short z0, nz;
 };
-
-__device__ inline void weno5f(float a, float b, float c, float d, float e, float &um, float &up)
-{
-#pragma clang fp contract(fast)
-    const float m2 = (a * a + b * b + c * c + d * d + e * e) * 0.2f + 1e-30f;
-    const float im2 = __builtin_amdgcn_rcpf(m2);
-    const float t0 = a - 2.f * b + c, s0 = a - 4.f * b + 3.f * c;
-    const float t1 = b - 2.f * c + d, s1 = b - d;
-    const float t2 = c - 2.f * d + e, s2 = 3.f * c - 4.f * d + e;
-    const float k13 = 13.f / 12.f;
-    const float b0 = (k13 * t0 * t0 + 0.25f * s0 * s0) * im2 + 1e-6f;
-    const float b1 = (k13 * t1 * t1 + 0.25f * s1 * s1) * im2 + 1e-6f;
-    const float b2 = (k13 * t2 * t2 + 0.25f * s2 * s2) * im2 + 1e-6f;
-    const float r0 = __builtin_amdgcn_rcpf(b0 * b0);
-    const float r1 = __builtin_amdgcn_rcpf(b1 * b1);
-    const float r2 = __builtin_amdgcn_rcpf(b2 * b2);
-    {
-        const float a0 = 0.1f * r0, a1 = 0.6f * r1, a2 = 0.3f * r2;
-        const float q0 = 2.f * a - 7.f * b + 11.f * c;
-        const float q1 = -b + 5.f * c + 2.f * d;
-        const float q2 = 2.f * c + 5.f * d - e;
-        um = (a0 * q0 + a1 * q1 + a2 * q2) * __builtin_amdgcn_rcpf(6.f * (a0 + a1 + a2));
-    }
-    {
-        const float a0 = 0.3f * r0, a1 = 0.6f * r1, a2 = 0.1f * r2;
-        const float q0 = -a + 5.f * b + 2.f * c;
-        const float q1 = 2.f * b + 5.f * c - d;
-        const float q2 = 11.f * c - 7.f * d + 2.f * e;
-        up = (a0 * q0 + a1 * q1 + a2 * q2) * __builtin_amdgcn_rcpf(6.f * (a0 + a1 + a2));
-    }
-}
 
 template <int WG>
 __global__ __launch_bounds__(WG) void k_batch(const BGrid *__restrict__ grids, const BItem *__restrict__ items,
@@ -134,18 +107,57 @@ __global__ __launch_bounds__(WG) void k_batch(const BGrid *__restrict__ grids, c
             } else {
                 float um, up;
                 float *oc = ob + (long long)(6 * c) * Gd.ocstride + o;
-                weno5f(lds[p - 2], lds[p - 1], cc, lds[p + 1], lds[p + 2], um, up);
+                weno5_fast(lds[p - 2], lds[p - 1], cc, lds[p + 1], lds[p + 2], um, up);
                 oc[0] = um;
                 oc[Gd.ocstride] = up;
-                weno5f(lds[p - 2 * Lx], lds[p - Lx], cc, lds[p + Lx], lds[p + 2 * Lx], um, up);
+                weno5_fast(lds[p - 2 * Lx], lds[p - Lx], cc, lds[p + Lx], lds[p + 2 * Lx], um, up);
                 oc[2 * Gd.ocstride] = um;
                 oc[3 * Gd.ocstride] = up;
-                weno5f(lds[p - 2 * LXY], lds[p - LXY], cc, lds[p + LXY], lds[p + 2 * LXY], um, up);
+                weno5_fast(lds[p - 2 * LXY], lds[p - LXY], cc, lds[p + LXY], lds[p + 2 * LXY], um, up);
                 oc[4 * Gd.ocstride] = um;
                 oc[5 * Gd.ocstride] = up;
             }
         }
         __syncthreads(); // the next item overwrites the tile
+    }
+}
+
+template <int OP, int NX, int WG>
+constexpr bool bm_valid()
+{
+    return OP == 0 ? S7Shape<float, NX, NX, WG>::valid : W5Shape<float, NX, NX, WG>::valid;
+}
+
+template <int OP, int NX, int WG>
+constexpr int bm_plane()
+{
+    return OP == 0 ? S7Shape<float, NX, NX, WG>::PLANE : W5Shape<float, NX, NX, WG>::PLANE;
+}
+
+// One item of an (operator, edge) class per workgroup: the single-grid tile
+// bodies on the item's grid, component, block and z-range.
+template <int OP, int NX, int WG>
+__global__ __launch_bounds__(WG) void k_bmarch(const BGrid *__restrict__ grids, const BItem *__restrict__ items,
+                                               int nitems)
+{
+    __shared__ alignas(16) float lds[2 * bm_plane<OP, NX, WG>()];
+    const BItem I = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const BGrid &Gd = grids[I.grid];
+    const int c = I.comp;
+    const float *cbase = Gd.in + (long long)c * Gd.G.cstride;
+    const float *ub = cbase + (long long)I.blk * Gd.G.bstride;
+    float *ob = Gd.out + (long long)I.blk * Gd.G.bstride;
+    if constexpr (OP == 0) {
+        FaceSrc<float> F[6];
+        block_faces<float>(Gd.G, Gd.heads[c], Gd.entries[c], Gd.has_bc[c], I.blk, cbase, F);
+        (void)s7_march<float, NX, NX, WG, 1, false>(ub, ob + (long long)c * Gd.ocstride, nullptr, F, NX, I.z0,
+                                                     I.z0 + I.nz, Gd.ih2[0], Gd.ih2[1], Gd.ih2[2], Gd.nudt, lds);
+    } else {
+        FaceL<float> F[6];
+        const int sb[3] = {-Gd.g, -Gd.g, -Gd.g}, se[3] = {Gd.g + 1, Gd.g + 1, Gd.g + 1};
+        make_faces<float>(Gd.G, Gd.heads[c], Gd.entries[c], Gd.has_bc[c], I.blk, cbase, sb, se, F);
+        w5_march<float, NX, NX, WG>(ub, ob + (long long)(6 * c) * Gd.ocstride, Gd.ocstride, F, NX, I.z0,
+                                    I.z0 + I.nz, lds);
     }
 }
 
@@ -155,11 +167,23 @@ __global__ __launch_bounds__(WG) void k_batch(const BGrid *__restrict__ grids, c
 
 using namespace cnv;
 
+// An (operator, block edge) class run by k_bmarch.
+struct BClass {
+    int op = 0, edge = 0;
+    int wg = 64;
+    int zdepth = 8;
+    std::vector<int> grids; // indices into cnv_batch_s::src
+    std::vector<BItem> items;
+    BItem *d_items = nullptr;
+    size_t d_cap = 0;
+};
+
 struct cnv_batch_s {
     std::vector<cnv_batch_item> src;
     std::vector<BGrid> grids;
     std::vector<std::vector<uint64_t>> versions; // BC versions the descriptors were built from
     BGrid *d_grids = nullptr;
+    // persistent kernel over the items of no class
     BItem *d_items = nullptr;
     int nitems = 0;
     int wg = 256;
@@ -167,9 +191,108 @@ struct cnv_batch_s {
     size_t lds_bytes = 0;
     std::map<std::pair<int, int>, int> zdepth; // (edge, ghost) -> planes per item
     std::vector<BItem> items;
+    std::vector<BClass> classes;
 };
 
 static int class_key_edge(const cnv_grid_s *g) { return g->desc.block_cells[0]; }
+
+// Items of this grid go to an (operator, edge) class: cubic blocks of edge 16 or 32.
+static bool marchable(const cnv_batch_item &it)
+{
+    const int *n = it.in->desc.block_cells;
+    return n[0] == n[1] && n[1] == n[2] && (n[0] == 16 || n[0] == 32);
+}
+
+static const int kWgCand[3] = {64, 128, 256};
+
+template <int OP, int NX, int WG>
+static bool bm_launch1(const BClass &c, const BGrid *g, hipStream_t st)
+{
+    if constexpr (bm_valid<OP, NX, WG>()) {
+        hipLaunchKernelGGL((k_bmarch<OP, NX, WG>), dim3((unsigned)c.items.size()), dim3(WG), 0, st, g, c.d_items,
+                           (int)c.items.size());
+        return true;
+    }
+    return false;
+}
+
+template <int OP, int NX>
+static bool bm_launch_wg(const BClass &c, const BGrid *g, hipStream_t st)
+{
+    switch (c.wg) {
+    case 64:
+        return bm_launch1<OP, NX, 64>(c, g, st);
+    case 128:
+        return bm_launch1<OP, NX, 128>(c, g, st);
+    case 256:
+        return bm_launch1<OP, NX, 256>(c, g, st);
+    }
+    return false;
+}
+
+static bool bm_launch(const BClass &c, const BGrid *g, hipStream_t st)
+{
+    if (c.op == 0)
+        return c.edge == 16 ? bm_launch_wg<0, 16>(c, g, st) : bm_launch_wg<0, 32>(c, g, st);
+    return c.edge == 16 ? bm_launch_wg<1, 16>(c, g, st) : bm_launch_wg<1, 32>(c, g, st);
+}
+
+static bool bm_wg_valid(int op, int edge, int wg)
+{
+    if (op == 0)
+        return edge == 16 ? (wg == 64 ? bm_valid<0, 16, 64>() : wg == 128 ? bm_valid<0, 16, 128>() : bm_valid<0, 16, 256>())
+                          : (wg == 64 ? bm_valid<0, 32, 64>() : wg == 128 ? bm_valid<0, 32, 128>() : bm_valid<0, 32, 256>());
+    return edge == 16 ? (wg == 64 ? bm_valid<1, 16, 64>() : wg == 128 ? bm_valid<1, 16, 128>() : bm_valid<1, 16, 256>())
+                      : (wg == 64 ? bm_valid<1, 32, 64>() : wg == 128 ? bm_valid<1, 32, 128>() : bm_valid<1, 32, 256>());
+}
+
+// Default (workgroup, z depth) of a class: the single-grid kernels' choices.
+static void class_defaults(BClass &c)
+{
+    if (c.op == 0) {
+        c.wg = 64;
+        c.zdepth = 8;
+    } else {
+        c.wg = c.edge == 32 ? 256 : 64;
+        c.zdepth = c.edge;
+    }
+}
+
+static int upload_items(BItem **d, size_t *cap, const std::vector<BItem> &v)
+{
+    if (v.size() > *cap) {
+        if (*d)
+            CNV_HIP(hipFree(*d));
+        *d = nullptr;
+        *cap = 0;
+        CNV_HIP(hipMalloc((void **)d, sizeof(BItem) * v.size()));
+        *cap = v.size();
+    }
+    if (!v.empty())
+        CNV_HIP(hipMemcpy(*d, v.data(), sizeof(BItem) * v.size(), hipMemcpyHostToDevice));
+    return CNV_OK;
+}
+
+static int build_class_items(cnv_batch_s *b, BClass &c)
+{
+    c.items.clear();
+    const int nz = std::max(1, std::min(c.zdepth, c.edge));
+    for (int gi : c.grids) {
+        const cnv_grid_s *g = b->src[gi].in;
+        for (int comp = 0; comp < g->desc.ncomp; ++comp)
+            for (long long blk = 0; blk < g->nblocks_total; ++blk)
+                for (int z0 = 0; z0 < c.edge; z0 += nz) {
+                    BItem I;
+                    I.grid = gi;
+                    I.comp = comp;
+                    I.blk = (int)blk;
+                    I.z0 = (short)z0;
+                    I.nz = (short)std::min(nz, c.edge - z0);
+                    c.items.push_back(I);
+                }
+    }
+    return upload_items(&c.d_items, &c.d_cap, c.items);
+}
 
 static size_t tile_bytes(const cnv_grid_s *g, int op, int nz)
 {
@@ -180,13 +303,16 @@ static size_t tile_bytes(const cnv_grid_s *g, int op, int nz)
 
 static constexpr size_t kMaxLds = 64 * 1024;
 
-// Items for the current depth table, sorted by cost (cells x op weight).
+// Items of the persistent kernel for the current depth table, sorted by cost
+// (cells x op weight).  only: restrict to these (edge, ghost) keys.
 static int build_items(cnv_batch_s *b, const std::map<std::pair<int, int>, int> *only)
 {
     b->items.clear();
     b->lds_bytes = 0;
     for (size_t gi = 0; gi < b->src.size(); ++gi) {
         const cnv_batch_item &it = b->src[gi];
+        if (marchable(it))
+            continue;
         const cnv_grid_s *g = it.in;
         const std::pair<int, int> key(class_key_edge(g), it.ghost);
         if (only && !only->count(key))
@@ -223,6 +349,33 @@ static int build_items(cnv_batch_s *b, const std::map<std::pair<int, int>, int> 
     if (b->nitems > 0) {
         CNV_HIP(hipMalloc((void **)&b->d_items, sizeof(BItem) * b->items.size()));
         CNV_HIP(hipMemcpy(b->d_items, b->items.data(), sizeof(BItem) * b->items.size(), hipMemcpyHostToDevice));
+    }
+    return CNV_OK;
+}
+
+static int build_classes(cnv_batch_s *b)
+{
+    b->classes.clear();
+    std::map<std::pair<int, int>, size_t> at;
+    for (size_t gi = 0; gi < b->src.size(); ++gi) {
+        const cnv_batch_item &it = b->src[gi];
+        if (!marchable(it))
+            continue;
+        const std::pair<int, int> key(it.op, it.in->desc.block_cells[0]);
+        if (!at.count(key)) {
+            at[key] = b->classes.size();
+            BClass c;
+            c.op = key.first;
+            c.edge = key.second;
+            class_defaults(c);
+            b->classes.push_back(c);
+        }
+        b->classes[at[key]].grids.push_back((int)gi);
+    }
+    for (auto &c : b->classes) {
+        const int rc = build_class_items(b, c);
+        if (rc != CNV_OK)
+            return rc;
     }
     return CNV_OK;
 }
@@ -280,7 +433,7 @@ static int build_grids(cnv_batch_s *b)
     return CNV_OK;
 }
 
-static int launch_batch(cnv_batch_s *b, hipStream_t st)
+static int launch_generic(cnv_batch_s *b, hipStream_t st)
 {
     if (b->nitems == 0)
         return CNV_OK;
@@ -298,6 +451,26 @@ static int launch_batch(cnv_batch_s *b, hipStream_t st)
         hipLaunchKernelGGL(k_batch<256>, dim3(nwg), dim3(256), b->lds_bytes, st, b->d_grids, b->d_items, b->nitems);
     CNV_HIP(hipGetLastError());
     return CNV_OK;
+}
+
+static int launch_class(cnv_batch_s *b, const BClass &c, hipStream_t st)
+{
+    if (c.items.empty())
+        return CNV_OK;
+    if (!bm_launch(c, b->d_grids, st))
+        return fail(CNV_ERR_UNSUPPORTED, "no batched kernel for this class and workgroup size");
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
+
+static int launch_batch(cnv_batch_s *b, hipStream_t st)
+{
+    for (const auto &c : b->classes) {
+        const int rc = launch_class(b, c, st);
+        if (rc != CNV_OK)
+            return rc;
+    }
+    return launch_generic(b, st);
 }
 
 static bool stale(const cnv_batch_s *b)
@@ -345,6 +518,8 @@ int cnv_batch_create(const cnv_batch_item *items, int n, cnv_batch *out)
         b->src.assign(items, items + n);
         int rc = build_grids(b);
         if (rc == CNV_OK)
+            rc = build_classes(b);
+        if (rc == CNV_OK)
             rc = build_items(b, nullptr);
         if (rc != CNV_OK) {
             cnv_batch_destroy(b);
@@ -365,6 +540,9 @@ int cnv_batch_destroy(cnv_batch b)
         (void)hipFree(b->d_grids);
     if (b->d_items)
         (void)hipFree(b->d_items);
+    for (auto &c : b->classes)
+        if (c.d_items)
+            (void)hipFree(c.d_items);
     delete b;
     return CNV_OK;
 }
@@ -390,8 +568,12 @@ int cnv_batch_info(cnv_batch b, int *nitems, int *wg, size_t *lds_bytes)
 {
     if (!b)
         return fail(CNV_ERR_INVALID, "batch is NULL");
-    if (nitems)
-        *nitems = b->nitems;
+    if (nitems) {
+        long long n = b->nitems;
+        for (const auto &c : b->classes)
+            n += (long long)c.items.size();
+        *nitems = (int)n;
+    }
     if (wg)
         *wg = b->wg;
     if (lds_bytes)
@@ -399,10 +581,32 @@ int cnv_batch_info(cnv_batch b, int *nitems, int *wg, size_t *lds_bytes)
     return CNV_OK;
 }
 
-// For every (block edge, ghost width) class: time the class's items alone for
-// each z depth, keep the fastest; then time the whole batch for each
-// workgroup size.  results: {edge, ghost, depth, microseconds} per trial
-// (workgroup trials are reported with edge = 0, ghost = wg).
+int cnv_batch_class_info(cnv_batch b, int max_classes, int *out, int *nclasses)
+{
+    if (!b)
+        return fail(CNV_ERR_INVALID, "batch is NULL");
+    if (max_classes < 0 || (max_classes > 0 && !out))
+        return fail(CNV_ERR_INVALID, "out is NULL");
+    const int n = (int)b->classes.size();
+    for (int i = 0; i < n && i < max_classes; ++i) {
+        const BClass &c = b->classes[(size_t)i];
+        out[5 * i] = c.op;
+        out[5 * i + 1] = c.edge;
+        out[5 * i + 2] = c.wg;
+        out[5 * i + 3] = c.zdepth;
+        out[5 * i + 4] = (int)c.items.size();
+    }
+    if (nclasses)
+        *nclasses = n;
+    return CNV_OK;
+}
+
+// For every (operator, edge) class: time the class's launch alone for every
+// valid workgroup size and z depth, keep the fastest.  Then, for the items of
+// the persistent kernel: per (block edge, ghost width) the z depth, then the
+// workgroup size.  results: records of 5 doubles {op, edge, wg, depth,
+// microseconds}; persistent-kernel trials carry op = -1 and edge = -1 for
+// the workgroup trials.
 int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results, int *nresults)
 {
     if (!b)
@@ -415,13 +619,24 @@ int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results,
     CNV_HIP(hipEventCreate(&e0));
     CNV_HIP(hipEventCreate(&e1));
     int cnt = 0, rc = CNV_OK;
-    auto time_it = [&](float &us) -> int {
-        int r = launch_batch(b, st); // warm
+    auto record = [&](double op, double edge, double wg, double depth, double us) {
+        if (results && cnt < max_results) {
+            double *r = results + 5 * cnt;
+            r[0] = op;
+            r[1] = edge;
+            r[2] = wg;
+            r[3] = depth;
+            r[4] = us;
+        }
+        ++cnt;
+    };
+    auto time_it = [&](auto &&launch, float &us) -> int {
+        int r = launch(); // warm
         if (r != CNV_OK)
             return r;
         (void)hipEventRecord(e0, st);
         for (int i = 0; i < iters && r == CNV_OK; ++i)
-            r = launch_batch(b, st);
+            r = launch();
         (void)hipEventRecord(e1, st);
         if (hipEventSynchronize(e1) != hipSuccess)
             return fail(CNV_ERR_HIP, "autotune: batch kernel failed");
@@ -430,11 +645,43 @@ int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results,
         us = ms * 1000.f / iters;
         return r;
     };
-    std::map<std::pair<int, int>, int> classes;
+    for (auto &c : b->classes) {
+        if (rc != CNV_OK)
+            break;
+        float best = 1e30f;
+        int bwg = c.wg, bz = c.zdepth;
+        for (int wg : kWgCand) {
+            if (!bm_wg_valid(c.op, c.edge, wg))
+                continue;
+            for (int z = 2; z <= c.edge && rc == CNV_OK; z *= 2) {
+                c.wg = wg;
+                c.zdepth = z;
+                rc = build_class_items(b, c);
+                if (rc != CNV_OK)
+                    break;
+                float us = 0.f;
+                rc = time_it([&]() { return launch_class(b, c, st); }, us);
+                if (rc != CNV_OK)
+                    break;
+                record(c.op, c.edge, wg, z, us);
+                if (us < best) {
+                    best = us;
+                    bwg = wg;
+                    bz = z;
+                }
+            }
+        }
+        c.wg = bwg;
+        c.zdepth = bz;
+        if (rc == CNV_OK)
+            rc = build_class_items(b, c);
+    }
+    std::map<std::pair<int, int>, int> keys;
     for (const auto &it : b->src)
-        classes[{class_key_edge(it.in), it.ghost}] = 1;
+        if (!marchable(it))
+            keys[{class_key_edge(it.in), it.ghost}] = 1;
     b->wg = 256;
-    for (const auto &kv : classes) {
+    for (const auto &kv : keys) {
         const auto key = kv.first;
         const int edge = key.first;
         float best = 1e30f;
@@ -446,16 +693,10 @@ int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results,
             if (rc != CNV_OK)
                 break;
             float us = 0.f;
-            rc = time_it(us);
+            rc = time_it([&]() { return launch_generic(b, st); }, us);
             if (rc != CNV_OK)
                 break;
-            if (results && cnt < max_results) {
-                results[4 * cnt] = edge;
-                results[4 * cnt + 1] = key.second;
-                results[4 * cnt + 2] = z;
-                results[4 * cnt + 3] = us;
-            }
-            ++cnt;
+            record(-1, edge, key.second, z, us);
             if (us < best) {
                 best = us;
                 bestz = z;
@@ -465,28 +706,24 @@ int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results,
     }
     if (rc == CNV_OK)
         rc = build_items(b, nullptr);
-    float best = 1e30f;
-    int bestwg = 256;
-    const int wgs[3] = {128, 256, 512};
-    for (int i = 0; i < 3 && rc == CNV_OK; ++i) {
-        b->wg = wgs[i];
-        float us = 0.f;
-        rc = time_it(us);
-        if (rc != CNV_OK)
-            break;
-        if (results && cnt < max_results) {
-            results[4 * cnt] = 0;
-            results[4 * cnt + 1] = wgs[i];
-            results[4 * cnt + 2] = 0;
-            results[4 * cnt + 3] = us;
+    if (!keys.empty()) {
+        float best = 1e30f;
+        int bestwg = 256;
+        const int wgs[3] = {128, 256, 512};
+        for (int i = 0; i < 3 && rc == CNV_OK; ++i) {
+            b->wg = wgs[i];
+            float us = 0.f;
+            rc = time_it([&]() { return launch_generic(b, st); }, us);
+            if (rc != CNV_OK)
+                break;
+            record(-1, -1, wgs[i], 0, us);
+            if (us < best) {
+                best = us;
+                bestwg = wgs[i];
+            }
         }
-        ++cnt;
-        if (us < best) {
-            best = us;
-            bestwg = wgs[i];
-        }
+        b->wg = bestwg;
     }
-    b->wg = bestwg;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(st);

@@ -105,6 +105,38 @@ __device__ inline void block_faces(const Geom &G, const DevBCHead *heads, const 
     }
 }
 
+// Fixed-order sum of n partials by one workgroup: strided per-thread sums
+// issued eight loads at a time, a fixed wave tree, the waves in order.
+template <int WG>
+__device__ inline void wg_sum_partials(const double *__restrict__ p, int n, double *out, double *red)
+{
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    for (int base = 0; base < n; base += 8 * WG) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int i = base + j * WG + tid;
+            v[j] = i < n ? p[i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            s += v[j];
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        s += __shfl_xor(s, o, 64);
+    if ((tid & 63) == 0)
+        red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < WG / 64; ++w)
+            t += red[w];
+        *out = t;
+    }
+    __syncthreads();
+}
+
 // Workgroup i is dispatched to XCD i % 8; give each XCD a contiguous range
 // of tiles so that the tiles sharing halo lines meet in the same L2.
 __device__ inline int xcd_remap(int i, int n)

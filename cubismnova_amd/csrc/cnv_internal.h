@@ -163,6 +163,16 @@ int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in
 
 int launch_sums(cnv_grid_s *g, int comp, double *out, hipStream_t st);
 int launch_diffusion_run(cnv_grid_s *u, cnv_grid_s *tmp, int nsteps, double nu_dt, double *norms, hipStream_t st);
+// two Euler steps per launch on periodic grids (cnv_stencil7x2.hip)
+bool x2_applicable(cnv_grid_s *u);
+long long x2_tiles(const cnv_grid_s *u, int zchunk);
+int launch_diffusion_x2(cnv_grid_s *u, cnv_grid_s *out, double nu_dt, double *part1, double *part2,
+                        const double *const prev_p[2], const int prev_n[2], double *const prev_out[2], int wg,
+                        int zchunk, int *nparts, hipStream_t st);
+int x2_set_tune(int dtype, int edge, int wg, int zchunk);
+int x2_get_tune(int dtype, int edge, int *wg, int *zchunk);
+int autotune_x2(cnv_grid_s *u, cnv_grid_s *out, int iters, int *best_wg, int *best_z, double *results,
+                int max_results, int *nresults);
 
 int autotune_stencil7(cnv_grid_s *u, cnv_grid_s *out, int iters, int *best_wg, int *best_z, double *results,
                       int max_results, int *nresults);

@@ -14,7 +14,7 @@
 // u + that (Block/Field.h:457-512, FieldOperator.h:43-114).  The library is
 // compiled with -ffp-contract=off so no step is fused into an FMA and FP64
 // results are bit-identical to the CPU evaluation of the same order.
-#include "cnv_device.h"
+#include "cnv_march.h"
 
 #include <cstdlib>
 #include <map>
@@ -43,209 +43,12 @@ struct S7Args {
     double *partials;
     double *sumsq;
     // deferred norm of the previous step of a multi-step run: workgroup 0
-    // sums prev_n partials (written by the previous launch) into *prev_out
-    const double *prev_partials;
-    int prev_n;
-    double *prev_out;
+    // sums prev_n[k] partials (written by the previous launch) into
+    // *prev_out[k], k = 0, 1 (unused sets have prev_p[k] == nullptr)
+    const double *prev_p[2];
+    int prev_n[2];
+    double *prev_out[2];
 };
-
-// Fixed-order sum of n partials by one workgroup: strided per-thread sums
-// issued eight loads at a time, a fixed wave tree, the waves in order.
-template <int WG>
-__device__ inline void wg_sum_partials(const double *__restrict__ p, int n, double *out, double *red)
-{
-    const int tid = threadIdx.x;
-    double s = 0.0;
-    for (int base = 0; base < n; base += 8 * WG) {
-        double v[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int i = base + j * WG + tid;
-            v[j] = i < n ? p[i] : 0.0;
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            s += v[j];
-    }
-    for (int o = 32; o > 0; o >>= 1)
-        s += __shfl_xor(s, o, 64);
-    if ((tid & 63) == 0)
-        red[tid >> 6] = s;
-    __syncthreads();
-    if (tid == 0) {
-        double t = 0.0;
-        for (int w = 0; w < WG / 64; ++w)
-            t += red[w];
-        *out = t;
-    }
-    __syncthreads();
-}
-
-#ifndef S7_PF
-#define S7_PF 2 // planes loaded ahead of the plane being computed
-#endif
-
-// Tile geometry of the z-marching 7-point kernels for an NX x NY plane and
-// WG threads: VEC-wide rows, R rows per thread, LDS planes of pitch P.
-template <typename T, int NX, int NY, int WG>
-struct S7Shape {
-    static constexpr int VEC = 16 / (int)sizeof(T);
-    static constexpr int TX = NX / VEC;  // threads per row
-    static constexpr int ROWS = WG / TX; // rows per pass
-    static constexpr int R = NY / ROWS;  // rows per thread
-    static constexpr int P = NX + 2 * VEC; // LDS row pitch (interior starts at VEC)
-    static constexpr int PLANE = (NY + 2) * P;
-    static constexpr bool valid = NX % VEC == 0 && WG % TX == 0 && ROWS <= NY && NY % ROWS == 0 &&
-                                  WG >= 2 * NY && WG >= 2 * TX;
-};
-
-// One tile of the fused 7-point kernel: planes [z0, z1) of the block at `ub`
-// (NZ planes), one-layer faces F, output at `ob` (and the Laplacian at `lb`
-// for MODE 2).  `lds` holds two planes (2 * PLANE elements).  Returns this
-// thread's FP64 sum of u_new^2 (SUMSQ).
-template <typename T, int NX, int NY, int WG, int MODE, bool SUMSQ>
-__device__ inline double s7_march(const T *__restrict__ ub, T *__restrict__ ob, T *__restrict__ lb,
-                                  const FaceSrc<T> F[6], int NZ, int z0, int z1, T ihx2, T ihy2, T ihz2, T nudt,
-                                  T *lds)
-{
-    using S = S7Shape<T, NX, NY, WG>;
-    constexpr int VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, P = S::P, PLANE = S::PLANE;
-    static_assert(S::valid, "tile shape");
-    using V = VecT<T, VEC>;
-    const int tid = threadIdx.x;
-    const int tx = tid % TX, ty = tid / TX;
-    const int x0 = tx * VEC;
-    constexpr long long SXY = (long long)NX * NY;
-
-    // halo roles: x-ghost scalar (tid < 2NY), y-ghost vector (tid < 2TX);
-    // faces are picked by value so no array is indexed at run time
-    const bool hx_on = tid < 2 * NY;
-    const int hx_side = tid / NY, hx_y = tid % NY;
-    const bool hy_on = tid < 2 * TX;
-    const int hy_side = tid / TX, hy_x = (tid % TX) * VEC;
-    const FaceSrc<T> FX = hx_side ? F[1] : F[0];
-    const FaceSrc<T> FY = hy_side ? F[3] : F[2];
-    const FaceSrc<T> FZ0 = F[4], FZ1 = F[5];
-
-    struct Plane {
-        V c[R];
-        T hx;
-        V hy;
-    };
-
-    auto load_plane = [&](int p, bool halo, Plane &P_) {
-        if (p >= 0 && p < NZ) {
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                P_.c[r] = *reinterpret_cast<const V *>(ub + x0 + NX * (ty + r * ROWS) + SXY * p);
-            if (halo) {
-                if (hx_on)
-                    P_.hx = face_val(FX, (long long)NX * hx_y + SXY * p);
-                if (hy_on) {
-                    const FaceSrc<T> f = FY;
-                    if (f.mode == 2) {
-#pragma unroll
-                        for (int v = 0; v < VEC; ++v)
-                            P_.hy.v[v] = f.s;
-                    } else {
-                        P_.hy = *reinterpret_cast<const V *>(f.p + hy_x + SXY * p);
-                        if (f.mode == 1)
-#pragma unroll
-                            for (int v = 0; v < VEC; ++v)
-                                P_.hy.v[v] = f.s * P_.hy.v[v];
-                    }
-                }
-            }
-        } else {
-            const FaceSrc<T> f = p < 0 ? FZ0 : FZ1;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if (f.mode == 2) {
-#pragma unroll
-                    for (int v = 0; v < VEC; ++v)
-                        P_.c[r].v[v] = f.s;
-                } else {
-                    P_.c[r] = *reinterpret_cast<const V *>(f.p + x0 + NX * (ty + r * ROWS));
-                    if (f.mode == 1)
-#pragma unroll
-                        for (int v = 0; v < VEC; ++v)
-                            P_.c[r].v[v] = f.s * P_.c[r].v[v];
-                }
-            }
-        }
-    };
-
-    double acc = 0.0;
-
-    // Ring of NQ planes in registers: plane p lives in slot (p - z0 + 1) % NQ.
-    // The z loop is unrolled by NQ so every slot index is a constant (no
-    // register moves); plane z + S7_PF is loaded while plane z is computed.
-    constexpr int NQ = S7_PF + 2;
-    Plane Q[NQ];
-#pragma unroll
-    for (int k = 0; k <= S7_PF; ++k)
-        if (z0 - 1 + k <= z1)
-            load_plane(z0 - 1 + k, k >= 1 && z0 - 1 + k < z1, Q[k]);
-
-    for (int zb = z0; zb < z1; zb += NQ) {
-#pragma unroll
-        for (int u = 0; u < NQ; ++u) {
-            const int z = zb + u;
-            if (z >= z1)
-                break;
-            const Plane &pm = Q[u % NQ];
-            const Plane &pc = Q[(u + 1) % NQ];
-            const Plane &pn = Q[(u + 2) % NQ];
-            if (z + S7_PF <= z1)
-                load_plane(z + S7_PF, z + S7_PF < z1, Q[(u + NQ - 1) % NQ]);
-            T *L = lds + ((z - z0) & 1) * PLANE;
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                *reinterpret_cast<V *>(L + (ty + r * ROWS + 1) * P + VEC + x0) = pc.c[r];
-            if (hx_on)
-                L[(hx_y + 1) * P + (hx_side ? VEC + NX : VEC - 1)] = pc.hx;
-            if (hy_on)
-                *reinterpret_cast<V *>(L + (hy_side ? NY + 1 : 0) * P + VEC + hy_x) = pc.hy;
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int y = ty + r * ROWS;
-                const T *row = L + (y + 1) * P + VEC;
-                const V ym = *reinterpret_cast<const V *>(row - P + x0);
-                const V yp = *reinterpret_cast<const V *>(row + P + x0);
-                const T xl = row[x0 - 1];
-                const T xr = row[x0 + VEC];
-                V res, lapv;
-#pragma unroll
-                for (int v = 0; v < VEC; ++v) {
-                    const T c = pc.c[r].v[v];
-                    const T xm = v == 0 ? xl : pc.c[r].v[v - 1];
-                    const T xp = v == VEC - 1 ? xr : pc.c[r].v[v + 1];
-                    const T ddx = ihx2 * (xp - 2 * c + xm);
-                    const T ddy = ihy2 * (yp.v[v] - 2 * c + ym.v[v]);
-                    const T ddz = ihz2 * (pn.c[r].v[v] - 2 * c + pm.c[r].v[v]);
-                    const T lap = ddx + ddy + ddz;
-                    lapv.v[v] = lap;
-                    if (MODE == 0) {
-                        res.v[v] = lap;
-                    } else {
-                        const T l = lap * nudt;
-                        const T un = c + l;
-                        res.v[v] = un;
-                        if (SUMSQ) // one rounding of un^2 + acc
-                            acc = __builtin_fma((double)un, (double)un, acc);
-                    }
-                }
-                const long long off = x0 + NX * y + SXY * z;
-                *reinterpret_cast<V *>(ob + off) = res;
-                if (MODE == 2)
-                    *reinterpret_cast<V *>(lb + off) = lapv;
-            }
-        }
-    }
-
-    return acc;
-}
 
 template <typename T, int NX, int NY, int WG, int MODE, bool SUMSQ>
 __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
@@ -255,8 +58,10 @@ __global__ __launch_bounds__(WG) void k_stencil7(const S7Args a)
     __shared__ double red[WG / 64];
 
     const int tid = threadIdx.x;
-    if (a.prev_partials && blockIdx.x == 0)
-        wg_sum_partials<WG>(a.prev_partials, a.prev_n, a.prev_out, red);
+    if (blockIdx.x == 0)
+        for (int k = 0; k < 2; ++k)
+            if (a.prev_p[k])
+                wg_sum_partials<WG>(a.prev_p[k], a.prev_n[k], a.prev_out[k], red);
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int comp = tile / a.tiles_per_comp;
     const int tt = tile - comp * a.tiles_per_comp;
@@ -335,8 +140,10 @@ __global__ __launch_bounds__(256) void k_stencil7_generic(const S7Args a)
 {
     __shared__ double red[4];
     const int tid = threadIdx.x;
-    if (a.prev_partials && blockIdx.x == 0 && blockIdx.y == 0)
-        wg_sum_partials<256>(a.prev_partials, a.prev_n, a.prev_out, red);
+    if (blockIdx.x == 0 && blockIdx.y == 0)
+        for (int k = 0; k < 2; ++k)
+            if (a.prev_p[k])
+                wg_sum_partials<256>(a.prev_p[k], a.prev_n[k], a.prev_out[k], red);
     const long long nb_tot = (long long)a.G.nb[0] * a.G.nb[1] * a.G.nb[2];
     const long long slot = blockIdx.y; // comp * nblocks + blk
     const int comp = (int)(slot / nb_tot);
@@ -559,12 +366,13 @@ static int check_faces(cnv_grid_s *g, int comp)
 
 // State of one launch inside a multi-step run (cnv_diffusion_run).
 struct S7Run {
-    double *partials;     // this launch's partial buffer
-    const double *prev;   // the previous launch's partials, summed by workgroup 0
-    int prev_n;
-    double *prev_out;
-    int *nparts;          // out: partials written by this launch
-    size_t cap;           // capacity of each half
+    double *partials;          // this launch's partial buffer (first set)
+    double *partials2;         // second set (two-step launches)
+    const double *prev[2];     // the previous launch's partial sets, summed by workgroup 0
+    int prev_n[2];
+    double *prev_out[2];
+    int *nparts;               // out: partials written per set by this launch
+    size_t cap;                // capacity of each buffer half
 };
 
 static int launch_stencil7_t(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
@@ -609,9 +417,11 @@ static int launch_stencil7_t(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, in
     a.nudt = nu_dt;
     a.sumsq = sumsq;
     a.partials = nullptr;
-    a.prev_partials = run ? run->prev : nullptr;
-    a.prev_n = run ? run->prev_n : 0;
-    a.prev_out = run ? run->prev_out : nullptr;
+    for (int k = 0; k < 2; ++k) {
+        a.prev_p[k] = run ? run->prev[k] : nullptr;
+        a.prev_n[k] = run ? run->prev_n[k] : 0;
+        a.prev_out[k] = run ? run->prev_out[k] : nullptr;
+    }
     a.zchunk = a.nzc = a.tiles_per_comp = 0;
     if (sumsq) {
         const long long nbt = u->nblocks_total;
@@ -656,33 +466,74 @@ int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in
     return launch_stencil7_t(u, out, lap, comp_in, comp_out, ncomp, mode, nu_dt, sumsq, st, nullptr);
 }
 
-// nsteps Euler steps ping-ponging u -> tmp -> u ...; the norm of step k is
-// reduced by workgroup 0 of step k+1 from the other half of u's partial
-// buffer, the last one by k_finalize_sum.
+// nsteps Euler steps alternating u -> tmp -> u ...  On periodic grids of
+// 16^3 or 32^3 blocks an even number of two-step launches covers the steps
+// up to a multiple of four, single steps the rest (so the field ends in u
+// for even nsteps, in tmp for odd).  The norms of a launch are reduced by
+// workgroup 0 of the next launch from the other half of u's partial buffer,
+// those of the last launch by k_finalize_sum.
 int launch_diffusion_run(cnv_grid_s *u, cnv_grid_s *tmp, int nsteps, double nu_dt, double *norms, hipStream_t st)
 {
     double *P[2] = {u->partials, u->partials + u->partials_cap};
-    int prev_n = 0;
-    for (int k = 0; k < nsteps; ++k) {
-        cnv_grid_s *src = (k % 2 == 0) ? u : tmp;
-        cnv_grid_s *dst = (k % 2 == 0) ? tmp : u;
-        S7Run run;
-        run.partials = P[k % 2];
-        run.prev = (norms && k > 0) ? P[(k + 1) % 2] : nullptr;
-        run.prev_n = prev_n;
-        run.prev_out = (norms && k > 0) ? norms + (k - 1) : nullptr;
-        run.nparts = &prev_n;
-        run.cap = u->partials_cap;
-        const int rc = launch_stencil7_t(src, dst, nullptr, 0, 0, u->desc.ncomp, 1, nu_dt, norms ? norms + k : nullptr,
-                                         st, nullptr, &run);
+    const size_t cap = u->partials_cap;
+    int nfused = 0, xwg = 0, xz = 0;
+    if (nsteps >= 4 && !force_generic() && x2_applicable(u) && x2_applicable(tmp)) {
+        x2_get_tune(u->desc.dtype, u->desc.block_cells[0], &xwg, &xz);
+        if (2 * x2_tiles(u, xz) <= (long long)cap)
+            nfused = 2 * (nsteps / 4);
+    }
+    const double *prev[2] = {nullptr, nullptr};
+    int prev_n[2] = {0, 0};
+    double *prev_out[2] = {nullptr, nullptr};
+    int k = 0, L = 0; // steps and launches done
+    auto grid_at = [&](int l) { return l % 2 == 0 ? u : tmp; };
+    for (int j = 0; j < nfused; ++j, ++L) {
+        double *p1 = norms ? P[L % 2] : nullptr;
+        double *p2 = norms ? p1 + x2_tiles(u, xz) : nullptr;
+        int nt = 0;
+        const int rc = launch_diffusion_x2(grid_at(L), grid_at(L + 1), nu_dt, p1, p2, prev, prev_n, prev_out, xwg, xz,
+                                           &nt, st);
         if (rc != CNV_OK)
             return rc;
+        if (norms) {
+            prev[0] = p1;
+            prev[1] = p2;
+            prev_n[0] = prev_n[1] = nt;
+            prev_out[0] = norms + k;
+            prev_out[1] = norms + k + 1;
+        }
+        k += 2;
     }
-    if (norms && nsteps > 0) {
-        hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, (const double *)P[(nsteps - 1) % 2], prev_n,
-                           norms + (nsteps - 1));
-        CNV_HIP(hipGetLastError());
+    for (; k < nsteps; ++k, ++L) {
+        S7Run run;
+        run.partials = P[L % 2];
+        run.partials2 = nullptr;
+        for (int s = 0; s < 2; ++s) {
+            run.prev[s] = prev[s];
+            run.prev_n[s] = prev_n[s];
+            run.prev_out[s] = prev_out[s];
+        }
+        int nt = 0;
+        run.nparts = &nt;
+        run.cap = cap;
+        const int rc = launch_stencil7_t(grid_at(L), grid_at(L + 1), nullptr, 0, 0, u->desc.ncomp, 1, nu_dt,
+                                         norms ? norms + k : nullptr, st, nullptr, &run);
+        if (rc != CNV_OK)
+            return rc;
+        if (norms) {
+            prev[0] = P[L % 2];
+            prev[1] = nullptr;
+            prev_n[0] = nt;
+            prev_n[1] = 0;
+            prev_out[0] = norms + k;
+            prev_out[1] = nullptr;
+        }
     }
+    for (int s = 0; s < 2; ++s)
+        if (prev[s]) {
+            hipLaunchKernelGGL(k_finalize_sum, dim3(1), dim3(1024), 0, st, prev[s], prev_n[s], prev_out[s]);
+            CNV_HIP(hipGetLastError());
+        }
     return CNV_OK;
 }
 

@@ -1,0 +1,512 @@
+// cnv_stencil7x2.hip -- two forward-Euler diffusion steps per launch on
+// periodic grids (every face of every block a neighbour copy, no BC list).
+//
+// A workgroup owns the x-y planes [z0, z1) of one block of one component.
+// It stages the planes of u^n it needs, each with a two-cell x/y halo from
+// the neighbour blocks (periodic wrap, the four diagonal cells included), in
+// a rolling set of three LDS planes; evaluates u^{n+1} on the block's cells
+// and on the one-cell ring around them (the face-neighbour cells that the
+// second step reads), and u^{n+2} on the block's cells.  Each cell of
+// u^{n+1} and u^{n+2} is computed with the operation order of the one-step
+// kernel (Laplacian.cpp:125-129, then the two Field.h:457-512 roundings), so
+// the result is bit-identical to two launches of it, at one HBM read and one
+// write for two steps.  The FP64 sums of squares of both steps are produced
+// as per-tile partials.
+#include "cnv_march.h"
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+namespace cnv
+{
+
+struct X2Args {
+    const char *u; // input grid base
+    char *out;     // output grid base
+    Geom G;
+    int nzc, zchunk, tiles_per_comp;
+    double ih2[3];
+    double nudt;
+    double *part1, *part2; // per-tile partials of |u^{n+1}|^2, |u^{n+2}|^2 (or null)
+    // deferred sums of the previous launch of a run (as in the one-step kernel)
+    const double *prev_p[2];
+    int prev_n[2];
+    double *prev_out[2];
+};
+
+template <typename T, int N, int WG>
+struct X2Shape {
+    static constexpr int VEC = 16 / (int)sizeof(T);
+    static constexpr int TX = N / VEC;
+    static constexpr int ROWS = WG / TX;
+    static constexpr int R = N / ROWS;
+    static constexpr int PA = N + 2 * VEC;   // row pitch; x = -VEC .. N+VEC-1
+    static constexpr int PLA = (N + 4) * PA; // u^n plane, y = -2 .. N+1
+    static constexpr int PLB = (N + 2) * PA; // u^{n+1} plane, y = -1 .. N
+    static constexpr int NHX = 4 * N;        // x halo: 2 layers x 2 sides x N rows (scalars)
+    static constexpr int NHY = 4 * TX;       // y halo: 2 layers x 2 sides x TX vectors
+    static constexpr int NH = NHX + NHY + 4; // + the four diagonal cells
+    static constexpr int HS = (NH + WG - 1) / WG;
+    static constexpr int NR = 4 * N; // ring cells of u^{n+1} per plane
+    static constexpr int RS = (NR + WG - 1) / WG;
+    // shapes with more than two rows per thread run out of registers
+    static constexpr bool valid =
+        N % VEC == 0 && WG % TX == 0 && ROWS <= N && N % ROWS == 0 && VEC >= 2 && R <= 2 && HS <= 2;
+};
+
+template <typename T, int N, int WG>
+__global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
+{
+    using S = X2Shape<T, N, WG>;
+    constexpr int VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, PA = S::PA, PLA = S::PLA, PLB = S::PLB;
+    constexpr int NHX = S::NHX, NHY = S::NHY, NH = S::NH, HS = S::HS, NR = S::NR, RS = S::RS;
+    static_assert(S::valid, "tile shape");
+    using V = VecT<T, VEC>;
+    constexpr long long NN = (long long)N * N;
+
+    __shared__ alignas(16) T sA[3 * PLA];
+    __shared__ alignas(16) T sB[2 * PLB];
+    __shared__ double red[WG / 64];
+
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0)
+        for (int k = 0; k < 2; ++k)
+            if (a.prev_p[k])
+                wg_sum_partials<WG>(a.prev_p[k], a.prev_n[k], a.prev_out[k], red);
+
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int comp = tile / a.tiles_per_comp;
+    const int tt = tile - comp * a.tiles_per_comp;
+    const long long blk = tt / a.nzc;
+    const int zc = tt - (int)blk * a.nzc;
+    const int z0 = zc * a.zchunk;
+    const int z1 = min(N, z0 + a.zchunk);
+    const int nbx = a.G.nb[0], nby = a.G.nb[1], nbz = a.G.nb[2];
+    const int bx = (int)(blk % nbx), by = (int)((blk / nbx) % nby), bz = (int)(blk / ((long long)nbx * nby));
+    const T *cbase = reinterpret_cast<const T *>(a.u) + (long long)comp * a.G.cstride;
+    T *ob = reinterpret_cast<T *>(a.out) + (long long)comp * a.G.cstride + blk * a.G.bstride;
+    const long long bstride = a.G.bstride;
+
+    // base of block (bx+dx, by+dy, bz+dz), periodic in every direction
+    auto bptr = [&](int dx, int dy, int dz) -> const T * {
+        int x = bx + dx, y = by + dy, z = bz + dz;
+        x = x < 0 ? x + nbx : (x >= nbx ? x - nbx : x);
+        y = y < 0 ? y + nby : (y >= nby ? y - nby : y);
+        z = z < 0 ? z + nbz : (z >= nbz ? z - nbz : z);
+        return cbase + ((long long)x + (long long)nbx * (y + (long long)nby * z)) * bstride;
+    };
+
+    const int tx = tid % TX, ty = tid / TX;
+    const int x0 = tx * VEC;
+
+    struct U0 {
+        V c[R];
+        V h[HS];
+    };
+    // plane p (z0-2 <= p <= z1+1, possibly in the z-neighbour block) into registers
+    auto load_u0 = [&](int p) -> U0 {
+        U0 Q = {};
+        const int dz = p < 0 ? -1 : (p >= N ? 1 : 0);
+        const long long lp = (long long)(p - dz * N) * NN;
+        const T *own = bptr(0, 0, dz) + lp;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            Q.c[r] = *reinterpret_cast<const V *>(own + x0 + N * (ty + r * ROWS));
+#pragma unroll
+        for (int h = 0; h < HS; ++h) {
+            const int s = tid + h * WG;
+            if (s < NHX) {
+                const int side = s / (2 * N), layer = (s / N) & 1, y = s % N;
+                const T *src = bptr(side ? 1 : -1, 0, dz) + lp;
+                V t = {};
+                t.v[0] = src[(side ? layer : N - 1 - layer) + N * y];
+                Q.h[h] = t;
+            } else if (s < NHX + NHY) {
+                const int s2 = s - NHX;
+                const int side = s2 / (2 * TX), layer = (s2 / TX) & 1, xs = (s2 % TX) * VEC;
+                const T *src = bptr(0, side ? 1 : -1, dz) + lp;
+                Q.h[h] = *reinterpret_cast<const V *>(src + xs + N * (side ? layer : N - 1 - layer));
+            } else if (s < NH) {
+                const int d = s - NHX - NHY, sx = d & 1, sy = d >> 1;
+                const T *src = bptr(sx ? 1 : -1, sy ? 1 : -1, dz) + lp;
+                V t = {};
+                t.v[0] = src[(sx ? 0 : N - 1) + N * (sy ? 0 : N - 1)];
+                Q.h[h] = t;
+            }
+        }
+        return Q;
+    };
+    // registers -> LDS plane A, A(x, y) = A[(y + 2) * PA + VEC + x]
+    auto store_u0 = [&](const U0 &Q, T *A) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            *reinterpret_cast<V *>(A + (ty + r * ROWS + 2) * PA + VEC + x0) = Q.c[r];
+#pragma unroll
+        for (int h = 0; h < HS; ++h) {
+            const int s = tid + h * WG;
+            if (s < NHX) {
+                const int side = s / (2 * N), layer = (s / N) & 1, y = s % N;
+                const int x = side ? N + layer : -1 - layer;
+                A[(y + 2) * PA + VEC + x] = Q.h[h].v[0];
+            } else if (s < NHX + NHY) {
+                const int s2 = s - NHX;
+                const int side = s2 / (2 * TX), layer = (s2 / TX) & 1, xs = (s2 % TX) * VEC;
+                const int y = side ? N + layer : -1 - layer;
+                *reinterpret_cast<V *>(A + (y + 2) * PA + VEC + xs) = Q.h[h];
+            } else if (s < NH) {
+                const int d = s - NHX - NHY, sx = d & 1, sy = d >> 1;
+                A[((sy ? N : -1) + 2) * PA + VEC + (sx ? N : -1)] = Q.h[h].v[0];
+            }
+        }
+    };
+
+    const T ihx2 = (T)a.ih2[0], ihy2 = (T)a.ih2[1], ihz2 = (T)a.ih2[2];
+    const T nudt = (T)a.nudt;
+    // one Euler step of one cell, in the order of the one-step kernel
+    auto euler = [&](T c, T xm, T xp, T ym, T yp, T zm, T zp) -> T {
+        const T ddx = ihx2 * (xp - 2 * c + xm);
+        const T ddy = ihy2 * (yp - 2 * c + ym);
+        const T ddz = ihz2 * (zp - 2 * c + zm);
+        const T lap = ddx + ddy + ddz;
+        const T l = lap * nudt;
+        return c + l;
+    };
+    auto slotA = [&](int p) { return sA + ((p - z0 + 3) % 3) * PLA; }; // p >= z0 - 2
+    auto slotB = [&](int q) { return sB + ((q - z0 + 2) & 1) * PLB; }; // q >= z0 - 1
+
+    store_u0(load_u0(z0 - 2), slotA(z0 - 2));
+    store_u0(load_u0(z0 - 1), slotA(z0 - 1));
+    U0 Pq = load_u0(z0);
+
+    V m1[R], c1[R], n1[R]; // u^{n+1} on this thread's cells of planes q-2, q-1, q
+    double acc1 = 0.0, acc2 = 0.0;
+    for (int q = z0 - 1; q <= z1; ++q) {
+        store_u0(Pq, slotA(q + 1));
+        if (q + 2 <= z1 + 1)
+            Pq = load_u0(q + 2);
+        __syncthreads();
+        const T *Am = slotA(q - 1), *Ac = slotA(q), *Ap = slotA(q + 1);
+        const bool inner = q >= z0 && q < z1;
+        T *Bq = slotB(q);
+        // u^{n+1} on the block's cells of plane q
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int y = ty + r * ROWS;
+            const int o = (y + 2) * PA + VEC + x0;
+            const V c = *reinterpret_cast<const V *>(Ac + o);
+            const V ym = *reinterpret_cast<const V *>(Ac + o - PA);
+            const V yp = *reinterpret_cast<const V *>(Ac + o + PA);
+            const V zm = *reinterpret_cast<const V *>(Am + o);
+            const V zp = *reinterpret_cast<const V *>(Ap + o);
+            const T xl = Ac[o - 1], xr = Ac[o + VEC];
+            V res;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+                const T xm = v == 0 ? xl : c.v[v - 1];
+                const T xp = v == VEC - 1 ? xr : c.v[v + 1];
+                res.v[v] = euler(c.v[v], xm, xp, ym.v[v], yp.v[v], zm.v[v], zp.v[v]);
+                if (inner)
+                    acc1 = __builtin_fma((double)res.v[v], (double)res.v[v], acc1);
+            }
+            n1[r] = res;
+            if (inner)
+                *reinterpret_cast<V *>(Bq + (y + 1) * PA + VEC + x0) = res;
+        }
+        // u^{n+1} on the ring of face-neighbour cells (planes the second step computes)
+        if (inner) {
+#pragma unroll
+            for (int h = 0; h < RS; ++h) {
+                const int s = tid + h * WG;
+                if (s < NR) {
+                    int x, y;
+                    if (s < 2 * N) {
+                        x = (s / N) ? N : -1;
+                        y = s % N;
+                    } else {
+                        y = ((s - 2 * N) / N) ? N : -1;
+                        x = (s - 2 * N) % N;
+                    }
+                    const int o = (y + 2) * PA + VEC + x;
+                    Bq[(y + 1) * PA + VEC + x] =
+                        euler(Ac[o], Ac[o - 1], Ac[o + 1], Ac[o - PA], Ac[o + PA], Am[o], Ap[o]);
+                }
+            }
+        }
+        __syncthreads();
+        // u^{n+2} on the block's cells of plane q - 1
+        if (q >= z0 + 1) {
+            const T *Br = slotB(q - 1);
+            T *op = ob + NN * (q - 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int y = ty + r * ROWS;
+                const int o = (y + 1) * PA + VEC + x0;
+                const V ym = *reinterpret_cast<const V *>(Br + o - PA);
+                const V yp = *reinterpret_cast<const V *>(Br + o + PA);
+                const T xl = Br[o - 1], xr = Br[o + VEC];
+                V res;
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    const T xm = v == 0 ? xl : c1[r].v[v - 1];
+                    const T xp = v == VEC - 1 ? xr : c1[r].v[v + 1];
+                    res.v[v] = euler(c1[r].v[v], xm, xp, ym.v[v], yp.v[v], m1[r].v[v], n1[r].v[v]);
+                    acc2 = __builtin_fma((double)res.v[v], (double)res.v[v], acc2);
+                }
+                *reinterpret_cast<V *>(op + x0 + N * y) = res;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            m1[r] = c1[r];
+            c1[r] = n1[r];
+        }
+    }
+
+    // per-tile partials in a fixed wave tree and wave order
+    auto tile_sum = [&](double v, double *part) {
+        for (int o = 32; o > 0; o >>= 1)
+            v += __shfl_xor(v, o, 64);
+        __syncthreads();
+        if ((tid & 63) == 0)
+            red[tid >> 6] = v;
+        __syncthreads();
+        if (tid == 0 && part) {
+            double t = 0.0;
+            for (int w = 0; w < WG / 64; ++w)
+                t += red[w];
+            part[tile] = t;
+        }
+    };
+    tile_sum(acc1, a.part1);
+    tile_sum(acc2, a.part2);
+}
+
+// ---------------------------------------------------------------- host side
+
+namespace
+{
+
+struct X2Tune {
+    int wg, zchunk;
+};
+std::mutex g_x2_mtx;
+std::map<std::pair<int, int>, X2Tune> g_x2_tune; // (dtype, edge) -> choice
+
+X2Tune x2_default(int dtype, int edge)
+{
+    if (edge == 32)
+        return dtype == CNV_F64 ? X2Tune{256, 16} : X2Tune{256, 16};
+    return dtype == CNV_F64 ? X2Tune{128, 16} : X2Tune{64, 16};
+}
+
+X2Tune x2_tune(int dtype, int edge)
+{
+    std::lock_guard<std::mutex> lk(g_x2_mtx);
+    auto it = g_x2_tune.find({dtype, edge});
+    return it == g_x2_tune.end() ? x2_default(dtype, edge) : it->second;
+}
+
+template <typename T, int N, int WG>
+bool x2_launch1(const X2Args &a, int ntiles, hipStream_t st)
+{
+    if constexpr (X2Shape<T, N, WG>::valid) {
+        hipLaunchKernelGGL((k_stencil7x2<T, N, WG>), dim3(ntiles), dim3(WG), 0, st, a);
+        return true;
+    }
+    return false;
+}
+
+template <typename T, int N>
+bool x2_dispatch_wg(const X2Args &a, int wg, int ntiles, hipStream_t st)
+{
+    switch (wg) {
+    case 64:
+        return x2_launch1<T, N, 64>(a, ntiles, st);
+    case 128:
+        return x2_launch1<T, N, 128>(a, ntiles, st);
+    case 256:
+        return x2_launch1<T, N, 256>(a, ntiles, st);
+    case 512:
+        return x2_launch1<T, N, 512>(a, ntiles, st);
+    }
+    return false;
+}
+
+template <typename T, int N>
+bool x2_valid_wg(int wg)
+{
+    switch (wg) {
+    case 64:
+        return X2Shape<T, N, 64>::valid;
+    case 128:
+        return X2Shape<T, N, 128>::valid;
+    case 256:
+        return X2Shape<T, N, 256>::valid;
+    case 512:
+        return X2Shape<T, N, 512>::valid;
+    }
+    return false;
+}
+
+bool x2_valid(int dtype, int edge, int wg)
+{
+    if (dtype == CNV_F64)
+        return edge == 16 ? x2_valid_wg<double, 16>(wg) : x2_valid_wg<double, 32>(wg);
+    return edge == 16 ? x2_valid_wg<float, 16>(wg) : x2_valid_wg<float, 32>(wg);
+}
+
+bool x2_dispatch(int dtype, int edge, int wg, const X2Args &a, int ntiles, hipStream_t st)
+{
+    if (dtype == CNV_F64)
+        return edge == 16 ? x2_dispatch_wg<double, 16>(a, wg, ntiles, st) : x2_dispatch_wg<double, 32>(a, wg, ntiles, st);
+    return edge == 16 ? x2_dispatch_wg<float, 16>(a, wg, ntiles, st) : x2_dispatch_wg<float, 32>(a, wg, ntiles, st);
+}
+
+} // namespace
+
+bool x2_applicable(cnv_grid_s *u)
+{
+    const int *n = u->desc.block_cells;
+    if (!(n[0] == n[1] && n[1] == n[2] && (n[0] == 16 || n[0] == 32)))
+        return false;
+    std::lock_guard<std::mutex> lk(u->mtx);
+    for (int c = 0; c < u->desc.ncomp; ++c)
+        for (const auto &l : u->bcs[c])
+            if (!l.empty())
+                return false;
+    return true;
+}
+
+long long x2_tiles(const cnv_grid_s *u, int zchunk)
+{
+    const int edge = u->desc.block_cells[0];
+    const int zc = zchunk < edge ? zchunk : edge;
+    return u->nblocks_total * u->desc.ncomp * ((edge + zc - 1) / zc);
+}
+
+int launch_diffusion_x2(cnv_grid_s *u, cnv_grid_s *out, double nu_dt, double *part1, double *part2,
+                        const double *const prev_p[2], const int prev_n[2], double *const prev_out[2], int wg,
+                        int zchunk, int *nparts, hipStream_t st)
+{
+    const int edge = u->desc.block_cells[0];
+    if (wg <= 0 || zchunk <= 0) {
+        const X2Tune t = x2_tune(u->desc.dtype, edge);
+        wg = t.wg;
+        zchunk = t.zchunk;
+    }
+    X2Args a;
+    a.u = static_cast<const char *>(u->data);
+    a.out = static_cast<char *>(out->data);
+    a.G = geom_of(u);
+    a.zchunk = zchunk < edge ? zchunk : edge;
+    a.nzc = (edge + a.zchunk - 1) / a.zchunk;
+    a.tiles_per_comp = (int)(u->nblocks_total * a.nzc);
+    for (int d = 0; d < 3; ++d)
+        a.ih2[d] = 1. / (u->h[d] * u->h[d]);
+    a.nudt = nu_dt;
+    a.part1 = part1;
+    a.part2 = part2;
+    for (int k = 0; k < 2; ++k) {
+        a.prev_p[k] = prev_p ? prev_p[k] : nullptr;
+        a.prev_n[k] = prev_n ? prev_n[k] : 0;
+        a.prev_out[k] = prev_out ? prev_out[k] : nullptr;
+    }
+    const long long ntiles = (long long)a.tiles_per_comp * u->desc.ncomp;
+    if (ntiles >= (1LL << 31))
+        return fail(CNV_ERR_UNSUPPORTED, "grid too large for the two-step kernel");
+    if (!x2_dispatch(u->desc.dtype, edge, wg, a, (int)ntiles, st))
+        return fail(CNV_ERR_UNSUPPORTED, "two-step tile shape not available for this block edge and dtype");
+    CNV_HIP(hipGetLastError());
+    if (nparts)
+        *nparts = (int)ntiles;
+    return CNV_OK;
+}
+
+int x2_set_tune(int dtype, int edge, int wg, int zchunk)
+{
+    if (edge != 16 && edge != 32)
+        return fail(CNV_ERR_INVALID, "two-step kernel: block edge must be 16 or 32");
+    if (zchunk < 1 || zchunk > edge)
+        return fail(CNV_ERR_INVALID, "zchunk must be in [1, edge]");
+    if (!x2_valid(dtype, edge, wg))
+        return fail(CNV_ERR_UNSUPPORTED, "two-step tile shape not available for this block edge and dtype");
+    std::lock_guard<std::mutex> lk(g_x2_mtx);
+    g_x2_tune[{dtype, edge}] = X2Tune{wg, zchunk};
+    return CNV_OK;
+}
+
+int x2_get_tune(int dtype, int edge, int *wg, int *zchunk)
+{
+    const X2Tune t = x2_tune(dtype, edge);
+    if (wg)
+        *wg = t.wg;
+    if (zchunk)
+        *zchunk = t.zchunk;
+    return CNV_OK;
+}
+
+// Times every valid (workgroup size, z-planes per tile) pair of the two-step
+// kernel on u -> out and keeps the fastest for (dtype, edge).
+int autotune_x2(cnv_grid_s *u, cnv_grid_s *out, int iters, int *best_wg, int *best_z, double *results,
+                int max_results, int *nresults)
+{
+    const int edge = u->desc.block_cells[0];
+    hipStream_t st;
+    CNV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    CNV_HIP(hipEventCreate(&e0));
+    CNV_HIP(hipEventCreate(&e1));
+    const int wgs[4] = {64, 128, 256, 512};
+    float best = 1e30f;
+    int bw = 0, bz = 0, cnt = 0, rc = CNV_OK;
+    for (int wi = 0; wi < 4 && rc == CNV_OK; ++wi)
+        for (int z = 4; z <= edge && rc == CNV_OK; z *= 2) {
+            if (x2_set_tune(u->desc.dtype, edge, wgs[wi], z) != CNV_OK)
+                continue;
+            rc = launch_diffusion_x2(u, out, 1e-9, nullptr, nullptr, nullptr, nullptr, nullptr, wgs[wi], z, nullptr,
+                                     st);
+            if (rc != CNV_OK)
+                break;
+            (void)hipEventRecord(e0, st);
+            for (int i = 0; i < iters && rc == CNV_OK; ++i)
+                rc = launch_diffusion_x2(u, out, 1e-9, nullptr, nullptr, nullptr, nullptr, nullptr, wgs[wi], z,
+                                         nullptr, st);
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                rc = fail(CNV_ERR_HIP, "autotune: two-step kernel failed");
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const float us = ms * 1000.f / iters;
+            if (results && cnt < max_results) {
+                results[3 * cnt] = wgs[wi];
+                results[3 * cnt + 1] = z;
+                results[3 * cnt + 2] = us;
+            }
+            ++cnt;
+            if (us < best) {
+                best = us;
+                bw = wgs[wi];
+                bz = z;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    if (rc != CNV_OK)
+        return rc;
+    if (bw == 0)
+        return fail(CNV_ERR_UNSUPPORTED, "no two-step tile shape for this grid");
+    x2_set_tune(u->desc.dtype, edge, bw, bz);
+    if (best_wg)
+        *best_wg = bw;
+    if (best_z)
+        *best_z = bz;
+    if (nresults)
+        *nresults = cnt < max_results ? cnt : max_results;
+    return CNV_OK;
+}
+
+} // namespace cnv

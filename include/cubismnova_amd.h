@@ -170,12 +170,14 @@ int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_ou
                        cnv_stream s);
 
 /* nsteps forward-Euler steps alternating u -> tmp -> u -> ... (the same
- * arithmetic as nsteps calls of cnv_diffusion_step; the field ends in u when
- * nsteps is even, in tmp when odd).  tmp must carry the same BC lists as u.
- * If norms is non-NULL (device array of nsteps doubles) norms[k] is the FP64
- * sum of squares after step k; the reduction of step k runs inside the
- * launch of step k+1 (one finalize launch after the last step).  The loop of
- * the reference's time stepping over Field.h:457-512 / FieldOperator.h. */
+ * arithmetic, bit for bit, as nsteps calls of cnv_diffusion_step; the field
+ * ends in u when nsteps is even, in tmp when odd).  tmp must carry the same
+ * BC lists as u.  If norms is non-NULL (device array of nsteps doubles)
+ * norms[k] is the FP64 sum of squares after step k; the reductions of a
+ * launch run inside the next launch (one finalize launch at the end).  On
+ * periodic grids (no BC list) of 16^3 or 32^3 blocks, steps go two per launch
+ * (one HBM read and write for two steps) up to a multiple of four.  The loop
+ * of the reference's time stepping over Field.h:457-512 / FieldOperator.h. */
 int cnv_diffusion_run(cnv_grid u, cnv_grid tmp, int nsteps, double nu_dt, double *norms, cnv_stream s);
 
 /* Fifth-order WENO face states (Jiang & Shu, J. Comput. Phys. 126:202-228,
@@ -251,11 +253,14 @@ int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts
 /* No counterpart in the reference (its applications loop over one grid's
  * blocks on the host, Laplacian.cpp:110-131).  Many independent FP32 grids
  * of uneven block counts, block sizes, component counts, ghost widths and
- * BCs, each with its own operator, processed by ONE persistent launch over a
- * device work list of (grid, component, block, z-range) items, largest first.
- * Ghosts follow the lab rules of cnv_lab_load for the grid's
- * Stencil(-ghost, ghost+1) and BC lists.  The launch is capturable in a HIP
- * graph (cnv_graph_begin/end) after one plain run. */
+ * BCs, each with its own operator, processed together over a device work
+ * list of (grid, component, block, z-range) items.  Items of cubic blocks of
+ * edge 16 or 32 are grouped per (operator, edge) class, one launch per class
+ * running the single-grid tile bodies; all other items go to one persistent
+ * launch that walks its list largest first.  Ghosts follow the lab rules of
+ * cnv_lab_load for the grid's Stencil(-ghost, ghost+1) and BC lists.  A run
+ * puts all its launches on one stream and is capturable in a HIP graph
+ * (cnv_graph_begin/end) after one plain run. */
 enum cnv_op {
     CNV_OP_DIFFUSION = 0, /* out = in + nu_dt * laplacian(in), as cnv_diffusion_step */
     CNV_OP_WENO5 = 1      /* out = WENO5 states of in, as cnv_weno5 */
@@ -271,13 +276,19 @@ typedef struct {
 
 typedef struct cnv_batch_s *cnv_batch;
 int cnv_batch_create(const cnv_batch_item *items, int n, cnv_batch *out);
-/* Autotuner: per (block edge, ghost width) class the z depth of the work
- * items, then the workgroup size of the launch.  results: up to max_results
- * records {edge, ghost, depth, microseconds} (workgroup trials: {0, wg, 0,
- * microseconds}). */
+/* Autotuner: per (operator, edge) class the workgroup size and z depth of
+ * its launch; for the persistent launch, per (block edge, ghost width) the z
+ * depth, then its workgroup size.  results: up to max_results records of 5
+ * doubles {op, edge, wg, depth, microseconds}; trials of the persistent
+ * launch have op = -1 (and edge = -1 for its workgroup trials, wg then being
+ * the workgroup size and depth 0; for its depth trials wg holds the ghost
+ * width). */
 int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results, int *nresults);
 int cnv_batch_run(cnv_batch b, cnv_stream s);
+/* nitems: all work items; wg, lds_bytes: of the persistent launch. */
 int cnv_batch_info(cnv_batch b, int *nitems, int *wg, size_t *lds_bytes);
+/* Per class 5 ints {op, edge, wg, z depth, items}, up to max_classes. */
+int cnv_batch_class_info(cnv_batch b, int max_classes, int *out, int *nclasses);
 int cnv_batch_destroy(cnv_batch b);
 
 /* ------------------------------------------------------------ autotuner */
@@ -291,6 +302,13 @@ int cnv_autotune_stencil7(cnv_grid u, cnv_grid out, int iters, int *wg, int *zch
                           int max_results, int *nresults);
 int cnv_tune_set_stencil7(int dtype, int edge, int wg, int zchunk);
 int cnv_tune_get_stencil7(int dtype, int edge, int *wg, int *zchunk);
+/* The same for the two-step kernel that cnv_diffusion_run uses on periodic
+ * grids (no BC list on any block) of 16^3 or 32^3 blocks: `tmp` is
+ * overwritten; CNV_ERR_UNSUPPORTED for other grids. */
+int cnv_autotune_diffusion_run(cnv_grid u, cnv_grid tmp, int iters, int *wg, int *zchunk, double *results,
+                               int max_results, int *nresults);
+int cnv_tune_set_diffusion_run(int dtype, int edge, int wg, int zchunk);
+int cnv_tune_get_diffusion_run(int dtype, int edge, int *wg, int *zchunk);
 /* The same for the WENO5 kernel (candidates: wg 64/128/256, z-chunks 4..64),
  * keyed by dtype. */
 int cnv_autotune_weno5(cnv_grid in, cnv_grid out, int iters, int *wg, int *zchunk, double *results,

@@ -1,8 +1,9 @@
-"""GPU parity of the batched persistent launch: every grid of a batch of
-uneven grids gives the result of its single-grid entry point (diffusion
-bit-exact, also against the oracle; WENO5 within FP32 rounding of the
-single-grid kernel and 1e-6 rel-L2 of the oracle's FP64 evaluation), and a
-graph replay of the batch reproduces the eager run every time."""
+"""GPU parity of the batched launches: every grid of a batch of uneven grids
+gives the result of its single-grid entry point bit for bit (diffusion also
+bit-exact against the oracle; WENO5 within 1e-6 rel-L2 of the oracle's FP64
+evaluation), whether its items run in an (operator, edge) class launch or in
+the persistent launch, and a graph replay of the batch reproduces the eager
+run every time."""
 import numpy as np
 import pytest
 
@@ -74,9 +75,21 @@ def test_batch_matches_single_grid_and_oracle(gpu, batch_setup):
                     got = cells_of(go.download(6 * c + k), nb, (e,) * 3, gi.block_stride).astype(np.float64)
                     one = cells_of(ref.download(6 * c + k), nb, (e,) * 3, gi.block_stride).astype(np.float64)
                     want = cells_of(o64[k], nb, (e,) * 3, gi.block_stride)
-                    scale = np.max(np.abs(want))
-                    assert np.max(np.abs(got - one)) <= 1e-5 * scale
+                    assert np.array_equal(got, one), (s["edge"], c, k)
                     assert np.linalg.norm(got - want) <= 1e-6 * np.linalg.norm(want)
+
+
+def test_batch_classes(gpu, batch_setup):
+    specs, grids, items = batch_setup
+    b = cnv.Batch(items)
+    cls = b.classes()
+    want = sorted({(0 if s["op"] == "diffusion" else 1, s["edge"]) for s in specs if s["edge"] in (16, 32)})
+    assert sorted((c[0], c[1]) for c in cls) == want
+    n_class = sum(c[4] for c in cls)
+    n8 = sum(s["nblocks"][0] * s["nblocks"][1] * s["nblocks"][2] * s["ncomp"] for s in specs if s["edge"] == 8)
+    assert b.info()["items"] >= n_class + n8      # 8^3 blocks: at least one item each
+    for op, edge, wg, z, n in cls:
+        assert wg in (64, 128, 256) and 1 <= z <= edge and n > 0
 
 
 def test_batch_autotune_and_graph_replays(gpu, batch_setup):

@@ -271,3 +271,62 @@ def test_run_argument_errors(gpu):
     with pytest.raises(cnv.CnvError):
         cnv.diffusion_run(u, t, -1, 1e-3)
     cnv.diffusion_run(t, cnv.Cartesian(2, 8), 0, 1e-3)
+
+
+TWO_STEP_SHAPES = {np.float32: [(16, 64), (32, 128), (32, 256)], np.float64: [(16, 64), (16, 128), (32, 256),
+                                                                             (32, 512)]}
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_two_step_kernel_matches_single_steps(gpu, dtype):
+    """Periodic grids take the two-steps-per-launch kernel inside diffusion_run:
+    fields bit-identical to single steps and norms within 1e-13 of the oracle,
+    for every tile shape and z depth, odd and even step counts, one-block
+    directions (self-periodic) and several blocks."""
+    for nb, e in [((2, 2, 2), 16), ((3, 2, 1), 32), ((1, 1, 1), 16), ((2, 1, 2), 32)]:
+        n = (e, e, e)
+        slab = random_slab(300 + e + nb[0], nb, n, dtype)
+        h = [1.0 / (nb[d] * e) for d in range(3)]
+        dt = 0.1 * min(h) ** 2
+        nsteps = 7
+        ref = [slab]
+        refn = []
+        for k in range(nsteps):
+            un, s = oracle.diffusion_step(oracle.Field(ref[-1], nb, n), h, dt)
+            ref.append(un)
+            refn.append(s)
+        for e2, wg in TWO_STEP_SHAPES[dtype]:
+            if e2 != e:
+                continue
+            for z in (4, 8, e):
+                cnv.set_tune_diffusion_run(dtype, e, wg, z)
+                for ns in (4, 5, nsteps):
+                    u = cnv.Cartesian(nb, n, dtype=dtype)
+                    t = cnv.Cartesian(nb, n, dtype=dtype)
+                    u.upload(0, slab)
+                    norms = cnv.DeviceBuffer(8 * ns)
+                    norms.fill(np.float64, np.nan)
+                    res = cnv.diffusion_run(u, t, ns, dt, norms=norms)
+                    assert_same(res.download(0), ref[ns], "%s wg %d z %d nsteps %d" % (nb, wg, z, ns))
+                    gn = norms.download(np.float64, ns)
+                    for k in range(ns):
+                        assert abs(gn[k] - refn[k]) <= 1e-13 * refn[k], (nb, wg, z, ns, k)
+    # back to the built-in choices for the tests that follow
+    cnv.set_tune_diffusion_run(dtype, 16, 64 if dtype == np.float32 else 128, 16)
+    cnv.set_tune_diffusion_run(dtype, 32, 256, 16)
+
+
+def test_two_step_autotune_and_fallback(gpu):
+    nb, n = (2, 2, 2), (32, 32, 32)
+    u = cnv.Cartesian(nb, n, dtype=np.float32)
+    t = cnv.Cartesian(nb, n, dtype=np.float32)
+    u.upload(0, random_slab(5, nb, n, np.float32))
+    wg, z, table = cnv.autotune_diffusion_run(u, t, iters=2)
+    assert (wg, z) == cnv.tune_diffusion_run(np.float32, 32)
+    assert len(table) >= 3 and all(r[2] > 0 for r in table)
+    # a grid with a BC list cannot use the two-step kernel
+    u.addFaceBC(cnv.Absorbing(0, 0))
+    with pytest.raises(cnv.CnvError):
+        cnv.autotune_diffusion_run(u, t, iters=1)
+    with pytest.raises(cnv.CnvError):
+        cnv.set_tune_diffusion_run(np.float32, 32, 64, 8)     # shape that runs out of registers
