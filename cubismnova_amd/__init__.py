@@ -490,6 +490,29 @@ def diffusion_run(u, tmp, nsteps, nu_dt, norms=None, stream=None):
     return u if nsteps % 2 == 0 else tmp
 
 
+FIELD_OPS = {"add": _abi.CNV_FIELD_ADD, "sub": _abi.CNV_FIELD_SUB, "mul": _abi.CNV_FIELD_MUL,
+             "div": _abi.CNV_FIELD_DIV}
+
+
+def _dt(dtype):
+    return _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32
+
+
+def field_op(op, a, b, dst, n, dtype=np.float64, stream=None):
+    """dst[i] = a[i] OP b[i] (b a device pointer) or a[i] OP b (b a number),
+    op in add/sub/mul/div, on n elements of device memory
+    (FieldOperator.h:11-141)."""
+    if isinstance(b, (int, float, np.floating)):
+        call("cnv_field_op1", FIELD_OPS[op], _dt(dtype), a, ctypes.c_double(float(b)), dst, int(n), _st(stream))
+    else:
+        call("cnv_field_op2", FIELD_OPS[op], _dt(dtype), a, b, dst, int(n), _st(stream))
+
+
+def field_rcp(a, scalar, dst, n, dtype=np.float64, stream=None):
+    """dst[i] = scalar * (1 / a[i]) on device memory (FieldOperator.h:25-30)."""
+    call("cnv_field_rcp", _dt(dtype), a, ctypes.c_double(float(scalar)), dst, int(n), _st(stream))
+
+
 def grid_sums(g, comp=0, stream=None):
     """(sum, sum of squares) of one component, FP64, deterministic."""
     buf = DeviceBuffer(16)
@@ -650,6 +673,13 @@ def grid_compare(a, b, comp_a=0, comp_b=0, stream=None):
     return float(r[0]), float(r[1]), float(r[2])
 
 
-def set_tune_weno5(dtype, wg, zchunk):
-    call("cnv_tune_set_weno5", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(wg),
-         int(zchunk))
+def set_tune_weno5(dtype, edge, wg, zchunk):
+    call("cnv_tune_set_weno5", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
+         int(wg), int(zchunk))
+
+
+def tune_weno5(dtype, edge):
+    wg, zc = ctypes.c_int(), ctypes.c_int()
+    call("cnv_tune_get_weno5", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
+         ctypes.byref(wg), ctypes.byref(zc))
+    return wg.value, zc.value

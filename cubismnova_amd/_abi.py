@@ -112,20 +112,33 @@ SIGNATURES = {
     "cnv_jacobi_solve": (i32, [vp, vp, vp, vp, vp, vp, i32, vp]),
     "cnv_weno5": (i32, [vp, vp, vp]),
     "cnv_autotune_weno5": (i32, [vp, vp, i32, c_int_p, c_int_p, vp, i32, c_int_p]),
-    "cnv_tune_set_weno5": (i32, [i32, i32, i32]),
-    "cnv_tune_get_weno5": (i32, [i32, c_int_p, c_int_p]),
+    "cnv_tune_set_weno5": (i32, [i32, i32, i32, i32]),
+    "cnv_tune_get_weno5": (i32, [i32, i32, c_int_p, c_int_p]),
     "cnv_batch_create": (i32, [vp, i32, ctypes.POINTER(vp)]),
     "cnv_batch_autotune": (i32, [vp, i32, vp, i32, c_int_p]),
     "cnv_batch_run": (i32, [vp, vp]),
     "cnv_batch_info": (i32, [vp, c_int_p, c_int_p, ctypes.POINTER(sz)]),
     "cnv_batch_class_info": (i32, [vp, i32, c_int_p, c_int_p]),
     "cnv_batch_destroy": (i32, [vp]),
+    "cnv_field_op2": (i32, [i32, i32, vp, vp, vp, sz, vp]),
+    "cnv_field_op1": (i32, [i32, i32, vp, f64, vp, sz, vp]),
+    "cnv_field_rcp": (i32, [i32, vp, f64, vp, sz, vp]),
     "cnv_grid_prepare": (i32, [vp]),
     "cnv_graph_begin": (i32, [vp]),
     "cnv_graph_end": (i32, [vp, ctypes.POINTER(vp)]),
     "cnv_graph_launch": (i32, [vp, vp]),
     "cnv_graph_destroy": (i32, [vp]),
 }
+
+# the reference's field-op hooks (FieldOperator.h:149-227): void, blocking
+for _op in ("Add", "Sub", "Mul", "Div"):
+    for _t, _ct in (("double", f64), ("float", ctypes.c_float)):
+        SIGNATURES["field%s2_%s" % (_op, _t)] = (None, [vp, vp, vp, sz])
+        SIGNATURES["field%s1_%s" % (_op, _t)] = (None, [vp, _ct, vp, sz])
+SIGNATURES["fieldRcp_double"] = (None, [vp, f64, vp, sz])
+SIGNATURES["fieldRcp_float"] = (None, [vp, ctypes.c_float, vp, sz])
+
+CNV_FIELD_ADD, CNV_FIELD_SUB, CNV_FIELD_MUL, CNV_FIELD_DIV = 0, 1, 2, 3
 
 _lib = None
 
@@ -141,7 +154,9 @@ def declared_symbols():
     import re
     hdr = os.path.join(os.path.dirname(HERE), "include", "cubismnova_amd.h")
     txt = open(hdr).read()
-    return sorted(set(re.findall(r"\b(cnv_[a-z0-9_]+)\s*\(", txt)))
+    names = set(re.findall(r"\b(cnv_[a-z0-9_]+)\s*\(", txt))
+    names |= set(re.findall(r"\b(field(?:Add|Sub|Mul|Div)[12]_(?:double|float)|fieldRcp_(?:double|float))\s*\(", txt))
+    return sorted(names)
 
 
 def lib():

@@ -657,22 +657,22 @@ int cnv_autotune_weno5(cnv_grid in, cnv_grid out, int iters, int *wg, int *zchun
     CNV_GUARD_END
 }
 
-int cnv_tune_set_weno5(int dtype, int wg, int zchunk)
+int cnv_tune_set_weno5(int dtype, int edge, int wg, int zchunk)
 {
     if (dtype != CNV_F32 && dtype != CNV_F64)
         return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
     if (wg != 64 && wg != 128 && wg != 256)
         return fail(CNV_ERR_INVALID, "workgroup size must be 64, 128 or 256");
-    if (zchunk < 1)
-        return fail(CNV_ERR_INVALID, "zchunk must be >= 1");
-    return weno5_set_tune(dtype, wg, zchunk);
+    return weno5_set_tune(dtype, edge, wg, zchunk);
 }
 
-int cnv_tune_get_weno5(int dtype, int *wg, int *zchunk)
+int cnv_tune_get_weno5(int dtype, int edge, int *wg, int *zchunk)
 {
     if (!wg || !zchunk)
         return fail(CNV_ERR_INVALID, "output pointer is NULL");
-    return weno5_get_tune(dtype, wg, zchunk);
+    if (dtype != CNV_F32 && dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    return weno5_get_tune(dtype, edge, wg, zchunk);
 }
 
 // ---------------------------------------------------------------- Jacobi

@@ -190,8 +190,8 @@ int jacobi_prepare(cnv_grid_s *g, int comp);
 int launch_weno5(cnv_grid_s *in, cnv_grid_s *out, hipStream_t st, const int *tile);
 int autotune_weno5(cnv_grid_s *in, cnv_grid_s *out, int iters, int *best_wg, int *best_z, double *results,
                    int max_results, int *nresults);
-int weno5_set_tune(int dtype, int wg, int zchunk);
-int weno5_get_tune(int dtype, int *wg, int *zchunk);
+int weno5_set_tune(int dtype, int edge, int wg, int zchunk);
+int weno5_get_tune(int dtype, int edge, int *wg, int *zchunk);
 
 int launch_fill(void *dst, int dtype, double value, size_t count, hipStream_t st);
 

@@ -4,7 +4,7 @@
 // A workgroup owns the x-y planes [z0, z1) of one block of one component.
 // It stages the planes of u^n it needs, each with a two-cell x/y halo from
 // the neighbour blocks (periodic wrap, the four diagonal cells included), in
-// a rolling set of three LDS planes; evaluates u^{n+1} on the block's cells
+// a rolling set of four LDS planes; evaluates u^{n+1} on the block's cells
 // and on the one-cell ring around them (the face-neighbour cells that the
 // second step reads), and u^{n+2} on the block's cells.  Each cell of
 // u^{n+1} and u^{n+2} is computed with the operation order of the one-step
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     using V = VecT<T, VEC>;
     constexpr long long NN = (long long)N * N;
 
-    __shared__ alignas(16) T sA[3 * PLA];
+    __shared__ alignas(16) T sA[4 * PLA];
     __shared__ alignas(16) T sB[2 * PLB];
     __shared__ double red[WG / 64];
 
@@ -172,40 +172,56 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
         const T l = lap * nudt;
         return c + l;
     };
-    auto slotA = [&](int p) { return sA + ((p - z0 + 3) % 3) * PLA; }; // p >= z0 - 2
+    // LDS: four u^n planes (q-1, q, q+1 read while plane q is evaluated, the
+    // next one being stored) and two u^{n+1} planes, so that one barrier per
+    // plane orders every write before its reads and every read before the
+    // write that reuses its slot
+    auto slotA = [&](int p) { return sA + ((p - z0 + 4) & 3) * PLA; }; // p >= z0 - 2
     auto slotB = [&](int q) { return sB + ((q - z0 + 2) & 1) * PLB; }; // q >= z0 - 1
 
-    store_u0(load_u0(z0 - 2), slotA(z0 - 2));
-    store_u0(load_u0(z0 - 1), slotA(z0 - 1));
+    // u^n on this thread's cells of planes q-1 and q; the pending plane holds q+1
+    V c0m[R], c0c[R];
+    {
+        const U0 Q0 = load_u0(z0 - 2);
+        store_u0(Q0, slotA(z0 - 2));
+        const U0 Q1 = load_u0(z0 - 1);
+        store_u0(Q1, slotA(z0 - 1));
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            c0m[r] = Q0.c[r];
+            c0c[r] = Q1.c[r];
+        }
+    }
     U0 Pq = load_u0(z0);
 
     V m1[R], c1[R], n1[R]; // u^{n+1} on this thread's cells of planes q-2, q-1, q
     double acc1 = 0.0, acc2 = 0.0;
     for (int q = z0 - 1; q <= z1; ++q) {
         store_u0(Pq, slotA(q + 1));
+        V c0p[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            c0p[r] = Pq.c[r];
         if (q + 2 <= z1 + 1)
             Pq = load_u0(q + 2);
         __syncthreads();
         const T *Am = slotA(q - 1), *Ac = slotA(q), *Ap = slotA(q + 1);
         const bool inner = q >= z0 && q < z1;
         T *Bq = slotB(q);
-        // u^{n+1} on the block's cells of plane q
+        // u^{n+1} on the block's cells of plane q (z neighbours from registers)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int y = ty + r * ROWS;
             const int o = (y + 2) * PA + VEC + x0;
-            const V c = *reinterpret_cast<const V *>(Ac + o);
             const V ym = *reinterpret_cast<const V *>(Ac + o - PA);
             const V yp = *reinterpret_cast<const V *>(Ac + o + PA);
-            const V zm = *reinterpret_cast<const V *>(Am + o);
-            const V zp = *reinterpret_cast<const V *>(Ap + o);
             const T xl = Ac[o - 1], xr = Ac[o + VEC];
             V res;
 #pragma unroll
             for (int v = 0; v < VEC; ++v) {
-                const T xm = v == 0 ? xl : c.v[v - 1];
-                const T xp = v == VEC - 1 ? xr : c.v[v + 1];
-                res.v[v] = euler(c.v[v], xm, xp, ym.v[v], yp.v[v], zm.v[v], zp.v[v]);
+                const T xm = v == 0 ? xl : c0c[r].v[v - 1];
+                const T xp = v == VEC - 1 ? xr : c0c[r].v[v + 1];
+                res.v[v] = euler(c0c[r].v[v], xm, xp, ym.v[v], yp.v[v], c0m[r].v[v], c0p[r].v[v]);
                 if (inner)
                     acc1 = __builtin_fma((double)res.v[v], (double)res.v[v], acc1);
             }
@@ -233,8 +249,8 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
                 }
             }
         }
-        __syncthreads();
-        // u^{n+2} on the block's cells of plane q - 1
+        // u^{n+2} on the block's cells of plane q - 1 (its u^{n+1} plane was
+        // completed before this plane's barrier)
         if (q >= z0 + 1) {
             const T *Br = slotB(q - 1);
             T *op = ob + NN * (q - 1);
@@ -258,6 +274,8 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
+            c0m[r] = c0c[r];
+            c0c[r] = c0p[r];
             m1[r] = c1[r];
             c1[r] = n1[r];
         }

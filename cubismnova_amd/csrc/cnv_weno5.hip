@@ -22,6 +22,10 @@
 // compiler fuse multiply-adds and use the hardware reciprocal.
 #include "cnv_march.h"
 
+#include <map>
+#include <type_traits>
+#include <utility>
+
 namespace cnv
 {
 
@@ -121,14 +125,29 @@ struct W5Tune {
     int wg, zchunk;
 };
 static std::mutex g_w5_mtx;
-static W5Tune g_w5_tune[2] = {{256, 32}, {128, 16}}; // [dtype]: measured best at 32^3 blocks
+static std::map<std::pair<int, int>, W5Tune> g_w5_tune; // (dtype, edge) -> choice
+
+// built-in choices (measured at 512^3 for edge 32)
+static W5Tune w5_default(int dtype, int edge)
+{
+    if (edge == 16)
+        return dtype == CNV_F64 ? W5Tune{128, 16} : W5Tune{64, 16};
+    if (edge == 32)
+        return dtype == CNV_F64 ? W5Tune{128, 16} : W5Tune{256, 32};
+    return W5Tune{256, 16};
+}
+
+static W5Tune w5_tune(int dtype, int edge)
+{
+    std::lock_guard<std::mutex> lk(g_w5_mtx);
+    auto it = g_w5_tune.find({dtype, edge});
+    return it == g_w5_tune.end() ? w5_default(dtype, edge) : it->second;
+}
 
 template <typename T, int NX, int WG>
 constexpr bool w5_valid()
 {
-    constexpr int VEC = 16 / (int)sizeof(T);
-    constexpr int TX = NX / VEC;
-    return NX % VEC == 0 && WG % TX == 0 && (WG / TX) <= NX && NX % (WG / TX) == 0;
+    return W5Shape<T, NX, NX, WG>::valid;
 }
 
 template <typename T, int NX, int WG>
@@ -162,11 +181,7 @@ static int w5_run(W5Args &a, hipStream_t st, const W5Tune *over)
     const int *n = a.G.n;
     const long long nbt = (long long)a.G.nb[0] * a.G.nb[1] * a.G.nb[2];
     if (n[0] == n[1] && (n[0] == 16 || n[0] == 32 || n[0] == 64)) {
-        W5Tune tu;
-        {
-            std::lock_guard<std::mutex> lk(g_w5_mtx);
-            tu = over ? *over : g_w5_tune[sizeof(T) == 8 ? 1 : 0];
-        }
+        const W5Tune tu = over ? *over : w5_tune(sizeof(T) == 8 ? CNV_F64 : CNV_F32, n[0]);
         a.zchunk = tu.zchunk < n[2] ? tu.zchunk : n[2];
         a.nzc = (n[2] + a.zchunk - 1) / a.zchunk;
         a.tiles_per_comp = (int)(nbt * a.nzc);
@@ -260,23 +275,46 @@ int launch_weno5(cnv_grid_s *in, cnv_grid_s *out, hipStream_t st, const int *til
     return w5_run<float>(a, st, tile ? &over : nullptr);
 }
 
-int weno5_set_tune(int dtype, int wg, int zchunk)
+template <typename T>
+static bool w5_valid_rt(int edge, int wg)
 {
+    auto pick = [&](auto e) -> bool {
+        constexpr int E = decltype(e)::value;
+        return wg == 64 ? w5_valid<T, E, 64>() : wg == 128 ? w5_valid<T, E, 128>() : wg == 256 ? w5_valid<T, E, 256>() : false;
+    };
+    if (edge == 16)
+        return pick(std::integral_constant<int, 16>());
+    if (edge == 32)
+        return pick(std::integral_constant<int, 32>());
+    if (edge == 64)
+        return pick(std::integral_constant<int, 64>());
+    return false;
+}
+
+int weno5_set_tune(int dtype, int edge, int wg, int zchunk)
+{
+    if (edge != 16 && edge != 32 && edge != 64)
+        return fail(CNV_ERR_INVALID, "WENO5 tiles exist for block edges 16, 32 and 64");
+    if (zchunk < 1)
+        return fail(CNV_ERR_INVALID, "zchunk must be >= 1");
+    const bool ok = dtype == CNV_F64 ? w5_valid_rt<double>(edge, wg) : w5_valid_rt<float>(edge, wg);
+    if (!ok)
+        return fail(CNV_ERR_UNSUPPORTED, "WENO5 tile shape not available for this block edge and dtype");
     std::lock_guard<std::mutex> lk(g_w5_mtx);
-    g_w5_tune[dtype == CNV_F64 ? 1 : 0] = {wg, zchunk};
+    g_w5_tune[{dtype, edge}] = {wg, zchunk};
     return CNV_OK;
 }
 
-int weno5_get_tune(int dtype, int *wg, int *zchunk)
+int weno5_get_tune(int dtype, int edge, int *wg, int *zchunk)
 {
-    std::lock_guard<std::mutex> lk(g_w5_mtx);
-    *wg = g_w5_tune[dtype == CNV_F64 ? 1 : 0].wg;
-    *zchunk = g_w5_tune[dtype == CNV_F64 ? 1 : 0].zchunk;
+    const W5Tune t = w5_tune(dtype, edge);
+    *wg = t.wg;
+    *zchunk = t.zchunk;
     return CNV_OK;
 }
 
 // Times every (workgroup size, z-chunk) candidate on in -> out and keeps the
-// fastest for the input dtype.
+// fastest for the input's (dtype, block edge).
 int autotune_weno5(cnv_grid_s *in, cnv_grid_s *out, int iters, int *best_wg, int *best_z, double *results,
                    int max_results, int *nresults)
 {
@@ -336,7 +374,7 @@ int autotune_weno5(cnv_grid_s *in, cnv_grid_s *out, int iters, int *best_wg, int
         return rc;
     if (bw == 0)
         return fail(CNV_ERR_UNSUPPORTED, "no tile candidate ran");
-    weno5_set_tune(in->desc.dtype, bw, bz);
+    weno5_set_tune(in->desc.dtype, n[0], bw, bz);
     if (best_wg)
         *best_wg = bw;
     if (best_z)

@@ -390,6 +390,7 @@ public:
     }
     cnv_grid handle() const { return g_; }
     MIndex blockExtent() const { return cells_; }
+    size_t getNumComponents() const { return ncomp_; }
 
 private:
     MIndex nblocks_, cells_;
@@ -431,6 +432,26 @@ double diffusionStep(Grid::Cartesian<T> &u, Grid::Cartesian<T> &u_new, const dou
     double s = 0;
     check(cnv_memcpy_d2h(&s, dev, sizeof(double), nullptr), "cnv_memcpy_d2h");
     return s;
+}
+
+// nsteps Euler steps u -> tmp -> u ...; returns the sum of squares after each
+// step; the field ends in u for even nsteps, in tmp for odd.
+template <typename T>
+std::vector<double> diffusionRun(Grid::Cartesian<T> &u, Grid::Cartesian<T> &tmp, const int nsteps, const double nu_dt)
+{
+    std::vector<double> norms((size_t)(nsteps > 0 ? nsteps : 0));
+    if (norms.empty())
+        return norms;
+    for (size_t c = 0; c < u.getNumComponents(); ++c) {
+        u.syncBCs(c);
+        tmp.syncBCs(c);
+    }
+    double *dev = nullptr;
+    check(cnv_malloc((void **)&dev, sizeof(double) * norms.size()), "cnv_malloc");
+    std::unique_ptr<void, int (*)(void *)> guard(dev, cnv_free);
+    check(cnv_diffusion_run(u.handle(), tmp.handle(), nsteps, nu_dt, dev, nullptr), "cnv_diffusion_run");
+    check(cnv_memcpy_d2h(norms.data(), dev, sizeof(double) * norms.size(), nullptr), "cnv_memcpy_d2h");
+    return norms;
 }
 } // namespace Device
 

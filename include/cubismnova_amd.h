@@ -249,6 +249,41 @@ typedef struct {
 int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts *opts, cnv_solve_info *info,
                      double *history, int history_cap, cnv_stream s);
 
+/* ------------------------------------------------------------ field ops */
+/* Element-wise arithmetic on device memory (a grid's component slab from
+ * cnv_grid_data, or cnv_malloc memory), one IEEE operation per element:
+ * op2: dst[i] = a[i] OP b[i];  op1: dst[i] = a[i] OP scalar;
+ * rcp: dst[i] = scalar * (1 / a[i]) (two roundings).  The loops of
+ * include/Cubism/Block/FieldOperator.h:11-30, 44-141, stream-ordered. */
+enum cnv_field_op { CNV_FIELD_ADD = 0, CNV_FIELD_SUB = 1, CNV_FIELD_MUL = 2, CNV_FIELD_DIV = 3 };
+int cnv_field_op2(int op, int dtype, const void *a, const void *b, void *dst, size_t n, cnv_stream s);
+int cnv_field_op1(int op, int dtype, const void *a, double scalar, void *dst, size_t n, cnv_stream s);
+int cnv_field_rcp(int dtype, const void *a, double scalar, void *dst, size_t n, cnv_stream s);
+
+/* The reference's own C-ABI hooks, same names and signatures as the extern
+ * "C" declarations of FieldOperator.h:149-227 (CUBISM_OPTIMIZED_FIELD_OP):
+ * linking this library provides them.  Pointers must be device-accessible;
+ * each call runs on the null stream and returns when done, like the host
+ * loops it replaces.  Failures are readable with cnv_last_error. */
+void fieldAdd2_double(const double *src0, const double *src1, double *dst, size_t n);
+void fieldSub2_double(const double *src0, const double *src1, double *dst, size_t n);
+void fieldMul2_double(const double *src0, const double *src1, double *dst, size_t n);
+void fieldDiv2_double(const double *src0, const double *src1, double *dst, size_t n);
+void fieldAdd2_float(const float *src0, const float *src1, float *dst, size_t n);
+void fieldSub2_float(const float *src0, const float *src1, float *dst, size_t n);
+void fieldMul2_float(const float *src0, const float *src1, float *dst, size_t n);
+void fieldDiv2_float(const float *src0, const float *src1, float *dst, size_t n);
+void fieldAdd1_double(const double *src0, double src1, double *dst, size_t n);
+void fieldSub1_double(const double *src0, double src1, double *dst, size_t n);
+void fieldMul1_double(const double *src0, double src1, double *dst, size_t n);
+void fieldDiv1_double(const double *src0, double src1, double *dst, size_t n);
+void fieldAdd1_float(const float *src0, float src1, float *dst, size_t n);
+void fieldSub1_float(const float *src0, float src1, float *dst, size_t n);
+void fieldMul1_float(const float *src0, float src1, float *dst, size_t n);
+void fieldDiv1_float(const float *src0, float src1, float *dst, size_t n);
+void fieldRcp_double(const double *src0, double src1, double *dst, size_t n);
+void fieldRcp_float(const float *src0, float src1, float *dst, size_t n);
+
 /* ------------------------------------------------------------ batches */
 /* No counterpart in the reference (its applications loop over one grid's
  * blocks on the host, Laplacian.cpp:110-131).  Many independent FP32 grids
@@ -310,11 +345,11 @@ int cnv_autotune_diffusion_run(cnv_grid u, cnv_grid tmp, int iters, int *wg, int
 int cnv_tune_set_diffusion_run(int dtype, int edge, int wg, int zchunk);
 int cnv_tune_get_diffusion_run(int dtype, int edge, int *wg, int *zchunk);
 /* The same for the WENO5 kernel (candidates: wg 64/128/256, z-chunks 4..64),
- * keyed by dtype. */
+ * keyed by (dtype, block edge); edges 16, 32 and 64 have tiles. */
 int cnv_autotune_weno5(cnv_grid in, cnv_grid out, int iters, int *wg, int *zchunk, double *results,
                        int max_results, int *nresults);
-int cnv_tune_set_weno5(int dtype, int wg, int zchunk);
-int cnv_tune_get_weno5(int dtype, int *wg, int *zchunk);
+int cnv_tune_set_weno5(int dtype, int edge, int wg, int zchunk);
+int cnv_tune_get_weno5(int dtype, int edge, int *wg, int *zchunk);
 
 /* ------------------------------------------------------------ HIP graphs */
 /* No counterpart in the reference (its loop is host code per block): record

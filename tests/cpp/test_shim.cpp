@@ -245,6 +245,38 @@ static void laplacian_vs_host_loop()
     }
 }
 
+// Device::diffusionRun (two steps per launch on this periodic grid) against
+// repeated Device::diffusionStep: identical fields, norms to 1e-14
+static void diffusion_run_vs_steps()
+{
+    using G = Grid::Cartesian<double>;
+    G a({2, 2, 2}, {16, 16, 16}), b({2, 2, 2}, {16, 16, 16}), c({2, 2, 2}, {16, 16, 16}), d({2, 2, 2}, {16, 16, 16});
+    for (size_t i = 0; i < a.size(); ++i)
+        for (Index z = 0; z < 16; ++z)
+            for (Index y = 0; y < 16; ++y)
+                for (Index x = 0; x < 16; ++x)
+                    a[i](x, y, z) = c[i](x, y, z) = std::sin(0.1 * (x + 3 * y + 7 * z) + (double)i);
+    a.upload(0);
+    c.upload(0);
+    const double dt = 1e-6;
+    const std::vector<double> n = Device::diffusionRun(a, b, 4, dt);
+    std::vector<double> m;
+    m.push_back(Device::diffusionStep(c, d, dt));
+    m.push_back(Device::diffusionStep(d, c, dt));
+    m.push_back(Device::diffusionStep(c, d, dt));
+    m.push_back(Device::diffusionStep(d, c, dt));
+    a.download(0);
+    c.download(0);
+    EXPECT(n.size() == 4);
+    for (int k = 0; k < 4; ++k)
+        EXPECT(std::fabs(n[(size_t)k] - m[(size_t)k]) <= 1e-14 * m[(size_t)k]);
+    for (size_t i = 0; i < a.size(); ++i)
+        for (Index z = 0; z < 16; ++z)
+            for (Index y = 0; y < 16; ++y)
+                for (Index x = 0; x < 16; ++x)
+                    EXPECT(a[i](x, y, z) == c[i](x, y, z));
+}
+
 int main()
 {
     fieldlab_ghosts(false);
@@ -252,6 +284,7 @@ int main()
     bc_tests();
     cartesian_lab_analytic();
     laplacian_vs_host_loop();
+    diffusion_run_vs_steps();
     std::printf("%d checks, %d failed\n", g_checks, g_fail);
     return g_fail == 0 ? 0 : 1;
 }
