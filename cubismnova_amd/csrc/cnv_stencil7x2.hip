@@ -88,17 +88,49 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     T *ob = reinterpret_cast<T *>(a.out) + (long long)comp * a.G.cstride + blk * a.G.bstride;
     const long long bstride = a.G.bstride;
 
-    // base of block (bx+dx, by+dy, bz+dz), periodic in every direction
-    auto bptr = [&](int dx, int dy, int dz) -> const T * {
-        int x = bx + dx, y = by + dy, z = bz + dz;
-        x = x < 0 ? x + nbx : (x >= nbx ? x - nbx : x);
-        y = y < 0 ? y + nby : (y >= nby ? y - nby : y);
-        z = z < 0 ? z + nbz : (z >= nbz ? z - nbz : z);
-        return cbase + ((long long)x + (long long)nbx * (y + (long long)nby * z)) * bstride;
-    };
+    auto wrap = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
+    const long long layer = (long long)nbx * nby * bstride; // elements per z-layer of blocks
+    const long long xyself = (bx + (long long)nbx * by) * bstride;
 
     const int tx = tid % TX, ty = tid / TX;
     const int x0 = tx * VEC;
+
+    // Fixed per-thread roles for the halo of a u^n plane, resolved once: the
+    // element offset of the source within its z-layer of blocks and plane
+    // (periodic neighbour blocks in x and y), the LDS offset, and the kind
+    // (0 none, 1 scalar, 2 vector).  A plane p then only adds its layer and
+    // plane offset.
+    long long hsrc[HS];
+    int hlds[HS], hkind[HS];
+#pragma unroll
+    for (int h = 0; h < HS; ++h) {
+        const int s = tid + h * WG;
+        hkind[h] = 0;
+        hsrc[h] = 0;
+        hlds[h] = 0;
+        if (s < NHX) {
+            const int side = s / (2 * N), lyr = (s / N) & 1, y = s % N;
+            const long long nb = (wrap(bx + (side ? 1 : -1), nbx) + (long long)nbx * by) * bstride;
+            hsrc[h] = nb + (side ? lyr : N - 1 - lyr) + N * y;
+            hlds[h] = (y + 2) * PA + VEC + (side ? N + lyr : -1 - lyr);
+            hkind[h] = 1;
+        } else if (s < NHX + NHY) {
+            const int s2 = s - NHX;
+            const int side = s2 / (2 * TX), lyr = (s2 / TX) & 1, xs = (s2 % TX) * VEC;
+            const long long nb = (bx + (long long)nbx * wrap(by + (side ? 1 : -1), nby)) * bstride;
+            hsrc[h] = nb + xs + N * (side ? lyr : N - 1 - lyr);
+            hlds[h] = ((side ? N + lyr : -1 - lyr) + 2) * PA + VEC + xs;
+            hkind[h] = 2;
+        } else if (s < NH) {
+            const int d = s - NHX - NHY, sx = d & 1, sy = d >> 1;
+            const long long nb =
+                (wrap(bx + (sx ? 1 : -1), nbx) + (long long)nbx * wrap(by + (sy ? 1 : -1), nby)) * bstride;
+            hsrc[h] = nb + (sx ? 0 : N - 1) + N * (sy ? 0 : N - 1);
+            hlds[h] = ((sy ? N : -1) + 2) * PA + VEC + (sx ? N : -1);
+            hkind[h] = 1;
+        }
+    }
+    const int own = x0 + N * ty; // this thread's first row offset within a plane
 
     struct U0 {
         V c[R];
@@ -108,30 +140,18 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     auto load_u0 = [&](int p) -> U0 {
         U0 Q = {};
         const int dz = p < 0 ? -1 : (p >= N ? 1 : 0);
-        const long long lp = (long long)(p - dz * N) * NN;
-        const T *own = bptr(0, 0, dz) + lp;
+        const T *base = cbase + (long long)wrap(bz + dz, nbz) * layer + (long long)(p - dz * N) * NN;
+        const T *self = base + xyself + own;
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            Q.c[r] = *reinterpret_cast<const V *>(own + x0 + N * (ty + r * ROWS));
+            Q.c[r] = *reinterpret_cast<const V *>(self + N * r * ROWS);
 #pragma unroll
         for (int h = 0; h < HS; ++h) {
-            const int s = tid + h * WG;
-            if (s < NHX) {
-                const int side = s / (2 * N), layer = (s / N) & 1, y = s % N;
-                const T *src = bptr(side ? 1 : -1, 0, dz) + lp;
+            if (hkind[h] == 2) {
+                Q.h[h] = *reinterpret_cast<const V *>(base + hsrc[h]);
+            } else if (hkind[h] == 1) {
                 V t = {};
-                t.v[0] = src[(side ? layer : N - 1 - layer) + N * y];
-                Q.h[h] = t;
-            } else if (s < NHX + NHY) {
-                const int s2 = s - NHX;
-                const int side = s2 / (2 * TX), layer = (s2 / TX) & 1, xs = (s2 % TX) * VEC;
-                const T *src = bptr(0, side ? 1 : -1, dz) + lp;
-                Q.h[h] = *reinterpret_cast<const V *>(src + xs + N * (side ? layer : N - 1 - layer));
-            } else if (s < NH) {
-                const int d = s - NHX - NHY, sx = d & 1, sy = d >> 1;
-                const T *src = bptr(sx ? 1 : -1, sy ? 1 : -1, dz) + lp;
-                V t = {};
-                t.v[0] = src[(sx ? 0 : N - 1) + N * (sy ? 0 : N - 1)];
+                t.v[0] = base[hsrc[h]];
                 Q.h[h] = t;
             }
         }
@@ -144,20 +164,10 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
             *reinterpret_cast<V *>(A + (ty + r * ROWS + 2) * PA + VEC + x0) = Q.c[r];
 #pragma unroll
         for (int h = 0; h < HS; ++h) {
-            const int s = tid + h * WG;
-            if (s < NHX) {
-                const int side = s / (2 * N), layer = (s / N) & 1, y = s % N;
-                const int x = side ? N + layer : -1 - layer;
-                A[(y + 2) * PA + VEC + x] = Q.h[h].v[0];
-            } else if (s < NHX + NHY) {
-                const int s2 = s - NHX;
-                const int side = s2 / (2 * TX), layer = (s2 / TX) & 1, xs = (s2 % TX) * VEC;
-                const int y = side ? N + layer : -1 - layer;
-                *reinterpret_cast<V *>(A + (y + 2) * PA + VEC + xs) = Q.h[h];
-            } else if (s < NH) {
-                const int d = s - NHX - NHY, sx = d & 1, sy = d >> 1;
-                A[((sy ? N : -1) + 2) * PA + VEC + (sx ? N : -1)] = Q.h[h].v[0];
-            }
+            if (hkind[h] == 2)
+                *reinterpret_cast<V *>(A + hlds[h]) = Q.h[h];
+            else if (hkind[h] == 1)
+                A[hlds[h]] = Q.h[h].v[0];
         }
     };
 
@@ -179,8 +189,11 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     auto slotA = [&](int p) { return sA + ((p - z0 + 4) & 3) * PLA; }; // p >= z0 - 2
     auto slotB = [&](int q) { return sB + ((q - z0 + 2) & 1) * PLB; }; // q >= z0 - 1
 
-    // u^n on this thread's cells of planes q-1 and q; the pending plane holds q+1
-    V c0m[R], c0c[R];
+    // Register rings of three planes, unrolled by three so that every slot
+    // index is a constant: U0r holds u^n on this thread's cells (plane p in
+    // slot (p - z0 + 2) % 3), U1r holds u^{n+1} (plane p in slot
+    // (p - z0 + 3) % 3); the pending registers hold the next u^n plane.
+    V U0r[3][R], U1r[3][R];
     {
         const U0 Q0 = load_u0(z0 - 2);
         store_u0(Q0, slotA(z0 - 2));
@@ -188,96 +201,99 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
         store_u0(Q1, slotA(z0 - 1));
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            c0m[r] = Q0.c[r];
-            c0c[r] = Q1.c[r];
+            U0r[0][r] = Q0.c[r];
+            U0r[1][r] = Q1.c[r];
         }
     }
     U0 Pq = load_u0(z0);
 
-    V m1[R], c1[R], n1[R]; // u^{n+1} on this thread's cells of planes q-2, q-1, q
     double acc1 = 0.0, acc2 = 0.0;
-    for (int q = z0 - 1; q <= z1; ++q) {
-        store_u0(Pq, slotA(q + 1));
-        V c0p[R];
+    for (int qb = z0 - 1; qb <= z1; qb += 3) {
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            c0p[r] = Pq.c[r];
-        if (q + 2 <= z1 + 1)
-            Pq = load_u0(q + 2);
-        __syncthreads();
-        const T *Am = slotA(q - 1), *Ac = slotA(q), *Ap = slotA(q + 1);
-        const bool inner = q >= z0 && q < z1;
-        T *Bq = slotB(q);
-        // u^{n+1} on the block's cells of plane q (z neighbours from registers)
+        for (int u = 0; u < 3; ++u) {
+            const int q = qb + u;
+            if (q > z1)
+                break;
+            V(&c0m)[R] = U0r[u % 3];
+            V(&c0c)[R] = U0r[(u + 1) % 3];
+            V(&c0p)[R] = U0r[(u + 2) % 3];
+            V(&m1)[R] = U1r[u % 3];
+            V(&c1)[R] = U1r[(u + 1) % 3];
+            V(&n1)[R] = U1r[(u + 2) % 3];
+            store_u0(Pq, slotA(q + 1));
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int y = ty + r * ROWS;
-            const int o = (y + 2) * PA + VEC + x0;
-            const V ym = *reinterpret_cast<const V *>(Ac + o - PA);
-            const V yp = *reinterpret_cast<const V *>(Ac + o + PA);
-            const T xl = Ac[o - 1], xr = Ac[o + VEC];
-            V res;
-#pragma unroll
-            for (int v = 0; v < VEC; ++v) {
-                const T xm = v == 0 ? xl : c0c[r].v[v - 1];
-                const T xp = v == VEC - 1 ? xr : c0c[r].v[v + 1];
-                res.v[v] = euler(c0c[r].v[v], xm, xp, ym.v[v], yp.v[v], c0m[r].v[v], c0p[r].v[v]);
-                if (inner)
-                    acc1 = __builtin_fma((double)res.v[v], (double)res.v[v], acc1);
-            }
-            n1[r] = res;
-            if (inner)
-                *reinterpret_cast<V *>(Bq + (y + 1) * PA + VEC + x0) = res;
-        }
-        // u^{n+1} on the ring of face-neighbour cells (planes the second step computes)
-        if (inner) {
-#pragma unroll
-            for (int h = 0; h < RS; ++h) {
-                const int s = tid + h * WG;
-                if (s < NR) {
-                    int x, y;
-                    if (s < 2 * N) {
-                        x = (s / N) ? N : -1;
-                        y = s % N;
-                    } else {
-                        y = ((s - 2 * N) / N) ? N : -1;
-                        x = (s - 2 * N) % N;
-                    }
-                    const int o = (y + 2) * PA + VEC + x;
-                    Bq[(y + 1) * PA + VEC + x] =
-                        euler(Ac[o], Ac[o - 1], Ac[o + 1], Ac[o - PA], Ac[o + PA], Am[o], Ap[o]);
-                }
-            }
-        }
-        // u^{n+2} on the block's cells of plane q - 1 (its u^{n+1} plane was
-        // completed before this plane's barrier)
-        if (q >= z0 + 1) {
-            const T *Br = slotB(q - 1);
-            T *op = ob + NN * (q - 1);
+            for (int r = 0; r < R; ++r)
+                c0p[r] = Pq.c[r];
+            if (q + 2 <= z1 + 1)
+                Pq = load_u0(q + 2);
+            __syncthreads();
+            const T *Am = slotA(q - 1), *Ac = slotA(q), *Ap = slotA(q + 1);
+            const bool inner = q >= z0 && q < z1;
+            T *Bq = slotB(q);
+            // u^{n+1} on the block's cells of plane q (z neighbours from registers)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int y = ty + r * ROWS;
-                const int o = (y + 1) * PA + VEC + x0;
-                const V ym = *reinterpret_cast<const V *>(Br + o - PA);
-                const V yp = *reinterpret_cast<const V *>(Br + o + PA);
-                const T xl = Br[o - 1], xr = Br[o + VEC];
+                const int o = (y + 2) * PA + VEC + x0;
+                const V ym = *reinterpret_cast<const V *>(Ac + o - PA);
+                const V yp = *reinterpret_cast<const V *>(Ac + o + PA);
+                const T xl = Ac[o - 1], xr = Ac[o + VEC];
                 V res;
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
-                    const T xm = v == 0 ? xl : c1[r].v[v - 1];
-                    const T xp = v == VEC - 1 ? xr : c1[r].v[v + 1];
-                    res.v[v] = euler(c1[r].v[v], xm, xp, ym.v[v], yp.v[v], m1[r].v[v], n1[r].v[v]);
-                    acc2 = __builtin_fma((double)res.v[v], (double)res.v[v], acc2);
+                    const T xm = v == 0 ? xl : c0c[r].v[v - 1];
+                    const T xp = v == VEC - 1 ? xr : c0c[r].v[v + 1];
+                    res.v[v] = euler(c0c[r].v[v], xm, xp, ym.v[v], yp.v[v], c0m[r].v[v], c0p[r].v[v]);
+                    if (inner)
+                        acc1 = __builtin_fma((double)res.v[v], (double)res.v[v], acc1);
                 }
-                *reinterpret_cast<V *>(op + x0 + N * y) = res;
+                n1[r] = res;
+                if (inner)
+                    *reinterpret_cast<V *>(Bq + (y + 1) * PA + VEC + x0) = res;
             }
-        }
+            // u^{n+1} on the ring of face-neighbour cells (planes the second step computes)
+            if (inner) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            c0m[r] = c0c[r];
-            c0c[r] = c0p[r];
-            m1[r] = c1[r];
-            c1[r] = n1[r];
+                for (int h = 0; h < RS; ++h) {
+                    const int s = tid + h * WG;
+                    if (s < NR) {
+                        int x, y;
+                        if (s < 2 * N) {
+                            x = (s / N) ? N : -1;
+                            y = s % N;
+                        } else {
+                            y = ((s - 2 * N) / N) ? N : -1;
+                            x = (s - 2 * N) % N;
+                        }
+                        const int o = (y + 2) * PA + VEC + x;
+                        Bq[(y + 1) * PA + VEC + x] =
+                            euler(Ac[o], Ac[o - 1], Ac[o + 1], Ac[o - PA], Ac[o + PA], Am[o], Ap[o]);
+                    }
+                }
+            }
+            // u^{n+2} on the block's cells of plane q - 1 (its u^{n+1} plane was
+            // completed before this plane's barrier)
+            if (q >= z0 + 1) {
+                const T *Br = slotB(q - 1);
+                T *op = ob + NN * (q - 1);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int y = ty + r * ROWS;
+                    const int o = (y + 1) * PA + VEC + x0;
+                    const V ym = *reinterpret_cast<const V *>(Br + o - PA);
+                    const V yp = *reinterpret_cast<const V *>(Br + o + PA);
+                    const T xl = Br[o - 1], xr = Br[o + VEC];
+                    V res;
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) {
+                        const T xm = v == 0 ? xl : c1[r].v[v - 1];
+                        const T xp = v == VEC - 1 ? xr : c1[r].v[v + 1];
+                        res.v[v] = euler(c1[r].v[v], xm, xp, ym.v[v], yp.v[v], m1[r].v[v], n1[r].v[v]);
+                        acc2 = __builtin_fma((double)res.v[v], (double)res.v[v], acc2);
+                    }
+                    *reinterpret_cast<V *>(op + x0 + N * y) = res;
+                }
+            }
         }
     }
 
