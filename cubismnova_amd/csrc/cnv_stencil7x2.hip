@@ -41,7 +41,11 @@ struct X2Shape {
     static constexpr int TX = N / VEC;
     static constexpr int ROWS = WG / TX;
     static constexpr int R = N / ROWS;
-    static constexpr int PA = N + 2 * VEC;   // row pitch; x = -VEC .. N+VEC-1
+    // row pitch: x = -VEC .. N+VEC-1 plus padding so that a column of 32
+    // rows spreads over 8 of the 32 LDS banks (pitch in 4-byte words = 4 mod 8)
+    static constexpr int WPE = (int)sizeof(T) / 4;
+    static constexpr int PA0 = N + 2 * VEC;
+    static constexpr int PA = (PA0 * WPE) % 8 == 4 ? PA0 : ((PA0 + VEC) * WPE) % 8 == 4 ? PA0 + VEC : PA0 + 2 * VEC;
     static constexpr int PLA = (N + 4) * PA; // u^n plane, y = -2 .. N+1
     static constexpr int PLB = (N + 2) * PA; // u^{n+1} plane, y = -1 .. N
     static constexpr int NHX = 4 * N;        // x halo: 2 layers x 2 sides x N rows (scalars)
@@ -51,9 +55,27 @@ struct X2Shape {
     static constexpr int NR = 4 * N; // ring cells of u^{n+1} per plane
     static constexpr int RS = (NR + WG - 1) / WG;
     // shapes with more than two rows per thread run out of registers
-    static constexpr bool valid =
-        N % VEC == 0 && WG % TX == 0 && ROWS <= N && N % ROWS == 0 && VEC >= 2 && R <= 2 && HS <= 2;
+    // rows of TX lanes must not straddle the 16-lane rows of DPP lane shifts
+    static constexpr bool valid = N % VEC == 0 && WG % TX == 0 && ROWS <= N && N % ROWS == 0 && VEC >= 2 &&
+                                  R <= 2 && HS <= 2 && TX <= 16 && 16 % TX == 0;
 };
+
+// Value held by lane i-1 (CTRL = row_shr:1) or lane i+1 (row_shl:1) within a
+// 16-lane DPP row; callers override the lanes at the ends of their rows.
+template <typename T, int CTRL>
+__device__ inline T dpp_shift(T v)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+    } else {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+        return __builtin_bit_cast(T, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    }
+}
+constexpr int kDppRowShr1 = 0x111; // lane i <- lane i-1
+constexpr int kDppRowShl1 = 0x101; // lane i <- lane i+1
 
 template <typename T, int N, int WG>
 __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
@@ -237,7 +259,14 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
                 const int o = (y + 2) * PA + VEC + x0;
                 const V ym = *reinterpret_cast<const V *>(Ac + o - PA);
                 const V yp = *reinterpret_cast<const V *>(Ac + o + PA);
-                const T xl = Ac[o - 1], xr = Ac[o + VEC];
+                // x neighbours of the end cells from the adjacent lanes; the
+                // first / last lane of a row reads the halo
+                T xl = dpp_shift<T, kDppRowShr1>(c0c[r].v[VEC - 1]);
+                T xr = dpp_shift<T, kDppRowShl1>(c0c[r].v[0]);
+                if (tx == 0)
+                    xl = Ac[o - 1];
+                if (tx == TX - 1)
+                    xr = Ac[o + VEC];
                 V res;
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
@@ -282,7 +311,12 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
                     const int o = (y + 1) * PA + VEC + x0;
                     const V ym = *reinterpret_cast<const V *>(Br + o - PA);
                     const V yp = *reinterpret_cast<const V *>(Br + o + PA);
-                    const T xl = Br[o - 1], xr = Br[o + VEC];
+                    T xl = dpp_shift<T, kDppRowShr1>(c1[r].v[VEC - 1]);
+                    T xr = dpp_shift<T, kDppRowShl1>(c1[r].v[0]);
+                    if (tx == 0)
+                        xl = Br[o - 1];
+                    if (tx == TX - 1)
+                        xr = Br[o + VEC];
                     V res;
 #pragma unroll
                     for (int v = 0; v < VEC; ++v) {
