@@ -217,10 +217,12 @@ __device__ inline void weno5_exact(R a, R b, R c, R d, R e, R &um, R &up)
 
 __device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// FP32 evaluation: same formulas, contracted, divisions by the hardware
-// reciprocal.  Every scaled weight stays within [1e-25, 1e10] for any data
-// (beta_k / m2 <= 170 since max u^2 <= 5 m2), so no product over- or
-// underflows in FP32.
+// FP32 evaluation: same weights, contracted, divisions by the hardware
+// reciprocal, and the linear weights d_k and the 1/6 of the candidate
+// polynomials folded into the polynomial coefficients (num = sum_k p_k Q_k
+// with Q_k = (d_k / 6) q_k, den = sum_k d_k p_k).  Every scaled weight stays
+// within [1e-25, 1e10] for any data (beta_k / m2 <= 170 since
+// max u^2 <= 5 m2), so no product over- or underflows in FP32.
 __device__ inline void weno5_fast(float a, float b, float c, float d, float e, float &um, float &up)
 {
 #pragma clang fp contract(fast)
@@ -236,18 +238,18 @@ __device__ inline void weno5_fast(float a, float b, float c, float d, float e, f
     const float g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
     const float p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
     {
-        const float w0 = 0.1f * p12, w1 = 0.6f * p02, w2 = 0.3f * p01;
-        const float q0 = 2.f * a - 7.f * b + 11.f * c;
-        const float q1 = -b + 5.f * c + 2.f * d;
-        const float q2 = 2.f * c + 5.f * d - e;
-        um = (w0 * q0 + w1 * q1 + w2 * q2) * rcpf(6.f * (w0 + w1 + w2));
+        // d = (0.1, 0.6, 0.3): Q0 = (2a - 7b + 11c) / 60, Q1 = (-b + 5c + 2d) / 10, Q2 = (2c + 5d - e) / 20
+        const float q0 = (2.f / 60.f) * a - (7.f / 60.f) * b + (11.f / 60.f) * c;
+        const float q1 = -0.1f * b + 0.5f * c + 0.2f * d;
+        const float q2 = 0.1f * c + 0.25f * d - 0.05f * e;
+        um = (p12 * q0 + p02 * q1 + p01 * q2) * rcpf(0.1f * p12 + 0.6f * p02 + 0.3f * p01);
     }
     {
-        const float w0 = 0.3f * p12, w1 = 0.6f * p02, w2 = 0.1f * p01;
-        const float q0 = -a + 5.f * b + 2.f * c;
-        const float q1 = 2.f * b + 5.f * c - d;
-        const float q2 = 11.f * c - 7.f * d + 2.f * e;
-        up = (w0 * q0 + w1 * q1 + w2 * q2) * rcpf(6.f * (w0 + w1 + w2));
+        // d = (0.3, 0.6, 0.1): R0 = (-a + 5b + 2c) / 20, R1 = (2b + 5c - d) / 10, R2 = (11c - 7d + 2e) / 60
+        const float r0 = -0.05f * a + 0.25f * b + 0.1f * c;
+        const float r1 = 0.2f * b + 0.5f * c - 0.1f * d;
+        const float r2 = (11.f / 60.f) * c - (7.f / 60.f) * d + (2.f / 60.f) * e;
+        up = (p12 * r0 + p02 * r1 + p01 * r2) * rcpf(0.3f * p12 + 0.6f * p02 + 0.1f * p01);
     }
 }
 

@@ -330,3 +330,32 @@ def test_two_step_autotune_and_fallback(gpu):
         cnv.autotune_diffusion_run(u, t, iters=1)
     with pytest.raises(cnv.CnvError):
         cnv.set_tune_diffusion_run(np.float32, 32, 64, 8)     # shape that runs out of registers
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_two_step_kernel_multicomponent(gpu, dtype):
+    """Three components in one periodic grid through diffusion_run (two steps
+    per launch): each component bit-identical to its own single steps, the
+    norms summed over components."""
+    nb, n = (2, 3, 2), (16, 16, 16)
+    u = cnv.Cartesian(nb, n, dtype=dtype, ncomp=3)
+    t = cnv.Cartesian(nb, n, dtype=dtype, ncomp=3)
+    h = [1.0 / (nb[d] * 16) for d in range(3)]
+    dt = 0.1 * min(h) ** 2
+    nsteps = 6
+    refs, tot = [], np.zeros(nsteps)
+    for c in range(3):
+        slab = random_slab(500 + c, nb, n, dtype)
+        u.upload(c, slab)
+        cur = slab
+        for k in range(nsteps):
+            cur, s = oracle.diffusion_step(oracle.Field(cur, nb, n), h, dt)
+            tot[k] += s
+        refs.append(cur)
+    norms = cnv.DeviceBuffer(8 * nsteps)
+    res = cnv.diffusion_run(u, t, nsteps, dt, norms=norms)
+    assert res is u
+    for c in range(3):
+        assert_same(res.download(c), refs[c], "component %d" % c)
+    gn = norms.download(np.float64, nsteps)
+    assert np.all(np.abs(gn - tot) <= 1e-13 * tot), (gn, tot)
