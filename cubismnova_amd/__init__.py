@@ -246,14 +246,22 @@ class _Mesh:
         return self.begin + (np.asarray(p, dtype=np.float64) + 0.5) * self.h
 
 
+ENTITIES = {"cell": _abi.CNV_CELL, "node": _abi.CNV_NODE, "face": _abi.CNV_FACE}
+
+
 class Cartesian:
-    """Grid::Cartesian for cell-centred scalar components on the GPU.
+    """Grid::Cartesian for scalar components on the GPU.
 
     Mirrors include/Cubism/Grid/Cartesian.h:44-550: one allocation, component
-    slabs of block_bytes-aligned blocks, blocks numbered x-fastest.
+    slabs of block_bytes-aligned blocks, blocks numbered x-fastest.  entity
+    "node" / "face" (Cubism::EntityType) gives (N+1)^3-element blocks whose
+    upper-end blocks hold N+1 along their plus dimensions; a face grid has
+    3 * ncomp slabs, slab d * ncomp + c (BlockFieldAssembler.h:111-185), and
+    every `comp` argument is a slab index.  The stencil operators take cell
+    grids only.
     """
 
-    def __init__(self, nblocks, block_cells, begin=0.0, end=1.0, dtype=np.float64, ncomp=1):
+    def __init__(self, nblocks, block_cells, begin=0.0, end=1.0, dtype=np.float64, ncomp=1, entity="cell"):
         nb = [int(nblocks)] * 3 if np.isscalar(nblocks) else [int(v) for v in nblocks]
         bc = [int(block_cells)] * 3 if np.isscalar(block_cells) else [int(v) for v in block_cells]
         b = [float(begin)] * 3 if np.isscalar(begin) else [float(v) for v in begin]
@@ -266,6 +274,11 @@ class Cartesian:
             d.nblocks[k], d.block_cells[k], d.begin[k], d.end[k] = nb[k], bc[k], b[k], e[k]
         d.ncomp = int(ncomp)
         d.dtype = _abi.CNV_F64 if self.dtype == np.float64 else _abi.CNV_F32
+        if entity not in ENTITIES:
+            raise ValueError("entity must be 'cell', 'node' or 'face'")
+        d.entity = ENTITIES[entity]
+        self.entity = entity
+        self.nslabs = 3 * int(ncomp) if entity == "face" else int(ncomp)
         h = ctypes.c_void_p()
         call("cnv_grid_create", ctypes.byref(d), ctypes.byref(h))
         self.handle = h
@@ -293,6 +306,29 @@ class Cartesian:
 
     def block_flat(self, bi):
         return bi[0] + self.nblocks[0] * (bi[1] + self.nblocks[1] * bi[2])
+
+    def plus(self, comp=0):
+        """Dimensions in which the upper-end blocks of slab `comp` hold one
+        more element (BlockFieldAssembler.h:123-131)."""
+        if self.entity == "node":
+            return (1, 1, 1)
+        if self.entity == "face":
+            d = comp // self.ncomp
+            return tuple(1 if e == d else 0 for e in range(3))
+        return (0, 0, 0)
+
+    def block_extent(self, bi, comp=0):
+        """Elements of block bi of slab `comp` per dimension."""
+        p = self.plus(comp)
+        return [self.block_cells[d] + (1 if p[d] and bi[d] == self.nblocks[d] - 1 else 0) for d in range(3)]
+
+    def lab_geometry(self, stencil, comp=0, block_index=None):
+        """(pitch, active): the slab's largest lab extent and block_index's own."""
+        pitch, active = (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+        bi = None if block_index is None else (ctypes.c_int * 3)(*block_index)
+        sc = stencil._c()
+        call("cnv_lab_geometry", self.handle, comp, bi, ctypes.byref(sc), pitch, active)
+        return list(pitch), list(active)
 
     # -- data movement (reference byte layout)
     def upload(self, comp, slab, stream=None):
@@ -345,7 +381,7 @@ class Cartesian:
         c = bc._c()
         call("cnv_grid_add_face_bc", self.handle, comp, ctypes.byref(c))
         d, s = bc.getBoundaryInfo().dir, bc.getBoundaryInfo().side
-        comps = range(self.ncomp) if comp == -1 else [comp]
+        comps = range(self.nslabs) if comp == -1 else [comp]
         for cc in comps:
             for bz in range(self.nblocks[2]):
                 for by in range(self.nblocks[1]):
@@ -367,10 +403,11 @@ class Cartesian:
         lab.loadData(block_index, self, comp=comp)
 
     def lab_all(self, stencil, comp=0, out=None, stream=None):
-        """Labs of every block (device buffer, blocks consecutive); returns the buffer."""
-        ext = (ctypes.c_int * 3)()
+        """Labs of every block (device buffer, blocks consecutive, one slot of
+        the largest block's lab extent each, a smaller block's lab in the low
+        corner); returns (buffer, slot extent)."""
         sc = stencil._c()
-        call("cnv_lab_extent", self.handle, ctypes.byref(sc), ext)
+        ext, _ = self.lab_geometry(stencil, comp)
         n = ext[0] * ext[1] * ext[2] * self.nblocks_total
         if out is None:
             out = DeviceBuffer(n * self.dtype.itemsize)
@@ -417,8 +454,7 @@ class FieldLab:
             raise RuntimeError("FieldLab: block larger than the allocated maximum range")
         s = self._stencil
         sc = s._c()
-        ext = (ctypes.c_int * 3)()
-        call("cnv_lab_extent", grid.handle, ctypes.byref(sc), ext)
+        _, ext = grid.lab_geometry(s, comp, block_index)
         n = ext[0] * ext[1] * ext[2]
         buf = DeviceBuffer(n * grid.dtype.itemsize)
         buf.fill(grid.dtype, np.nan if poison is None else poison)
@@ -432,7 +468,7 @@ class FieldLab:
                  None)
         self._data = buf.download(grid.dtype).reshape(ext[2], ext[1], ext[0])
         self._ext = list(ext)
-        self._N = list(grid.block_cells)
+        self._N = grid.block_extent(block_index, comp)
         self._field = (grid, tuple(block_index), comp)
         return self._data
 

@@ -21,6 +21,7 @@ CNV_ERR_SIZE = 6
 
 CNV_F32 = 0
 CNV_F64 = 1
+CNV_CELL, CNV_NODE, CNV_FACE = 0, 1, 2
 
 CNV_BC_BASE = 0
 CNV_BC_ABSORBING = 1
@@ -40,7 +41,8 @@ class cnv_stencil(ctypes.Structure):
 class cnv_grid_desc(ctypes.Structure):
     _fields_ = [("nblocks", ctypes.c_int * 3), ("block_cells", ctypes.c_int * 3),
                 ("ncomp", ctypes.c_int), ("dtype", ctypes.c_int),
-                ("begin", ctypes.c_double * 3), ("end", ctypes.c_double * 3)]
+                ("begin", ctypes.c_double * 3), ("end", ctypes.c_double * 3),
+                ("entity", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
 class cnv_solve_opts(ctypes.Structure):
@@ -91,6 +93,7 @@ SIGNATURES = {
     "cnv_grid_add_face_bc": (i32, [vp, i32, ctypes.POINTER(cnv_bc)]),
     "cnv_grid_clear_bcs": (i32, [vp, i32]),
     "cnv_lab_extent": (i32, [vp, ctypes.POINTER(cnv_stencil), c_int_p]),
+    "cnv_lab_geometry": (i32, [vp, i32, c_int_p, ctypes.POINTER(cnv_stencil), c_int_p, c_int_p]),
     "cnv_lab_load": (i32, [vp, i32, c_int_p, ctypes.POINTER(cnv_stencil), i32, ctypes.POINTER(cnv_bc), i32,
                            vp, vp]),
     "cnv_lab_load_all": (i32, [vp, i32, ctypes.POINTER(cnv_stencil), vp, vp]),

@@ -108,8 +108,16 @@ static int check_grid(const cnv_grid_s *g)
 
 static int check_comp(const cnv_grid_s *g, int comp)
 {
-    if (comp < 0 || comp >= g->desc.ncomp)
+    if (comp < 0 || comp >= g->nslabs)
         return fail(CNV_ERR_INVALID, "component index out of range");
+    return CNV_OK;
+}
+
+// the stencil operators are defined on cell-centred data
+static int check_cell(const cnv_grid_s *g)
+{
+    if (g && g->desc.entity != CNV_CELL)
+        return fail(CNV_ERR_UNSUPPORTED, "operator takes cell-centred grids only");
     return CNV_OK;
 }
 
@@ -266,7 +274,16 @@ int cnv_grid_create(const cnv_grid_desc *desc, cnv_grid *out)
         return fail(CNV_ERR_INVALID, "ncomp must be >= 1");
     if (desc->dtype != CNV_F32 && desc->dtype != CNV_F64)
         return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    if (desc->entity != CNV_CELL && desc->entity != CNV_NODE && desc->entity != CNV_FACE)
+        return fail(CNV_ERR_INVALID, "entity must be CNV_CELL, CNV_NODE or CNV_FACE");
     const long long cells = (long long)desc->block_cells[0] * desc->block_cells[1] * desc->block_cells[2];
+    // Cartesian::alloc_ (Grid/Cartesian.h:490-503): (N+1)^3 elements per
+    // block for node and face data, 3 face slabs per component
+    const long long elements = desc->entity == CNV_CELL
+                                   ? cells
+                                   : (long long)(desc->block_cells[0] + 1) * (desc->block_cells[1] + 1) *
+                                         (desc->block_cells[2] + 1);
+    const int nslabs = desc->entity == CNV_FACE ? 3 * desc->ncomp : desc->ncomp;
     const long long nbt = (long long)desc->nblocks[0] * desc->nblocks[1] * desc->nblocks[2];
     if (cells > (1LL << 30) || nbt > (1LL << 30))
         return fail(CNV_ERR_INVALID, "grid too large");
@@ -275,14 +292,15 @@ int cnv_grid_create(const cnv_grid_desc *desc, cnv_grid *out)
     g->desc = *desc;
     g->elem = desc->dtype == CNV_F64 ? 8 : 4;
     // Cartesian::alloc_ (Grid/Cartesian.h:490-529): 32-byte aligned blocks
-    g->block_bytes = ((size_t)cells * g->elem + 31) / 32 * 32;
+    g->block_bytes = ((size_t)elements * g->elem + 31) / 32 * 32;
     g->nblocks_total = nbt;
+    g->nslabs = nslabs;
     g->component_bytes = g->block_bytes * (size_t)nbt;
     for (int d = 0; d < 3; ++d)
         g->h[d] = (desc->end[d] - desc->begin[d]) / (double)(desc->nblocks[d] * desc->block_cells[d]);
-    g->bcs.assign(desc->ncomp, std::vector<std::vector<cnv_bc>>((size_t)nbt));
-    g->tables.resize(desc->ncomp);
-    const size_t all = g->component_bytes * (size_t)desc->ncomp;
+    g->bcs.assign((size_t)nslabs, std::vector<std::vector<cnv_bc>>((size_t)nbt));
+    g->tables.resize((size_t)nslabs);
+    const size_t all = g->component_bytes * (size_t)nslabs;
     hipError_t e = hipMalloc(&g->data, all);
     if (e != hipSuccess) {
         delete g;
@@ -418,7 +436,7 @@ int cnv_grid_add_face_bc(cnv_grid g, int comp, const cnv_bc *bc)
         CNV_CHECK(check_comp(g, comp));
     std::lock_guard<std::mutex> lk(g->mtx);
     const int *nb = g->desc.nblocks;
-    for (int c = 0; c < g->desc.ncomp; ++c) {
+    for (int c = 0; c < g->nslabs; ++c) {
         if (comp != -1 && c != comp)
             continue;
         for (int bz = 0; bz < nb[2]; ++bz)
@@ -443,7 +461,7 @@ int cnv_grid_clear_bcs(cnv_grid g, int comp)
     if (comp != -1)
         CNV_CHECK(check_comp(g, comp));
     std::lock_guard<std::mutex> lk(g->mtx);
-    for (int c = 0; c < g->desc.ncomp; ++c)
+    for (int c = 0; c < g->nslabs; ++c)
         if (comp == -1 || c == comp)
             for (auto &l : g->bcs[c])
                 l.clear();
@@ -459,8 +477,31 @@ int cnv_lab_extent(cnv_grid g, const cnv_stencil *s, int extent[3])
     CNV_CHECK(check_stencil(s));
     if (!extent)
         return fail(CNV_ERR_INVALID, "extent is NULL");
+    const int plus = g->desc.entity == CNV_CELL ? 0 : 1;
     for (int d = 0; d < 3; ++d)
-        extent[d] = g->desc.block_cells[d] + s->end[d] - 1 - s->begin[d];
+        extent[d] = g->desc.block_cells[d] + plus + s->end[d] - 1 - s->begin[d];
+    return CNV_OK;
+}
+
+int cnv_lab_geometry(cnv_grid g, int comp, const int block[3], const cnv_stencil *s, int pitch[3], int active[3])
+{
+    CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_comp(g, comp));
+    CNV_CHECK(check_stencil(s));
+    if (block)
+        for (int d = 0; d < 3; ++d)
+            if (block[d] < 0 || block[d] >= g->desc.nblocks[d])
+                return fail(CNV_ERR_INVALID, "block index out of range");
+    const int pm = plus_mask(g, comp);
+    for (int d = 0; d < 3; ++d) {
+        const int p = (pm >> d) & 1;
+        const int wide = p && (!block || block[d] == g->desc.nblocks[d] - 1);
+        const int w = s->end[d] - 1 - s->begin[d];
+        if (pitch)
+            pitch[d] = g->desc.block_cells[d] + p + w;
+        if (active)
+            active[d] = g->desc.block_cells[d] + wide + w;
+    }
     return CNV_OK;
 }
 
@@ -541,7 +582,9 @@ int cnv_laplacian(cnv_grid in, int comp_in, cnv_grid out, int comp_out, cnv_stre
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(in));
+    CNV_CHECK(check_cell(in));
     CNV_CHECK(check_grid(out));
+    CNV_CHECK(check_cell(out));
     CNV_CHECK(check_comp(in, comp_in));
     CNV_CHECK(check_comp(out, comp_out));
     CNV_CHECK(same_shape(in, out));
@@ -556,7 +599,9 @@ int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_ou
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_cell(u));
     CNV_CHECK(check_grid(u_new));
+    CNV_CHECK(check_cell(u_new));
     CNV_CHECK(same_shape(u, u_new));
     if (u == u_new)
         return fail(CNV_ERR_INVALID, "u and u_new must be different grids");
@@ -576,7 +621,9 @@ int cnv_diffusion_run(cnv_grid u, cnv_grid tmp, int nsteps, double nu_dt, double
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_cell(u));
     CNV_CHECK(check_grid(tmp));
+    CNV_CHECK(check_cell(tmp));
     CNV_CHECK(same_shape(u, tmp));
     if (u == tmp)
         return fail(CNV_ERR_INVALID, "u and tmp must be different grids");
@@ -607,6 +654,7 @@ int cnv_grid_sums(cnv_grid g, int comp, double *out, cnv_stream s)
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_cell(g));
     CNV_CHECK(check_comp(g, comp));
     if (!out)
         return fail(CNV_ERR_INVALID, "out is NULL");
@@ -618,7 +666,9 @@ int cnv_grid_compare(cnv_grid a, int comp_a, cnv_grid b, int comp_b, double *out
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(a));
+    CNV_CHECK(check_cell(a));
     CNV_CHECK(check_grid(b));
+    CNV_CHECK(check_cell(b));
     CNV_CHECK(check_comp(a, comp_a));
     CNV_CHECK(check_comp(b, comp_b));
     if (!out)
@@ -634,7 +684,9 @@ int cnv_weno5(cnv_grid in, cnv_grid out, cnv_stream s)
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(in));
+    CNV_CHECK(check_cell(in));
     CNV_CHECK(check_grid(out));
+    CNV_CHECK(check_cell(out));
     CNV_CHECK(same_shape(in, out));
     if (in == out)
         return fail(CNV_ERR_INVALID, "input and output grids must differ");
@@ -649,7 +701,9 @@ int cnv_autotune_weno5(cnv_grid in, cnv_grid out, int iters, int *wg, int *zchun
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(in));
+    CNV_CHECK(check_cell(in));
     CNV_CHECK(check_grid(out));
+    CNV_CHECK(check_cell(out));
     CNV_CHECK(same_shape(in, out));
     if (in == out || out->desc.ncomp < 6 * in->desc.ncomp)
         return fail(CNV_ERR_INVALID, "out must be a separate grid with 6 components per input component");
@@ -681,6 +735,7 @@ int cnv_grid_set_zfaces(cnv_grid g, const double *zfaces, int n)
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(g));
+    CNV_CHECK(check_cell(g));
     if (!zfaces)
         return fail(CNV_ERR_INVALID, "zfaces is NULL");
     std::lock_guard<std::mutex> lk(g->mtx);
@@ -691,8 +746,11 @@ int cnv_grid_set_zfaces(cnv_grid g, const double *zfaces, int n)
 static int check_jacobi_grids(const cnv_grid_s *u, int cu, const cnv_grid_s *f, int cf, const cnv_grid_s *o, int co)
 {
     CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_cell(u));
     CNV_CHECK(check_grid(f));
+    CNV_CHECK(check_cell(f));
     CNV_CHECK(check_grid(o));
+    CNV_CHECK(check_cell(o));
     CNV_CHECK(check_comp(u, cu));
     CNV_CHECK(check_comp(f, cf));
     CNV_CHECK(check_comp(o, co));
@@ -725,7 +783,9 @@ int cnv_add_correction(cnv_grid u, int cu, cnv_grid e, int ce, cnv_stream s)
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_cell(u));
     CNV_CHECK(check_grid(e));
+    CNV_CHECK(check_cell(e));
     CNV_CHECK(check_comp(u, cu));
     CNV_CHECK(check_comp(e, ce));
     if (u->desc.dtype != CNV_F64)
@@ -758,8 +818,11 @@ int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_cell(u));
     CNV_CHECK(check_grid(f));
+    CNV_CHECK(check_cell(f));
     CNV_CHECK(check_grid(work));
+    CNV_CHECK(check_cell(work));
     if (!o || !info)
         return fail(CNV_ERR_INVALID, "options or info is NULL");
     if (o->sweeps_per_cycle < 2 || o->sweeps_per_cycle % 2 != 0)
@@ -894,8 +957,14 @@ int cnv_grid_prepare(cnv_grid g)
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(g));
-    for (int c = 0; c < g->desc.ncomp; ++c)
-        CNV_CHECK(jacobi_prepare(g, c));
+    for (int c = 0; c < g->nslabs; ++c) {
+        if (g->desc.entity == CNV_CELL) {
+            CNV_CHECK(jacobi_prepare(g, c));
+        } else {
+            std::lock_guard<std::mutex> lk(g->mtx);
+            CNV_CHECK(ensure_bc_table(g, c));
+        }
+    }
     return CNV_OK;
     CNV_GUARD_END
 }
@@ -907,7 +976,9 @@ int cnv_autotune_stencil7(cnv_grid u, cnv_grid out, int iters, int *wg, int *zch
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_cell(u));
     CNV_CHECK(check_grid(out));
+    CNV_CHECK(check_cell(out));
     CNV_CHECK(same_shape(u, out));
     if (u == out)
         return fail(CNV_ERR_INVALID, "autotune needs two different grids");
@@ -922,7 +993,9 @@ int cnv_autotune_diffusion_run(cnv_grid u, cnv_grid tmp, int iters, int *wg, int
 {
     CNV_GUARD_BEGIN
     CNV_CHECK(check_grid(u));
+    CNV_CHECK(check_cell(u));
     CNV_CHECK(check_grid(tmp));
+    CNV_CHECK(check_cell(tmp));
     CNV_CHECK(same_shape(u, tmp));
     if (u == tmp)
         return fail(CNV_ERR_INVALID, "autotune needs two different grids");

@@ -495,6 +495,8 @@ int cnv_batch_create(const cnv_batch_item *items, int n, cnv_batch *out)
                 return fail(CNV_ERR_INVALID, "batch item needs distinct input and output grids");
             if (it.in->desc.dtype != CNV_F32 || it.out->desc.dtype != CNV_F32)
                 return fail(CNV_ERR_UNSUPPORTED, "the batched kernel runs FP32 grids");
+            if (it.in->desc.entity != CNV_CELL || it.out->desc.entity != CNV_CELL)
+                return fail(CNV_ERR_UNSUPPORTED, "operator takes cell-centred grids only");
             if (it.op != CNV_OP_DIFFUSION && it.op != CNV_OP_WENO5)
                 return fail(CNV_ERR_INVALID, "unknown batch operator");
             if (it.ghost < (it.op == CNV_OP_WENO5 ? 2 : 1) || it.ghost > 8)

@@ -73,6 +73,7 @@ struct cnv_grid_s {
     size_t block_bytes = 0;
     size_t component_bytes = 0;
     long long nblocks_total = 0;
+    int nslabs = 0;             // ncomp, 3 * ncomp for face grids
     void *data = nullptr;       // device allocation, ncomp component slabs
     double h[3] = {0, 0, 0};    // mesh spacing (extent / cells)
 
@@ -95,6 +96,18 @@ struct cnv_grid_s {
 
 namespace cnv
 {
+
+// Dimensions in which the blocks at the upper end hold one more element:
+// bit d for node grids in every d, for face slab s in d = s / ncomp
+// (BlockFieldAssembler.h:123-131).
+inline int plus_mask(const cnv_grid_s *g, int slab)
+{
+    if (g->desc.entity == CNV_NODE)
+        return 7;
+    if (g->desc.entity == CNV_FACE)
+        return 1 << (slab / g->desc.ncomp);
+    return 0;
+}
 
 inline cnv::Geom geom_of(const cnv_grid_s *g)
 {

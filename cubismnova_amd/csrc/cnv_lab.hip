@@ -10,6 +10,12 @@
 // labs; sides skipped by a non-periodic BC).  That is the same result as the
 // sequential passes because every BC pass reads only cells it does not write
 // (BC/Absorbing.h:180-188, BC/Symmetry.h:107-113).
+//
+// Node and face slabs: a block at the upper end of a "plus" dimension holds
+// one more element there, so each block's extent (and its lab range) is its
+// own, and ghosts below a block count back from the neighbour's own end
+// (FieldLabLoader.h:217-219, 236-241).  The all-blocks launch lays every lab
+// out in the largest block's geometry, the block's lab in the low corner.
 #include "cnv_internal.h"
 
 namespace cnv
@@ -24,7 +30,8 @@ struct LabArgs {
     int sb[3], se[3];
     int tens;
     int apply_bc;
-    int L[3];       // lab extent
+    int plus;       // bit d: upper-end blocks one element larger along d
+    int L[3];       // lab slot extent (the largest block's lab)
     long long lcells; // L0*L1*L2
     T *out;
     // single-block mode: block index and the list by value
@@ -38,11 +45,18 @@ struct LabArgs {
     const DevBC *entries;
 };
 
+// extent of block B along d (node / face slabs: one more at the upper end)
+template <typename T>
+__device__ inline int blk_extent(const LabArgs<T> &a, const int *B, int d)
+{
+    return a.G.n[d] + (((a.plus >> d) & 1) && B[d] == a.G.nb[d] - 1 ? 1 : 0);
+}
+
 template <typename T>
 __device__ bool lab_value(const LabArgs<T> &a, const int B[3], const DevBC *list, int n,
                           long long q0, long long q1, long long q2, T &out)
 {
-    const int *N = a.G.n;
+    const int N[3] = {blk_extent(a, B, 0), blk_extent(a, B, 1), blk_extent(a, B, 2)};
     long long q[3] = {q0, q1, q2};
     bool periodic[3] = {true, true, true};
     int skip[3] = {1, 1, 1};
@@ -112,10 +126,15 @@ __device__ bool lab_value(const LabArgs<T> &a, const int B[3], const DevBC *list
         if (!a.tens && l1 > 1)
             return false; // edge/corner of a non-tensorial lab
         const long long bf = nb[0] + (long long)a.G.nb[0] * (nb[1] + (long long)a.G.nb[1] * nb[2]);
-        const long long x = q[0] - (long long)bi[0] * N[0];
-        const long long y = q[1] - (long long)bi[1] * N[1];
-        const long long z = q[2] - (long long)bi[2] * N[2];
-        v = a.base[bf * a.G.bstride + x + (long long)N[0] * (y + (long long)N[1] * z)];
+        // source in the neighbour's own extent NE; below the block it counts
+        // back from the neighbour's end (shift NE - N)
+        long long s[3];
+        int NE[3];
+        for (int d = 0; d < 3; ++d) {
+            NE[d] = blk_extent(a, nb, d);
+            s[d] = q[d] - (long long)bi[d] * N[d] + (bi[d] < 0 ? NE[d] - N[d] : 0);
+        }
+        v = a.base[bf * a.G.bstride + s[0] + (long long)NE[0] * (s[1] + (long long)NE[1] * s[2])];
     }
     for (int i = depth - 1; i >= 0; --i)
         v = signs[i] * v;
@@ -161,6 +180,11 @@ __global__ __launch_bounds__(256) void k_lab(const LabArgs<T> a)
     const long long x = c % a.L[0];
     const long long y = (c / a.L[0]) % a.L[1];
     const long long z = c / ((long long)a.L[0] * a.L[1]);
+    if (a.plus) { // smaller blocks fill the low corner of the slot
+        if (x >= blk_extent(a, B, 0) + a.se[0] - 1 - a.sb[0] || y >= blk_extent(a, B, 1) + a.se[1] - 1 - a.sb[1] ||
+            z >= blk_extent(a, B, 2) + a.se[2] - 1 - a.sb[2])
+            return;
+    }
     T v;
     if (lab_value(a, B, s_list, s_n, x + a.sb[0], y + a.sb[1], z + a.sb[2], v))
         a.out[slot * a.lcells + c] = v;
@@ -174,10 +198,13 @@ static int launch_lab_t(const cnv_grid_s *g, int comp, const cnv_stencil &s, boo
     LabArgs<T> a;
     a.G = geom_of(g);
     a.base = reinterpret_cast<const T *>((const char *)g->data + (size_t)comp * g->component_bytes);
+    a.plus = plus_mask(g, comp);
     for (int d = 0; d < 3; ++d) {
         a.sb[d] = s.begin[d];
         a.se[d] = s.end[d];
-        a.L[d] = a.G.n[d] + s.end[d] - 1 - s.begin[d];
+        // one block: its own lab extent; all blocks: the largest block's
+        const int wide = ((a.plus >> d) & 1) && (!single || block[d] == a.G.nb[d] - 1);
+        a.L[d] = a.G.n[d] + wide + s.end[d] - 1 - s.begin[d];
     }
     a.tens = s.tensorial ? 1 : 0;
     a.apply_bc = apply_bc ? 1 : 0;

@@ -316,7 +316,7 @@ public:
               const std::array<double, 3> &end = {1, 1, 1}, const size_t ncomp = 1)
         : nblocks_(nblocks), cells_(block_cells), ncomp_(ncomp)
     {
-        cnv_grid_desc d;
+        cnv_grid_desc d{}; // entity: cell
         for (int k = 0; k < 3; ++k) {
             d.nblocks[k] = nblocks[k];
             d.block_cells[k] = block_cells[k];

@@ -68,9 +68,20 @@ typedef struct {
     int tensorial;
 } cnv_stencil;
 
-/* Grid::Cartesian(nblocks, block_cells, begin, end) for cell-centred scalar
- * components (include/Cubism/Grid/Cartesian.h:132-143). ncomp independent
- * scalar components stored as consecutive component slabs. */
+/* Data entity of a grid: Cubism::EntityType (include/Cubism/Common.h). */
+enum cnv_entity { CNV_CELL = 0, CNV_NODE = 1, CNV_FACE = 2 };
+
+/* Grid::Cartesian(nblocks, block_cells, begin, end) with scalar components
+ * (include/Cubism/Grid/Cartesian.h:132-143). ncomp independent scalar
+ * components stored as consecutive component slabs.  Node and face grids
+ * (Cartesian.h:490-529, BlockFieldAssembler.h:111-185) hold (N+1)^3 elements
+ * per block; a block at the upper end of dimension d stores N[d]+1 elements
+ * along d (every d for nodes, d = the face direction for faces), the others
+ * N[d], each block x-fastest with its own extent as pitch.  A face grid holds
+ * 3*ncomp slabs, slab d*ncomp + c the faces normal to d of component c; every
+ * call taking a component index takes a slab index there.  The operators
+ * (Laplacian, diffusion, WENO5, Jacobi, sums, batches) take cell grids only;
+ * labs, BCs, uploads and the field operations take any entity. */
 typedef struct {
     int nblocks[3];
     int block_cells[3];
@@ -78,6 +89,8 @@ typedef struct {
     int dtype; /* enum cnv_dtype */
     double begin[3];
     double end[3];
+    int entity; /* enum cnv_entity; 0 (cell) when zero-initialised */
+    int reserved;
 } cnv_grid_desc;
 
 /* --------------------------------------------------------------- errors */
@@ -133,21 +146,33 @@ int cnv_grid_clear_bcs(cnv_grid g, int comp);
 /* ------------------------------------------------------------------- lab */
 
 /* Extent of the active lab range of a block for stencil s:
- * N + end - 1 - begin per dimension (Block/FieldLab.h:198-202). */
+ * N + end - 1 - begin per dimension (Block/FieldLab.h:198-202).  On node
+ * and face grids N is the largest block's extent over all slabs. */
 int cnv_lab_extent(cnv_grid g, const cnv_stencil *s, int extent[3]);
+/* Lab geometry of slab `comp` for stencil s: pitch = the lab extent of the
+ * slab's largest block (the per-block layout of cnv_lab_load_all), active =
+ * the lab extent of block `block` (the layout cnv_lab_load writes; in
+ * cnv_lab_load_all the block's lab occupies the low corner [0, active) of
+ * its pitch-sized slot).  block may be NULL (active = pitch).  Cell grids:
+ * both N + end - 1 - begin. */
+int cnv_lab_geometry(cnv_grid g, int comp, const int block[3], const cnv_stencil *s, int pitch[3],
+                     int active[3]);
 
 /* Replaces Grid::Cartesian::loadLab + FieldLab::loadData
  * (Grid/Cartesian.h:411-427, Block/FieldLab.h:180-248, FieldLabLoader.h
  * :129-287): assembles the lab of block `block` of component `comp` and
  * writes it to device memory `out` in compact x-fastest layout over the
- * active lab range [begin, N + end - 1). Cells the reference leaves unwritten
+ * active lab range [begin, N + end - 1), N the block's own extent (node and
+ * face blocks below the upper end are one smaller; their ghosts below the
+ * block count back from the neighbour's own end, FieldLabLoader.h:217-219). Cells the reference leaves unwritten
  * (edges/corners of non-tensorial labs, skipped sides without BC) are left
  * untouched in `out`. n_extern < 0 uses the block's own BC list, otherwise
  * the `extern_bcs` list (FieldLab.h:184,206,241-248). */
 int cnv_lab_load(cnv_grid g, int comp, const int block[3], const cnv_stencil *s, int apply_bc,
                  const cnv_bc *extern_bcs, int n_extern, void *out, cnv_stream st);
 /* The same for every block of the grid, blocks consecutive in block order
- * (the reference's per-block loop over a grid, Laplacian.cpp:110-117). */
+ * (the reference's per-block loop over a grid, Laplacian.cpp:110-117), one
+ * slot of pitch cells each (cnv_lab_geometry). */
 int cnv_lab_load_all(cnv_grid g, int comp, const cnv_stencil *s, void *out, cnv_stream st);
 
 /* ------------------------------------------------------------- operators */

@@ -28,7 +28,7 @@ class _bc(ctypes.Structure):
 
 class _field(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("dtype", ctypes.c_int), ("nblocks", ctypes.c_int * 3),
-                ("cells", ctypes.c_int * 3), ("reserved", ctypes.c_int), ("block_stride", ctypes.c_longlong),
+                ("cells", ctypes.c_int * 3), ("plus", ctypes.c_int), ("block_stride", ctypes.c_longlong),
                 ("bc_offsets", ctypes.c_void_p), ("bcs", ctypes.c_void_p)]
 
 
@@ -79,11 +79,16 @@ class Field:
     bcs: dict block-flat-index -> list of (kind, dir, side, value).
     """
 
-    def __init__(self, slab, nblocks, cells, block_stride=None, bcs=None):
+    def __init__(self, slab, nblocks, cells, block_stride=None, bcs=None, plus=(0, 0, 0)):
+        """plus[d] = 1: blocks at the upper end of dim d hold cells[d] + 1
+        elements (node fields in every dim, face fields in their direction)."""
         self.slab = np.ascontiguousarray(slab)
         assert self.slab.dtype in (np.float32, np.float64)
         self.nblocks, self.cells = _s3(nblocks), _s3(cells)
-        self.block_stride = int(np.prod(self.cells)) if block_stride is None else int(block_stride)
+        self.plus = [int(bool(x)) for x in plus]
+        if block_stride is None:
+            block_stride = int(np.prod([self.cells[d] + self.plus[d] for d in range(3)]))
+        self.block_stride = int(block_stride)
         nbt = int(np.prod(self.nblocks))
         offs = np.zeros(nbt + 1, dtype=np.int32)
         flat = []
@@ -103,6 +108,7 @@ class Field:
         for d in range(3):
             f.nblocks[d], f.cells[d] = self.nblocks[d], self.cells[d]
         f.block_stride = self.block_stride
+        f.plus = self.plus[0] | (self.plus[1] << 1) | (self.plus[2] << 2)
         f.bc_offsets = self._offs.ctypes.data if self._has else None
         f.bcs = ctypes.cast(self._bcs, ctypes.c_void_p).value if self._has else None
         self._f = f
@@ -125,9 +131,14 @@ def lab_extent(cells, begin, end):
     return [c[d] + e[d] - 1 - b[d] for d in range(3)]
 
 
+def block_extent(field, bidx):
+    """Element extent of block bidx (one more at the upper end of plus dims)."""
+    return [field.cells[d] + (1 if field.plus[d] and bidx[d] == field.nblocks[d] - 1 else 0) for d in range(3)]
+
+
 def lab_load(field, bidx, begin, end, tensorial=False, apply_bc=True, extern=None, poison=np.nan):
     """Lab of block `bidx` over the active lab range, as a (Z, Y, X) array."""
-    L = lab_extent(field.cells, begin, end)
+    L = lab_extent(block_extent(field, bidx), begin, end)
     out = np.empty(L[2] * L[1] * L[0], dtype=field.slab.dtype)
     st = stencil(begin, end, tensorial)
     bi = (ctypes.c_int * 3)(*bidx)

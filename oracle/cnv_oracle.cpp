@@ -59,7 +59,8 @@ typedef struct {
     int dtype;             // 0 = float, 1 = double
     int nblocks[3];
     int cells[3];
-    int reserved;
+    int plus;               // bit d set: blocks at the upper end of dim d hold one
+                            // more element (node / face fields, BlockFieldAssembler.h:121-131)
     long long block_stride; // elements between consecutive blocks
     const int *bc_offsets;  // nblocks_total + 1 prefix offsets, or NULL
     const orc_bc *bcs;      // flattened per-block BC lists
@@ -86,6 +87,13 @@ inline int wrap_block(int p, int n) { return (p + n) % n; }
 inline long long block_flat(const int b[3], const int nb[3])
 {
     return b[0] + (long long)nb[0] * (b[1] + (long long)nb[1] * b[2]);
+}
+
+// element extent of block b in dim d: cells, plus one at the upper end of
+// dims flagged in f.plus (node fields, or face fields in their direction)
+inline int block_extent(const orc_field &f, const int b[3], int d)
+{
+    return f.cells[d] + (((f.plus >> d) & 1) && b[d] == f.nblocks[d] - 1 ? 1 : 0);
 }
 
 template <typename T>
@@ -159,9 +167,12 @@ struct OLab {
                 continue;
             if (!tens && (std::abs(bi[0]) + std::abs(bi[1]) + std::abs(bi[2]) > 1))
                 continue;
-            int nb[3];
-            for (int d = 0; d < 3; ++d)
+            int nb[3], ne[3], shift[3];
+            for (int d = 0; d < 3; ++d) {
                 nb[d] = wrap_block(bidx[d] + bi[d], f.nblocks[d]);
+                ne[d] = block_extent(f, nb, d);
+                shift[d] = bi[d] < 0 ? ne[d] - N[d] : 0; // FieldLabLoader.h:217-219
+            }
             const T *src = base + block_flat(nb, f.nblocks) * f.block_stride;
             int b[3], e[3];
             for (int d = 0; d < 3; ++d) {
@@ -179,13 +190,14 @@ struct OLab {
             const int nx = e[0] - b[0];
             if (nx <= 0)
                 continue;
-            // cell fields: neighbour extent equals own extent, shift is zero
+            // source index p - bi N + shift in the neighbour's own extent
+            // (FieldLabLoader.h:236-241)
             for (int iz = b[2]; iz < e[2]; ++iz)
                 for (int iy = b[1]; iy < e[1]; ++iy) {
-                    const long long sx = b[0] - (long long)bi[0] * N[0];
-                    const long long sy = iy - (long long)bi[1] * N[1];
-                    const long long sz = iz - (long long)bi[2] * N[2];
-                    std::memcpy(&at(b[0], iy, iz), src + sx + (long long)N[0] * (sy + (long long)N[1] * sz),
+                    const long long sx = b[0] - (long long)bi[0] * N[0] + shift[0];
+                    const long long sy = iy - (long long)bi[1] * N[1] + shift[1];
+                    const long long sz = iz - (long long)bi[2] * N[2] + shift[2];
+                    std::memcpy(&at(b[0], iy, iz), src + sx + (long long)ne[0] * (sy + (long long)ne[1] * sz),
                                 sizeof(T) * nx);
                 }
         }
@@ -284,7 +296,7 @@ struct OLab {
                   bool apply_bc)
     {
         for (int d = 0; d < 3; ++d)
-            N[d] = f.cells[d];
+            N[d] = block_extent(f, bidx, d);
         const T *base = static_cast<const T *>(f.data);
         loadInner(base + block_flat(bidx, f.nblocks) * f.block_stride);
         bool periodic[3] = {true, true, true};

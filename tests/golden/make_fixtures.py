@@ -16,6 +16,11 @@ oracle or the HIP kernels):
   Laplacian   Laplacian.cpp:123-130: ih2 * ((l[+] - 2 c) + l[-]), x + y + z
   Euler       Field.h:457-512: lap *= nu_dt; u += lap (two roundings)
 
+Node / face cases (plus flags) store blocks at the upper end of a plus
+dimension with one more element and copy ghosts below a block from the
+neighbour's own end (FieldLabLoader.h:217-219); their labs are laid out in
+the largest block's geometry, cells outside a block's active range NaN.
+
 Inputs are multiples of 1/64 in [-1, 1] drawn from a fixed splitmix64 stream,
 exact in FP32 and FP64, so one input serves both precisions.  Cells the
 reference never writes, and BC values read from such cells (tensorial edges
@@ -43,28 +48,36 @@ def splitmix(seed, n):
     return out
 
 
-def field_values(seed, nblocks, n):
-    cnt = nblocks * n[0] * n[1] * n[2]
+def field_values(seed, nblocks, n, plus=(0, 0, 0)):
+    cnt = nblocks * (n[0] + plus[0]) * (n[1] + plus[1]) * (n[2] + plus[2])
     return np.array([(v % 129) - 64 for v in splitmix(seed, cnt)], dtype=np.float64) / 64.0
 
 
 class Grid:
-    def __init__(self, nb, n, values):
-        self.nb, self.n = tuple(nb), tuple(n)
-        self.cells = n[0] * n[1] * n[2]
-        self.v = values        # block b at [b*cells, (b+1)*cells), x fastest
+    """Blocks of n elements per dim; with plus[d] the blocks at the upper end
+    of dim d hold n[d] + 1 (node / face fields, BlockFieldAssembler.h:121-131).
+    Block b occupies [b*stride, ...) x-fastest in its own extent."""
+
+    def __init__(self, nb, n, values, plus=(0, 0, 0)):
+        self.nb, self.n, self.plus = tuple(nb), tuple(n), tuple(plus)
+        self.stride = (n[0] + plus[0]) * (n[1] + plus[1]) * (n[2] + plus[2])
+        self.v = values
 
     def block_id(self, B):
         return B[0] + self.nb[0] * (B[1] + self.nb[1] * B[2])
 
+    def extent(self, B):
+        return tuple(self.n[d] + (1 if self.plus[d] and B[d] == self.nb[d] - 1 else 0) for d in range(3))
+
     def cell(self, B, p):
-        n = self.n
-        return self.v[self.block_id(B) * self.cells + p[0] + n[0] * (p[1] + n[1] * p[2])]
+        e = self.extent(B)
+        assert all(0 <= p[d] < e[d] for d in range(3))
+        return self.v[self.block_id(B) * self.stride + p[0] + e[0] * (p[1] + e[1] * p[2])]
 
 
 def lab_of(g, B, sb, se, tensorial, bcs):
     """dict lab[(x, y, z)] for block B (tuple), stencil [sb, se)."""
-    n = g.n
+    n = g.extent(B)
     lab = {}
     # inner
     for p in itertools.product(*[range(n[d]) for d in range(3)]):
@@ -90,8 +103,12 @@ def lab_of(g, B, sb, se, tensorial, bcs):
             else:
                 rng.append(range(n[d], n[d] + se[d] - 1))
         NB = tuple((B[d] + bi[d]) % g.nb[d] for d in range(3))
+        ne = g.extent(NB)
+        # below the block the source counts back from the neighbour's own end
+        # (FieldLabLoader.h:217-219)
+        shift = [ne[d] - n[d] if bi[d] < 0 else 0 for d in range(3)]
         for q in itertools.product(*rng):
-            lab[q] = g.cell(NB, tuple(q[d] - bi[d] * n[d] for d in range(3)))
+            lab[q] = g.cell(NB, tuple(q[d] - bi[d] * n[d] + shift[d] for d in range(3)))
     # BCs in list order
     for (kind, d, side, val) in bcs:
         if kind == "base" or sb[d] == 0 or se[d] == 1:
@@ -148,8 +165,8 @@ def diffusion(g, bcl, h, nu_dt, T):
     ih = [T(1.0 / (h[d] * h[d])) for d in range(3)]
     s = T(nu_dt)
     nbt = g.nb[0] * g.nb[1] * g.nb[2]
-    lap = np.zeros(nbt * g.cells, dtype=T)
-    un = np.zeros(nbt * g.cells, dtype=T)
+    lap = np.zeros(nbt * g.stride, dtype=T)
+    un = np.zeros(nbt * g.stride, dtype=T)
     two = T(2)
     for B in itertools.product(*[range(g.nb[d]) for d in range(3)]):
         b = g.block_id(B)
@@ -160,7 +177,7 @@ def diffusion(g, bcl, h, nu_dt, T):
             ddx = ih[0] * ((L[(x + 1, y, z)] - two * c) + L[(x - 1, y, z)])
             ddy = ih[1] * ((L[(x, y + 1, z)] - two * c) + L[(x, y - 1, z)])
             ddz = ih[2] * ((L[(x, y, z + 1)] - two * c) + L[(x, y, z - 1)])
-            i = b * g.cells + x + n[0] * (y + n[1] * z)
+            i = b * g.stride + x + n[0] * (y + n[1] * z)
             lp = (ddx + ddy) + ddz
             lap[i] = lp
             un[i] = c + lp * s
@@ -183,21 +200,33 @@ CASES = [
      [("dirichlet", 0, 0, -1.0), ("dirichlet", 0, 1, 2.0), ("symmetry", 2, 0, 1.0), ("absorbing", 2, 1, 0.0),
       ("base", 1, 0, 0.0)],
      [((-2, -2, -2), (3, 3, 3), True), ((-2, -2, -2), (3, 3, 3), False)], True),
+    # node and face fields: upper-end blocks one element larger, shifted copies
+    ("node_periodic", (2, 2, 2), (4, 4, 4), 5, [],
+     [((-1, -1, -1), (2, 2, 2), False), ((-2, -2, -2), (3, 3, 3), True)], False, (1, 1, 1)),
+    ("face_y_mixed", (2, 3, 1), (4, 3, 4), 6,
+     [("absorbing", 0, 0, 0.0), ("dirichlet", 1, 1, 0.5), ("symmetry", 2, 0, -1.0)],
+     [((-2, -2, -2), (3, 3, 3), True), ((-1, -1, -1), (2, 2, 2), False)], False, (0, 1, 0)),
+    ("node_one_block", (1, 1, 1), (3, 3, 3), 7,
+     [("dirichlet", 0, 1, 1.5), ("absorbing", 2, 0, 0.0)],
+     [((-2, -2, -2), (3, 3, 3), True)], False, (1, 1, 1)),
 ]
 
 
 def main():
     data, meta = {}, []
-    for name, nb, n, seed, faces, labs, diff in CASES:
+    for case in CASES:
+        name, nb, n, seed, faces, labs, diff = case[:7]
+        plus = case[7] if len(case) > 7 else (0, 0, 0)
         nbt = nb[0] * nb[1] * nb[2]
-        vals = field_values(0x5EED0100 + seed, nbt, n)
-        g = Grid(nb, n, vals)
+        vals = field_values(0x5EED0100 + seed, nbt, n, plus)
+        g = Grid(nb, n, vals, plus)
         bcl = face_lists(nb, faces)
         data[name + "/input"] = vals
-        entry = {"name": name, "nblocks": nb, "cells": n, "faces": faces,
+        entry = {"name": name, "nblocks": nb, "cells": n, "plus": plus, "faces": faces,
                  "bc_lists": {str(k): v for k, v in bcl.items()}, "labs": []}
+        nmax = tuple(n[d] + plus[d] for d in range(3))  # labs of all blocks in the largest block's geometry
         for li, (sb, se, tens) in enumerate(labs):
-            arr = np.stack([lab_array(lab_of(g, B, sb, se, tens, bcl.get(g.block_id(B), [])), n, sb, se)
+            arr = np.stack([lab_array(lab_of(g, B, sb, se, tens, bcl.get(g.block_id(B), [])), nmax, sb, se)
                             for B in [(b % nb[0], (b // nb[0]) % nb[1], b // (nb[0] * nb[1])) for b in range(nbt)]])
             data["%s/lab%d" % (name, li)] = arr
             entry["labs"].append({"begin": sb, "end": se, "tensorial": tens})

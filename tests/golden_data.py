@@ -19,8 +19,24 @@ def bc_lists(entry):
     return {int(b): [(KIND[k], d, s, v) for (k, d, s, v) in lst] for b, lst in entry["bc_lists"].items()}
 
 
+def elements(entry):
+    """Elements per block of a fixture grid: (cells + plus) per dimension."""
+    return tuple(c + p for c, p in zip(entry["cells"], plus(entry)))
+
+
+def plus(entry):
+    return tuple(entry.get("plus", (0, 0, 0)))
+
+
+def extent(entry, bi):
+    """Extent of block bi: one more in a plus dimension at the upper end."""
+    p, nb = plus(entry), entry["nblocks"]
+    return tuple(entry["cells"][d] + (1 if p[d] and bi[d] == nb[d] - 1 else 0) for d in range(3))
+
+
 def padded(values, nblocks, cells, block_stride, dtype):
-    """Block-major values -> slab with the grid's block stride."""
+    """Block-major values (cells = elements per block) -> slab with the
+    grid's block stride."""
     nbt = int(np.prod(nblocks))
     n = int(np.prod(cells))
     out = np.zeros((nbt, block_stride), dtype=dtype)

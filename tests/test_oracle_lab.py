@@ -190,6 +190,61 @@ def test_cartesian_lab_analytic(dtype):
         assert np.max(np.abs(L - fexact(XX, YY, ZZ))) <= 8 * eps
 
 
+def entity_slab(nb, n, plus, fexact, h):
+    """Slab of a node / face field in the reference layout: block b stores its
+    elements x-fastest in its own extent (n, plus one at the upper end of the
+    plus dims), blocks (n+1)^3 elements apart (Cartesian.h:490-507,
+    BlockFieldAssembler.h:121-131); element i of a plus dim sits at i h, of a
+    cell dim at (i + 1/2) h."""
+    stride = (n + 1) ** 3
+    slab = np.zeros(nb ** 3 * stride)
+    for bz, by, bx in itertools.product(range(nb), repeat=3):
+        b = (bx, by, bz)
+        ext = [n + (1 if plus[d] and b[d] == nb - 1 else 0) for d in range(3)]
+        co = [(b[d] * n + np.arange(ext[d]) + (0.0 if plus[d] else 0.5)) * h for d in range(3)]
+        ZZ, YY, XX = np.meshgrid(co[2], co[1], co[0], indexing="ij")
+        flat = bx + nb * (by + nb * bz)
+        slab[flat * stride:flat * stride + ext[0] * ext[1] * ext[2]] = fexact(XX, YY, ZZ).ravel()
+    return slab, stride
+
+
+@pytest.mark.parametrize("plus", [(1, 1, 1), (1, 0, 0), (0, 1, 0), (0, 0, 1)],
+                         ids=["node", "face_x", "face_y", "face_z"])
+@pytest.mark.parametrize("tensorial", [True, False])
+def test_node_face_lab_analytic(plus, tensorial):
+    """CartesianTest.cpp:172-318 for node fields (and the same construction per
+    face direction): 3^3 blocks of 8^3, Stencil(-2, 3), periodic.  Lab values
+    equal the function at the element coordinates, where a coordinate of a
+    plus dim outside [0, 1] is moved inwards by h -- the periodic images the
+    shifted neighbour copies (FieldLabLoader.h:217-219) produce -- to 8 eps."""
+    nb, n = 3, 8
+    h = 1.0 / (nb * n)
+    fx = lambda x, y, z: np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.sin(2 * np.pi * z)
+
+    def fexact(X, Y, Z):
+        c = [X, Y, Z]
+        for d in range(3):
+            if plus[d]:
+                c[d] = np.where(c[d] < 0, c[d] + h, np.where(c[d] > 1, c[d] - h, c[d]))
+        return fx(*c)
+
+    slab, stride = entity_slab(nb, n, plus, fx, h)
+    f = oracle.Field(slab, nb, n, stride, plus=plus)
+    eps = np.finfo(np.float64).eps
+    checked = 0
+    for bz, by, bx in itertools.product(range(nb), repeat=3):
+        b = (bx, by, bz)
+        L = oracle.lab_load(f, b, -2, 3, tensorial)
+        ext = oracle.block_extent(f, b)
+        co = [(b[d] * n + np.arange(-2, ext[d] + 2) + (0.0 if plus[d] else 0.5)) * h for d in range(3)]
+        ZZ, YY, XX = np.meshgrid(co[2], co[1], co[0], indexing="ij")
+        m = defined_mask(f, b, -2, 3, tensorial)
+        assert L.shape == XX.shape
+        assert np.max(np.abs(L[m] - fexact(XX, YY, ZZ)[m])) <= 8 * eps
+        checked += int(m.sum())
+    assert checked > 0
+
+
 # ------------------------------------------------- independent formulations
 
 
