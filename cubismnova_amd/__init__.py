@@ -549,6 +549,20 @@ def field_rcp(a, scalar, dst, n, dtype=np.float64, stream=None):
     call("cnv_field_rcp", _dt(dtype), a, ctypes.c_double(float(scalar)), dst, int(n), _st(stream))
 
 
+GRID_OPS = dict(FIELD_OPS, rcp=_abi.CNV_FIELD_RCP, neg=_abi.CNV_FIELD_NEG)
+
+
+def grid_field_op(op, dst, a, b=None, scalar=0.0, comps=None, stream=None):
+    """Field / FieldContainer arithmetic over whole grids (Field.h:444-556,
+    1039-1154), every block over its own elements:
+    op add/sub/mul/div: dst = a OP b (b a Cartesian) or a OP scalar (b None);
+    rcp: dst = scalar * (1 / a); neg: dst = -a.  comps = (cd, ca, cb) slab
+    indices, or None for every slab.  dst may be a or b."""
+    cd, ca, cb = (-1, -1, -1) if comps is None else (tuple(comps) + (0,))[:3]
+    call("cnv_grid_field_op", GRID_OPS[op], dst.handle, int(cd), a.handle, int(ca),
+         b.handle if b is not None else None, int(cb), ctypes.c_double(float(scalar)), _st(stream))
+
+
 def grid_sums(g, comp=0, stream=None):
     """(sum, sum of squares) of one component, FP64, deterministic."""
     buf = DeviceBuffer(16)

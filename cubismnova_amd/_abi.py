@@ -126,6 +126,7 @@ SIGNATURES = {
     "cnv_field_op2": (i32, [i32, i32, vp, vp, vp, sz, vp]),
     "cnv_field_op1": (i32, [i32, i32, vp, f64, vp, sz, vp]),
     "cnv_field_rcp": (i32, [i32, vp, f64, vp, sz, vp]),
+    "cnv_grid_field_op": (i32, [i32, vp, i32, vp, i32, vp, i32, ctypes.c_double, vp]),
     "cnv_grid_prepare": (i32, [vp]),
     "cnv_graph_begin": (i32, [vp]),
     "cnv_graph_end": (i32, [vp, ctypes.POINTER(vp)]),
@@ -141,7 +142,7 @@ for _op in ("Add", "Sub", "Mul", "Div"):
 SIGNATURES["fieldRcp_double"] = (None, [vp, f64, vp, sz])
 SIGNATURES["fieldRcp_float"] = (None, [vp, ctypes.c_float, vp, sz])
 
-CNV_FIELD_ADD, CNV_FIELD_SUB, CNV_FIELD_MUL, CNV_FIELD_DIV = 0, 1, 2, 3
+CNV_FIELD_ADD, CNV_FIELD_SUB, CNV_FIELD_MUL, CNV_FIELD_DIV, CNV_FIELD_RCP, CNV_FIELD_NEG = 0, 1, 2, 3, 4, 5
 
 _lib = None
 

@@ -9,16 +9,21 @@
 // dst[i] = a[i] OP s, and dst[i] = s * (1 / a[i]) (two roundings).  This file
 // provides those symbols for pointers the GPU can address (the data of a
 // cnv_grid, cnv_malloc memory), blocking like the reference's loops, and
-// stream-ordered variants with a status (cnv_field_op2 / op1 / rcp).  Every
+// stream-ordered variants with a status (cnv_field_op2 / op1 / rcp).  The
+// Field and FieldContainer operators of whole grids (Field.h:444-556,
+// 1039-1154: each block over its own range_.size() elements, each
+// component) run as one launch over (slab, block) rows (cnv_grid_field_op).  Every
 // element is one IEEE operation (the library is built without contraction
 // and with correctly rounded division), so results equal the host loops bit
 // for bit.
-#include "cnv_internal.h"
+#include "cnv_device.h"
+
+#include <algorithm>
 
 namespace cnv
 {
 
-enum { FOP_ADD = 0, FOP_SUB = 1, FOP_MUL = 2, FOP_DIV = 3, FOP_RCP = 4 };
+enum { FOP_ADD = 0, FOP_SUB = 1, FOP_MUL = 2, FOP_DIV = 3, FOP_RCP = 4, FOP_NEG = 5 };
 
 template <typename T, int OP>
 __device__ inline T fop(T a, T b)
@@ -31,8 +36,10 @@ __device__ inline T fop(T a, T b)
         return a * b;
     else if constexpr (OP == FOP_DIV)
         return a / b;
+    else if constexpr (OP == FOP_RCP)
+        return b * ((T)1 / a); // s * (1 / a)
     else
-        return b * ((T)1 / a); // RCP: s * (1 / a)
+        return -a;
 }
 
 // dst[i] = a[i] OP (b ? b[i] : s); four elements per thread per pass
@@ -116,6 +123,121 @@ static void fieldop_blocking(int op, int dtype, const void *a, const void *b, do
     }
 }
 
+// ------------------------------------------------------------ whole grids
+
+template <typename T>
+struct GridOpArgs {
+    T *dst;
+    const T *a, *b; // b null: the scalar operand
+    T s;
+    long long cstride, bstride; // elements between slabs, between blocks
+    int cd, ca, cb;             // first slab of each operand
+    int nb[3], n[3];
+    int entity, ncomp;
+    long long nbt, rows; // rows = slabs * blocks
+};
+
+// one workgroup row per (slab, block); 4 elements per thread per pass over
+// the block's own extent, 16- or 32-byte vectors where the block has them
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void k_grid_fieldop(const GridOpArgs<T> g)
+{
+    using V = VecT<T, 4>;
+    for (long long r = blockIdx.y; r < g.rows; r += gridDim.y) {
+        const int sl = (int)(r / g.nbt);
+        const long long blk = r - (long long)sl * g.nbt;
+        const int B[3] = {(int)(blk % g.nb[0]), (int)((blk / g.nb[0]) % g.nb[1]),
+                          (int)(blk / ((long long)g.nb[0] * g.nb[1]))};
+        // plus dimensions of this slab (BlockFieldAssembler.h:123-131)
+        const int pm = g.entity == CNV_NODE ? 7 : (g.entity == CNV_FACE ? 1 << ((g.cd + sl) / g.ncomp) : 0);
+        long long ne = 1;
+        for (int d = 0; d < 3; ++d)
+            ne *= g.n[d] + (((pm >> d) & 1) && B[d] == g.nb[d] - 1 ? 1 : 0);
+        T *dst = g.dst + (g.cd + sl) * g.cstride + blk * g.bstride;
+        const T *a = g.a + (g.ca + sl) * g.cstride + blk * g.bstride;
+        const T *b = g.b ? g.b + (g.cb + sl) * g.cstride + blk * g.bstride : nullptr;
+        for (long long o = ((long long)blockIdx.x * 256 + threadIdx.x) * 4; o < ne;
+             o += (long long)gridDim.x * 256 * 4) {
+            if (o + 4 <= ne) {
+                const V va = load_vec<T, 4>(a + o);
+                V vb;
+                if (b)
+                    vb = load_vec<T, 4>(b + o);
+                V r4;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    r4.v[j] = fop<T, OP>(va.v[j], b ? vb.v[j] : g.s);
+                store_vec<T, 4>(dst + o, r4);
+            } else {
+                for (long long i = o; i < ne; ++i)
+                    dst[i] = fop<T, OP>(a[i], b ? b[i] : g.s);
+            }
+        }
+    }
+}
+
+template <typename T>
+static int grid_fieldop_t(int op, cnv_grid_s *dst, int cd, const cnv_grid_s *a, int ca, const cnv_grid_s *b, int cb,
+                          double s, int nsl, hipStream_t st)
+{
+    GridOpArgs<T> g;
+    g.dst = reinterpret_cast<T *>(dst->data);
+    g.a = reinterpret_cast<const T *>(a->data);
+    g.b = b ? reinterpret_cast<const T *>(b->data) : nullptr;
+    g.s = (T)s;
+    g.cstride = (long long)(a->component_bytes / a->elem);
+    g.bstride = (long long)(a->block_bytes / a->elem);
+    g.cd = cd;
+    g.ca = ca;
+    g.cb = cb;
+    for (int d = 0; d < 3; ++d) {
+        g.nb[d] = a->desc.nblocks[d];
+        g.n[d] = a->desc.block_cells[d];
+    }
+    g.entity = a->desc.entity;
+    g.ncomp = a->desc.ncomp;
+    g.nbt = a->nblocks_total;
+    g.rows = g.nbt * nsl;
+    long long maxe = 1;
+    for (int d = 0; d < 3; ++d)
+        maxe *= g.n[d] + (g.entity == CNV_CELL ? 0 : 1);
+    const long long gx = std::min<long long>((maxe + 1023) / 1024, 64);
+    const long long gy = std::min<long long>(g.rows, 65535);
+    const dim3 grid((unsigned)gx, (unsigned)gy), blk(256);
+    switch (op) {
+    case FOP_ADD:
+        hipLaunchKernelGGL((k_grid_fieldop<T, FOP_ADD>), grid, blk, 0, st, g);
+        break;
+    case FOP_SUB:
+        hipLaunchKernelGGL((k_grid_fieldop<T, FOP_SUB>), grid, blk, 0, st, g);
+        break;
+    case FOP_MUL:
+        hipLaunchKernelGGL((k_grid_fieldop<T, FOP_MUL>), grid, blk, 0, st, g);
+        break;
+    case FOP_DIV:
+        hipLaunchKernelGGL((k_grid_fieldop<T, FOP_DIV>), grid, blk, 0, st, g);
+        break;
+    case FOP_RCP:
+        hipLaunchKernelGGL((k_grid_fieldop<T, FOP_RCP>), grid, blk, 0, st, g);
+        break;
+    default:
+        hipLaunchKernelGGL((k_grid_fieldop<T, FOP_NEG>), grid, blk, 0, st, g);
+        break;
+    }
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
+
+static int same_layout(const cnv_grid_s *x, const cnv_grid_s *y)
+{
+    for (int d = 0; d < 3; ++d)
+        if (x->desc.nblocks[d] != y->desc.nblocks[d] || x->desc.block_cells[d] != y->desc.block_cells[d])
+            return fail(CNV_ERR_SIZE, "grids differ in block count or block size");
+    if (x->desc.dtype != y->desc.dtype || x->desc.entity != y->desc.entity || x->nslabs != y->nslabs)
+        return fail(CNV_ERR_SIZE, "grids differ in dtype, entity or slab count");
+    return CNV_OK;
+}
+
 } // namespace cnv
 
 using namespace cnv;
@@ -141,6 +263,38 @@ int cnv_field_op1(int op, int dtype, const void *a, double scalar, void *dst, si
 int cnv_field_rcp(int dtype, const void *a, double scalar, void *dst, size_t n, cnv_stream s)
 {
     return fieldop(FOP_RCP, dtype, a, nullptr, scalar, dst, n, (hipStream_t)s);
+}
+
+int cnv_grid_field_op(int op, cnv_grid dst, int cd, cnv_grid a, int ca, cnv_grid b, int cb, double scalar,
+                      cnv_stream s)
+{
+    if (!dst || !a)
+        return fail(CNV_ERR_INVALID, "grid handle is NULL");
+    if (op < FOP_ADD || op > FOP_NEG)
+        return fail(CNV_ERR_INVALID, "unknown field operation");
+    if (b && op >= FOP_RCP)
+        return fail(CNV_ERR_INVALID, "reciprocal and negation take no second grid");
+    int rc = same_layout(dst, a);
+    if (rc == CNV_OK && b)
+        rc = same_layout(a, b);
+    if (rc != CNV_OK)
+        return rc;
+    int nsl = 1;
+    if (cd == -1 && ca == -1 && (!b || cb == -1)) {
+        cd = ca = cb = 0;
+        nsl = a->nslabs;
+    } else {
+        if (cd < 0 || cd >= dst->nslabs || ca < 0 || ca >= a->nslabs || (b && (cb < 0 || cb >= b->nslabs)))
+            return fail(CNV_ERR_INVALID, "component index out of range");
+        if (a->desc.entity == CNV_FACE &&
+            (cd / a->desc.ncomp != ca / a->desc.ncomp || (b && cb / a->desc.ncomp != ca / a->desc.ncomp)))
+            return fail(CNV_ERR_SIZE, "face slabs of different directions differ in shape");
+        if (!b)
+            cb = 0;
+    }
+    if (a->desc.dtype == CNV_F64)
+        return grid_fieldop_t<double>(op, dst, cd, a, ca, b, cb, scalar, nsl, (hipStream_t)s);
+    return grid_fieldop_t<float>(op, dst, cd, a, ca, b, cb, scalar, nsl, (hipStream_t)s);
 }
 
 #define CNV_FIELD2(NAME, OPC, TYPE, DT)                                                                 \

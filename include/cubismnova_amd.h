@@ -280,10 +280,32 @@ int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts
  * op2: dst[i] = a[i] OP b[i];  op1: dst[i] = a[i] OP scalar;
  * rcp: dst[i] = scalar * (1 / a[i]) (two roundings).  The loops of
  * include/Cubism/Block/FieldOperator.h:11-30, 44-141, stream-ordered. */
-enum cnv_field_op { CNV_FIELD_ADD = 0, CNV_FIELD_SUB = 1, CNV_FIELD_MUL = 2, CNV_FIELD_DIV = 3 };
+enum cnv_field_op {
+    CNV_FIELD_ADD = 0,
+    CNV_FIELD_SUB = 1,
+    CNV_FIELD_MUL = 2,
+    CNV_FIELD_DIV = 3,
+    CNV_FIELD_RCP = 4, /* cnv_grid_field_op only */
+    CNV_FIELD_NEG = 5  /* cnv_grid_field_op only */
+};
 int cnv_field_op2(int op, int dtype, const void *a, const void *b, void *dst, size_t n, cnv_stream s);
 int cnv_field_op1(int op, int dtype, const void *a, double scalar, void *dst, size_t n, cnv_stream s);
 int cnv_field_rcp(int dtype, const void *a, double scalar, void *dst, size_t n, cnv_stream s);
+
+/* Field and FieldContainer arithmetic over whole grids, one launch: every
+ * block over its own elements only (range_.size(): N^3 for cells, the
+ * block's extent for node and face slabs), as Field::operator+= and the
+ * other Field operators (Block/Field.h:444-556) applied to each block of a
+ * grid, and FieldContainer's operators (Field.h:1039-1154) to each of its
+ * components:
+ *   CNV_FIELD_ADD..DIV: dst = a OP b (b a grid) or a OP scalar (b NULL);
+ *   CNV_FIELD_RCP: dst = scalar * (1 / a), two roundings (Field::reciprocal);
+ *   CNV_FIELD_NEG: dst = -a (Field::operator-()).
+ * cd, ca, cb are slab indices, or all -1 for every slab.  The grids have one
+ * shape, entity, dtype and slab count; dst may be a or b.  Elements past a
+ * block's own extent (padding) are not touched. */
+int cnv_grid_field_op(int op, cnv_grid dst, int cd, cnv_grid a, int ca, cnv_grid b, int cb, double scalar,
+                      cnv_stream s);
 
 /* The reference's own C-ABI hooks, same names and signatures as the extern
  * "C" declarations of FieldOperator.h:149-227 (CUBISM_OPTIMIZED_FIELD_OP):

@@ -75,3 +75,85 @@ def test_field_op_errors(gpu):
     with pytest.raises(cnv.CnvError):
         _abi.call("cnv_field_op2", 7, _abi.CNV_F64, None, None, None, 0, None)
     cnv.field_op("add", None, None, None, 0)      # empty: no-op
+
+
+# ------------------------------------------------ whole grids (Field.h:444-556, 1039-1154)
+
+GRID_SHAPES = [("cell", (2, 3, 2), (8, 8, 8), 1), ("cell", (3, 1, 2), (5, 7, 3), 2),
+               ("node", (2, 2, 3), (4, 5, 3), 1), ("face", (2, 2, 2), (3, 4, 5), 2)]
+
+
+def grid_expected(op, g, ain, bin_, s, dst_before, comps):
+    """numpy over each block's own extent; padding keeps dst's old values."""
+    out = {c: dst_before[c].copy() for c in dst_before}
+    nb = g.nblocks
+    todo = [(c, c, c) for c in range(g.nslabs)] if comps is None else [comps]
+    T = g.dtype.type
+    for (cd, ca, cb) in todo:
+        o = out[cd].reshape(g.nblocks_total, g.block_stride)
+        x = ain[ca].reshape(g.nblocks_total, g.block_stride)
+        y = None if bin_ is None else bin_[cb].reshape(g.nblocks_total, g.block_stride)
+        for blk in range(g.nblocks_total):
+            bi = (blk % nb[0], (blk // nb[0]) % nb[1], blk // (nb[0] * nb[1]))
+            ne = int(np.prod(g.block_extent(bi, ca)))
+            xa = x[blk, :ne]
+            if op == "neg":
+                o[blk, :ne] = -xa
+            elif op == "rcp":
+                o[blk, :ne] = T(s) * (T(1) / xa)
+            else:
+                o[blk, :ne] = REF[op](xa, y[blk, :ne] if y is not None else T(s))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", GRID_SHAPES, ids=[s[0] + str(i) for i, s in enumerate(GRID_SHAPES)])
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div", "rcp", "neg"])
+def test_grid_field_op_every_slab(gpu, dtype, shape, op):
+    ent, nb, n, nc = shape
+    mk = lambda: cnv.Cartesian(nb, n, dtype=dtype, ncomp=nc, entity=ent)  # noqa: E731
+    a, b, d = mk(), mk(), mk()
+    cnt = a.component_bytes // a.dtype.itemsize
+    A = {c: values(10 + c, cnt, dtype) for c in range(a.nslabs)}
+    Bv = {c: values(20 + c, cnt, dtype) for c in range(a.nslabs)}
+    D = {c: np.full(cnt, 7.0, dtype=dtype) for c in range(a.nslabs)}
+    for c in range(a.nslabs):
+        a.upload(c, A[c])
+        b.upload(c, Bv[c])
+        d.upload(c, D[c])
+    two = op in REF
+    s = -1.75
+    cnv.grid_field_op(op, d, a, b if two else None, s)
+    want = grid_expected(op, a, A, Bv if two else None, s, D, None)
+    for c in range(a.nslabs):
+        assert np.array_equal(d.download(c), want[c], equal_nan=True), (op, c)
+    if two:  # grid OP scalar
+        cnv.grid_field_op(op, d, a, None, s)
+        want = grid_expected(op, a, A, None, s, want, None)
+        for c in range(a.nslabs):
+            assert np.array_equal(d.download(c), want[c], equal_nan=True), (op, "scalar", c)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_grid_field_op_one_slab_in_place(gpu, dtype):
+    # a += b on slab 3 from slab 5 of a face grid (both faces normal to z, ncomp = 2)
+    nb, n = (2, 3, 2), (4, 4, 6)
+    a = cnv.Cartesian(nb, n, dtype=dtype, ncomp=2, entity="face")
+    b = cnv.Cartesian(nb, n, dtype=dtype, ncomp=2, entity="face")
+    cnt = a.component_bytes // a.dtype.itemsize
+    A = {c: values(30 + c, cnt, dtype) for c in range(6)}
+    Bv = {c: values(40 + c, cnt, dtype) for c in range(6)}
+    for c in range(6):
+        a.upload(c, A[c])
+        b.upload(c, Bv[c])
+    cnv.grid_field_op("add", a, a, b, comps=(5, 5, 4))
+    want = grid_expected("add", a, A, Bv, 0.0, A, (5, 5, 4))
+    for c in range(6):
+        assert np.array_equal(a.download(c), want[c], equal_nan=True), c
+    with pytest.raises(cnv.CnvError):  # x faces and z faces differ in shape
+        cnv.grid_field_op("add", a, a, b, comps=(0, 0, 5))
+    c = cnv.Cartesian(nb, n, dtype=dtype, ncomp=2, entity="node")
+    with pytest.raises(cnv.CnvError):
+        cnv.grid_field_op("add", a, a, c)
+    with pytest.raises(cnv.CnvError):
+        cnv.grid_field_op("neg", a, a, b)
