@@ -25,7 +25,7 @@ struct X2Args {
     const char *u; // input grid base
     char *out;     // output grid base
     Geom G;
-    int nzc, zchunk, tiles_per_comp;
+    int nzc, tiles_per_comp; // z-chunks per block, tiles per component
     double ih2[3];
     double nudt;
     double *part1, *part2; // per-tile partials of |u^{n+1}|^2, |u^{n+2}|^2 (or null)
@@ -77,10 +77,13 @@ __device__ inline T dpp_shift(T v)
 constexpr int kDppRowShr1 = 0x111; // lane i <- lane i-1
 constexpr int kDppRowShl1 = 0x101; // lane i <- lane i+1
 
-template <typename T, int N, int WG>
+// ZC planes per tile; ZC + 2 planes of u^{n+1} (a multiple of the three
+// register slots) make every slot index and the trip count static.
+template <typename T, int N, int WG, int ZC>
 __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
 {
     using S = X2Shape<T, N, WG>;
+    static_assert((ZC + 2) % 3 == 0 && N % ZC == 0, "z chunk");
     constexpr int VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, PA = S::PA, PLA = S::PLA, PLB = S::PLB;
     constexpr int NHX = S::NHX, NHY = S::NHY, NH = S::NH, HS = S::HS, NR = S::NR, RS = S::RS;
     static_assert(S::valid, "tile shape");
@@ -102,8 +105,8 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     const int tt = tile - comp * a.tiles_per_comp;
     const long long blk = tt / a.nzc;
     const int zc = tt - (int)blk * a.nzc;
-    const int z0 = zc * a.zchunk;
-    const int z1 = min(N, z0 + a.zchunk);
+    const int z0 = zc * ZC;
+    const int z1 = z0 + ZC;
     const int nbx = a.G.nb[0], nby = a.G.nb[1], nbz = a.G.nb[2];
     const int bx = (int)(blk % nbx), by = (int)((blk / nbx) % nby), bz = (int)(blk / ((long long)nbx * nby));
     const T *cbase = reinterpret_cast<const T *>(a.u) + (long long)comp * a.G.cstride;
@@ -160,7 +163,7 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     };
     // plane p (z0-2 <= p <= z1+1, possibly in the z-neighbour block) into registers
     auto load_u0 = [&](int p) -> U0 {
-        U0 Q = {};
+        U0 Q; // scalar halo entries use element 0 only
         const int dz = p < 0 ? -1 : (p >= N ? 1 : 0);
         const T *base = cbase + (long long)wrap(bz + dz, nbz) * layer + (long long)(p - dz * N) * NN;
         const T *self = base + xyself + own;
@@ -172,9 +175,7 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
             if (hkind[h] == 2) {
                 Q.h[h] = *reinterpret_cast<const V *>(base + hsrc[h]);
             } else if (hkind[h] == 1) {
-                V t = {};
-                t.v[0] = base[hsrc[h]];
-                Q.h[h] = t;
+                Q.h[h].v[0] = base[hsrc[h]];
             }
         }
         return Q;
@@ -195,11 +196,19 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
 
     const T ihx2 = (T)a.ih2[0], ihy2 = (T)a.ih2[1], ihz2 = (T)a.ih2[2];
     const T nudt = (T)a.nudt;
-    // one Euler step of one cell, in the order of the one-step kernel
+    // one Euler step of one cell, in the order of the one-step kernel;
+    // xp - 2c as fma(-2, c, xp): 2c is exact, so the single rounding of the
+    // fused form is the rounding of the subtraction
+    auto fms2 = [](T c, T p) -> T {
+        if constexpr (sizeof(T) == 4)
+            return __builtin_fmaf(-2.f, c, p);
+        else
+            return __builtin_fma(-2.0, c, p);
+    };
     auto euler = [&](T c, T xm, T xp, T ym, T yp, T zm, T zp) -> T {
-        const T ddx = ihx2 * (xp - 2 * c + xm);
-        const T ddy = ihy2 * (yp - 2 * c + ym);
-        const T ddz = ihz2 * (zp - 2 * c + zm);
+        const T ddx = ihx2 * (fms2(c, xp) + xm);
+        const T ddy = ihy2 * (fms2(c, yp) + ym);
+        const T ddz = ihz2 * (fms2(c, zp) + zm);
         const T lap = ddx + ddy + ddz;
         const T l = lap * nudt;
         return c + l;
@@ -230,12 +239,11 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     U0 Pq = load_u0(z0);
 
     double acc1 = 0.0, acc2 = 0.0;
-    for (int qb = z0 - 1; qb <= z1; qb += 3) {
+    for (int it = 0; it < (ZC + 2) / 3; ++it) {
+        const int qb = z0 - 1 + 3 * it;
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
             const int q = qb + u;
-            if (q > z1)
-                break;
             V(&c0m)[R] = U0r[u % 3];
             V(&c0c)[R] = U0r[(u + 1) % 3];
             V(&c0p)[R] = U0r[(u + 2) % 3];
@@ -273,8 +281,6 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
                     const T xm = v == 0 ? xl : c0c[r].v[v - 1];
                     const T xp = v == VEC - 1 ? xr : c0c[r].v[v + 1];
                     res.v[v] = euler(c0c[r].v[v], xm, xp, ym.v[v], yp.v[v], c0m[r].v[v], c0p[r].v[v]);
-                    if (inner)
-                        acc1 = __builtin_fma((double)res.v[v], (double)res.v[v], acc1);
                 }
                 n1[r] = res;
                 if (inner)
@@ -301,7 +307,8 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
                 }
             }
             // u^{n+2} on the block's cells of plane q - 1 (its u^{n+1} plane was
-            // completed before this plane's barrier)
+            // completed before this plane's barrier); the sums of squares of
+            // u^{n+1} and u^{n+2} over the tile's cells
             if (q >= z0 + 1) {
                 const T *Br = slotB(q - 1);
                 T *op = ob + NN * (q - 1);
@@ -323,6 +330,7 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
                         const T xm = v == 0 ? xl : c1[r].v[v - 1];
                         const T xp = v == VEC - 1 ? xr : c1[r].v[v + 1];
                         res.v[v] = euler(c1[r].v[v], xm, xp, ym.v[v], yp.v[v], m1[r].v[v], n1[r].v[v]);
+                        acc1 = __builtin_fma((double)c1[r].v[v], (double)c1[r].v[v], acc1);
                         acc2 = __builtin_fma((double)res.v[v], (double)res.v[v], acc2);
                     }
                     *reinterpret_cast<V *>(op + x0 + N * y) = res;
@@ -375,28 +383,36 @@ X2Tune x2_tune(int dtype, int edge)
     return it == g_x2_tune.end() ? x2_default(dtype, edge) : it->second;
 }
 
+// z-chunks with a static trip count: ZC + 2 a multiple of 3, dividing 16 and 32
+inline bool x2_zchunk_ok(int zc) { return zc == 4 || zc == 16; }
+
 template <typename T, int N, int WG>
-bool x2_launch1(const X2Args &a, int ntiles, hipStream_t st)
+bool x2_launch1(const X2Args &a, int zc, int ntiles, hipStream_t st)
 {
     if constexpr (X2Shape<T, N, WG>::valid) {
-        hipLaunchKernelGGL((k_stencil7x2<T, N, WG>), dim3(ntiles), dim3(WG), 0, st, a);
+        if (zc == 16)
+            hipLaunchKernelGGL((k_stencil7x2<T, N, WG, 16>), dim3(ntiles), dim3(WG), 0, st, a);
+        else if (zc == 4)
+            hipLaunchKernelGGL((k_stencil7x2<T, N, WG, 4>), dim3(ntiles), dim3(WG), 0, st, a);
+        else
+            return false;
         return true;
     }
     return false;
 }
 
 template <typename T, int N>
-bool x2_dispatch_wg(const X2Args &a, int wg, int ntiles, hipStream_t st)
+bool x2_dispatch_wg(const X2Args &a, int wg, int zc, int ntiles, hipStream_t st)
 {
     switch (wg) {
     case 64:
-        return x2_launch1<T, N, 64>(a, ntiles, st);
+        return x2_launch1<T, N, 64>(a, zc, ntiles, st);
     case 128:
-        return x2_launch1<T, N, 128>(a, ntiles, st);
+        return x2_launch1<T, N, 128>(a, zc, ntiles, st);
     case 256:
-        return x2_launch1<T, N, 256>(a, ntiles, st);
+        return x2_launch1<T, N, 256>(a, zc, ntiles, st);
     case 512:
-        return x2_launch1<T, N, 512>(a, ntiles, st);
+        return x2_launch1<T, N, 512>(a, zc, ntiles, st);
     }
     return false;
 }
@@ -424,11 +440,13 @@ bool x2_valid(int dtype, int edge, int wg)
     return edge == 16 ? x2_valid_wg<float, 16>(wg) : x2_valid_wg<float, 32>(wg);
 }
 
-bool x2_dispatch(int dtype, int edge, int wg, const X2Args &a, int ntiles, hipStream_t st)
+bool x2_dispatch(int dtype, int edge, int wg, int zc, const X2Args &a, int ntiles, hipStream_t st)
 {
     if (dtype == CNV_F64)
-        return edge == 16 ? x2_dispatch_wg<double, 16>(a, wg, ntiles, st) : x2_dispatch_wg<double, 32>(a, wg, ntiles, st);
-    return edge == 16 ? x2_dispatch_wg<float, 16>(a, wg, ntiles, st) : x2_dispatch_wg<float, 32>(a, wg, ntiles, st);
+        return edge == 16 ? x2_dispatch_wg<double, 16>(a, wg, zc, ntiles, st)
+                          : x2_dispatch_wg<double, 32>(a, wg, zc, ntiles, st);
+    return edge == 16 ? x2_dispatch_wg<float, 16>(a, wg, zc, ntiles, st)
+                      : x2_dispatch_wg<float, 32>(a, wg, zc, ntiles, st);
 }
 
 } // namespace
@@ -467,8 +485,9 @@ int launch_diffusion_x2(cnv_grid_s *u, cnv_grid_s *out, double nu_dt, double *pa
     a.u = static_cast<const char *>(u->data);
     a.out = static_cast<char *>(out->data);
     a.G = geom_of(u);
-    a.zchunk = zchunk < edge ? zchunk : edge;
-    a.nzc = (edge + a.zchunk - 1) / a.zchunk;
+    if (!x2_zchunk_ok(zchunk))
+        return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: zchunk must be 4 or 16");
+    a.nzc = edge / zchunk;
     a.tiles_per_comp = (int)(u->nblocks_total * a.nzc);
     for (int d = 0; d < 3; ++d)
         a.ih2[d] = 1. / (u->h[d] * u->h[d]);
@@ -483,7 +502,7 @@ int launch_diffusion_x2(cnv_grid_s *u, cnv_grid_s *out, double nu_dt, double *pa
     const long long ntiles = (long long)a.tiles_per_comp * u->desc.ncomp;
     if (ntiles >= (1LL << 31))
         return fail(CNV_ERR_UNSUPPORTED, "grid too large for the two-step kernel");
-    if (!x2_dispatch(u->desc.dtype, edge, wg, a, (int)ntiles, st))
+    if (!x2_dispatch(u->desc.dtype, edge, wg, zchunk, a, (int)ntiles, st))
         return fail(CNV_ERR_UNSUPPORTED, "two-step tile shape not available for this block edge and dtype");
     CNV_HIP(hipGetLastError());
     if (nparts)
@@ -495,8 +514,8 @@ int x2_set_tune(int dtype, int edge, int wg, int zchunk)
 {
     if (edge != 16 && edge != 32)
         return fail(CNV_ERR_INVALID, "two-step kernel: block edge must be 16 or 32");
-    if (zchunk < 1 || zchunk > edge)
-        return fail(CNV_ERR_INVALID, "zchunk must be in [1, edge]");
+    if (!x2_zchunk_ok(zchunk))
+        return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: zchunk must be 4 or 16");
     if (!x2_valid(dtype, edge, wg))
         return fail(CNV_ERR_UNSUPPORTED, "two-step tile shape not available for this block edge and dtype");
     std::lock_guard<std::mutex> lk(g_x2_mtx);
@@ -529,7 +548,7 @@ int autotune_x2(cnv_grid_s *u, cnv_grid_s *out, int iters, int *best_wg, int *be
     float best = 1e30f;
     int bw = 0, bz = 0, cnt = 0, rc = CNV_OK;
     for (int wi = 0; wi < 4 && rc == CNV_OK; ++wi)
-        for (int z = 4; z <= edge && rc == CNV_OK; z *= 2) {
+        for (int z = 4; z <= 16 && rc == CNV_OK; z *= 4) {
             if (x2_set_tune(u->desc.dtype, edge, wgs[wi], z) != CNV_OK)
                 continue;
             rc = launch_diffusion_x2(u, out, 1e-9, nullptr, nullptr, nullptr, nullptr, nullptr, wgs[wi], z, nullptr,
