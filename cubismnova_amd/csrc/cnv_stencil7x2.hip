@@ -16,6 +16,7 @@
 
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 
 namespace cnv
@@ -77,13 +78,13 @@ __device__ inline T dpp_shift(T v)
 constexpr int kDppRowShr1 = 0x111; // lane i <- lane i-1
 constexpr int kDppRowShl1 = 0x101; // lane i <- lane i+1
 
-// ZC planes per tile; ZC + 2 planes of u^{n+1} (a multiple of the three
-// register slots) make every slot index and the trip count static.
+// ZC planes per tile: the ZC + 2 planes of u^{n+1} run as a static rotation
+// of the three register slots, so every slot index is a constant.
 template <typename T, int N, int WG, int ZC>
 __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
 {
     using S = X2Shape<T, N, WG>;
-    static_assert((ZC + 2) % 3 == 0 && N % ZC == 0, "z chunk");
+    static_assert(N % ZC == 0, "z chunk");
     constexpr int VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, PA = S::PA, PLA = S::PLA, PLB = S::PLB;
     constexpr int NHX = S::NHX, NHY = S::NHY, NH = S::NH, HS = S::HS, NR = S::NR, RS = S::RS;
     static_assert(S::valid, "tile shape");
@@ -239,105 +240,128 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     U0 Pq = load_u0(z0);
 
     double acc1 = 0.0, acc2 = 0.0;
-    for (int it = 0; it < (ZC + 2) / 3; ++it) {
-        const int qb = z0 - 1 + 3 * it;
+    // plane q of the tile, register slots in the static rotation u = (q - z0 + 1) % 3
+    auto plane = [&](auto U, const int q) __attribute__((always_inline)) {
+        constexpr int u = decltype(U)::value;
+        V(&c0m)[R] = U0r[u % 3];
+        V(&c0c)[R] = U0r[(u + 1) % 3];
+        V(&c0p)[R] = U0r[(u + 2) % 3];
+        V(&m1)[R] = U1r[u % 3];
+        V(&c1)[R] = U1r[(u + 1) % 3];
+        V(&n1)[R] = U1r[(u + 2) % 3];
+        store_u0(Pq, slotA(q + 1));
 #pragma unroll
-        for (int u = 0; u < 3; ++u) {
-            const int q = qb + u;
-            V(&c0m)[R] = U0r[u % 3];
-            V(&c0c)[R] = U0r[(u + 1) % 3];
-            V(&c0p)[R] = U0r[(u + 2) % 3];
-            V(&m1)[R] = U1r[u % 3];
-            V(&c1)[R] = U1r[(u + 1) % 3];
-            V(&n1)[R] = U1r[(u + 2) % 3];
-            store_u0(Pq, slotA(q + 1));
+        for (int r = 0; r < R; ++r)
+            c0p[r] = Pq.c[r];
+        if (q + 2 <= z1 + 1)
+            Pq = load_u0(q + 2);
+        __syncthreads();
+        const T *Am = slotA(q - 1), *Ac = slotA(q), *Ap = slotA(q + 1);
+        const bool inner = q >= z0 && q < z1;
+        T *Bq = slotB(q);
+        // u^{n+1} on the block's cells of plane q (z neighbours from registers)
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                c0p[r] = Pq.c[r];
-            if (q + 2 <= z1 + 1)
-                Pq = load_u0(q + 2);
-            __syncthreads();
-            const T *Am = slotA(q - 1), *Ac = slotA(q), *Ap = slotA(q + 1);
-            const bool inner = q >= z0 && q < z1;
-            T *Bq = slotB(q);
-            // u^{n+1} on the block's cells of plane q (z neighbours from registers)
+        for (int r = 0; r < R; ++r) {
+            const int y = ty + r * ROWS;
+            const int o = (y + 2) * PA + VEC + x0;
+            const V ym = *reinterpret_cast<const V *>(Ac + o - PA);
+            const V yp = *reinterpret_cast<const V *>(Ac + o + PA);
+            // x neighbours of the end cells from the adjacent lanes; the
+            // first / last lane of a row reads the halo
+            T xl = dpp_shift<T, kDppRowShr1>(c0c[r].v[VEC - 1]);
+            T xr = dpp_shift<T, kDppRowShl1>(c0c[r].v[0]);
+            if (tx == 0)
+                xl = Ac[o - 1];
+            if (tx == TX - 1)
+                xr = Ac[o + VEC];
+            V res;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int y = ty + r * ROWS;
-                const int o = (y + 2) * PA + VEC + x0;
-                const V ym = *reinterpret_cast<const V *>(Ac + o - PA);
-                const V yp = *reinterpret_cast<const V *>(Ac + o + PA);
-                // x neighbours of the end cells from the adjacent lanes; the
-                // first / last lane of a row reads the halo
-                T xl = dpp_shift<T, kDppRowShr1>(c0c[r].v[VEC - 1]);
-                T xr = dpp_shift<T, kDppRowShl1>(c0c[r].v[0]);
-                if (tx == 0)
-                    xl = Ac[o - 1];
-                if (tx == TX - 1)
-                    xr = Ac[o + VEC];
-                V res;
-#pragma unroll
-                for (int v = 0; v < VEC; ++v) {
-                    const T xm = v == 0 ? xl : c0c[r].v[v - 1];
-                    const T xp = v == VEC - 1 ? xr : c0c[r].v[v + 1];
-                    res.v[v] = euler(c0c[r].v[v], xm, xp, ym.v[v], yp.v[v], c0m[r].v[v], c0p[r].v[v]);
-                }
-                n1[r] = res;
-                if (inner)
-                    *reinterpret_cast<V *>(Bq + (y + 1) * PA + VEC + x0) = res;
+            for (int v = 0; v < VEC; ++v) {
+                const T xm = v == 0 ? xl : c0c[r].v[v - 1];
+                const T xp = v == VEC - 1 ? xr : c0c[r].v[v + 1];
+                res.v[v] = euler(c0c[r].v[v], xm, xp, ym.v[v], yp.v[v], c0m[r].v[v], c0p[r].v[v]);
             }
-            // u^{n+1} on the ring of face-neighbour cells (planes the second step computes)
-            if (inner) {
+            n1[r] = res;
+            if (inner)
+                *reinterpret_cast<V *>(Bq + (y + 1) * PA + VEC + x0) = res;
+        }
+        // u^{n+1} on the ring of face-neighbour cells (planes the second step computes)
+        if (inner) {
 #pragma unroll
-                for (int h = 0; h < RS; ++h) {
-                    const int s = tid + h * WG;
-                    if (s < NR) {
-                        int x, y;
-                        if (s < 2 * N) {
-                            x = (s / N) ? N : -1;
-                            y = s % N;
-                        } else {
-                            y = ((s - 2 * N) / N) ? N : -1;
-                            x = (s - 2 * N) % N;
-                        }
-                        const int o = (y + 2) * PA + VEC + x;
-                        Bq[(y + 1) * PA + VEC + x] =
-                            euler(Ac[o], Ac[o - 1], Ac[o + 1], Ac[o - PA], Ac[o + PA], Am[o], Ap[o]);
+            for (int h = 0; h < RS; ++h) {
+                const int s = tid + h * WG;
+                if (s < NR) {
+                    int x, y;
+                    if (s < 2 * N) {
+                        x = (s / N) ? N : -1;
+                        y = s % N;
+                    } else {
+                        y = ((s - 2 * N) / N) ? N : -1;
+                        x = (s - 2 * N) % N;
                     }
-                }
-            }
-            // u^{n+2} on the block's cells of plane q - 1 (its u^{n+1} plane was
-            // completed before this plane's barrier); the sums of squares of
-            // u^{n+1} and u^{n+2} over the tile's cells
-            if (q >= z0 + 1) {
-                const T *Br = slotB(q - 1);
-                T *op = ob + NN * (q - 1);
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int y = ty + r * ROWS;
-                    const int o = (y + 1) * PA + VEC + x0;
-                    const V ym = *reinterpret_cast<const V *>(Br + o - PA);
-                    const V yp = *reinterpret_cast<const V *>(Br + o + PA);
-                    T xl = dpp_shift<T, kDppRowShr1>(c1[r].v[VEC - 1]);
-                    T xr = dpp_shift<T, kDppRowShl1>(c1[r].v[0]);
-                    if (tx == 0)
-                        xl = Br[o - 1];
-                    if (tx == TX - 1)
-                        xr = Br[o + VEC];
-                    V res;
-#pragma unroll
-                    for (int v = 0; v < VEC; ++v) {
-                        const T xm = v == 0 ? xl : c1[r].v[v - 1];
-                        const T xp = v == VEC - 1 ? xr : c1[r].v[v + 1];
-                        res.v[v] = euler(c1[r].v[v], xm, xp, ym.v[v], yp.v[v], m1[r].v[v], n1[r].v[v]);
-                        acc1 = __builtin_fma((double)c1[r].v[v], (double)c1[r].v[v], acc1);
-                        acc2 = __builtin_fma((double)res.v[v], (double)res.v[v], acc2);
-                    }
-                    *reinterpret_cast<V *>(op + x0 + N * y) = res;
+                    const int o = (y + 2) * PA + VEC + x;
+                    Bq[(y + 1) * PA + VEC + x] =
+                        euler(Ac[o], Ac[o - 1], Ac[o + 1], Ac[o - PA], Ac[o + PA], Am[o], Ap[o]);
                 }
             }
         }
+        // u^{n+2} on the block's cells of plane q - 1 (its u^{n+1} plane was
+        // completed before this plane's barrier); the sums of squares of
+        // u^{n+1} and u^{n+2} over the tile's cells
+        if (q >= z0 + 1) {
+            const T *Br = slotB(q - 1);
+            T *op = ob + NN * (q - 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int y = ty + r * ROWS;
+                const int o = (y + 1) * PA + VEC + x0;
+                const V ym = *reinterpret_cast<const V *>(Br + o - PA);
+                const V yp = *reinterpret_cast<const V *>(Br + o + PA);
+                T xl = dpp_shift<T, kDppRowShr1>(c1[r].v[VEC - 1]);
+                T xr = dpp_shift<T, kDppRowShl1>(c1[r].v[0]);
+                if (tx == 0)
+                    xl = Br[o - 1];
+                if (tx == TX - 1)
+                    xr = Br[o + VEC];
+                V res;
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    const T xm = v == 0 ? xl : c1[r].v[v - 1];
+                    const T xp = v == VEC - 1 ? xr : c1[r].v[v + 1];
+                    res.v[v] = euler(c1[r].v[v], xm, xp, ym.v[v], yp.v[v], m1[r].v[v], n1[r].v[v]);
+                    acc1 = __builtin_fma((double)c1[r].v[v], (double)c1[r].v[v], acc1);
+                    acc2 = __builtin_fma((double)res.v[v], (double)res.v[v], acc2);
+                }
+                *reinterpret_cast<V *>(op + x0 + N * y) = res;
+            }
+        }
+    };
+    // Wave priority falls from 3 to 0 over the tile, so that on a compute
+    // unit the waves that lag behind are issued first and the workgroups
+    // sharing it finish together instead of the oldest first.
+    constexpr int kIters = (ZC + 2) / 3;
+    for (int it = 0; it < kIters; ++it) {
+        const int qb = z0 - 1 + 3 * it;
+        const int lvl = (kIters - 1 - it) * 4 / kIters;
+        if (lvl >= 3)
+            __builtin_amdgcn_s_setprio(3);
+        else if (lvl == 2)
+            __builtin_amdgcn_s_setprio(2);
+        else if (lvl == 1)
+            __builtin_amdgcn_s_setprio(1);
+        else
+            __builtin_amdgcn_s_setprio(0);
+        plane(std::integral_constant<int, 0>{}, qb);
+        plane(std::integral_constant<int, 1>{}, qb + 1);
+        plane(std::integral_constant<int, 2>{}, qb + 2);
     }
+    // the last (ZC + 2) % 3 planes continue the rotation
+    constexpr int kTail = (ZC + 2) % 3;
+    const int qt = z0 - 1 + 3 * ((ZC + 2) / 3);
+    if constexpr (kTail >= 1)
+        plane(std::integral_constant<int, 0>{}, qt);
+    if constexpr (kTail >= 2)
+        plane(std::integral_constant<int, 1>{}, qt + 1);
 
     // per-tile partials in a fixed wave tree and wave order
     auto tile_sum = [&](double v, double *part) {
@@ -372,7 +396,7 @@ std::map<std::pair<int, int>, X2Tune> g_x2_tune; // (dtype, edge) -> choice
 X2Tune x2_default(int dtype, int edge)
 {
     if (edge == 32)
-        return dtype == CNV_F64 ? X2Tune{256, 16} : X2Tune{256, 16};
+        return dtype == CNV_F64 ? X2Tune{512, 32} : X2Tune{256, 16};
     return dtype == CNV_F64 ? X2Tune{128, 16} : X2Tune{64, 16};
 }
 
@@ -383,8 +407,8 @@ X2Tune x2_tune(int dtype, int edge)
     return it == g_x2_tune.end() ? x2_default(dtype, edge) : it->second;
 }
 
-// z-chunks with a static trip count: ZC + 2 a multiple of 3, dividing 16 and 32
-inline bool x2_zchunk_ok(int zc) { return zc == 4 || zc == 16; }
+// z-chunks compiled: 8, 16 and 32 planes (up to the block edge)
+inline bool x2_zchunk_ok(int zc) { return zc == 8 || zc == 16 || zc == 32; }
 
 template <typename T, int N, int WG>
 bool x2_launch1(const X2Args &a, int zc, int ntiles, hipStream_t st)
@@ -392,9 +416,13 @@ bool x2_launch1(const X2Args &a, int zc, int ntiles, hipStream_t st)
     if constexpr (X2Shape<T, N, WG>::valid) {
         if (zc == 16)
             hipLaunchKernelGGL((k_stencil7x2<T, N, WG, 16>), dim3(ntiles), dim3(WG), 0, st, a);
-        else if (zc == 4)
-            hipLaunchKernelGGL((k_stencil7x2<T, N, WG, 4>), dim3(ntiles), dim3(WG), 0, st, a);
-        else
+        else if (zc == 8)
+            hipLaunchKernelGGL((k_stencil7x2<T, N, WG, 8>), dim3(ntiles), dim3(WG), 0, st, a);
+        else if constexpr (N == 32) {
+            if (zc != 32)
+                return false;
+            hipLaunchKernelGGL((k_stencil7x2<T, N, WG, 32>), dim3(ntiles), dim3(WG), 0, st, a);
+        } else
             return false;
         return true;
     }
@@ -486,7 +514,9 @@ int launch_diffusion_x2(cnv_grid_s *u, cnv_grid_s *out, double nu_dt, double *pa
     a.out = static_cast<char *>(out->data);
     a.G = geom_of(u);
     if (!x2_zchunk_ok(zchunk))
-        return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: zchunk must be 4 or 16");
+        return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: zchunk must be 8, 16 or 32");
+    if (zchunk > edge)
+        return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: zchunk larger than the block edge");
     a.nzc = edge / zchunk;
     a.tiles_per_comp = (int)(u->nblocks_total * a.nzc);
     for (int d = 0; d < 3; ++d)
@@ -514,8 +544,8 @@ int x2_set_tune(int dtype, int edge, int wg, int zchunk)
 {
     if (edge != 16 && edge != 32)
         return fail(CNV_ERR_INVALID, "two-step kernel: block edge must be 16 or 32");
-    if (!x2_zchunk_ok(zchunk))
-        return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: zchunk must be 4 or 16");
+    if (!x2_zchunk_ok(zchunk) || zchunk > edge)
+        return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: zchunk must be 8, 16 or 32, at most the edge");
     if (!x2_valid(dtype, edge, wg))
         return fail(CNV_ERR_UNSUPPORTED, "two-step tile shape not available for this block edge and dtype");
     std::lock_guard<std::mutex> lk(g_x2_mtx);
@@ -548,7 +578,7 @@ int autotune_x2(cnv_grid_s *u, cnv_grid_s *out, int iters, int *best_wg, int *be
     float best = 1e30f;
     int bw = 0, bz = 0, cnt = 0, rc = CNV_OK;
     for (int wi = 0; wi < 4 && rc == CNV_OK; ++wi)
-        for (int z = 4; z <= 16 && rc == CNV_OK; z *= 4) {
+        for (int z = 8; z <= edge && rc == CNV_OK; z *= 2) {
             if (x2_set_tune(u->desc.dtype, edge, wgs[wi], z) != CNV_OK)
                 continue;
             rc = launch_diffusion_x2(u, out, 1e-9, nullptr, nullptr, nullptr, nullptr, nullptr, wgs[wi], z, nullptr,

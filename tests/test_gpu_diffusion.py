@@ -281,7 +281,7 @@ TWO_STEP_SHAPES = {np.float32: [(16, 64), (32, 128), (32, 256)], np.float64: [(1
 def test_two_step_kernel_matches_single_steps(gpu, dtype):
     """Periodic grids take the two-steps-per-launch kernel inside diffusion_run:
     fields bit-identical to single steps and norms within 1e-13 of the oracle,
-    for every tile shape and z depth (4 and 16 planes), odd and even step counts, one-block
+    for every tile shape and z depth (8, 16 and 32 planes), odd and even step counts, one-block
     directions (self-periodic) and several blocks."""
     for nb, e in [((2, 2, 2), 16), ((3, 2, 1), 32), ((1, 1, 1), 16), ((2, 1, 2), 32)]:
         n = (e, e, e)
@@ -298,7 +298,7 @@ def test_two_step_kernel_matches_single_steps(gpu, dtype):
         for e2, wg in TWO_STEP_SHAPES[dtype]:
             if e2 != e:
                 continue
-            for z in (4, 16):
+            for z in sorted({8, 16, e}):
                 cnv.set_tune_diffusion_run(dtype, e, wg, z)
                 for ns in (4, 5, nsteps):
                     u = cnv.Cartesian(nb, n, dtype=dtype)
@@ -313,7 +313,7 @@ def test_two_step_kernel_matches_single_steps(gpu, dtype):
                         assert abs(gn[k] - refn[k]) <= 1e-13 * refn[k], (nb, wg, z, ns, k)
     # back to the built-in choices for the tests that follow
     cnv.set_tune_diffusion_run(dtype, 16, 64 if dtype == np.float32 else 128, 16)
-    cnv.set_tune_diffusion_run(dtype, 32, 256, 16)
+    cnv.set_tune_diffusion_run(dtype, 32, 256 if dtype == np.float32 else 512, 16 if dtype == np.float32 else 32)
 
 
 def test_two_step_autotune_and_fallback(gpu):
