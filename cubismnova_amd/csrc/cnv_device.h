@@ -11,6 +11,23 @@ struct alignas(16) VecT {
     T v[N];
 };
 
+// Wave priority from 3 at the start of a workgroup's z loop down to 0 at
+// its end (done = planes finished, total = planes of the tile): on a compute
+// unit the waves that lag behind are issued first, so the workgroups that
+// share it finish together instead of the oldest first.
+__device__ inline void prio_ramp(int done, int total)
+{
+    const int lvl = (total - 1 - done) * 4 / total;
+    if (lvl >= 3)
+        __builtin_amdgcn_s_setprio(3);
+    else if (lvl == 2)
+        __builtin_amdgcn_s_setprio(2);
+    else if (lvl == 1)
+        __builtin_amdgcn_s_setprio(1);
+    else
+        __builtin_amdgcn_s_setprio(0);
+}
+
 // Plain vector load / store of exactly N elements (N * sizeof(T) bytes,
 // naturally aligned), whatever the padding of VecT.
 template <typename T, int N>
@@ -34,6 +51,20 @@ __device__ inline void store_vec(T *p, const VecT<T, N> &v)
     for (int i = 0; i < N; ++i)
         x[i] = v.v[i];
     *reinterpret_cast<vt *>(p) = x;
+}
+
+// Streaming (non-temporal) vector load of exactly N elements, for data read
+// once per launch.
+template <typename T, int N>
+__device__ inline VecT<T, N> load_nt(const T *p)
+{
+    typedef T vt __attribute__((ext_vector_type(N)));
+    const vt x = __builtin_nontemporal_load(reinterpret_cast<const vt *>(p));
+    VecT<T, N> r;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        r.v[i] = x[i];
+    return r;
 }
 
 // Streaming (non-temporal) 16-byte store of a VecT.

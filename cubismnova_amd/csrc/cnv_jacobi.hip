@@ -139,12 +139,13 @@ __global__ __launch_bounds__(WG) void k_jacobi(const JArgs a)
     load_plane(z0, true, pc);
     load_plane(z0 + 1, z0 + 1 < z1, pn);
     for (int z = z0; z < z1; ++z) {
+        prio_ramp(z - z0, z1 - z0);
         if (z + 2 <= z1)
             load_plane(z + 2, z + 2 < z1, pnn);
         VF fr[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            fr[r] = load_vec<TF, VEC>(fb + x0 + NX * (ty + r * ROWS) + SXY * z);
+            fr[r] = load_nt<TF, VEC>(fb + x0 + NX * (ty + r * ROWS) + SXY * z);
         T *L = lds[(z - z0) & 1];
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(WG) void k_jacobi(const JArgs a)
                     acc = __builtin_fma((double)rr, (double)rr, acc);
                 }
             }
-            store_vec<TO, VEC>(ob + x0 + NX * y + SXY * z, res);
+            store_nt<TO, VEC>(ob + x0 + NX * y + SXY * z, res);
         }
         pm = pc;
         pc = pn;

@@ -121,6 +121,7 @@ __device__ inline double s7_march(const T *__restrict__ ub, T *__restrict__ ob, 
             load_plane(z0 - 1 + k, k >= 1 && z0 - 1 + k < z1, Q[k]);
 
     for (int zb = z0; zb < z1; zb += NQ) {
+        prio_ramp(zb - z0, z1 - z0);
 #pragma unroll
         for (int u = 0; u < NQ; ++u) {
             const int z = zb + u;
@@ -217,40 +218,70 @@ __device__ inline void weno5_exact(R a, R b, R c, R d, R e, R &um, R &up)
 
 __device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// FP32 evaluation: same weights, contracted, divisions by the hardware
-// reciprocal, and the linear weights d_k and the 1/6 of the candidate
-// polynomials folded into the polynomial coefficients (num = sum_k p_k Q_k
-// with Q_k = (d_k / 6) q_k, den = sum_k d_k p_k).  Every scaled weight stays
-// within [1e-25, 1e10] for any data (beta_k / m2 <= 170 since
-// max u^2 <= 5 m2), so no product over- or underflows in FP32.
-__device__ inline void weno5_fast(float a, float b, float c, float d, float e, float &um, float &up)
+// FP32 evaluation in difference form.  With the stencil (a, b, c, d, e) =
+// u_{i-2..i+2}: first differences dm2 = b - a, dm1 = c - b, d0 = d - c,
+// dp1 = e - d; second differences tm = dm1 - dm2, t0 = d0 - dm1,
+// tp = dp1 - d0 and Tk = (65/12) tk^2; S = sum of (u_j^2 + 1e-30) over the
+// stencil, so that beta_k / m2 = 5 beta_k / S with
+// 5 beta_k = Tk + 1.25 sk^2, s0 = a - 4b + 3c = tm + 2 dm1,
+// s1 = d - b = dm1 + d0, s2 = 3c - 4d + e = tp - 2 d0.  The candidates enter
+// as offsets from c times their linear weight:
+//   u- = c + sum_k p_k Dk / sum_k d_k p_k,  D = (0.1 (5 dm1 - 2 dm2), 0.6 (dm1 + 2 d0), 0.3 (4 d0 - dp1)) / 6
+//   u+ = c + sum_k p_k Ek / sum_k d'_k p_k, E = (0.3 (dm2 - 4 dm1), 0.6 (-2 dm1 - d0), 0.1 (2 dp1 - 5 d0)) / 6
+// with p_k the product of the other two squared scaled weights.  Cells of
+// one row share the differences.  Every operation is explicit (fmaf or one
+// rounding), so any caller that forms the same differences gets the same
+// bits.  Scaled weights stay within [1e-6, 170] and their products within
+// [1e-25, 1e10], so nothing over- or underflows at any data magnitude.
+constexpr float kW5T = 65.f / 12.f; // 5 * 13/12
+
+__device__ inline float w5_sqr(float u) { return __builtin_fmaf(u, u, 1e-30f); }
+__device__ inline float w5_T(float t) { return (kW5T * t) * t; }
+// S of a stencil from its five squares, in one fixed association
+__device__ inline float w5_S(float sa, float sb, float sc, float sd, float se) { return ((sa + sb) + (sc + sd)) + se; }
+
+__device__ inline void weno5_core(float c, float dm2, float dm1, float d0, float dp1, float tm, float t0, float tp,
+                                  float Tm, float T0, float Tp, float S, float &um, float &up)
 {
-#pragma clang fp contract(fast)
-    const float m2 = (a * a + b * b + c * c + d * d + e * e) * 0.2f + 1e-30f;
-    const float im2 = rcpf(m2);
-    const float t0 = a - 2.f * b + c, s0 = a - 4.f * b + 3.f * c;
-    const float t1 = b - 2.f * c + d, s1 = b - d;
-    const float t2 = c - 2.f * d + e, s2 = 3.f * c - 4.f * d + e;
-    const float k13 = 13.f / 12.f;
-    const float b0 = (k13 * t0 * t0 + 0.25f * s0 * s0) * im2 + 1e-6f;
-    const float b1 = (k13 * t1 * t1 + 0.25f * s1 * s1) * im2 + 1e-6f;
-    const float b2 = (k13 * t2 * t2 + 0.25f * s2 * s2) * im2 + 1e-6f;
+    const float r = rcpf(S);
+    const float s0 = __builtin_fmaf(2.f, dm1, tm);
+    const float s1 = dm1 + d0;
+    const float s2 = __builtin_fmaf(-2.f, d0, tp);
+    const float b0 = __builtin_fmaf(__builtin_fmaf(1.25f * s0, s0, Tm), r, 1e-6f);
+    const float b1 = __builtin_fmaf(__builtin_fmaf(1.25f * s1, s1, T0), r, 1e-6f);
+    const float b2 = __builtin_fmaf(__builtin_fmaf(1.25f * s2, s2, Tp), r, 1e-6f);
     const float g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
     const float p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
     {
-        // d = (0.1, 0.6, 0.3): Q0 = (2a - 7b + 11c) / 60, Q1 = (-b + 5c + 2d) / 10, Q2 = (2c + 5d - e) / 20
-        const float q0 = (2.f / 60.f) * a - (7.f / 60.f) * b + (11.f / 60.f) * c;
-        const float q1 = -0.1f * b + 0.5f * c + 0.2f * d;
-        const float q2 = 0.1f * c + 0.25f * d - 0.05f * e;
-        um = (p12 * q0 + p02 * q1 + p01 * q2) * rcpf(0.1f * p12 + 0.6f * p02 + 0.3f * p01);
+        const float D0 = __builtin_fmaf(5.f / 60.f, dm1, (-2.f / 60.f) * dm2);
+        const float D1 = __builtin_fmaf(0.1f, dm1, 0.2f * d0);
+        const float D2 = __builtin_fmaf(0.2f, d0, -0.05f * dp1);
+        const float num = __builtin_fmaf(p12, D0, __builtin_fmaf(p02, D1, p01 * D2));
+        const float den = __builtin_fmaf(0.1f, p12, __builtin_fmaf(0.6f, p02, 0.3f * p01));
+        um = __builtin_fmaf(num, rcpf(den), c);
     }
     {
-        // d = (0.3, 0.6, 0.1): R0 = (-a + 5b + 2c) / 20, R1 = (2b + 5c - d) / 10, R2 = (11c - 7d + 2e) / 60
-        const float r0 = -0.05f * a + 0.25f * b + 0.1f * c;
-        const float r1 = 0.2f * b + 0.5f * c - 0.1f * d;
-        const float r2 = (11.f / 60.f) * c - (7.f / 60.f) * d + (2.f / 60.f) * e;
-        up = (p12 * r0 + p02 * r1 + p01 * r2) * rcpf(0.3f * p12 + 0.6f * p02 + 0.1f * p01);
+        const float E0 = __builtin_fmaf(0.05f, dm2, -0.2f * dm1);
+        const float E1 = __builtin_fmaf(-0.2f, dm1, -0.1f * d0);
+        const float E2 = __builtin_fmaf(2.f / 60.f, dp1, (-5.f / 60.f) * d0);
+        const float num = __builtin_fmaf(p12, E0, __builtin_fmaf(p02, E1, p01 * E2));
+        const float den = __builtin_fmaf(0.3f, p12, __builtin_fmaf(0.6f, p02, 0.1f * p01));
+        up = __builtin_fmaf(num, rcpf(den), c);
     }
+}
+
+// one cell from its stencil and the stencil's squares (sc shared between directions)
+__device__ inline void weno5_sq(float a, float b, float c, float d, float e, float sa, float sb, float sc, float sd,
+                                float se, float &um, float &up)
+{
+    const float dm2 = b - a, dm1 = c - b, d0 = d - c, dp1 = e - d;
+    const float tm = dm1 - dm2, t0 = d0 - dm1, tp = dp1 - d0;
+    weno5_core(c, dm2, dm1, d0, dp1, tm, t0, tp, w5_T(tm), w5_T(t0), w5_T(tp), w5_S(sa, sb, sc, sd, se), um, up);
+}
+
+__device__ inline void weno5_fast(float a, float b, float c, float d, float e, float &um, float &up)
+{
+    weno5_sq(a, b, c, d, e, w5_sqr(a), w5_sqr(b), w5_sqr(c), w5_sqr(d), w5_sqr(e), um, up);
 }
 
 template <typename T>
@@ -372,6 +403,7 @@ __device__ inline void w5_march(const T *__restrict__ ub, T *__restrict__ ob, lo
     load_plane(z0 + 2, z0 + 2 < z1, q4);
 
     for (int z = z0; z < z1; ++z) {
+        prio_ramp(z - z0, z1 - z0);
         if (z + 3 <= z1 + 1)
             load_plane(z + 3, z + 3 < z1, q5);
         T *L = lds + ((z - z0) & 1) * PLANE;
@@ -413,12 +445,42 @@ __device__ inline void w5_march(const T *__restrict__ ub, T *__restrict__ ob, lo
             xs[VEC + 2] = row[x0 + VEC];
             xs[VEC + 3] = row[x0 + VEC + 1];
             V o[6];
+            if constexpr (sizeof(T) == 4) {
+                // x: differences, second differences and squares shared along the row
+                T sq[VEC + 4], dx[VEC + 3], tx[VEC + 3], Tx[VEC + 3], pp[VEC + 3];
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) {
-                weno5<T>(xs[v], xs[v + 1], xs[v + 2], xs[v + 3], xs[v + 4], o[0].v[v], o[1].v[v]);
-                weno5<T>(ym2.v[v], ym1.v[v], q2.c[r].v[v], yp1.v[v], yp2.v[v], o[2].v[v], o[3].v[v]);
-                weno5<T>(q0.c[r].v[v], q1.c[r].v[v], q2.c[r].v[v], q3.c[r].v[v], q4.c[r].v[v], o[4].v[v],
-                         o[5].v[v]);
+                for (int j = 0; j < VEC + 4; ++j)
+                    sq[j] = w5_sqr(xs[j]);
+#pragma unroll
+                for (int j = 0; j < VEC + 3; ++j) {
+                    dx[j] = xs[j + 1] - xs[j];
+                    pp[j] = sq[j] + sq[j + 1];
+                }
+#pragma unroll
+                for (int j = 1; j < VEC + 3; ++j) {
+                    tx[j] = dx[j] - dx[j - 1];
+                    Tx[j] = w5_T(tx[j]);
+                }
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    const int i = v + 2;
+                    weno5_core(xs[i], dx[i - 2], dx[i - 1], dx[i], dx[i + 1], tx[i - 1], tx[i], tx[i + 1], Tx[i - 1],
+                               Tx[i], Tx[i + 1], (pp[i - 2] + pp[i]) + sq[i + 2], o[0].v[v], o[1].v[v]);
+                    const T c = q2.c[r].v[v], sc = sq[i];
+                    weno5_sq(ym2.v[v], ym1.v[v], c, yp1.v[v], yp2.v[v], w5_sqr(ym2.v[v]), w5_sqr(ym1.v[v]), sc,
+                             w5_sqr(yp1.v[v]), w5_sqr(yp2.v[v]), o[2].v[v], o[3].v[v]);
+                    weno5_sq(q0.c[r].v[v], q1.c[r].v[v], c, q3.c[r].v[v], q4.c[r].v[v], w5_sqr(q0.c[r].v[v]),
+                             w5_sqr(q1.c[r].v[v]), sc, w5_sqr(q3.c[r].v[v]), w5_sqr(q4.c[r].v[v]), o[4].v[v],
+                             o[5].v[v]);
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    weno5<T>(xs[v], xs[v + 1], xs[v + 2], xs[v + 3], xs[v + 4], o[0].v[v], o[1].v[v]);
+                    weno5<T>(ym2.v[v], ym1.v[v], q2.c[r].v[v], yp1.v[v], yp2.v[v], o[2].v[v], o[3].v[v]);
+                    weno5<T>(q0.c[r].v[v], q1.c[r].v[v], q2.c[r].v[v], q3.c[r].v[v], q4.c[r].v[v], o[4].v[v],
+                             o[5].v[v]);
+                }
             }
             const long long off = x0 + NX * y + SXY * z;
 #pragma unroll

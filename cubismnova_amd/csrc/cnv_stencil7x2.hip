@@ -336,21 +336,10 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
             }
         }
     };
-    // Wave priority falls from 3 to 0 over the tile, so that on a compute
-    // unit the waves that lag behind are issued first and the workgroups
-    // sharing it finish together instead of the oldest first.
     constexpr int kIters = (ZC + 2) / 3;
     for (int it = 0; it < kIters; ++it) {
         const int qb = z0 - 1 + 3 * it;
-        const int lvl = (kIters - 1 - it) * 4 / kIters;
-        if (lvl >= 3)
-            __builtin_amdgcn_s_setprio(3);
-        else if (lvl == 2)
-            __builtin_amdgcn_s_setprio(2);
-        else if (lvl == 1)
-            __builtin_amdgcn_s_setprio(1);
-        else
-            __builtin_amdgcn_s_setprio(0);
+        prio_ramp(it, kIters);
         plane(std::integral_constant<int, 0>{}, qb);
         plane(std::integral_constant<int, 1>{}, qb + 1);
         plane(std::integral_constant<int, 2>{}, qb + 2);
