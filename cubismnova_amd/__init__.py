@@ -261,7 +261,11 @@ class Cartesian:
     grids only.
     """
 
-    def __init__(self, nblocks, block_cells, begin=0.0, end=1.0, dtype=np.float64, ncomp=1, entity="cell"):
+    def __init__(self, nblocks, block_cells, begin=0.0, end=1.0, dtype=np.float64, ncomp=1, entity="cell",
+                 device_data=None, device_bytes=0):
+        """device_data (an address, with device_bytes): build the grid over
+        that device memory instead of a new allocation (cnv_grid_wrap); the
+        memory must stay alive as long as the grid."""
         nb = [int(nblocks)] * 3 if np.isscalar(nblocks) else [int(v) for v in nblocks]
         bc = [int(block_cells)] * 3 if np.isscalar(block_cells) else [int(v) for v in block_cells]
         b = [float(begin)] * 3 if np.isscalar(begin) else [float(v) for v in begin]
@@ -280,8 +284,14 @@ class Cartesian:
         self.entity = entity
         self.nslabs = 3 * int(ncomp) if entity == "face" else int(ncomp)
         h = ctypes.c_void_p()
-        call("cnv_grid_create", ctypes.byref(d), ctypes.byref(h))
+        if device_data is None:
+            call("cnv_grid_create", ctypes.byref(d), ctypes.byref(h))
+        else:
+            addr = device_data.value if isinstance(device_data, ctypes.c_void_p) else int(device_data)
+            call("cnv_grid_wrap", ctypes.byref(d), ctypes.c_void_p(addr), ctypes.c_size_t(int(device_bytes)),
+                 ctypes.byref(h))
         self.handle = h
+        self._owner = None
         bb, cb = ctypes.c_size_t(), ctypes.c_size_t()
         call("cnv_grid_info", h, None, ctypes.byref(bb), ctypes.byref(cb))
         self.block_bytes, self.component_bytes = bb.value, cb.value
@@ -290,6 +300,25 @@ class Cartesian:
         self.nblocks_total = nb[0] * nb[1] * nb[2]
         self._mesh = _Mesh(b, e, [nb[k] * bc[k] for k in range(3)])
         self._bcs = {}
+
+    @classmethod
+    def from_tensor(cls, t, nblocks, block_cells, begin=0.0, end=1.0, ncomp=1, entity="cell"):
+        """Grid over the storage of a contiguous torch tensor on the GPU
+        (float32 or float64), no copy.  The tensor holds the reference
+        layout (component slabs of 32-byte rounded blocks, x-fastest) and is
+        kept alive by the grid.  torch brings its own HIP runtime, which sees
+        the device only if it initialises before this library's first GPU
+        call in the process: touch the GPU with torch first."""
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError("need a contiguous tensor on the GPU")
+        import torch
+        dt = {torch.float32: np.float32, torch.float64: np.float64}.get(t.dtype)
+        if dt is None:
+            raise ValueError("tensor dtype must be float32 or float64")
+        g = cls(nblocks, block_cells, begin, end, dtype=dt, ncomp=ncomp, entity=entity, device_data=t.data_ptr(),
+                device_bytes=t.numel() * t.element_size())
+        g._owner = t
+        return g
 
     # -- reference queries
     def size(self):

@@ -258,9 +258,10 @@ int cnv_stream_synchronize(cnv_stream s)
 
 // ---------------------------------------------------------------- grids
 
-int cnv_grid_create(const cnv_grid_desc *desc, cnv_grid *out)
+// Grid over `data` (device memory of at least the grid's bytes) or, with
+// data == nullptr, over a new allocation the grid owns.
+static int grid_make(const cnv_grid_desc *desc, void *data, size_t data_bytes, cnv_grid *out)
 {
-    CNV_GUARD_BEGIN
     if (!desc || !out)
         return fail(CNV_ERR_INVALID, "descriptor or output handle is NULL");
     *out = nullptr;
@@ -287,37 +288,72 @@ int cnv_grid_create(const cnv_grid_desc *desc, cnv_grid *out)
     const long long nbt = (long long)desc->nblocks[0] * desc->nblocks[1] * desc->nblocks[2];
     if (cells > (1LL << 30) || nbt > (1LL << 30))
         return fail(CNV_ERR_INVALID, "grid too large");
+    const size_t elem = desc->dtype == CNV_F64 ? 8 : 4;
+    // Cartesian::alloc_ (Grid/Cartesian.h:490-529): 32-byte aligned blocks
+    const size_t block_bytes = ((size_t)elements * elem + 31) / 32 * 32;
+    const size_t all = block_bytes * (size_t)nbt * (size_t)nslabs;
+    if (data) {
+        if ((reinterpret_cast<uintptr_t>(data) & 31) != 0)
+            return fail(CNV_ERR_INVALID, "wrapped grid memory must be 32-byte aligned");
+        if (data_bytes < all)
+            return fail(CNV_ERR_SIZE, "wrapped memory smaller than the grid");
+        hipPointerAttribute_t attr;
+        if (hipPointerGetAttributes(&attr, data) != hipSuccess ||
+            (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged)) {
+            (void)hipGetLastError();
+            return fail(CNV_ERR_INVALID, "wrapped grid memory is not device memory");
+        }
+    }
 
     cnv_grid_s *g = new cnv_grid_s();
     g->desc = *desc;
-    g->elem = desc->dtype == CNV_F64 ? 8 : 4;
-    // Cartesian::alloc_ (Grid/Cartesian.h:490-529): 32-byte aligned blocks
-    g->block_bytes = ((size_t)elements * g->elem + 31) / 32 * 32;
+    g->elem = elem;
+    g->block_bytes = block_bytes;
     g->nblocks_total = nbt;
     g->nslabs = nslabs;
-    g->component_bytes = g->block_bytes * (size_t)nbt;
+    g->component_bytes = block_bytes * (size_t)nbt;
     for (int d = 0; d < 3; ++d)
         g->h[d] = (desc->end[d] - desc->begin[d]) / (double)(desc->nblocks[d] * desc->block_cells[d]);
     g->bcs.assign((size_t)nslabs, std::vector<std::vector<cnv_bc>>((size_t)nbt));
     g->tables.resize((size_t)nslabs);
-    const size_t all = g->component_bytes * (size_t)nslabs;
-    hipError_t e = hipMalloc(&g->data, all);
-    if (e != hipSuccess) {
-        delete g;
-        return fail(CNV_ERR_ALLOC, std::string("grid allocation: ") + hipGetErrorString(e));
+    if (data) {
+        g->data = data;
+        g->owns_data = false;
+    } else {
+        const hipError_t e = hipMalloc(&g->data, all);
+        if (e != hipSuccess) {
+            delete g;
+            return fail(CNV_ERR_ALLOC, std::string("grid allocation: ") + hipGetErrorString(e));
+        }
     }
     // reduction scratch sized for the largest launch of any operator, so that
     // no call allocates later (calls may be recorded into a HIP graph)
     const size_t nparts = (size_t)nbt * desc->ncomp * ((size_t)(cells + 255) / 256 + desc->block_cells[2]) + 4096;
-    e = hipMalloc((void **)&g->partials, 2 * sizeof(double) * nparts); // two halves: multi-step runs
+    const hipError_t e = hipMalloc((void **)&g->partials, 2 * sizeof(double) * nparts); // two halves
     if (e != hipSuccess) {
-        (void)hipFree(g->data);
+        if (g->owns_data)
+            (void)hipFree(g->data);
         delete g;
         return fail(CNV_ERR_ALLOC, std::string("scratch allocation: ") + hipGetErrorString(e));
     }
     g->partials_cap = nparts;
     *out = g;
     return CNV_OK;
+}
+
+int cnv_grid_create(const cnv_grid_desc *desc, cnv_grid *out)
+{
+    CNV_GUARD_BEGIN
+    return grid_make(desc, nullptr, 0, out);
+    CNV_GUARD_END
+}
+
+int cnv_grid_wrap(const cnv_grid_desc *desc, void *device_data, size_t bytes, cnv_grid *out)
+{
+    CNV_GUARD_BEGIN
+    if (!device_data)
+        return fail(CNV_ERR_INVALID, "wrapped grid memory is NULL");
+    return grid_make(desc, device_data, bytes, out);
     CNV_GUARD_END
 }
 
@@ -325,7 +361,8 @@ int cnv_grid_destroy(cnv_grid g)
 {
     if (!g)
         return CNV_OK;
-    (void)hipFree(g->data);
+    if (g->owns_data)
+        (void)hipFree(g->data);
     for (auto &t : g->tables) {
         if (t.heads)
             (void)hipFree(t.heads);

@@ -74,7 +74,8 @@ struct cnv_grid_s {
     size_t component_bytes = 0;
     long long nblocks_total = 0;
     int nslabs = 0;             // ncomp, 3 * ncomp for face grids
-    void *data = nullptr;       // device allocation, ncomp component slabs
+    void *data = nullptr;       // device memory, nslabs component slabs
+    bool owns_data = true;      // false for cnv_grid_wrap
     double h[3] = {0, 0, 0};    // mesh spacing (extent / cells)
 
     // BC lists per component per block (host truth)

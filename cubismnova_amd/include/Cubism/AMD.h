@@ -409,6 +409,66 @@ MIndex Block::Field<T>::extent() const
     return grid_->blockExtent();
 }
 
+// ------------------------------------------------------------ device storage
+// The Alloc seam of the reference's Grid::Cartesian (Cartesian.h:481,527,537),
+// with the contract of Alloc/AlignedBlockAllocator.h:20-57: allocate()
+// rounds `bytes` up to the alignment and returns the block memory, here in
+// device memory.  A reference Cartesian<T, Mesh, Cell, 0, State,
+// DeviceBlockAllocator> then keeps its blocks in HBM (its constructor does
+// not touch the data, Cartesian.h:466-469); host code must not dereference
+// them, and a DeviceGrid over the same allocation runs the operators in place.
+template <typename T>
+class DeviceBlockAllocator
+{
+public:
+    using DataType = T;
+    static constexpr size_t Alignment = 256;
+
+    DataType *allocate(size_t &bytes) const
+    {
+        bytes = (bytes + Alignment - 1) / Alignment * Alignment;
+        void *p = nullptr;
+        check(cnv_malloc(&p, bytes), "cnv_malloc");
+        return static_cast<DataType *>(p);
+    }
+    void deallocate(DataType *block) const
+    {
+        if (block != nullptr)
+            cnv_free(block);
+    }
+};
+
+// Non-owning device grid over an allocation in Grid::Cartesian's layout
+// (component slabs of 32-byte rounded blocks, blocks x-fastest).
+template <typename T>
+class DeviceGrid
+{
+public:
+    DeviceGrid(T *data, const size_t bytes, const MIndex &nblocks, const MIndex &block_cells, const size_t ncomp = 1,
+               const std::array<double, 3> &begin = {0, 0, 0}, const std::array<double, 3> &end = {1, 1, 1},
+               const int entity = CNV_CELL)
+    {
+        cnv_grid_desc d{};
+        for (int k = 0; k < 3; ++k) {
+            d.nblocks[k] = nblocks[k];
+            d.block_cells[k] = block_cells[k];
+            d.begin[k] = begin[k];
+            d.end[k] = end[k];
+        }
+        d.ncomp = (int)ncomp;
+        d.dtype = sizeof(T) == 8 ? CNV_F64 : CNV_F32;
+        d.entity = entity;
+        check(cnv_grid_wrap(&d, data, bytes, &g_), "cnv_grid_wrap");
+    }
+    DeviceGrid(const DeviceGrid &) = delete;
+    DeviceGrid &operator=(const DeviceGrid &) = delete;
+    ~DeviceGrid() { cnv_grid_destroy(g_); }
+    cnv_grid handle() const { return g_; }
+
+private:
+    cnv_grid g_ = nullptr;
+};
+
 // ------------------------------------------------------------ device operators
 // The per-block loops of the reference applications as single device calls.
 namespace Device

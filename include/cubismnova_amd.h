@@ -120,6 +120,14 @@ int cnv_stream_synchronize(cnv_stream s);
  * (Grid/Cartesian.h:132-143, 444-529): one device allocation holding every
  * component slab, reference byte layout. Memory is not initialised. */
 int cnv_grid_create(const cnv_grid_desc *desc, cnv_grid *out);
+/* The same grid over memory the caller owns: the `Alloc` seam of
+ * Grid::Cartesian (Cartesian.h:481,527,537; the AlignedBlockAllocator
+ * contract of Alloc/AlignedBlockAllocator.h:34-56).  A reference Cartesian
+ * built with a device allocator lays its data out exactly as described
+ * here, so its allocation can be handed over as is and the operators run on
+ * it in place.  device_data: device (or managed) memory, 32-byte aligned,
+ * at least `bytes` >= the grid's bytes; cnv_grid_destroy leaves it alone. */
+int cnv_grid_wrap(const cnv_grid_desc *desc, void *device_data, size_t bytes, cnv_grid *out);
 int cnv_grid_destroy(cnv_grid g);
 /* Layout query: block_bytes and component_bytes as in Cartesian.h:490-529. */
 int cnv_grid_info(cnv_grid g, cnv_grid_desc *desc, size_t *block_bytes, size_t *component_bytes);

@@ -277,6 +277,51 @@ static void diffusion_run_vs_steps()
                     EXPECT(a[i](x, y, z) == c[i](x, y, z));
 }
 
+// The Alloc seam: a grid laid out in memory from DeviceBlockAllocator, wrapped
+// without a copy, gives the same Laplacian as the library's own grid
+static void device_allocator_wrap()
+{
+    Grid64 u({2, 2, 2}, {16, 16, 16});
+    for (size_t i = 0; i < u.size(); ++i)
+        for (Index z = 0; z < 16; ++z)
+            for (Index y = 0; y < 16; ++y)
+                for (Index x = 0; x < 16; ++x)
+                    u[i](x, y, z) = std::cos(0.2 * (x + 5 * y) - 0.3 * z + (double)i);
+    u.upload(0);
+    DeviceBlockAllocator<double> alloc;
+    size_t bytes = 8 * 16 * 16 * 16 * sizeof(double); // 8 blocks of 32 KiB, no padding
+    double *p = alloc.allocate(bytes);
+    EXPECT(bytes % DeviceBlockAllocator<double>::Alignment == 0);
+    void *src = nullptr;
+    check(cnv_grid_data(u.handle(), 0, &src), "cnv_grid_data");
+    check(cnv_memcpy_d2d(p, src, 8 * 16 * 16 * 16 * sizeof(double), nullptr), "cnv_memcpy_d2d");
+    Grid64 a({2, 2, 2}, {16, 16, 16}), b({2, 2, 2}, {16, 16, 16});
+    {
+        DeviceGrid<double> w(p, bytes, {2, 2, 2}, {16, 16, 16});
+        check(cnv_laplacian(w.handle(), 0, a.handle(), 0, nullptr), "cnv_laplacian");
+    }
+    Device::laplacian(u, b);
+    a.download(0);
+    b.download(0);
+    for (size_t i = 0; i < a.size(); ++i)
+        for (Index z = 0; z < 16; ++z)
+            for (Index y = 0; y < 16; ++y)
+                for (Index x = 0; x < 16; ++x)
+                    EXPECT(a[i](x, y, z) == b[i](x, y, z));
+    // the wrapped memory outlives its grid
+    double first = 0;
+    check(cnv_memcpy_d2h(&first, p, sizeof(double), nullptr), "cnv_memcpy_d2h");
+    EXPECT(first == u[0](0, 0, 0));
+    bool threw = false;
+    try {
+        DeviceGrid<double> small(p, 1024, {2, 2, 2}, {16, 16, 16});
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    EXPECT(threw);
+    alloc.deallocate(p);
+}
+
 int main()
 {
     fieldlab_ghosts(false);
@@ -285,6 +330,7 @@ int main()
     cartesian_lab_analytic();
     laplacian_vs_host_loop();
     diffusion_run_vs_steps();
+    device_allocator_wrap();
     std::printf("%d checks, %d failed\n", g_checks, g_fail);
     return g_fail == 0 ? 0 : 1;
 }
