@@ -592,6 +592,40 @@ def grid_field_op(op, dst, a, b=None, scalar=0.0, comps=None, stream=None):
          b.handle if b is not None else None, int(cb), ctypes.c_double(float(scalar)), _st(stream))
 
 
+def _box(v):
+    return None if v is None else (ctypes.c_int * 3)(*[int(x) for x in v])
+
+
+def write_hdf(grid, fname, aname="data", time=0.0, file_dtype=None, face_dir=0, create_xdmf=True, begin=None,
+              end=None, stream=None):
+    """<fname>.h5 (dataset "data", Z x Y x X x ncomp) and <fname>.xmf of the
+    grid's entity span, or of the global index box [begin, end)
+    (IO/CartesianHDF.h:46-136).  file_dtype: numpy dtype of the file
+    elements (default: the grid's)."""
+    ft = _dt(grid.dtype if file_dtype is None else file_dtype)
+    call("cnv_grid_write_hdf", grid.handle, str(fname).encode(), str(aname).encode(), ft, ctypes.c_double(time),
+         int(face_dir), 1 if create_xdmf else 0, _box(begin), _box(end), _st(stream))
+
+
+def read_hdf(grid, fname, file_dtype=None, face_dir=0, begin=None, end=None, stream=None):
+    """Fill the grid (or the box [begin, end) of it) from <fname>.h5
+    (IO/CartesianHDF.h:138-215)."""
+    ft = _dt(grid.dtype if file_dtype is None else file_dtype)
+    call("cnv_grid_read_hdf", grid.handle, str(fname).encode(), ft, int(face_dir), _box(begin), _box(end),
+         _st(stream))
+
+
+def CartesianWriteHDF(fname, aname, grid, time, face_dir=0, create_xdmf=True, file_dtype=None):
+    """IO::CartesianWriteHDF<FileDataType>(fname, aname, grid, time, face_dir,
+    create_xdmf) (IO/CartesianHDF.h:120-136)."""
+    write_hdf(grid, fname, aname, time, file_dtype, face_dir, create_xdmf)
+
+
+def CartesianReadHDF(fname, grid, face_dir=0, file_dtype=None):
+    """IO::CartesianReadHDF<FileDataType>(fname, grid, face_dir) (IO/CartesianHDF.h:203-210)."""
+    read_hdf(grid, fname, file_dtype, face_dir)
+
+
 def grid_sums(g, comp=0, stream=None):
     """(sum, sum of squares) of one component, FP64, deterministic."""
     buf = DeviceBuffer(16)

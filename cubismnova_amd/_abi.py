@@ -85,6 +85,8 @@ SIGNATURES = {
     "cnv_stream_synchronize": (i32, [vp]),
     "cnv_grid_create": (i32, [ctypes.POINTER(cnv_grid_desc), ctypes.POINTER(vp)]),
     "cnv_grid_wrap": (i32, [ctypes.POINTER(cnv_grid_desc), vp, sz, ctypes.POINTER(vp)]),
+    "cnv_grid_write_hdf": (i32, [vp, ctypes.c_char_p, ctypes.c_char_p, i32, f64, i32, i32, c_int_p, c_int_p, vp]),
+    "cnv_grid_read_hdf": (i32, [vp, ctypes.c_char_p, i32, i32, c_int_p, c_int_p, vp]),
     "cnv_grid_destroy": (i32, [vp]),
     "cnv_grid_info": (i32, [vp, ctypes.POINTER(cnv_grid_desc), ctypes.POINTER(sz), ctypes.POINTER(sz)]),
     "cnv_grid_data": (i32, [vp, i32, ctypes.POINTER(vp)]),

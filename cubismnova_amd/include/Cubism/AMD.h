@@ -515,6 +515,32 @@ std::vector<double> diffusionRun(Grid::Cartesian<T> &u, Grid::Cartesian<T> &tmp,
 }
 } // namespace Device
 
+// ------------------------------------------------------------ on-disk format
+// IO::CartesianWriteHDF / CartesianReadHDF (IO/CartesianHDF.h:120-136,
+// 203-210): the device grid's data (upload() it first) to <fname>.h5 +
+// <fname>.xmf, and back (download() afterwards for the host mirror).
+namespace IO
+{
+template <typename FileDataType, typename T>
+void CartesianWriteHDF(const std::string &fname, const std::string &aname, const Grid::Cartesian<T> &grid,
+                       const double time, const size_t face_dir = 0, const bool create_xdmf = true)
+{
+    static_assert(sizeof(FileDataType) == 4 || sizeof(FileDataType) == 8, "FP32 or FP64 files");
+    check(cnv_grid_write_hdf(grid.handle(), fname.c_str(), aname.c_str(), sizeof(FileDataType) == 8 ? CNV_F64 : CNV_F32,
+                             time, (int)face_dir, create_xdmf ? 1 : 0, nullptr, nullptr, nullptr),
+          "CartesianWriteHDF");
+}
+
+template <typename FileDataType, typename T>
+void CartesianReadHDF(const std::string &fname, Grid::Cartesian<T> &grid, const size_t face_dir = 0)
+{
+    static_assert(sizeof(FileDataType) == 4 || sizeof(FileDataType) == 8, "FP32 or FP64 files");
+    check(cnv_grid_read_hdf(grid.handle(), fname.c_str(), sizeof(FileDataType) == 8 ? CNV_F64 : CNV_F32,
+                            (int)face_dir, nullptr, nullptr, nullptr),
+          "CartesianReadHDF");
+}
+} // namespace IO
+
 } // namespace Cubism
 
 #endif // CUBISM_AMD_H

@@ -339,6 +339,24 @@ void fieldDiv1_float(const float *src0, float src1, float *dst, size_t n);
 void fieldRcp_double(const double *src0, double src1, double *dst, size_t n);
 void fieldRcp_float(const float *src0, float src1, float *dst, size_t n);
 
+/* ------------------------------------------------------------ on-disk format */
+/* Replaces IO::CartesianWriteHDF / CartesianReadHDF (IO/CartesianHDF.h:46-215,
+ * src/IO/HDFDriver.cpp:26-148, IO/XDMFDriver.h:29-160): <fname>.h5 holds the
+ * dataset "data" of shape (Z, Y, X, ncomp) over the global index span of the
+ * grid's entity (cells, nodes, or the faces normal to face_dir), components
+ * interleaved per point (IO/FieldAOS.h), elements of file_dtype (CNV_F32 or
+ * CNV_F64, converted from or to the grid's dtype); <fname>.xmf is the XDMF
+ * wrapper (cell and node grids, when create_xdmf).  begin / end (global
+ * indices, NULL for the whole span) clip the written or read box, as the
+ * reference's sub-mesh argument does.  The reordering between block layout
+ * and file order runs on the GPU; the HDF5 C library is loaded at the first
+ * call (CNV_ERR_UNSUPPORTED without it; CNV_HDF5_LIBRARY names its path).
+ * Both calls return when the file, or the grid, is complete. */
+int cnv_grid_write_hdf(cnv_grid g, const char *fname, const char *aname, int file_dtype, double time, int face_dir,
+                       int create_xdmf, const int begin[3], const int end[3], cnv_stream s);
+int cnv_grid_read_hdf(cnv_grid g, const char *fname, int file_dtype, int face_dir, const int begin[3],
+                      const int end[3], cnv_stream s);
+
 /* ------------------------------------------------------------ batches */
 /* No counterpart in the reference (its applications loop over one grid's
  * blocks on the host, Laplacian.cpp:110-131).  Many independent FP32 grids

@@ -322,6 +322,37 @@ static void device_allocator_wrap()
     alloc.deallocate(p);
 }
 
+// CartesianHDFTest.cpp:221-252 through the C++ mirror: write, read back,
+// every block equal to its index
+static void hdf_read_back()
+{
+    Grid64 src({3, 3, 3}, {8, 8, 8}), dst({3, 3, 3}, {8, 8, 8});
+    for (size_t i = 0; i < src.size(); ++i)
+        for (Index z = 0; z < 8; ++z)
+            for (Index y = 0; y < 8; ++y)
+                for (Index x = 0; x < 8; ++x)
+                    src[i](x, y, z) = (double)i;
+    src.upload(0);
+    const std::string fn = "shim_framen";
+    IO::CartesianWriteHDF<double>(fn, "ramen", src, 0.0);
+    IO::CartesianReadHDF<double>(fn, dst);
+    dst.download(0);
+    for (size_t i = 0; i < dst.size(); ++i)
+        for (Index z = 0; z < 8; ++z)
+            for (Index y = 0; y < 8; ++y)
+                for (Index x = 0; x < 8; ++x)
+                    EXPECT(dst[i](x, y, z) == (double)i);
+    std::remove((fn + ".h5").c_str());
+    std::remove((fn + ".xmf").c_str());
+    bool threw = false;
+    try {
+        IO::CartesianReadHDF<double>("shim_no_such_file", dst);
+    } catch (const std::runtime_error &e) {
+        threw = std::string(e.what()).find("does not exist") != std::string::npos;
+    }
+    EXPECT(threw);
+}
+
 int main()
 {
     fieldlab_ghosts(false);
@@ -331,6 +362,7 @@ int main()
     laplacian_vs_host_loop();
     diffusion_run_vs_steps();
     device_allocator_wrap();
+    hdf_read_back();
     std::printf("%d checks, %d failed\n", g_checks, g_fail);
     return g_fail == 0 ? 0 : 1;
 }

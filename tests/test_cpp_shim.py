@@ -26,7 +26,7 @@ def test_shim_compiles(tmp_path):
 def test_shim_reference_unit_tests_on_gpu(gpu, tmp_path):
     exe = str(tmp_path / "test_shim")
     build(exe)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240, cwd=str(tmp_path))
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert " 0 failed" in r.stdout
