@@ -53,6 +53,52 @@ __device__ inline void store_vec(T *p, const VecT<T, N> &v)
     *reinterpret_cast<vt *>(p) = x;
 }
 
+// Raw buffer access: a buffer resource over [base, base + bytes) in scalar
+// registers, addressed by a 32-bit per-lane byte offset plus a 32-bit uniform
+// one, so no lane computes a 64-bit address.  Accesses past `bytes` read 0
+// and write nothing.  The resource's fourth word is the gfx9 data format
+// for raw (untyped) access.
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename T, int N>
+__device__ inline VecT<T, N> buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    static_assert(sizeof(T) * N == 16, "16-byte vectors");
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    typedef T vt __attribute__((ext_vector_type(N)));
+    const vt x = __builtin_bit_cast(vt, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+    VecT<T, N> o;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        o.v[i] = x[i];
+    return o;
+}
+
+template <typename T>
+__device__ inline T buf_load1(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
+template <typename T, int N>
+__device__ inline void buf_store(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const VecT<T, N> &v)
+{
+    static_assert(sizeof(T) * N == 16, "16-byte vectors");
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    typedef T vt __attribute__((ext_vector_type(N)));
+    vt x;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        x[i] = v.v[i];
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, x), r, (int)voff, (int)soff, 0);
+}
+
 // Streaming (non-temporal) vector load of exactly N elements, for data read
 // once per launch.
 template <typename T, int N>

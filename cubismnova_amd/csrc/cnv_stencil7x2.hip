@@ -111,8 +111,14 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     const int nbx = a.G.nb[0], nby = a.G.nb[1], nbz = a.G.nb[2];
     const int bx = (int)(blk % nbx), by = (int)((blk / nbx) % nby), bz = (int)(blk / ((long long)nbx * nby));
     const T *cbase = reinterpret_cast<const T *>(a.u) + (long long)comp * a.G.cstride;
-    T *ob = reinterpret_cast<T *>(a.out) + (long long)comp * a.G.cstride + blk * a.G.bstride;
+    T *obase = reinterpret_cast<T *>(a.out) + (long long)comp * a.G.cstride;
     const long long bstride = a.G.bstride;
+    // buffer resources over the input and output component slabs (fewer
+    // than 2^31 bytes, host-checked): scalar bases, 32-bit lane offsets
+    constexpr unsigned TS = sizeof(T);
+    const auto rin = buf_rsrc(cbase, (unsigned)(a.G.cstride * TS));
+    const auto rout = buf_rsrc(obase, (unsigned)(a.G.cstride * TS));
+    const unsigned oblk = (unsigned)(blk * bstride * TS); // this block in the output slab
 
     auto wrap = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
     const long long layer = (long long)nbx * nby * bstride; // elements per z-layer of blocks
@@ -126,7 +132,7 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     // (periodic neighbour blocks in x and y), the LDS offset, and the kind
     // (0 none, 1 scalar, 2 vector).  A plane p then only adds its layer and
     // plane offset.
-    long long hsrc[HS];
+    unsigned hsrc[HS]; // byte offset within the source plane's z-layer of blocks
     int hlds[HS], hkind[HS];
 #pragma unroll
     for (int h = 0; h < HS; ++h) {
@@ -137,21 +143,21 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
         if (s < NHX) {
             const int side = s / (2 * N), lyr = (s / N) & 1, y = s % N;
             const long long nb = (wrap(bx + (side ? 1 : -1), nbx) + (long long)nbx * by) * bstride;
-            hsrc[h] = nb + (side ? lyr : N - 1 - lyr) + N * y;
+            hsrc[h] = (unsigned)((nb + (side ? lyr : N - 1 - lyr) + N * y) * TS);
             hlds[h] = (y + 2) * PA + VEC + (side ? N + lyr : -1 - lyr);
             hkind[h] = 1;
         } else if (s < NHX + NHY) {
             const int s2 = s - NHX;
             const int side = s2 / (2 * TX), lyr = (s2 / TX) & 1, xs = (s2 % TX) * VEC;
             const long long nb = (bx + (long long)nbx * wrap(by + (side ? 1 : -1), nby)) * bstride;
-            hsrc[h] = nb + xs + N * (side ? lyr : N - 1 - lyr);
+            hsrc[h] = (unsigned)((nb + xs + N * (side ? lyr : N - 1 - lyr)) * TS);
             hlds[h] = ((side ? N + lyr : -1 - lyr) + 2) * PA + VEC + xs;
             hkind[h] = 2;
         } else if (s < NH) {
             const int d = s - NHX - NHY, sx = d & 1, sy = d >> 1;
             const long long nb =
                 (wrap(bx + (sx ? 1 : -1), nbx) + (long long)nbx * wrap(by + (sy ? 1 : -1), nby)) * bstride;
-            hsrc[h] = nb + (sx ? 0 : N - 1) + N * (sy ? 0 : N - 1);
+            hsrc[h] = (unsigned)((nb + (sx ? 0 : N - 1) + N * (sy ? 0 : N - 1)) * TS);
             hlds[h] = ((sy ? N : -1) + 2) * PA + VEC + (sx ? N : -1);
             hkind[h] = 1;
         }
@@ -166,17 +172,18 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     auto load_u0 = [&](int p) -> U0 {
         U0 Q; // scalar halo entries use element 0 only
         const int dz = p < 0 ? -1 : (p >= N ? 1 : 0);
-        const T *base = cbase + (long long)wrap(bz + dz, nbz) * layer + (long long)(p - dz * N) * NN;
-        const T *self = base + xyself + own;
+        // byte offset of plane p of this block row's z-layer (uniform)
+        const unsigned lb = (unsigned)(((long long)wrap(bz + dz, nbz) * layer + (long long)(p - dz * N) * NN) * TS);
+        const unsigned sb = lb + (unsigned)(xyself * TS);
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            Q.c[r] = *reinterpret_cast<const V *>(self + N * r * ROWS);
+            Q.c[r] = buf_load<T, VEC>(rin, (unsigned)((own + N * r * ROWS) * TS), sb);
 #pragma unroll
         for (int h = 0; h < HS; ++h) {
             if (hkind[h] == 2) {
-                Q.h[h] = *reinterpret_cast<const V *>(base + hsrc[h]);
+                Q.h[h] = buf_load<T, VEC>(rin, hsrc[h], lb);
             } else if (hkind[h] == 1) {
-                Q.h[h].v[0] = base[hsrc[h]];
+                Q.h[h].v[0] = buf_load1<T>(rin, hsrc[h], lb);
             }
         }
         return Q;
@@ -310,7 +317,7 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
         // u^{n+1} and u^{n+2} over the tile's cells
         if (q >= z0 + 1) {
             const T *Br = slotB(q - 1);
-            T *op = ob + NN * (q - 1);
+            const unsigned ops = oblk + (unsigned)(NN * (q - 1) * TS); // plane q - 1 of the block
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int y = ty + r * ROWS;
@@ -332,7 +339,7 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
                     acc1 = __builtin_fma((double)c1[r].v[v], (double)c1[r].v[v], acc1);
                     acc2 = __builtin_fma((double)res.v[v], (double)res.v[v], acc2);
                 }
-                *reinterpret_cast<V *>(op + x0 + N * y) = res;
+                buf_store<T, VEC>(rout, (unsigned)((x0 + N * y) * TS), ops, res);
             }
         }
     };
@@ -472,6 +479,9 @@ bool x2_applicable(cnv_grid_s *u)
 {
     const int *n = u->desc.block_cells;
     if (!(n[0] == n[1] && n[1] == n[2] && (n[0] == 16 || n[0] == 32)))
+        return false;
+    // the kernel addresses a component slab with 32-bit byte offsets
+    if (u->component_bytes >= ((size_t)1 << 31))
         return false;
     std::lock_guard<std::mutex> lk(u->mtx);
     for (int c = 0; c < u->desc.ncomp; ++c)
