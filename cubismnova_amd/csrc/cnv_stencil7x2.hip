@@ -67,11 +67,11 @@ template <typename T, int CTRL>
 __device__ inline T dpp_shift(T v)
 {
     if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+        return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
     } else {
         const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-        const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+        const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
         return __builtin_bit_cast(T, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
     }
 }
@@ -91,8 +91,8 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
     using V = VecT<T, VEC>;
     constexpr long long NN = (long long)N * N;
 
-    __shared__ alignas(16) T sA[4 * PLA];
-    __shared__ alignas(16) T sB[2 * PLB];
+    __shared__ alignas(16) T sA[3 * PLA];
+    __shared__ alignas(16) T sB[3 * PLB];
     __shared__ double red[WG / 64];
 
     const int tid = threadIdx.x;
@@ -221,30 +221,35 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
         const T l = lap * nudt;
         return c + l;
     };
-    // LDS: four u^n planes (q-1, q, q+1 read while plane q is evaluated, the
-    // next one being stored) and two u^{n+1} planes, so that one barrier per
-    // plane orders every write before its reads and every read before the
-    // write that reuses its slot
-    auto slotA = [&](int p) { return sA + ((p - z0 + 4) & 3) * PLA; }; // p >= z0 - 2
-    auto slotB = [&](int q) { return sB + ((q - z0 + 2) & 1) * PLB; }; // q >= z0 - 1
+    // LDS: three u^n planes (q and q + 1 read while plane q is evaluated,
+    // q + 2 stored at the next plane) and three u^{n+1} planes, in slots
+    // (p - z0 + 1) % 3; one barrier per plane orders every write before its
+    // reads and every read before the write that reuses its slot
+    auto slotA = [&](int k) { return sA + k * PLA; };
+    auto slotB = [&](int k) { return sB + k * PLB; };
 
     // Register rings of three planes, unrolled by three so that every slot
-    // index is a constant: U0r holds u^n on this thread's cells (plane p in
-    // slot (p - z0 + 2) % 3), U1r holds u^{n+1} (plane p in slot
-    // (p - z0 + 3) % 3); the pending registers hold the next u^n plane.
+    // index is a constant (plane p in slot (p - z0 + 2) % 3): U0r holds u^n on
+    // this thread's cells, U1r u^{n+1}, rz u^n on this thread's ring cells;
+    // Ph the halo of the next u^n plane.
     V U0r[3][R], U1r[3][R];
+    T rz[3][RS];
+    V Ph[HS];
     {
         const U0 Q0 = load_u0(z0 - 2);
-        store_u0(Q0, slotA(z0 - 2));
         const U0 Q1 = load_u0(z0 - 1);
-        store_u0(Q1, slotA(z0 - 1));
+        store_u0(Q1, slotA(0));
+        const U0 Q2 = load_u0(z0);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             U0r[0][r] = Q0.c[r];
             U0r[1][r] = Q1.c[r];
+            U0r[2][r] = Q2.c[r];
         }
+#pragma unroll
+        for (int h = 0; h < HS; ++h)
+            Ph[h] = Q2.h[h];
     }
-    U0 Pq = load_u0(z0);
 
     double acc1 = 0.0, acc2 = 0.0;
     // plane q of the tile, register slots in the static rotation u = (q - z0 + 1) % 3
@@ -256,16 +261,20 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
         V(&m1)[R] = U1r[u % 3];
         V(&c1)[R] = U1r[(u + 1) % 3];
         V(&n1)[R] = U1r[(u + 2) % 3];
-        store_u0(Pq, slotA(q + 1));
+        {
+            U0 P;
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            c0p[r] = Pq.c[r];
-        if (q + 2 <= z1 + 1)
-            Pq = load_u0(q + 2);
+            for (int r = 0; r < R; ++r)
+                P.c[r] = c0p[r];
+#pragma unroll
+            for (int h = 0; h < HS; ++h)
+                P.h[h] = Ph[h];
+            store_u0(P, slotA((u + 1) % 3)); // plane q + 1
+        }
         __syncthreads();
-        const T *Am = slotA(q - 1), *Ac = slotA(q), *Ap = slotA(q + 1);
+        const T *Ac = slotA(u), *Ap = slotA((u + 1) % 3);
         const bool inner = q >= z0 && q < z1;
-        T *Bq = slotB(q);
+        T *Bq = slotB(u);
         // u^{n+1} on the block's cells of plane q (z neighbours from registers)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -292,8 +301,19 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
             if (inner)
                 *reinterpret_cast<V *>(Bq + (y + 1) * PA + VEC + x0) = res;
         }
-        // u^{n+1} on the ring of face-neighbour cells (planes the second step computes)
-        if (inner) {
+        // plane q - 1 has served: the next u^n plane goes into its slot
+        if (q + 2 <= z1 + 1) {
+            const U0 Q = load_u0(q + 2);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                c0m[r] = Q.c[r];
+#pragma unroll
+            for (int h = 0; h < HS; ++h)
+                Ph[h] = Q.h[h];
+        }
+        // u^{n+1} on the ring of face-neighbour cells (planes the second step
+        // computes); their z neighbours come from the register ring rz
+        if (q < z1) {
 #pragma unroll
             for (int h = 0; h < RS; ++h) {
                 const int s = tid + h * WG;
@@ -307,8 +327,12 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
                         x = (s - 2 * N) % N;
                     }
                     const int o = (y + 2) * PA + VEC + x;
-                    Bq[(y + 1) * PA + VEC + x] =
-                        euler(Ac[o], Ac[o - 1], Ac[o + 1], Ac[o - PA], Ac[o + PA], Am[o], Ap[o]);
+                    rz[(u + 2) % 3][h] = Ap[o];
+                    if (q == z0 - 1)
+                        rz[(u + 1) % 3][h] = Ac[o];
+                    if (inner)
+                        Bq[(y + 1) * PA + VEC + x] = euler(rz[(u + 1) % 3][h], Ac[o - 1], Ac[o + 1], Ac[o - PA],
+                                                           Ac[o + PA], rz[u % 3][h], rz[(u + 2) % 3][h]);
                 }
             }
         }
@@ -316,7 +340,7 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
         // completed before this plane's barrier); the sums of squares of
         // u^{n+1} and u^{n+2} over the tile's cells
         if (q >= z0 + 1) {
-            const T *Br = slotB(q - 1);
+            const T *Br = slotB((u + 2) % 3);
             const unsigned ops = oblk + (unsigned)(NN * (q - 1) * TS); // plane q - 1 of the block
 #pragma unroll
             for (int r = 0; r < R; ++r) {
