@@ -599,7 +599,7 @@ def main():
     ap.add_argument("--tile-f64", type=tile, default=None, help="WG,ZCHUNK for the FP64 kernel of the diffusion or WENO5 workload (skips autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (got --gpus %d)" % args.gpus,
+        print("bench.py runs on one GPU only: the library has no multi-GPU path (got --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     if args.steps < 1:

@@ -2,8 +2,8 @@
 
 The oracle restates CubismNova's lab assembly, BCs and stencil operators on
 the CPU (oracle/cnv_oracle.cpp).  Only tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg use it, and only as the checker or as the timed
-CPU baseline; the product never imports it.
+bench.py's cpu_baseline leg use it, and only to check results or as the
+timed CPU baseline; the product never imports it.
 
 BCs are plain tuples (kind, dir, side, value) with kind 0 Base, 1 Absorbing,
 2 Dirichlet, 3 Symmetry.  Slabs are 1-D numpy arrays in the grid's block

@@ -3,7 +3,7 @@
 // CPU restatement of CubismNova's block-structured stencil path, written from
 // the reference's algorithm so that the HIP kernels can be checked against it.
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
-// load this library, and only as the checker or as the timed CPU baseline.
+// load this library, and only to check results or as the timed CPU baseline.
 // The product library (libcubismnova_amd.so) never links or calls it.
 //
 // Functions and the reference code they restate (paths relative to the
