@@ -98,9 +98,9 @@ __global__ __launch_bounds__(WG) void k_batch(const BGrid *__restrict__ grids, c
             const long long o = x + sx * y + sxy * z;
             const float cc = lds[p];
             if (Gd.op == 0) {
-                const float ddx = Gd.ih2[0] * (lds[p + 1] - 2 * cc + lds[p - 1]);
-                const float ddy = Gd.ih2[1] * (lds[p + Lx] - 2 * cc + lds[p - Lx]);
-                const float ddz = Gd.ih2[2] * (lds[p + LXY] - 2 * cc + lds[p - LXY]);
+                const float ddx = Gd.ih2[0] * (fms2(cc, lds[p + 1]) + lds[p - 1]);
+                const float ddy = Gd.ih2[1] * (fms2(cc, lds[p + Lx]) + lds[p - Lx]);
+                const float ddz = Gd.ih2[2] * (fms2(cc, lds[p + LXY]) + lds[p - LXY]);
                 const float lap = ddx + ddy + ddz;
                 const float lt = lap * Gd.nudt;
                 ob[(long long)c * Gd.ocstride + o] = cc + lt;

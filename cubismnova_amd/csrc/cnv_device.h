@@ -11,6 +11,18 @@ struct alignas(16) VecT {
     T v[N];
 };
 
+// p - 2c with one rounding: 2c is exact (a power-of-two scaling, finite for
+// |c| below half the largest value), so the fused form rounds exactly like
+// the subtraction p - (2c) of the 7-point operator
+template <typename T>
+__device__ inline T fms2(T c, T p)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_fmaf(-2.f, c, p);
+    else
+        return __builtin_fma(-2.0, c, p);
+}
+
 // Wave priority from 3 at the start of a workgroup's z loop down to 0 at
 // its end (done = planes finished, total = planes of the tile): on a compute
 // unit the waves that lag behind are issued first, so the workgroups that

@@ -173,8 +173,8 @@ __global__ __launch_bounds__(WG) void k_jacobi(const JArgs a)
                 const T c = pc.c[r].v[v];
                 const T xm = v == 0 ? xl : pc.c[r].v[v - 1];
                 const T xp = v == VEC - 1 ? xr : pc.c[r].v[v + 1];
-                const T ax = ihx2 * (xp - 2 * c + xm);
-                const T ay = ihy2 * (yp.v[v] - 2 * c + ym.v[v]);
+                const T ax = ihx2 * (fms2(c, xp) + xm);
+                const T ay = ihy2 * (fms2(c, yp.v[v]) + ym.v[v]);
                 const T az = cp * (pn.c[r].v[v] - c) - cm * (c - pm.c[r].v[v]);
                 const T au = ax + ay + az;
                 const T rr = (T)fr[r].v[v] - au;

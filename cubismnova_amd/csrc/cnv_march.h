@@ -155,9 +155,10 @@ __device__ inline double s7_march(const T *__restrict__ ub, T *__restrict__ ob, 
                     const T c = pc.c[r].v[v];
                     const T xm = v == 0 ? xl : pc.c[r].v[v - 1];
                     const T xp = v == VEC - 1 ? xr : pc.c[r].v[v + 1];
-                    const T ddx = ihx2 * (xp - 2 * c + xm);
-                    const T ddy = ihy2 * (yp.v[v] - 2 * c + ym.v[v]);
-                    const T ddz = ihz2 * (pn.c[r].v[v] - 2 * c + pm.c[r].v[v]);
+                    // p - 2c as one fma: 2c is exact, so it rounds like the subtraction
+                    const T ddx = ihx2 * (fms2(c, xp) + xm);
+                    const T ddy = ihy2 * (fms2(c, yp.v[v]) + ym.v[v]);
+                    const T ddz = ihz2 * (fms2(c, pn.c[r].v[v]) + pm.c[r].v[v]);
                     const T lap = ddx + ddy + ddz;
                     lapv.v[v] = lap;
                     if (MODE == 0) {

@@ -179,9 +179,9 @@ __global__ __launch_bounds__(256) void k_stencil7_generic(const S7Args a)
         };
         const T ihx2 = (T)a.ih2[0], ihy2 = (T)a.ih2[1], ihz2 = (T)a.ih2[2];
         const T c = ub[i];
-        const T ddx = ihx2 * (at(x + 1, y, z) - 2 * c + at(x - 1, y, z));
-        const T ddy = ihy2 * (at(x, y + 1, z) - 2 * c + at(x, y - 1, z));
-        const T ddz = ihz2 * (at(x, y, z + 1) - 2 * c + at(x, y, z - 1));
+        const T ddx = ihx2 * (fms2(c, at(x + 1, y, z)) + at(x - 1, y, z));
+        const T ddy = ihy2 * (fms2(c, at(x, y + 1, z)) + at(x, y - 1, z));
+        const T ddz = ihz2 * (fms2(c, at(x, y, z + 1)) + at(x, y, z - 1));
         const T lap = ddx + ddy + ddz;
         if (MODE == 0) {
             ob[i] = lap;

@@ -204,19 +204,11 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
 
     const T ihx2 = (T)a.ih2[0], ihy2 = (T)a.ih2[1], ihz2 = (T)a.ih2[2];
     const T nudt = (T)a.nudt;
-    // one Euler step of one cell, in the order of the one-step kernel;
-    // xp - 2c as fma(-2, c, xp): 2c is exact, so the single rounding of the
-    // fused form is the rounding of the subtraction
-    auto fms2 = [](T c, T p) -> T {
-        if constexpr (sizeof(T) == 4)
-            return __builtin_fmaf(-2.f, c, p);
-        else
-            return __builtin_fma(-2.0, c, p);
-    };
+    // one Euler step of one cell, in the order of the one-step kernel
     auto euler = [&](T c, T xm, T xp, T ym, T yp, T zm, T zp) -> T {
-        const T ddx = ihx2 * (fms2(c, xp) + xm);
-        const T ddy = ihy2 * (fms2(c, yp) + ym);
-        const T ddz = ihz2 * (fms2(c, zp) + zm);
+        const T ddx = ihx2 * (fms2<T>(c, xp) + xm);
+        const T ddy = ihy2 * (fms2<T>(c, yp) + ym);
+        const T ddz = ihz2 * (fms2<T>(c, zp) + zm);
         const T lap = ddx + ddy + ddz;
         const T l = lap * nudt;
         return c + l;
