@@ -581,6 +581,26 @@ def bench_copy(args):
             "roofline": roofline(2 * nbytes, avg)}
 
 
+def copy_ceiling():
+    """Practical streaming ceiling of this GPU: a 1 GiB device-to-device copy
+    (past the last-level cache), GB/s of bytes read + written."""
+    import cubismnova_amd as cnv
+    from cubismnova_amd import _abi
+    ev = HipEvents()
+    nbytes = 1 << 30
+    a, b = cnv.DeviceBuffer(nbytes), cnv.DeviceBuffer(nbytes)
+    st = cnv.Stream()
+    for _ in range(2):
+        _abi.call("cnv_memcpy_d2d", b.ptr, a.ptr, nbytes, st.handle)
+    st.synchronize()
+    e0, e1 = ev.new(), ev.new()
+    ev.record(e0, st)
+    for _ in range(5):
+        _abi.call("cnv_memcpy_d2d", b.ptr, a.ptr, nbytes, st.handle)
+    ev.record(e1, st)
+    return 2 * nbytes / (ev.elapsed_ms(e0, e1) / 1e3 / 5) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -606,6 +626,10 @@ def main():
         sys.exit("--steps must be >= 1")
     out = {"weno5": bench_weno5, "batch": bench_batch, "jacobi": bench_jacobi,
            "copy": bench_copy}.get(args.workload, bench_diffusion)(args)
+    if args.workload != "copy":
+        c = copy_ceiling()
+        out["hbm_ceiling"] = {"copy_1GiB_GB_s": round(c, 1), "frac_of_peak": round(c / HBM_PEAK_GBS, 4),
+                              "kernel_frac_of_copy": round(out["roofline"]["achieved"] / c, 4)}
     print(json.dumps(out), flush=True)
 
 
