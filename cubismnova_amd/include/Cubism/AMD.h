@@ -513,6 +513,33 @@ std::vector<double> diffusionRun(Grid::Cartesian<T> &u, Grid::Cartesian<T> &tmp,
     check(cnv_memcpy_d2h(norms.data(), dev, sizeof(double) * norms.size(), nullptr), "cnv_memcpy_d2h");
     return norms;
 }
+// WENO5 face states of every component of `in` into `out` (6 per component:
+// u- and u+ in x, y, z)
+template <typename T>
+void weno5(Grid::Cartesian<T> &in, Grid::Cartesian<T> &out)
+{
+    for (size_t c = 0; c < in.getNumComponents(); ++c)
+        in.syncBCs(c);
+    check(cnv_weno5(in.handle(), out.handle(), nullptr), "cnv_weno5");
+}
+
+// Field / FieldContainer arithmetic on whole grids (Block/Field.h:444-556,
+// 1039-1154): dst = a OP b for every block (and every component when
+// comp = -1); op one of CNV_FIELD_ADD, _SUB, _MUL, _DIV
+template <typename T>
+void fieldOp(const int op, Grid::Cartesian<T> &dst, const Grid::Cartesian<T> &a, const Grid::Cartesian<T> &b,
+             const int comp = -1)
+{
+    check(cnv_grid_field_op(op, dst.handle(), comp, a.handle(), comp, b.handle(), comp, 0.0, nullptr),
+          "cnv_grid_field_op");
+}
+// dst = a OP s; CNV_FIELD_RCP: dst = s * (1 / a); CNV_FIELD_NEG: dst = -a
+template <typename T>
+void fieldOp(const int op, Grid::Cartesian<T> &dst, const Grid::Cartesian<T> &a, const T s, const int comp = -1)
+{
+    check(cnv_grid_field_op(op, dst.handle(), comp, a.handle(), comp, nullptr, comp, (double)s, nullptr),
+          "cnv_grid_field_op");
+}
 } // namespace Device
 
 // ------------------------------------------------------------ on-disk format

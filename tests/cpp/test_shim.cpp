@@ -353,6 +353,38 @@ static void hdf_read_back()
     EXPECT(threw);
 }
 
+// Device::fieldOp against the host Field arithmetic (Field.h:457-556)
+static void device_field_ops()
+{
+    Grid64 a({2, 1, 2}, {8, 8, 8}, {0, 0, 0}, {1, 1, 1}, 2), b({2, 1, 2}, {8, 8, 8}, {0, 0, 0}, {1, 1, 1}, 2),
+        d({2, 1, 2}, {8, 8, 8}, {0, 0, 0}, {1, 1, 1}, 2);
+    for (size_t c = 0; c < 2; ++c)
+        for (size_t i = 0; i < a.size(); ++i)
+            for (Index z = 0; z < 8; ++z)
+                for (Index y = 0; y < 8; ++y)
+                    for (Index x = 0; x < 8; ++x) {
+                        a.field(c, i)(x, y, z) = std::sin(0.3 * x + y - 0.7 * z + (double)(i + 3 * c));
+                        b.field(c, i)(x, y, z) = 1.5 + std::cos(0.1 * x * y + (double)c);
+                    }
+    for (size_t c = 0; c < 2; ++c) {
+        a.upload(c);
+        b.upload(c);
+    }
+    Device::fieldOp(CNV_FIELD_DIV, d, a, b);
+    for (size_t c = 0; c < 2; ++c)
+        d.download(c);
+    for (size_t c = 0; c < 2; ++c)
+        for (size_t i = 0; i < a.size(); ++i)
+            for (Index z = 0; z < 8; ++z)
+                for (Index y = 0; y < 8; ++y)
+                    for (Index x = 0; x < 8; ++x)
+                        EXPECT(d.field(c, i)(x, y, z) == a.field(c, i)(x, y, z) / b.field(c, i)(x, y, z));
+    Device::fieldOp(CNV_FIELD_RCP, d, b, 2.0, 1);
+    d.download(1);
+    for (size_t i = 0; i < a.size(); ++i)
+        EXPECT(d.field(1, i)(3, 4, 5) == 2.0 * (1.0 / b.field(1, i)(3, 4, 5)));
+}
+
 int main()
 {
     fieldlab_ghosts(false);
@@ -363,6 +395,7 @@ int main()
     diffusion_run_vs_steps();
     device_allocator_wrap();
     hdf_read_back();
+    device_field_ops();
     std::printf("%d checks, %d failed\n", g_checks, g_fail);
     return g_fail == 0 ? 0 : 1;
 }
