@@ -222,51 +222,56 @@ __device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 // FP32 evaluation in difference form.  With the stencil (a, b, c, d, e) =
 // u_{i-2..i+2}: first differences dm2 = b - a, dm1 = c - b, d0 = d - c,
 // dp1 = e - d; second differences tm = dm1 - dm2, t0 = d0 - dm1,
-// tp = dp1 - d0 and Tk = (65/12) tk^2; S = sum of (u_j^2 + 1e-30) over the
-// stencil, so that beta_k / m2 = 5 beta_k / S with
-// 5 beta_k = Tk + 1.25 sk^2, s0 = a - 4b + 3c = tm + 2 dm1,
-// s1 = d - b = dm1 + d0, s2 = 3c - 4d + e = tp - 2 d0.  The candidates enter
-// as offsets from c times their linear weight:
-//   u- = c + sum_k p_k Dk / sum_k d_k p_k,  D = (0.1 (5 dm1 - 2 dm2), 0.6 (dm1 + 2 d0), 0.3 (4 d0 - dp1)) / 6
-//   u+ = c + sum_k p_k Ek / sum_k d'_k p_k, E = (0.3 (dm2 - 4 dm1), 0.6 (-2 dm1 - d0), 0.1 (2 dp1 - 5 d0)) / 6
-// with p_k the product of the other two squared scaled weights.  Cells of
-// one row share the differences.  Every operation is explicit (fmaf or one
-// rounding), so any caller that forms the same differences gets the same
-// bits.  Scaled weights stay within [1e-6, 170] and their products within
+// tp = dp1 - d0 and Tk = (13/3) tk^2; S = sum of (u_j^2 + 1e-30) over the
+// stencil, so that 0.8 beta_k / m2 = 4 beta_k / S with
+// 4 beta_k = Tk + sk^2, s0 = a - 4b + 3c = 3 dm1 - dm2, s1 = d - b (sign
+// free) = dm1 + d0, s2 = 3c - 4d + e = dp1 - 3 d0.  The nonlinear weights
+// depend on the ratios of the bk = 4 beta_k / S + 0.8e-6 only, so this
+// common factor 0.8 of the indicators and the epsilon changes nothing.
+// The candidates enter as offsets from c; with p_k the product of the other
+// two squared indicators and the linear weights (1, 6, 3) / 10:
+//   u- = c + ((p12/6) C0 + p02 C1 + (p01/2) C2) / (p12 + 6 p02 + 3 p01)
+//        C0 = 2 s0 - dm1, C1 = dm1 + 2 d0, C2 = 4 d0 - dp1
+//   u+ = c + ((p12/2) E0 + p02 E1 + (p01/6) E2) / (3 p12 + 6 p02 + p01)
+//        E0 = dm2 - 4 dm1, E1 = -2 dm1 - d0, E2 = 2 s2 + d0
+// (6 (q_k - c) of Jiang & Shu's candidates q_k).  Cells of one row share
+// the differences.  Every operation is explicit (fmaf or one rounding), so
+// any caller that forms the same differences gets the same bits.  The
+// indicators stay within [0.8e-6, 136] and their products within
 // [1e-25, 1e10], so nothing over- or underflows at any data magnitude.
-constexpr float kW5T = 65.f / 12.f; // 5 * 13/12
+constexpr float kW5T = 13.f / 3.f;
 
 __device__ inline float w5_sqr(float u) { return __builtin_fmaf(u, u, 1e-30f); }
 __device__ inline float w5_T(float t) { return (kW5T * t) * t; }
 // S of a stencil from its five squares, in one fixed association
 __device__ inline float w5_S(float sa, float sb, float sc, float sd, float se) { return ((sa + sb) + (sc + sd)) + se; }
 
-__device__ inline void weno5_core(float c, float dm2, float dm1, float d0, float dp1, float tm, float t0, float tp,
-                                  float Tm, float T0, float Tp, float S, float &um, float &up)
+__device__ inline void weno5_core(float c, float dm2, float dm1, float d0, float dp1, float Tm, float T0, float Tp,
+                                  float S, float &um, float &up)
 {
     const float r = rcpf(S);
-    const float s0 = __builtin_fmaf(2.f, dm1, tm);
+    const float s0 = __builtin_fmaf(3.f, dm1, -dm2);
     const float s1 = dm1 + d0;
-    const float s2 = __builtin_fmaf(-2.f, d0, tp);
-    const float b0 = __builtin_fmaf(__builtin_fmaf(1.25f * s0, s0, Tm), r, 1e-6f);
-    const float b1 = __builtin_fmaf(__builtin_fmaf(1.25f * s1, s1, T0), r, 1e-6f);
-    const float b2 = __builtin_fmaf(__builtin_fmaf(1.25f * s2, s2, Tp), r, 1e-6f);
+    const float s2 = __builtin_fmaf(-3.f, d0, dp1);
+    const float b0 = __builtin_fmaf(__builtin_fmaf(s0, s0, Tm), r, 0.8e-6f);
+    const float b1 = __builtin_fmaf(__builtin_fmaf(s1, s1, T0), r, 0.8e-6f);
+    const float b2 = __builtin_fmaf(__builtin_fmaf(s2, s2, Tp), r, 0.8e-6f);
     const float g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
     const float p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
     {
-        const float D0 = __builtin_fmaf(5.f / 60.f, dm1, (-2.f / 60.f) * dm2);
-        const float D1 = __builtin_fmaf(0.1f, dm1, 0.2f * d0);
-        const float D2 = __builtin_fmaf(0.2f, d0, -0.05f * dp1);
-        const float num = __builtin_fmaf(p12, D0, __builtin_fmaf(p02, D1, p01 * D2));
-        const float den = __builtin_fmaf(0.1f, p12, __builtin_fmaf(0.6f, p02, 0.3f * p01));
+        const float C0 = __builtin_fmaf(2.f, s0, -dm1);
+        const float C1 = __builtin_fmaf(2.f, d0, dm1);
+        const float C2 = __builtin_fmaf(4.f, d0, -dp1);
+        const float num = __builtin_fmaf(p12 * (1.f / 6.f), C0, __builtin_fmaf(p02, C1, (0.5f * p01) * C2));
+        const float den = __builtin_fmaf(6.f, p02, __builtin_fmaf(3.f, p01, p12));
         um = __builtin_fmaf(num, rcpf(den), c);
     }
     {
-        const float E0 = __builtin_fmaf(0.05f, dm2, -0.2f * dm1);
-        const float E1 = __builtin_fmaf(-0.2f, dm1, -0.1f * d0);
-        const float E2 = __builtin_fmaf(2.f / 60.f, dp1, (-5.f / 60.f) * d0);
-        const float num = __builtin_fmaf(p12, E0, __builtin_fmaf(p02, E1, p01 * E2));
-        const float den = __builtin_fmaf(0.3f, p12, __builtin_fmaf(0.6f, p02, 0.1f * p01));
+        const float E0 = __builtin_fmaf(-4.f, dm1, dm2);
+        const float E1 = __builtin_fmaf(-2.f, dm1, -d0);
+        const float E2 = __builtin_fmaf(2.f, s2, d0);
+        const float num = __builtin_fmaf(0.5f * p12, E0, __builtin_fmaf(p02, E1, (p01 * (1.f / 6.f)) * E2));
+        const float den = __builtin_fmaf(6.f, p02, __builtin_fmaf(3.f, p12, p01));
         up = __builtin_fmaf(num, rcpf(den), c);
     }
 }
@@ -277,7 +282,7 @@ __device__ inline void weno5_sq(float a, float b, float c, float d, float e, flo
 {
     const float dm2 = b - a, dm1 = c - b, d0 = d - c, dp1 = e - d;
     const float tm = dm1 - dm2, t0 = d0 - dm1, tp = dp1 - d0;
-    weno5_core(c, dm2, dm1, d0, dp1, tm, t0, tp, w5_T(tm), w5_T(t0), w5_T(tp), w5_S(sa, sb, sc, sd, se), um, up);
+    weno5_core(c, dm2, dm1, d0, dp1, w5_T(tm), w5_T(t0), w5_T(tp), w5_S(sa, sb, sc, sd, se), um, up);
 }
 
 __device__ inline void weno5_fast(float a, float b, float c, float d, float e, float &um, float &up)
@@ -465,8 +470,8 @@ __device__ inline void w5_march(const T *__restrict__ ub, T *__restrict__ ob, lo
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
                     const int i = v + 2;
-                    weno5_core(xs[i], dx[i - 2], dx[i - 1], dx[i], dx[i + 1], tx[i - 1], tx[i], tx[i + 1], Tx[i - 1],
-                               Tx[i], Tx[i + 1], (pp[i - 2] + pp[i]) + sq[i + 2], o[0].v[v], o[1].v[v]);
+                    weno5_core(xs[i], dx[i - 2], dx[i - 1], dx[i], dx[i + 1], Tx[i - 1], Tx[i], Tx[i + 1],
+                               (pp[i - 2] + pp[i]) + sq[i + 2], o[0].v[v], o[1].v[v]);
                     const T c = q2.c[r].v[v], sc = sq[i];
                     weno5_sq(ym2.v[v], ym1.v[v], c, yp1.v[v], yp2.v[v], w5_sqr(ym2.v[v]), w5_sqr(ym1.v[v]), sc,
                              w5_sqr(yp1.v[v]), w5_sqr(yp2.v[v]), o[2].v[v], o[3].v[v]);
