@@ -107,13 +107,13 @@ __global__ __launch_bounds__(WG) void k_batch(const BGrid *__restrict__ grids, c
             } else {
                 float um, up;
                 float *oc = ob + (long long)(6 * c) * Gd.ocstride + o;
-                weno5_fast(lds[p - 2], lds[p - 1], cc, lds[p + 1], lds[p + 2], um, up);
+                weno5<float>(lds[p - 2], lds[p - 1], cc, lds[p + 1], lds[p + 2], um, up);
                 oc[0] = um;
                 oc[Gd.ocstride] = up;
-                weno5_fast(lds[p - 2 * Lx], lds[p - Lx], cc, lds[p + Lx], lds[p + 2 * Lx], um, up);
+                weno5<float>(lds[p - 2 * Lx], lds[p - Lx], cc, lds[p + Lx], lds[p + 2 * Lx], um, up);
                 oc[2 * Gd.ocstride] = um;
                 oc[3 * Gd.ocstride] = up;
-                weno5_fast(lds[p - 2 * LXY], lds[p - LXY], cc, lds[p + LXY], lds[p + 2 * LXY], um, up);
+                weno5<float>(lds[p - 2 * LXY], lds[p - LXY], cc, lds[p + LXY], lds[p + 2 * LXY], um, up);
                 oc[4 * Gd.ocstride] = um;
                 oc[5 * Gd.ocstride] = up;
             }

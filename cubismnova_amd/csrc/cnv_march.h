@@ -182,121 +182,90 @@ __device__ inline double s7_march(const T *__restrict__ ub, T *__restrict__ ob, 
     return acc;
 }
 
-// IEEE evaluation in the operation order of the CPU restatement.  The
-// weights alpha_k = d_k / (eps' + beta_k / m2)^2 are scaled by the common
-// factor prod_j (eps' + beta_j / m2)^2, so only 1/m2 and the two
-// normalisations divide; the 1/6 of the candidate polynomials goes into the
-// normalisation.
-template <typename R>
-__device__ inline void weno5_exact(R a, R b, R c, R d, R e, R &um, R &up)
-{
-    const R m2 = (a * a + b * b + c * c + d * d + e * e) * (R)0.2 + (R)1e-30;
-    const R im2 = (R)1 / m2;
-    const R t0 = a - 2 * b + c, s0 = a - 4 * b + 3 * c;
-    const R t1 = b - 2 * c + d, s1 = b - d;
-    const R t2 = c - 2 * d + e, s2 = 3 * c - 4 * d + e;
-    const R k13 = (R)13 / (R)12;
-    const R b0 = (k13 * t0 * t0 + (R)0.25 * s0 * s0) * im2 + (R)1e-6;
-    const R b1 = (k13 * t1 * t1 + (R)0.25 * s1 * s1) * im2 + (R)1e-6;
-    const R b2 = (k13 * t2 * t2 + (R)0.25 * s2 * s2) * im2 + (R)1e-6;
-    const R g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
-    const R p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
-    {
-        const R w0 = (R)0.1 * p12, w1 = (R)0.6 * p02, w2 = (R)0.3 * p01;
-        const R q0 = 2 * a - 7 * b + 11 * c;
-        const R q1 = -b + 5 * c + 2 * d;
-        const R q2 = 2 * c + 5 * d - e;
-        um = (w0 * q0 + w1 * q1 + w2 * q2) / ((R)6 * (w0 + w1 + w2));
-    }
-    {
-        const R w0 = (R)0.3 * p12, w1 = (R)0.6 * p02, w2 = (R)0.1 * p01;
-        const R q0 = -a + 5 * b + 2 * c;
-        const R q1 = 2 * b + 5 * c - d;
-        const R q2 = 11 * c - 7 * d + 2 * e;
-        up = (w0 * q0 + w1 * q1 + w2 * q2) / ((R)6 * (w0 + w1 + w2));
-    }
-}
-
 __device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// FP32 evaluation in difference form.  With the stencil (a, b, c, d, e) =
-// u_{i-2..i+2}: first differences dm2 = b - a, dm1 = c - b, d0 = d - c,
-// dp1 = e - d; second differences tm = dm1 - dm2, t0 = d0 - dm1,
-// tp = dp1 - d0 and Tk = (13/3) tk^2; S = sum of (u_j^2 + 1e-30) over the
-// stencil, so that 0.8 beta_k / m2 = 4 beta_k / S with
-// 4 beta_k = Tk + sk^2, s0 = a - 4b + 3c = 3 dm1 - dm2, s1 = d - b (sign
-// free) = dm1 + d0, s2 = 3c - 4d + e = dp1 - 3 d0.  The nonlinear weights
-// depend on the ratios of the bk = 4 beta_k / S + 0.8e-6 only, so this
-// common factor 0.8 of the indicators and the epsilon changes nothing.
+__device__ inline float w5_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double w5_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// 1/S: the hardware reciprocal in FP32, IEEE division in FP64
+__device__ inline float w5_inv(float x) { return rcpf(x); }
+__device__ inline double w5_inv(double x) { return 1.0 / x; }
+// c + num / den: one multiply-add with the hardware reciprocal in FP32; an
+// IEEE quotient and sum in FP64
+__device__ inline float w5_fin(float num, float den, float c) { return __builtin_fmaf(num, rcpf(den), c); }
+__device__ inline double w5_fin(double num, double den, double c) { return c + num / den; }
+
+// WENO5 (Jiang & Shu weights) in difference form.  With the stencil
+// (a, b, c, d, e) = u_{i-2..i+2}: first differences dm2 = b - a,
+// dm1 = c - b, d0 = d - c, dp1 = e - d; second differences tm = dm1 - dm2,
+// t0 = d0 - dm1, tp = dp1 - d0 and Tk = (13/3) tk^2; S = sum of
+// (u_j^2 + 1e-30) over the stencil, so that 0.8 beta_k / m2 = 4 beta_k / S
+// with 4 beta_k = Tk + sk^2, s0 = a - 4b + 3c = 3 dm1 - dm2, s1 = d - b
+// (sign free) = dm1 + d0, s2 = 3c - 4d + e = dp1 - 3 d0.  The nonlinear
+// weights depend on the ratios of the bk = 4 beta_k / S + 0.8e-6 only, so
+// this common factor 0.8 of the indicators and the epsilon changes nothing.
 // The candidates enter as offsets from c; with p_k the product of the other
 // two squared indicators and the linear weights (1, 6, 3) / 10:
 //   u- = c + ((p12/6) C0 + p02 C1 + (p01/2) C2) / (p12 + 6 p02 + 3 p01)
 //        C0 = 2 s0 - dm1, C1 = dm1 + 2 d0, C2 = 4 d0 - dp1
 //   u+ = c + ((p12/2) E0 + p02 E1 + (p01/6) E2) / (3 p12 + 6 p02 + p01)
 //        E0 = dm2 - 4 dm1, E1 = -2 dm1 - d0, E2 = 2 s2 + d0
-// (6 (q_k - c) of Jiang & Shu's candidates q_k).  Cells of one row share
-// the differences.  Every operation is explicit (fmaf or one rounding), so
-// any caller that forms the same differences gets the same bits.  The
-// indicators stay within [0.8e-6, 136] and their products within
+// (6 (q_k - c) of Jiang & Shu's candidates q_k).  Every operation is
+// explicit (one fma or one rounding), so any caller that forms the same
+// differences gets the same bits: cells of one row share the differences,
+// and the FP64 evaluation is the CPU restatement's operation for operation.
+// The indicators stay within [0.8e-6, 136] and their products within
 // [1e-25, 1e10], so nothing over- or underflows at any data magnitude.
-constexpr float kW5T = 13.f / 3.f;
-
-__device__ inline float w5_sqr(float u) { return __builtin_fmaf(u, u, 1e-30f); }
-__device__ inline float w5_T(float t) { return (kW5T * t) * t; }
+template <typename R>
+__device__ inline R w5_sqr(R u) { return w5_fma(u, u, (R)1e-30); }
+template <typename R>
+__device__ inline R w5_T(R t) { return (((R)13 / (R)3) * t) * t; }
 // S of a stencil from its five squares, in one fixed association
-__device__ inline float w5_S(float sa, float sb, float sc, float sd, float se) { return ((sa + sb) + (sc + sd)) + se; }
+template <typename R>
+__device__ inline R w5_S(R sa, R sb, R sc, R sd, R se) { return ((sa + sb) + (sc + sd)) + se; }
 
-__device__ inline void weno5_core(float c, float dm2, float dm1, float d0, float dp1, float Tm, float T0, float Tp,
-                                  float S, float &um, float &up)
+template <typename R>
+__device__ inline void weno5_core(R c, R dm2, R dm1, R d0, R dp1, R Tm, R T0, R Tp, R S, R &um, R &up)
 {
-    const float r = rcpf(S);
-    const float s0 = __builtin_fmaf(3.f, dm1, -dm2);
-    const float s1 = dm1 + d0;
-    const float s2 = __builtin_fmaf(-3.f, d0, dp1);
-    const float b0 = __builtin_fmaf(__builtin_fmaf(s0, s0, Tm), r, 0.8e-6f);
-    const float b1 = __builtin_fmaf(__builtin_fmaf(s1, s1, T0), r, 0.8e-6f);
-    const float b2 = __builtin_fmaf(__builtin_fmaf(s2, s2, Tp), r, 0.8e-6f);
-    const float g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
-    const float p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
+    const R r = w5_inv(S);
+    const R s0 = w5_fma((R)3, dm1, -dm2);
+    const R s1 = dm1 + d0;
+    const R s2 = w5_fma((R)-3, d0, dp1);
+    const R b0 = w5_fma(w5_fma(s0, s0, Tm), r, (R)0.8e-6);
+    const R b1 = w5_fma(w5_fma(s1, s1, T0), r, (R)0.8e-6);
+    const R b2 = w5_fma(w5_fma(s2, s2, Tp), r, (R)0.8e-6);
+    const R g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
+    const R p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
     {
-        const float C0 = __builtin_fmaf(2.f, s0, -dm1);
-        const float C1 = __builtin_fmaf(2.f, d0, dm1);
-        const float C2 = __builtin_fmaf(4.f, d0, -dp1);
-        const float num = __builtin_fmaf(p12 * (1.f / 6.f), C0, __builtin_fmaf(p02, C1, (0.5f * p01) * C2));
-        const float den = __builtin_fmaf(6.f, p02, __builtin_fmaf(3.f, p01, p12));
-        um = __builtin_fmaf(num, rcpf(den), c);
+        const R C0 = w5_fma((R)2, s0, -dm1);
+        const R C1 = w5_fma((R)2, d0, dm1);
+        const R C2 = w5_fma((R)4, d0, -dp1);
+        const R num = w5_fma(p12 * ((R)1 / (R)6), C0, w5_fma(p02, C1, ((R)0.5 * p01) * C2));
+        const R den = w5_fma((R)6, p02, w5_fma((R)3, p01, p12));
+        um = w5_fin(num, den, c);
     }
     {
-        const float E0 = __builtin_fmaf(-4.f, dm1, dm2);
-        const float E1 = __builtin_fmaf(-2.f, dm1, -d0);
-        const float E2 = __builtin_fmaf(2.f, s2, d0);
-        const float num = __builtin_fmaf(0.5f * p12, E0, __builtin_fmaf(p02, E1, (p01 * (1.f / 6.f)) * E2));
-        const float den = __builtin_fmaf(6.f, p02, __builtin_fmaf(3.f, p12, p01));
-        up = __builtin_fmaf(num, rcpf(den), c);
+        const R E0 = w5_fma((R)-4, dm1, dm2);
+        const R E1 = w5_fma((R)-2, dm1, -d0);
+        const R E2 = w5_fma((R)2, s2, d0);
+        const R num = w5_fma((R)0.5 * p12, E0, w5_fma(p02, E1, (p01 * ((R)1 / (R)6)) * E2));
+        const R den = w5_fma((R)6, p02, w5_fma((R)3, p12, p01));
+        up = w5_fin(num, den, c);
     }
 }
 
 // one cell from its stencil and the stencil's squares (sc shared between directions)
-__device__ inline void weno5_sq(float a, float b, float c, float d, float e, float sa, float sb, float sc, float sd,
-                                float se, float &um, float &up)
+template <typename R>
+__device__ inline void weno5_sq(R a, R b, R c, R d, R e, R sa, R sb, R sc, R sd, R se, R &um, R &up)
 {
-    const float dm2 = b - a, dm1 = c - b, d0 = d - c, dp1 = e - d;
-    const float tm = dm1 - dm2, t0 = d0 - dm1, tp = dp1 - d0;
-    weno5_core(c, dm2, dm1, d0, dp1, w5_T(tm), w5_T(t0), w5_T(tp), w5_S(sa, sb, sc, sd, se), um, up);
-}
-
-__device__ inline void weno5_fast(float a, float b, float c, float d, float e, float &um, float &up)
-{
-    weno5_sq(a, b, c, d, e, w5_sqr(a), w5_sqr(b), w5_sqr(c), w5_sqr(d), w5_sqr(e), um, up);
+    const R dm2 = b - a, dm1 = c - b, d0 = d - c, dp1 = e - d;
+    const R tm = dm1 - dm2, t0 = d0 - dm1, tp = dp1 - d0;
+    weno5_core<R>(c, dm2, dm1, d0, dp1, w5_T(tm), w5_T(t0), w5_T(tp), w5_S(sa, sb, sc, sd, se), um, up);
 }
 
 template <typename T>
 __device__ inline void weno5(T a, T b, T c, T d, T e, T &um, T &up)
 {
-    if constexpr (sizeof(T) == 4)
-        weno5_fast(a, b, c, d, e, um, up);
-    else
-        weno5_exact<T>(a, b, c, d, e, um, up);
+    weno5_sq<T>(a, b, c, d, e, w5_sqr(a), w5_sqr(b), w5_sqr(c), w5_sqr(d), w5_sqr(e), um, up);
 }
 
 // Tile geometry of the z-marching WENO5 kernels (two ghost layers read).
@@ -451,42 +420,32 @@ __device__ inline void w5_march(const T *__restrict__ ub, T *__restrict__ ob, lo
             xs[VEC + 2] = row[x0 + VEC];
             xs[VEC + 3] = row[x0 + VEC + 1];
             V o[6];
-            if constexpr (sizeof(T) == 4) {
-                // x: differences, second differences and squares shared along the row
-                T sq[VEC + 4], dx[VEC + 3], tx[VEC + 3], Tx[VEC + 3], pp[VEC + 3];
+            // x: differences, second differences and squares shared along the row
+            T sq[VEC + 4], dx[VEC + 3], tx[VEC + 3], Tx[VEC + 3], pp[VEC + 3];
 #pragma unroll
-                for (int j = 0; j < VEC + 4; ++j)
-                    sq[j] = w5_sqr(xs[j]);
+            for (int j = 0; j < VEC + 4; ++j)
+                sq[j] = w5_sqr(xs[j]);
 #pragma unroll
-                for (int j = 0; j < VEC + 3; ++j) {
-                    dx[j] = xs[j + 1] - xs[j];
-                    pp[j] = sq[j] + sq[j + 1];
-                }
+            for (int j = 0; j < VEC + 3; ++j) {
+                dx[j] = xs[j + 1] - xs[j];
+                pp[j] = sq[j] + sq[j + 1];
+            }
 #pragma unroll
-                for (int j = 1; j < VEC + 3; ++j) {
-                    tx[j] = dx[j] - dx[j - 1];
-                    Tx[j] = w5_T(tx[j]);
-                }
+            for (int j = 1; j < VEC + 3; ++j) {
+                tx[j] = dx[j] - dx[j - 1];
+                Tx[j] = w5_T(tx[j]);
+            }
 #pragma unroll
-                for (int v = 0; v < VEC; ++v) {
-                    const int i = v + 2;
-                    weno5_core(xs[i], dx[i - 2], dx[i - 1], dx[i], dx[i + 1], Tx[i - 1], Tx[i], Tx[i + 1],
-                               (pp[i - 2] + pp[i]) + sq[i + 2], o[0].v[v], o[1].v[v]);
-                    const T c = q2.c[r].v[v], sc = sq[i];
-                    weno5_sq(ym2.v[v], ym1.v[v], c, yp1.v[v], yp2.v[v], w5_sqr(ym2.v[v]), w5_sqr(ym1.v[v]), sc,
-                             w5_sqr(yp1.v[v]), w5_sqr(yp2.v[v]), o[2].v[v], o[3].v[v]);
-                    weno5_sq(q0.c[r].v[v], q1.c[r].v[v], c, q3.c[r].v[v], q4.c[r].v[v], w5_sqr(q0.c[r].v[v]),
-                             w5_sqr(q1.c[r].v[v]), sc, w5_sqr(q3.c[r].v[v]), w5_sqr(q4.c[r].v[v]), o[4].v[v],
-                             o[5].v[v]);
-                }
-            } else {
-#pragma unroll
-                for (int v = 0; v < VEC; ++v) {
-                    weno5<T>(xs[v], xs[v + 1], xs[v + 2], xs[v + 3], xs[v + 4], o[0].v[v], o[1].v[v]);
-                    weno5<T>(ym2.v[v], ym1.v[v], q2.c[r].v[v], yp1.v[v], yp2.v[v], o[2].v[v], o[3].v[v]);
-                    weno5<T>(q0.c[r].v[v], q1.c[r].v[v], q2.c[r].v[v], q3.c[r].v[v], q4.c[r].v[v], o[4].v[v],
-                             o[5].v[v]);
-                }
+            for (int v = 0; v < VEC; ++v) {
+                const int i = v + 2;
+                weno5_core<T>(xs[i], dx[i - 2], dx[i - 1], dx[i], dx[i + 1], Tx[i - 1], Tx[i], Tx[i + 1],
+                              (pp[i - 2] + pp[i]) + sq[i + 2], o[0].v[v], o[1].v[v]);
+                const T c = q2.c[r].v[v], sc = sq[i];
+                weno5_sq<T>(ym2.v[v], ym1.v[v], c, yp1.v[v], yp2.v[v], w5_sqr(ym2.v[v]), w5_sqr(ym1.v[v]), sc,
+                            w5_sqr(yp1.v[v]), w5_sqr(yp2.v[v]), o[2].v[v], o[3].v[v]);
+                weno5_sq<T>(q0.c[r].v[v], q1.c[r].v[v], c, q3.c[r].v[v], q4.c[r].v[v], w5_sqr(q0.c[r].v[v]),
+                            w5_sqr(q1.c[r].v[v]), sc, w5_sqr(q3.c[r].v[v]), w5_sqr(q4.c[r].v[v]), o[4].v[v],
+                            o[5].v[v]);
             }
             const long long off = x0 + NX * y + SXY * z;
 #pragma unroll

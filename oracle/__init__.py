@@ -58,6 +58,8 @@ def lib():
         L.orc_weno5.restype = None
         L.orc_weno5_point.argtypes = [vp, vp, vp]
         L.orc_weno5_point.restype = None
+        L.orc_weno5_point_textbook.argtypes = [vp, vp, vp]
+        L.orc_weno5_point_textbook.restype = None
         L.orc_jacobi.argtypes = [vp, vp, ll, vp, ll, vp, vp, vp, f64, i32, i32]
         L.orc_jacobi.restype = f64
         L.orc_max_threads.restype = i32
@@ -185,10 +187,13 @@ def weno5(field, prec=1, nthreads=1, blocks=None):
     return outs
 
 
-def weno5_point(v):
+def weno5_point(v, textbook=False):
+    """(u- at i+1/2, u+ at i-1/2) of one 5-point stencil in FP64: the
+    difference form the kernels evaluate, or the textbook statement."""
     v = np.ascontiguousarray(v, dtype=np.float64)
     um, up = ctypes.c_double(), ctypes.c_double()
-    lib().orc_weno5_point(_p(v), ctypes.byref(um), ctypes.byref(up))
+    fn = lib().orc_weno5_point_textbook if textbook else lib().orc_weno5_point
+    fn(_p(v), ctypes.byref(um), ctypes.byref(up))
     return um.value, up.value
 
 

@@ -23,13 +23,17 @@
 //   weno5 / jacobi      no reference implementation exists (README.rst:28-31
 //                       advertises WENO only); restated from Jiang & Shu,
 //                       J. Comput. Phys. 126:202-228 (1996), and from the
-//                       textbook weighted-Jacobi iteration.
+//                       textbook weighted-Jacobi iteration.  WENO5 has two
+//                       statements: the textbook one (weno5_cell_textbook)
+//                       and the difference form the kernels evaluate
+//                       (weno5_cell), checked against each other.
 //
 // Pinning (what checks these restatements independently of their own text):
 // the reference's own unit-test properties (FieldLabTest.cpp, BC/*Test.cpp,
 // CartesianTest.cpp), an independent global-index formulation of periodic
 // ghosts, closed-form Fourier eigenvalues of the 7-point operator, WENO5
-// polynomial exactness / mirror symmetry / convergence order, and manufactured
+// polynomial exactness / mirror symmetry / convergence order and agreement of
+// its two statements to rounding, and manufactured
 // solutions for the Jacobi operator.  All of it lives in tests/.
 
 #include <algorithm>
@@ -454,9 +458,10 @@ double diffusion(const orc_field &in, void *out_v, long long out_stride, void *l
 // ---------------------------------------------------------------- WENO5
 // Per-cell left/right states with Jiang-Shu weights; eps = 1e-6 * m2 where m2
 // is the mean of the squared values of the five stencil cells (plus 1e-30),
-// evaluated in the scale-free form beta_k / m2.
+// evaluated in the scale-free form beta_k / m2.  The textbook statement: it
+// pins the difference form below (agreement to rounding).
 template <typename R>
-inline void weno5_cell(R a, R b, R c, R d, R e, R &um, R &up)
+inline void weno5_cell_textbook(R a, R b, R c, R d, R e, R &um, R &up)
 {
     // Jiang & Shu weights alpha_k = d_k / (eps' + beta_k / m2)^2, written
     // with the common factor prod_j (eps' + beta_j / m2)^2 multiplied out so
@@ -488,6 +493,60 @@ inline void weno5_cell(R a, R b, R c, R d, R e, R &um, R &up)
         const R q1 = 2 * b + 5 * c - d;
         const R q2 = 11 * c - 7 * d + 2 * e;
         up = (w0 * q0 + w1 * q1 + w2 * q2) / ((R)6 * (w0 + w1 + w2));
+    }
+}
+
+// The same states in difference form, the evaluation the GPU kernels follow
+// operation for operation in FP64.  With dm2 = b - a, dm1 = c - b,
+// d0 = d - c, dp1 = e - d, tk the second differences and S the sum of
+// (u^2 + 1e-30) over the stencil: 4 beta_k = (13/3) tk^2 + sk^2 with
+// s0 = 3 dm1 - dm2, s1 = dm1 + d0, s2 = dp1 - 3 d0, and the weights use
+// bk = 4 beta_k / S + 0.8e-6 = 0.8 (beta_k / m2 + 1e-6): a common factor
+// of the three indicators, which the normalised weights do not see.  The
+// candidates enter as offsets from c:
+//   u- = c + ((p12/6) C0 + p02 C1 + (p01/2) C2) / (p12 + 6 p02 + 3 p01)
+//   u+ = c + ((p12/2) E0 + p02 E1 + (p01/6) E2) / (3 p12 + 6 p02 + p01)
+// with C = (2 s0 - dm1, dm1 + 2 d0, 4 d0 - dp1) and
+// E = (dm2 - 4 dm1, -2 dm1 - d0, 2 s2 + d0), i.e. 6 (q_k - c) of the
+// textbook candidates, and p_k the product of the other two bj^2.
+template <typename R>
+inline R w5_T(R t)
+{
+    return (((R)13 / (R)3) * t) * t;
+}
+
+template <typename R>
+inline void weno5_cell(R a, R b, R c, R d, R e, R &um, R &up)
+{
+    const R sa = std::fma(a, a, (R)1e-30), sb = std::fma(b, b, (R)1e-30), sc = std::fma(c, c, (R)1e-30);
+    const R sd = std::fma(d, d, (R)1e-30), se = std::fma(e, e, (R)1e-30);
+    const R S = ((sa + sb) + (sc + sd)) + se;
+    const R dm2 = b - a, dm1 = c - b, d0 = d - c, dp1 = e - d;
+    const R Tm = w5_T(dm1 - dm2), T0 = w5_T(d0 - dm1), Tp = w5_T(dp1 - d0);
+    const R r = (R)1 / S;
+    const R s0 = std::fma((R)3, dm1, -dm2);
+    const R s1 = dm1 + d0;
+    const R s2 = std::fma((R)-3, d0, dp1);
+    const R b0 = std::fma(std::fma(s0, s0, Tm), r, (R)0.8e-6);
+    const R b1 = std::fma(std::fma(s1, s1, T0), r, (R)0.8e-6);
+    const R b2 = std::fma(std::fma(s2, s2, Tp), r, (R)0.8e-6);
+    const R g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
+    const R p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
+    {
+        const R C0 = std::fma((R)2, s0, -dm1);
+        const R C1 = std::fma((R)2, d0, dm1);
+        const R C2 = std::fma((R)4, d0, -dp1);
+        const R num = std::fma(p12 * ((R)1 / (R)6), C0, std::fma(p02, C1, ((R)0.5 * p01) * C2));
+        const R den = std::fma((R)6, p02, std::fma((R)3, p01, p12));
+        um = c + num / den;
+    }
+    {
+        const R E0 = std::fma((R)-4, dm1, dm2);
+        const R E1 = std::fma((R)-2, dm1, -d0);
+        const R E2 = std::fma((R)2, s2, d0);
+        const R num = std::fma((R)0.5 * p12, E0, std::fma(p02, E1, (p01 * ((R)1 / (R)6)) * E2));
+        const R den = std::fma((R)6, p02, std::fma((R)3, p12, p01));
+        up = c + num / den;
     }
 }
 
@@ -663,6 +722,12 @@ void orc_weno5(const orc_field *f, void *const *out6, long long out_stride, int 
 void orc_weno5_point(const double *v, double *um, double *up)
 {
     weno5_cell<double>(v[0], v[1], v[2], v[3], v[4], *um, *up);
+}
+
+// The textbook statement of the same stencil (pins the difference form).
+void orc_weno5_point_textbook(const double *v, double *um, double *up)
+{
+    weno5_cell_textbook<double>(v[0], v[1], v[2], v[3], v[4], *um, *up);
 }
 
 // Jacobi sweep (mode 0) or defect (mode 1, returns sum of squares).

@@ -104,35 +104,39 @@ def poly(c, x):
 
 
 @pytest.mark.parametrize("coef", [(1.0, 0.0, 0.0), (0.3, -2.0, 0.0), (1.5, 0.7, -0.4), (-2.0, 3.0, 5.0)])
-def test_weno5_quadratic_exact(coef):
+@pytest.mark.parametrize("tb", [False, True], ids=["diff", "textbook"])
+def test_weno5_quadratic_exact(coef, tb):
     j = np.arange(-2, 3, dtype=np.float64)
     v = cell_avg_poly(coef, j)
-    um, up = oracle.weno5_point(v)
+    um, up = oracle.weno5_point(v, tb)
     assert abs(um - poly(coef, 0.5)) < 1e-12 * (1 + abs(poly(coef, 0.5)))
     assert abs(up - poly(coef, -0.5)) < 1e-12 * (1 + abs(poly(coef, -0.5)))
 
 
-def test_weno5_mirror_symmetry():
+@pytest.mark.parametrize("tb", [False, True], ids=["diff", "textbook"])
+def test_weno5_mirror_symmetry(tb):
     rng = np.random.Generator(np.random.PCG64(1))
     for _ in range(200):
         v = rng.standard_normal(5)
-        um, up = oracle.weno5_point(v)
-        um2, up2 = oracle.weno5_point(v[::-1].copy())
+        um, up = oracle.weno5_point(v, tb)
+        um2, up2 = oracle.weno5_point(v[::-1].copy(), tb)
         assert abs(um - up2) <= 1e-14 * (1 + abs(um)) and abs(up - um2) <= 1e-14 * (1 + abs(up))
 
 
-def test_weno5_linear_weights_on_smooth_data():
+@pytest.mark.parametrize("tb", [False, True], ids=["diff", "textbook"])
+def test_weno5_linear_weights_on_smooth_data(tb):
     # smooth data: the result approaches the optimal 5-point linear scheme
     h = 1e-3
     j = np.arange(-2, 3)
     v = np.array([(np.sin((k + 0.5) * h) - np.sin((k - 0.5) * h)) / h for k in j])
     a, b, c, d, e = v
     lin = (2 * a - 13 * b + 47 * c + 27 * d - 3 * e) / 60.0
-    um, _ = oracle.weno5_point(v)
+    um, _ = oracle.weno5_point(v, tb)
     assert abs(um - lin) < 1e-10
 
 
-def test_weno5_fifth_order():
+@pytest.mark.parametrize("tb", [False, True], ids=["diff", "textbook"])
+def test_weno5_fifth_order(tb):
     errs = []
     for N in (32, 64, 128):
         h = 2 * np.pi / N
@@ -141,18 +145,42 @@ def test_weno5_fifth_order():
         err = 0.0
         for i in range(N):
             v = avg[[(i + k) % N for k in range(-2, 3)]]
-            um, _ = oracle.weno5_point(v)
+            um, _ = oracle.weno5_point(v, tb)
             err = max(err, abs(um - np.sin(xs[i] + h / 2)))
         errs.append(err)
     orders = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
     assert orders[-1] > 4.5, (errs, orders)
 
 
-def test_weno5_eno_at_jump():
-    um, up = oracle.weno5_point(np.array([0.0, 0.0, 0.0, 1.0, 1.0]))
+@pytest.mark.parametrize("tb", [False, True], ids=["diff", "textbook"])
+def test_weno5_eno_at_jump(tb):
+    um, up = oracle.weno5_point(np.array([0.0, 0.0, 0.0, 1.0, 1.0]), tb)
     assert abs(um) < 1e-6          # stencil {i-2,i-1,i} dominates: no overshoot
-    um, up = oracle.weno5_point(np.array([0.0, 0.0, 1.0, 1.0, 1.0]))
+    um, up = oracle.weno5_point(np.array([0.0, 0.0, 1.0, 1.0, 1.0]), tb)
     assert abs(up - 1.0) < 1e-6     # right state taken from the smooth side
+
+
+def test_weno5_two_statements_agree():
+    """The difference form (what the kernels evaluate) and the textbook
+    statement give the same states to rounding, 1e-14 of the stencil's
+    largest magnitude, over smooth, rough, discontinuous, constant and zero
+    stencils at magnitudes from 1e-30 to 1e30."""
+    rng = np.random.Generator(np.random.PCG64(7))
+    cases = [np.zeros(5), np.full(5, 3.25), np.array([0.0, 0.0, 0.0, 1.0, 1.0]), np.array([1.0, 1.0, 0.0, 0.0, 0.0])]
+    for _ in range(3000):
+        kind = rng.integers(3)
+        if kind == 0:
+            v = rng.standard_normal(5)
+        elif kind == 1:
+            v = np.sin(rng.uniform(0, 6) + np.arange(5) * rng.uniform(1e-3, 0.5)) + rng.uniform(-2, 2)
+        else:
+            v = np.where(np.arange(5) < rng.integers(1, 5), rng.uniform(-1, 1), rng.uniform(-10, 10)) + 0.0
+        cases.append(v * 10.0 ** rng.uniform(-30, 30))
+    for v in cases:
+        a = oracle.weno5_point(v)
+        b = oracle.weno5_point(v, textbook=True)
+        tol = 1e-14 * max(np.max(np.abs(v)), 1e-300)
+        assert abs(a[0] - b[0]) <= tol and abs(a[1] - b[1]) <= tol, (v, a, b)
 
 
 def test_weno5_field_matches_points():
