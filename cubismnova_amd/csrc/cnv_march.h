@@ -189,10 +189,20 @@ __device__ inline double w5_fma(double a, double b, double c) { return __builtin
 // 1/S: the hardware reciprocal in FP32, IEEE division in FP64
 __device__ inline float w5_inv(float x) { return rcpf(x); }
 __device__ inline double w5_inv(double x) { return 1.0 / x; }
-// c + num / den: one multiply-add with the hardware reciprocal in FP32; an
-// IEEE quotient and sum in FP64
-__device__ inline float w5_fin(float num, float den, float c) { return __builtin_fmaf(num, rcpf(den), c); }
-__device__ inline double w5_fin(double num, double den, double c) { return c + num / den; }
+// The two states c + nm / dm and c + np / dp: a multiply-add with the
+// hardware reciprocal each in FP32; in FP64 one IEEE quotient q = 1/(dm dp)
+// shared by both, c + (nm dp) q and c + (np dm) q
+__device__ inline void w5_fin(float nm, float dm, float np, float dp, float c, float &um, float &up)
+{
+    um = __builtin_fmaf(nm, rcpf(dm), c);
+    up = __builtin_fmaf(np, rcpf(dp), c);
+}
+__device__ inline void w5_fin(double nm, double dm, double np, double dp, double c, double &um, double &up)
+{
+    const double q = 1.0 / (dm * dp);
+    um = c + (nm * dp) * q;
+    up = c + (np * dm) * q;
+}
 
 // WENO5 (Jiang & Shu weights) in difference form.  With the stencil
 // (a, b, c, d, e) = u_{i-2..i+2}: first differences dm2 = b - a,
@@ -209,7 +219,8 @@ __device__ inline double w5_fin(double num, double den, double c) { return c + n
 //        C0 = 2 s0 - dm1, C1 = dm1 + 2 d0, C2 = 4 d0 - dp1
 //   u+ = c + ((p12/2) E0 + p02 E1 + (p01/6) E2) / (3 p12 + 6 p02 + p01)
 //        E0 = dm2 - 4 dm1, E1 = -2 dm1 - d0, E2 = 2 s2 + d0
-// (6 (q_k - c) of Jiang & Shu's candidates q_k).  Every operation is
+// (6 (q_k - c) of Jiang & Shu's candidates q_k); FP64 divides once for both
+// states, by the product of the two denominators.  Every operation is
 // explicit (one fma or one rounding), so any caller that forms the same
 // differences gets the same bits: cells of one row share the differences,
 // and the FP64 evaluation is the CPU restatement's operation for operation.
@@ -235,22 +246,17 @@ __device__ inline void weno5_core(R c, R dm2, R dm1, R d0, R dp1, R Tm, R T0, R 
     const R b2 = w5_fma(w5_fma(s2, s2, Tp), r, (R)0.8e-6);
     const R g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
     const R p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
-    {
-        const R C0 = w5_fma((R)2, s0, -dm1);
-        const R C1 = w5_fma((R)2, d0, dm1);
-        const R C2 = w5_fma((R)4, d0, -dp1);
-        const R num = w5_fma(p12 * ((R)1 / (R)6), C0, w5_fma(p02, C1, ((R)0.5 * p01) * C2));
-        const R den = w5_fma((R)6, p02, w5_fma((R)3, p01, p12));
-        um = w5_fin(num, den, c);
-    }
-    {
-        const R E0 = w5_fma((R)-4, dm1, dm2);
-        const R E1 = w5_fma((R)-2, dm1, -d0);
-        const R E2 = w5_fma((R)2, s2, d0);
-        const R num = w5_fma((R)0.5 * p12, E0, w5_fma(p02, E1, (p01 * ((R)1 / (R)6)) * E2));
-        const R den = w5_fma((R)6, p02, w5_fma((R)3, p12, p01));
-        up = w5_fin(num, den, c);
-    }
+    const R C0 = w5_fma((R)2, s0, -dm1);
+    const R C1 = w5_fma((R)2, d0, dm1);
+    const R C2 = w5_fma((R)4, d0, -dp1);
+    const R nm = w5_fma(p12 * ((R)1 / (R)6), C0, w5_fma(p02, C1, ((R)0.5 * p01) * C2));
+    const R dm = w5_fma((R)6, p02, w5_fma((R)3, p01, p12));
+    const R E0 = w5_fma((R)-4, dm1, dm2);
+    const R E1 = w5_fma((R)-2, dm1, -d0);
+    const R E2 = w5_fma((R)2, s2, d0);
+    const R np = w5_fma((R)0.5 * p12, E0, w5_fma(p02, E1, (p01 * ((R)1 / (R)6)) * E2));
+    const R dp = w5_fma((R)6, p02, w5_fma((R)3, p12, p01));
+    w5_fin(nm, dm, np, dp, c, um, up);
 }
 
 // one cell from its stencil and the stencil's squares (sc shared between directions)

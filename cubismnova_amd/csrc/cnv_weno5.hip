@@ -133,7 +133,7 @@ static W5Tune w5_default(int dtype, int edge)
     if (edge == 16)
         return dtype == CNV_F64 ? W5Tune{128, 16} : W5Tune{64, 16};
     if (edge == 32)
-        return dtype == CNV_F64 ? W5Tune{128, 16} : W5Tune{256, 32};
+        return dtype == CNV_F64 ? W5Tune{256, 32} : W5Tune{256, 32};
     return W5Tune{256, 16};
 }
 
@@ -170,6 +170,8 @@ static bool w5_dispatch_wg(const W5Args &a, int wg, int ntiles, hipStream_t st)
         return w5_launch<T, NX, 128>(a, ntiles, st);
     case 256:
         return w5_launch<T, NX, 256>(a, ntiles, st);
+    case 512:
+        return w5_launch<T, NX, 512>(a, ntiles, st);
     default:
         return false;
     }
@@ -280,7 +282,11 @@ static bool w5_valid_rt(int edge, int wg)
 {
     auto pick = [&](auto e) -> bool {
         constexpr int E = decltype(e)::value;
-        return wg == 64 ? w5_valid<T, E, 64>() : wg == 128 ? w5_valid<T, E, 128>() : wg == 256 ? w5_valid<T, E, 256>() : false;
+        return wg == 64    ? w5_valid<T, E, 64>()
+               : wg == 128 ? w5_valid<T, E, 128>()
+               : wg == 256 ? w5_valid<T, E, 256>()
+               : wg == 512 ? w5_valid<T, E, 512>()
+                           : false;
     };
     if (edge == 16)
         return pick(std::integral_constant<int, 16>());
@@ -328,11 +334,11 @@ int autotune_weno5(cnv_grid_s *in, cnv_grid_s *out, int iters, int *best_wg, int
     hipEvent_t e0, e1;
     CNV_HIP(hipEventCreate(&e0));
     CNV_HIP(hipEventCreate(&e1));
-    const int wgs[3] = {64, 128, 256};
+    const int wgs[4] = {64, 128, 256, 512};
     const int zcs[5] = {4, 8, 16, 32, 64};
     float best = 1e30f;
     int bw = 0, bz = 0, cnt = 0, rc = CNV_OK;
-    for (int wi = 0; wi < 3 && rc == CNV_OK; ++wi)
+    for (int wi = 0; wi < 4 && rc == CNV_OK; ++wi)
         for (int zi = 0; zi < 5; ++zi) {
             if (zcs[zi] > n[2])
                 continue;

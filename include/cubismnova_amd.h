@@ -417,7 +417,7 @@ int cnv_autotune_diffusion_run(cnv_grid u, cnv_grid tmp, int iters, int *wg, int
                                int max_results, int *nresults);
 int cnv_tune_set_diffusion_run(int dtype, int edge, int wg, int zchunk);
 int cnv_tune_get_diffusion_run(int dtype, int edge, int *wg, int *zchunk);
-/* The same for the WENO5 kernel (candidates: wg 64/128/256, z-chunks 4..64),
+/* The same for the WENO5 kernel (candidates: wg 64/128/256/512, z-chunks 4..64),
  * keyed by (dtype, block edge); edges 16, 32 and 64 have tiles. */
 int cnv_autotune_weno5(cnv_grid in, cnv_grid out, int iters, int *wg, int *zchunk, double *results,
                        int max_results, int *nresults);

@@ -55,7 +55,7 @@ def lib():
         L.orc_diffusion.argtypes = [vp, vp, ll, vp, vp, f64, i32, i32]
         L.orc_diffusion.restype = f64
         L.orc_weno5.argtypes = [vp, vp, ll, i32, i32, vp, i32]
-        L.orc_weno5.restype = None
+        L.orc_weno5.restype = i32
         L.orc_weno5_point.argtypes = [vp, vp, vp]
         L.orc_weno5_point.restype = None
         L.orc_weno5_point_textbook.argtypes = [vp, vp, vp]
@@ -172,18 +172,21 @@ def diffusion_step(field, h, nu_dt, nthreads=1, want_lap=False):
 
 
 def weno5(field, prec=1, nthreads=1, blocks=None):
-    """Six state slabs (u- x, u+ x, u- y, u+ y, u- z, u+ z) of one component,
-    float64 (prec=1) or float32.  With `blocks` (flat block indices) only those
-    blocks are computed and returned consecutively, one block_stride each."""
-    dt = np.float64 if prec == 1 else np.float32
+    """Six state slabs (u- x, u+ x, u- y, u+ y, u- z, u+ z) of one component
+    in float64 (prec=1, the only precision the oracle evaluates).  With
+    `blocks` (flat block indices) only those blocks are computed and returned
+    consecutively, one block_stride each."""
+    if prec != 1:
+        raise ValueError("the WENO5 oracle evaluates in float64 only (prec=1)")
     n = field.slab.shape[0] if blocks is None else len(blocks) * field.block_stride
-    outs = [np.zeros(n, dtype=dt) for _ in range(6)]
+    outs = [np.zeros(n, dtype=np.float64) for _ in range(6)]
     ptrs = (ctypes.c_void_p * 6)(*[o.ctypes.data for o in outs])
     if blocks is None:
-        lib().orc_weno5(field.ref(), ptrs, field.block_stride, int(prec), int(nthreads), None, 0)
+        rc = lib().orc_weno5(field.ref(), ptrs, field.block_stride, 1, int(nthreads), None, 0)
     else:
         sel = np.ascontiguousarray(blocks, dtype=np.int64)
-        lib().orc_weno5(field.ref(), ptrs, field.block_stride, int(prec), int(nthreads), _p(sel), len(sel))
+        rc = lib().orc_weno5(field.ref(), ptrs, field.block_stride, 1, int(nthreads), _p(sel), len(sel))
+    assert rc == 0
     return outs
 
 

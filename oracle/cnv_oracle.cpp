@@ -508,7 +508,8 @@ inline void weno5_cell_textbook(R a, R b, R c, R d, R e, R &um, R &up)
 //   u+ = c + ((p12/2) E0 + p02 E1 + (p01/6) E2) / (3 p12 + 6 p02 + p01)
 // with C = (2 s0 - dm1, dm1 + 2 d0, 4 d0 - dp1) and
 // E = (dm2 - 4 dm1, -2 dm1 - d0, 2 s2 + d0), i.e. 6 (q_k - c) of the
-// textbook candidates, and p_k the product of the other two bj^2.
+// textbook candidates, and p_k the product of the other two bj^2; both
+// states divide once, by the product of the two denominators.
 template <typename R>
 inline R w5_T(R t)
 {
@@ -532,22 +533,20 @@ inline void weno5_cell(R a, R b, R c, R d, R e, R &um, R &up)
     const R b2 = std::fma(std::fma(s2, s2, Tp), r, (R)0.8e-6);
     const R g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
     const R p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
-    {
-        const R C0 = std::fma((R)2, s0, -dm1);
-        const R C1 = std::fma((R)2, d0, dm1);
-        const R C2 = std::fma((R)4, d0, -dp1);
-        const R num = std::fma(p12 * ((R)1 / (R)6), C0, std::fma(p02, C1, ((R)0.5 * p01) * C2));
-        const R den = std::fma((R)6, p02, std::fma((R)3, p01, p12));
-        um = c + num / den;
-    }
-    {
-        const R E0 = std::fma((R)-4, dm1, dm2);
-        const R E1 = std::fma((R)-2, dm1, -d0);
-        const R E2 = std::fma((R)2, s2, d0);
-        const R num = std::fma((R)0.5 * p12, E0, std::fma(p02, E1, (p01 * ((R)1 / (R)6)) * E2));
-        const R den = std::fma((R)6, p02, std::fma((R)3, p12, p01));
-        up = c + num / den;
-    }
+    const R C0 = std::fma((R)2, s0, -dm1);
+    const R C1 = std::fma((R)2, d0, dm1);
+    const R C2 = std::fma((R)4, d0, -dp1);
+    const R nm = std::fma(p12 * ((R)1 / (R)6), C0, std::fma(p02, C1, ((R)0.5 * p01) * C2));
+    const R dm = std::fma((R)6, p02, std::fma((R)3, p01, p12));
+    const R E0 = std::fma((R)-4, dm1, dm2);
+    const R E1 = std::fma((R)-2, dm1, -d0);
+    const R E2 = std::fma((R)2, s2, d0);
+    const R np = std::fma((R)0.5 * p12, E0, std::fma(p02, E1, (p01 * ((R)1 / (R)6)) * E2));
+    const R dp = std::fma((R)6, p02, std::fma((R)3, p12, p01));
+    // one quotient for both states
+    const R q = (R)1 / (dm * dp);
+    um = c + (nm * dp) * q;
+    up = c + (np * dm) * q;
 }
 
 // in: field of one component (input type T); out: 6 slabs (xm,xp,ym,yp,zm,zp)
@@ -698,24 +697,20 @@ double orc_diffusion(const orc_field *in, void *out, long long out_stride, void 
     return diffusion<float>(*in, out, out_stride, lap_out, h, nu_dt, mode, nthreads);
 }
 
-// WENO5 states of one component. prec: 0 = float arithmetic/outputs,
-// 1 = double arithmetic/outputs (input dtype from the field).
-void orc_weno5(const orc_field *f, void *const *out6, long long out_stride, int prec, int nthreads,
-               const long long *sel, int nsel)
+// WENO5 states of one component in double arithmetic and outputs (input
+// dtype from the field); prec must be 1 (the FP32 kernels are checked
+// against this FP64 evaluation).  Returns 0, or -1 for another prec.
+int orc_weno5(const orc_field *f, void *const *out6, long long out_stride, int prec, int nthreads,
+              const long long *sel, int nsel)
 {
-    if (prec == 1) {
-        double *const *o = reinterpret_cast<double *const *>(out6);
-        if (f->dtype == 1)
-            weno5_field<double, double>(*f, o, out_stride, nthreads, sel, nsel);
-        else
-            weno5_field<float, double>(*f, o, out_stride, nthreads, sel, nsel);
-    } else {
-        float *const *o = reinterpret_cast<float *const *>(out6);
-        if (f->dtype == 1)
-            weno5_field<double, float>(*f, o, out_stride, nthreads, sel, nsel);
-        else
-            weno5_field<float, float>(*f, o, out_stride, nthreads, sel, nsel);
-    }
+    if (prec != 1)
+        return -1;
+    double *const *o = reinterpret_cast<double *const *>(out6);
+    if (f->dtype == 1)
+        weno5_field<double, double>(*f, o, out_stride, nthreads, sel, nsel);
+    else
+        weno5_field<float, double>(*f, o, out_stride, nthreads, sel, nsel);
+    return 0;
 }
 
 // Scalar WENO5 of one 5-point stencil, double precision.
