@@ -186,9 +186,21 @@ __device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
 __device__ inline float w5_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ inline double w5_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
-// 1/S: the hardware reciprocal in FP32, IEEE division in FP64
-__device__ inline float w5_inv(float x) { return rcpf(x); }
-__device__ inline double w5_inv(double x) { return 1.0 / x; }
+// The indicators are bk = (4 beta_k + 0.8e-6 S) r with r = 1/S up to a
+// factor common to the three of them.  FP32: r = the hardware reciprocal of S.
+// FP64: with S = m 2^e, m in [0.5, 1), r = 2^-e exactly and the epsilon term
+// 0.8e-6 S r = 0.8e-6 m, which replaces a division by two exponent
+// operations and a product.
+__device__ inline void w5_scale(float S, float &r, float &eps)
+{
+    r = rcpf(S);
+    eps = 0.8e-6f;
+}
+__device__ inline void w5_scale(double S, double &r, double &eps)
+{
+    r = __builtin_ldexp(1.0, -__builtin_amdgcn_frexp_exp(S));
+    eps = 0.8e-6 * __builtin_amdgcn_frexp_mant(S);
+}
 // The two states c + nm / dm and c + np / dp: a multiply-add with the
 // hardware reciprocal each in FP32; in FP64 one IEEE quotient q = 1/(dm dp)
 // shared by both, c + (nm dp) q and c + (np dm) q
@@ -212,7 +224,8 @@ __device__ inline void w5_fin(double nm, double dm, double np, double dp, double
 // with 4 beta_k = Tk + sk^2, s0 = a - 4b + 3c = 3 dm1 - dm2, s1 = d - b
 // (sign free) = dm1 + d0, s2 = 3c - 4d + e = dp1 - 3 d0.  The nonlinear
 // weights depend on the ratios of the bk = 4 beta_k / S + 0.8e-6 only, so
-// this common factor 0.8 of the indicators and the epsilon changes nothing.
+// this common factor 0.8 of the indicators and the epsilon changes nothing
+// (nor does FP64's factor m in [0.5, 1), see w5_scale).
 // The candidates enter as offsets from c; with p_k the product of the other
 // two squared indicators and the linear weights (1, 6, 3) / 10:
 //   u- = c + ((p12/6) C0 + p02 C1 + (p01/2) C2) / (p12 + 6 p02 + 3 p01)
@@ -224,8 +237,8 @@ __device__ inline void w5_fin(double nm, double dm, double np, double dp, double
 // explicit (one fma or one rounding), so any caller that forms the same
 // differences gets the same bits: cells of one row share the differences,
 // and the FP64 evaluation is the CPU restatement's operation for operation.
-// The indicators stay within [0.8e-6, 136] and their products within
-// [1e-25, 1e10], so nothing over- or underflows at any data magnitude.
+// The indicators stay within [0.4e-6, 136] and their products within
+// [1e-27, 1e10], so nothing over- or underflows at any data magnitude.
 template <typename R>
 __device__ inline R w5_sqr(R u) { return w5_fma(u, u, (R)1e-30); }
 template <typename R>
@@ -237,13 +250,14 @@ __device__ inline R w5_S(R sa, R sb, R sc, R sd, R se) { return ((sa + sb) + (sc
 template <typename R>
 __device__ inline void weno5_core(R c, R dm2, R dm1, R d0, R dp1, R Tm, R T0, R Tp, R S, R &um, R &up)
 {
-    const R r = w5_inv(S);
+    R r, eps;
+    w5_scale(S, r, eps);
     const R s0 = w5_fma((R)3, dm1, -dm2);
     const R s1 = dm1 + d0;
     const R s2 = w5_fma((R)-3, d0, dp1);
-    const R b0 = w5_fma(w5_fma(s0, s0, Tm), r, (R)0.8e-6);
-    const R b1 = w5_fma(w5_fma(s1, s1, T0), r, (R)0.8e-6);
-    const R b2 = w5_fma(w5_fma(s2, s2, Tp), r, (R)0.8e-6);
+    const R b0 = w5_fma(w5_fma(s0, s0, Tm), r, eps);
+    const R b1 = w5_fma(w5_fma(s1, s1, T0), r, eps);
+    const R b2 = w5_fma(w5_fma(s2, s2, Tp), r, eps);
     const R g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
     const R p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
     const R C0 = w5_fma((R)2, s0, -dm1);

@@ -501,7 +501,8 @@ inline void weno5_cell_textbook(R a, R b, R c, R d, R e, R &um, R &up)
 // d0 = d - c, dp1 = e - d, tk the second differences and S the sum of
 // (u^2 + 1e-30) over the stencil: 4 beta_k = (13/3) tk^2 + sk^2 with
 // s0 = 3 dm1 - dm2, s1 = dm1 + d0, s2 = dp1 - 3 d0, and the weights use
-// bk = 4 beta_k / S + 0.8e-6 = 0.8 (beta_k / m2 + 1e-6): a common factor
+// bk = 4 beta_k / S + 0.8e-6 = 0.8 (beta_k / m2 + 1e-6), evaluated as
+// (4 beta_k + 0.8e-6 S) 2^-e with S = m 2^e, m in [0.5, 1): common factors
 // of the three indicators, which the normalised weights do not see.  The
 // candidates enter as offsets from c:
 //   u- = c + ((p12/6) C0 + p02 C1 + (p01/2) C2) / (p12 + 6 p02 + 3 p01)
@@ -524,13 +525,17 @@ inline void weno5_cell(R a, R b, R c, R d, R e, R &um, R &up)
     const R S = ((sa + sb) + (sc + sd)) + se;
     const R dm2 = b - a, dm1 = c - b, d0 = d - c, dp1 = e - d;
     const R Tm = w5_T(dm1 - dm2), T0 = w5_T(d0 - dm1), Tp = w5_T(dp1 - d0);
-    const R r = (R)1 / S;
+    // S = m 2^e, m in [0.5, 1): the indicators times the common factor m
+    int ex = 0;
+    const R m = std::frexp(S, &ex);
+    const R r = std::ldexp((R)1, -ex);
+    const R eps = (R)0.8e-6 * m;
     const R s0 = std::fma((R)3, dm1, -dm2);
     const R s1 = dm1 + d0;
     const R s2 = std::fma((R)-3, d0, dp1);
-    const R b0 = std::fma(std::fma(s0, s0, Tm), r, (R)0.8e-6);
-    const R b1 = std::fma(std::fma(s1, s1, T0), r, (R)0.8e-6);
-    const R b2 = std::fma(std::fma(s2, s2, Tp), r, (R)0.8e-6);
+    const R b0 = std::fma(std::fma(s0, s0, Tm), r, eps);
+    const R b1 = std::fma(std::fma(s1, s1, T0), r, eps);
+    const R b2 = std::fma(std::fma(s2, s2, Tp), r, eps);
     const R g0 = b0 * b0, g1 = b1 * b1, g2 = b2 * b2;
     const R p12 = g1 * g2, p02 = g0 * g2, p01 = g0 * g1;
     const R C0 = std::fma((R)2, s0, -dm1);
