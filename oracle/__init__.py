@@ -48,6 +48,8 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
+        if os.path.exists("/proc/cpuinfo") and " fma" not in open("/proc/cpuinfo").read():
+            raise RuntimeError("the oracle is built for x86-64 hosts with FMA3 instructions")
         L = ctypes.CDLL(LIB_PATH)
         vp, i32, f64, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
         L.orc_lab_load.argtypes = [vp, vp, vp, i32, vp, i32, f64, vp]
