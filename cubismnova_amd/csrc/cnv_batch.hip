@@ -15,8 +15,10 @@
 // Operators: op 0: u_new = u + nu_dt * lap(u) in the Laplacian.cpp:125-129
 // order; op 1: WENO5 u^-_{i+1/2}, u^+_{i-1/2} in x, y, z (outputs 6c+2d,
 // 6c+2d+1).  Ghosts follow the grid's Stencil(-g, g+1) and BC lists.
-// FP32 grids only.  All launches of a batch go to one stream, so a batch run
-// is capturable as one unit in a HIP graph.
+// FP32 grids only.  When no item's output is another item's input or output,
+// the launches fork from the caller's stream onto streams of the batch and
+// join back into it, so they overlap; either way a batch run is capturable
+// as one unit in a HIP graph on the caller's stream.
 #include "cnv_march.h"
 
 #include <algorithm>
@@ -192,6 +194,13 @@ struct cnv_batch_s {
     std::map<std::pair<int, int>, int> zdepth; // (edge, ghost) -> planes per item
     std::vector<BItem> items;
     std::vector<BClass> classes;
+    // launches of the classes and the persistent launch run concurrently
+    // when no item writes what another item reads or writes: forked from
+    // the caller's stream onto `side` and joined back (capturable)
+    bool concurrent = false;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
 };
 
 static int class_key_edge(const cnv_grid_s *g) { return g->desc.block_cells[0]; }
@@ -465,12 +474,71 @@ static int launch_class(cnv_batch_s *b, const BClass &c, hipStream_t st)
 
 static int launch_batch(cnv_batch_s *b, hipStream_t st)
 {
-    for (const auto &c : b->classes) {
-        const int rc = launch_class(b, c, st);
-        if (rc != CNV_OK)
-            return rc;
+    // the non-empty launches: classes in order, then the persistent one (-1)
+    std::vector<int> parts;
+    for (size_t k = 0; k < b->classes.size(); ++k)
+        if (!b->classes[k].items.empty())
+            parts.push_back((int)k);
+    if (b->nitems > 0)
+        parts.push_back(-1);
+    auto run = [&](int k, hipStream_t s) { return k < 0 ? launch_generic(b, s) : launch_class(b, b->classes[k], s); };
+    if (!b->concurrent || parts.size() < 2 || b->side.size() + 1 < parts.size()) {
+        for (int k : parts) {
+            const int rc = run(k, st);
+            if (rc != CNV_OK)
+                return rc;
+        }
+        return CNV_OK;
     }
-    return launch_generic(b, st);
+    // fork: every launch after the first waits for the work before it on
+    // st, runs on its own stream and is joined back into st
+    CNV_HIP(hipEventRecord(b->fork, st));
+    int rc = CNV_OK;
+    size_t forked = 0;
+    for (size_t i = 1; i < parts.size() && rc == CNV_OK; ++i) {
+        hipStream_t s = b->side[i - 1];
+        CNV_HIP(hipStreamWaitEvent(s, b->fork, 0));
+        rc = run(parts[i], s);
+        CNV_HIP(hipEventRecord(b->joins[i - 1], s));
+        ++forked;
+    }
+    if (rc == CNV_OK)
+        rc = run(parts[0], st);
+    for (size_t i = 0; i < forked; ++i)
+        CNV_HIP(hipStreamWaitEvent(st, b->joins[i], 0));
+    return rc;
+}
+
+// Concurrent launches need every output grid distinct from every other
+// item's output and from every input.
+static bool independent_items(const cnv_batch_s *b)
+{
+    for (size_t i = 0; i < b->src.size(); ++i)
+        for (size_t j = 0; j < b->src.size(); ++j) {
+            if (i != j && b->src[i].out == b->src[j].out)
+                return false;
+            if (b->src[i].out == b->src[j].in)
+                return false;
+        }
+    return true;
+}
+
+static int make_side_streams(cnv_batch_s *b)
+{
+    b->concurrent = independent_items(b);
+    if (!b->concurrent)
+        return CNV_OK;
+    const size_t n = b->classes.size(); // launches beyond the first: at most #classes
+    CNV_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (size_t i = 0; i < n; ++i) {
+        hipStream_t s = nullptr;
+        CNV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        b->side.push_back(s);
+        hipEvent_t e = nullptr;
+        CNV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->joins.push_back(e);
+    }
+    return CNV_OK;
 }
 
 static bool stale(const cnv_batch_s *b)
@@ -523,6 +591,8 @@ int cnv_batch_create(const cnv_batch_item *items, int n, cnv_batch *out)
             rc = build_classes(b);
         if (rc == CNV_OK)
             rc = build_items(b, nullptr);
+        if (rc == CNV_OK)
+            rc = make_side_streams(b);
         if (rc != CNV_OK) {
             cnv_batch_destroy(b);
             return rc;
@@ -545,6 +615,12 @@ int cnv_batch_destroy(cnv_batch b)
     for (auto &c : b->classes)
         if (c.d_items)
             (void)hipFree(c.d_items);
+    for (hipStream_t s : b->side)
+        (void)hipStreamDestroy(s);
+    for (hipEvent_t e : b->joins)
+        (void)hipEventDestroy(e);
+    if (b->fork)
+        (void)hipEventDestroy(b->fork);
     delete b;
     return CNV_OK;
 }

@@ -367,7 +367,10 @@ int cnv_grid_read_hdf(cnv_grid g, const char *fname, int file_dtype, int face_di
  * running the single-grid tile bodies; all other items go to one persistent
  * launch that walks its list largest first.  Ghosts follow the lab rules of
  * cnv_lab_load for the grid's Stencil(-ghost, ghost+1) and BC lists.  A run
- * puts all its launches on one stream and is capturable in a HIP graph
+ * is ordered on the caller's stream: when no item's output grid is another
+ * item's output or any item's input, its launches fork from that stream onto
+ * streams of the batch and join back into it, so they overlap; otherwise
+ * they run one after another.  Either way it is capturable in a HIP graph
  * (cnv_graph_begin/end) after one plain run. */
 enum cnv_op {
     CNV_OP_DIFFUSION = 0, /* out = in + nu_dt * laplacian(in), as cnv_diffusion_step */
