@@ -4,7 +4,7 @@
 // A workgroup owns the x-y planes [z0, z1) of one block of one component.
 // It stages the planes of u^n it needs, each with a two-cell x/y halo from
 // the neighbour blocks (periodic wrap, the four diagonal cells included), in
-// a rolling set of four LDS planes; evaluates u^{n+1} on the block's cells
+// a rolling set of three LDS planes; evaluates u^{n+1} on the block's cells
 // and on the one-cell ring around them (the face-neighbour cells that the
 // second step reads), and u^{n+2} on the block's cells.  Each cell of
 // u^{n+1} and u^{n+2} is computed with the operation order of the one-step
