@@ -408,7 +408,7 @@ std::map<std::pair<int, int>, X2Tune> g_x2_tune; // (dtype, edge) -> choice
 X2Tune x2_default(int dtype, int edge)
 {
     if (edge == 32)
-        return dtype == CNV_F64 ? X2Tune{512, 32} : X2Tune{256, 16};
+        return dtype == CNV_F64 ? X2Tune{256, 32} : X2Tune{256, 16};
     return dtype == CNV_F64 ? X2Tune{128, 16} : X2Tune{64, 16};
 }
 
