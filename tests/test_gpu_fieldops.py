@@ -97,12 +97,13 @@ def grid_expected(op, g, ain, bin_, s, dst_before, comps):
             bi = (blk % nb[0], (blk // nb[0]) % nb[1], blk // (nb[0] * nb[1]))
             ne = int(np.prod(g.block_extent(bi, ca)))
             xa = x[blk, :ne]
-            if op == "neg":
-                o[blk, :ne] = -xa
-            elif op == "rcp":
-                o[blk, :ne] = T(s) * (T(1) / xa)
-            else:
-                o[blk, :ne] = REF[op](xa, y[blk, :ne] if y is not None else T(s))
+            with np.errstate(divide="ignore", invalid="ignore"):  # IEEE inf/nan are part of the contract
+                if op == "neg":
+                    o[blk, :ne] = -xa
+                elif op == "rcp":
+                    o[blk, :ne] = T(s) * (T(1) / xa)
+                else:
+                    o[blk, :ne] = REF[op](xa, y[blk, :ne] if y is not None else T(s))
     return out
 
 
