@@ -752,8 +752,8 @@ int cnv_tune_set_weno5(int dtype, int edge, int wg, int zchunk)
 {
     if (dtype != CNV_F32 && dtype != CNV_F64)
         return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
-    if (wg != 64 && wg != 128 && wg != 256)
-        return fail(CNV_ERR_INVALID, "workgroup size must be 64, 128 or 256");
+    if (wg != 64 && wg != 128 && wg != 256 && wg != 512)
+        return fail(CNV_ERR_INVALID, "workgroup size must be 64, 128, 256 or 512");
     return weno5_set_tune(dtype, edge, wg, zchunk);
 }
 
