@@ -97,10 +97,16 @@ def test_batch_classes(gpu, batch_setup):
 def test_batch_autotune_and_graph_replays(gpu, batch_setup):
     specs, grids, items = batch_setup
     b = cnv.Batch(items)
+    b.run()
+    builtin = [[go.download(c) for c in range(go.ncomp)] for _, go in grids]
     table = b.autotune(iters=2)
     assert len(table) > 0
     b.run()
     eager = [[go.download(c) for c in range(go.ncomp)] for _, go in grids]
+    # the tuned class and persistent launches give the built-in choices' bits
+    for ref, got in zip(builtin, eager):
+        for c in range(len(ref)):
+            assert np.array_equal(got[c], ref[c]), c
     st = cnv.Stream()
     gr = cnv.Graph(st).capture()
     b.run(stream=st)
