@@ -78,10 +78,13 @@ __device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, unsigned byt
 template <typename T, int N>
 __device__ inline VecT<T, N> buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
-    static_assert(sizeof(T) * N == 16, "16-byte vectors");
-    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    static_assert(sizeof(T) * N == 16 || sizeof(T) * N == 8, "8- or 16-byte vectors");
     typedef T vt __attribute__((ext_vector_type(N)));
-    const vt x = __builtin_bit_cast(vt, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+    vt x;
+    if constexpr (sizeof(T) * N == 16)
+        x = __builtin_bit_cast(vt, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+    else
+        x = __builtin_bit_cast(vt, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
     VecT<T, N> o;
 #pragma unroll
     for (int i = 0; i < N; ++i)
@@ -101,14 +104,18 @@ __device__ inline T buf_load1(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned 
 template <typename T, int N>
 __device__ inline void buf_store(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const VecT<T, N> &v)
 {
-    static_assert(sizeof(T) * N == 16, "16-byte vectors");
+    static_assert(sizeof(T) * N == 16 || sizeof(T) * N == 8, "8- or 16-byte vectors");
     typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
     typedef T vt __attribute__((ext_vector_type(N)));
     vt x;
 #pragma unroll
     for (int i = 0; i < N; ++i)
         x[i] = v.v[i];
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, x), r, (int)voff, (int)soff, 0);
+    if constexpr (sizeof(T) * N == 16)
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, x), r, (int)voff, (int)soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, (int)voff, (int)soff, 0);
 }
 
 // Streaming (non-temporal) vector load of exactly N elements, for data read

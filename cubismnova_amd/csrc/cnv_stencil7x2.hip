@@ -38,7 +38,10 @@ struct X2Args {
 
 template <typename T, int N, int WG>
 struct X2Shape {
-    static constexpr int VEC = 16 / (int)sizeof(T);
+    // 16-byte vectors per thread, or 8-byte ones when a plane has fewer
+    // 16-byte vectors than the workgroup has threads (more waves per tile)
+    static constexpr int VB = N * N * (int)sizeof(T) / 16 >= WG ? 16 : 8;
+    static constexpr int VEC = VB / (int)sizeof(T);
     static constexpr int TX = N / VEC;
     static constexpr int ROWS = WG / TX;
     static constexpr int R = N / ROWS;
