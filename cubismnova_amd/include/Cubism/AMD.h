@@ -540,6 +540,84 @@ void fieldOp(const int op, Grid::Cartesian<T> &dst, const Grid::Cartesian<T> &a,
     check(cnv_grid_field_op(op, dst.handle(), comp, a.handle(), comp, nullptr, comp, (double)s, nullptr),
           "cnv_grid_field_op");
 }
+
+// No counterpart in the reference (it names MeshClass::Stretched,
+// Common.h:86, but ships no stretched mesh or solver): the z faces of a
+// stretched mesh, nz_total + 1 values, used by the Jacobi operator of g
+template <typename T>
+void setZFaces(Grid::Cartesian<T> &g, const std::vector<double> &zfaces)
+{
+    check(cnv_grid_set_zfaces(g.handle(), zfaces.data(), (int)zfaces.size()), "cnv_grid_set_zfaces");
+}
+
+struct SolveResult {
+    int cycles = 0;
+    long long sweeps = 0;
+    double rel_residual = 0;
+    std::vector<double> history; // relative residual before cycle 1 and after every cycle
+};
+
+// Weighted-Jacobi solve of A u = f (component 0 of u, in place) with u's BC
+// lists.  opts.refine = 1: u and f FP64, work an FP32 grid of 3 components
+// (FP32 sweeps, FP64 defect and correction); 0: work of u's dtype with 2
+// components.
+template <typename TU, typename TW>
+SolveResult jacobiSolve(Grid::Cartesian<TU> &u, Grid::Cartesian<TU> &f, Grid::Cartesian<TW> &work,
+                        const cnv_solve_opts &opts)
+{
+    u.syncBCs(0);
+    SolveResult r;
+    r.history.assign((size_t)(opts.max_cycles > 0 ? opts.max_cycles : 0) + 1, 0.0);
+    cnv_solve_info info{};
+    check(cnv_jacobi_solve(u.handle(), f.handle(), work.handle(), &opts, &info, r.history.data(),
+                           (int)r.history.size(), nullptr),
+          "cnv_jacobi_solve");
+    r.cycles = info.cycles;
+    r.sweeps = info.sweeps;
+    r.rel_residual = info.rel_residual;
+    r.history.resize((size_t)info.cycles + 1);
+    return r;
+}
+
+// No counterpart in the reference: many independent FP32 grids, each with
+// its operator (CNV_OP_DIFFUSION: out = in + nu_dt lap(in); CNV_OP_WENO5: 6
+// states per component), processed together.  The grids must outlive the
+// batch; BC lists are taken from the grids when the batch is built.
+class Batch
+{
+public:
+    struct Item {
+        Grid::Cartesian<float> *in;
+        Grid::Cartesian<float> *out;
+        int op;
+        int ghost;
+        double nu_dt;
+    };
+    explicit Batch(const std::vector<Item> &items)
+    {
+        std::vector<cnv_batch_item> v;
+        for (const Item &it : items) {
+            for (size_t c = 0; c < it.in->getNumComponents(); ++c)
+                it.in->syncBCs(c);
+            v.push_back(cnv_batch_item{it.in->handle(), it.out->handle(), it.op, it.ghost, it.nu_dt});
+        }
+        check(cnv_batch_create(v.data(), (int)v.size(), &b_), "cnv_batch_create");
+    }
+    Batch(const Batch &) = delete;
+    Batch &operator=(const Batch &) = delete;
+    ~Batch() { cnv_batch_destroy(b_); }
+    void run(cnv_stream s = nullptr) { check(cnv_batch_run(b_, s), "cnv_batch_run"); }
+    // times the launch choices on the batch's own grids and keeps the fastest
+    void autotune(const int iters = 3)
+    {
+        int n = 0;
+        check(cnv_batch_autotune(b_, iters, nullptr, 0, &n), "cnv_batch_autotune");
+    }
+    cnv_batch handle() const { return b_; }
+
+private:
+    cnv_batch b_ = nullptr;
+};
 } // namespace Device
 
 // ------------------------------------------------------------ on-disk format

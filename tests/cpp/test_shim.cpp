@@ -6,6 +6,7 @@
 // the device operator.
 #include "Cubism/AMD.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <limits>
@@ -385,6 +386,128 @@ static void device_field_ops()
         EXPECT(d.field(1, i)(3, 4, 5) == 2.0 * (1.0 / b.field(1, i)(3, 4, 5)));
 }
 
+// Device::jacobiSolve: the mixed-precision refined solve reaches 1e-12 and
+// agrees with the all-FP64 solve; Dirichlet 0 on every face, tanh-stretched z
+static void jacobi_refined_vs_fp64()
+{
+    using G32 = Grid::Cartesian<float>;
+    const Index n = 16, N = 32;
+    Grid64 u({2, 2, 2}, {n, n, n}), f({2, 2, 2}, {n, n, n}), u2({2, 2, 2}, {n, n, n});
+    G32 w32({2, 2, 2}, {n, n, n}, {0, 0, 0}, {1, 1, 1}, 3);
+    Grid64 w64({2, 2, 2}, {n, n, n}, {0, 0, 0}, {1, 1, 1}, 2);
+    std::vector<double> zf((size_t)N + 1);
+    for (Index k = 0; k <= N; ++k) // faces clustered at both ends
+        zf[(size_t)k] = 0.5 * (1.0 + std::tanh(2.0 * (2.0 * k / N - 1.0)) / std::tanh(2.0));
+    std::vector<Field64::BCVector *> lists;
+    std::vector<BC::Dirichlet<Lab64> *> owned;
+    for (Grid64 *g : {&u, &u2})
+        for (size_t i = 0; i < g->size(); ++i) {
+            Field64 &fl = (*g)[i];
+            const MIndex bi = fl.getState().block_index;
+            for (size_t d = 0; d < 3; ++d)
+                for (size_t side = 0; side < 2; ++side)
+                    if (bi[d] == (side ? 1 : 0)) {
+                        owned.push_back(new BC::Dirichlet<Lab64>(d, side, 0.0));
+                        fl.getBC().push_back(owned.back());
+                    }
+        }
+    double mean = 0;
+    for (size_t i = 0; i < f.size(); ++i)
+        for (Index z = 0; z < n; ++z)
+            for (Index y = 0; y < n; ++y)
+                for (Index x = 0; x < n; ++x) {
+                    const MIndex bi = f[i].getState().block_index;
+                    const double gx = (bi[0] * n + x + 0.5) / N, gy = (bi[1] * n + y + 0.5) / N;
+                    const double gz = (bi[2] * n + z + 0.5) / N;
+                    f[i](x, y, z) = std::exp(-((gx - 0.4) * (gx - 0.4) + (gy - 0.6) * (gy - 0.6) +
+                                               (gz - 0.5) * (gz - 0.5)) / 0.02);
+                    mean += f[i](x, y, z);
+                    u[i](x, y, z) = u2[i](x, y, z) = 0.0;
+                }
+    mean /= (double)(N * N * N);
+    for (size_t i = 0; i < f.size(); ++i)
+        for (Index z = 0; z < n; ++z)
+            for (Index y = 0; y < n; ++y)
+                for (Index x = 0; x < n; ++x)
+                    f[i](x, y, z) -= mean;
+    u.upload(0);
+    u2.upload(0);
+    f.upload(0);
+    for (Grid64 *g : {&u, &f, &u2})
+        Device::setZFaces(*g, zf);
+    cnv_solve_opts o{1, 50, 4000, 2.0 / 3.0, 1e-12};
+    const Device::SolveResult r = Device::jacobiSolve(u, f, w32, o);
+    o.refine = 0;
+    const Device::SolveResult r2 = Device::jacobiSolve(u2, f, w64, o);
+    EXPECT(r.rel_residual <= 1e-12 && r2.rel_residual <= 1e-12);
+    EXPECT(r.history.size() == (size_t)r.cycles + 1 && std::fabs(r.history[0] - 1.0) <= 1e-12);
+    EXPECT(r.sweeps <= (long long)(1.2 * (double)r2.sweeps) + 100);
+    u.download(0);
+    u2.download(0);
+    double scale = 0, diff = 0;
+    for (size_t i = 0; i < u.size(); ++i)
+        for (Index z = 0; z < n; ++z)
+            for (Index y = 0; y < n; ++y)
+                for (Index x = 0; x < n; ++x) {
+                    scale = std::max(scale, std::fabs(u2[i](x, y, z)));
+                    diff = std::max(diff, std::fabs(u[i](x, y, z) - u2[i](x, y, z)));
+                }
+    EXPECT(scale > 0 && diff <= 1e-9 * scale);
+    for (auto *b : owned)
+        delete b;
+}
+
+// Device::Batch: a diffusion grid and a WENO5 grid in one batch give their
+// single-grid entry points' results (diffusion bit for bit, WENO5 states
+// within 1e-6), before and after autotuning
+static void batch_vs_single_grids()
+{
+    using G32 = Grid::Cartesian<float>;
+    G32 a({2, 2, 1}, {16, 16, 16}), ao({2, 2, 1}, {16, 16, 16}), ar({2, 2, 1}, {16, 16, 16});
+    G32 w({1, 2, 2}, {16, 16, 16}), wo({1, 2, 2}, {16, 16, 16}, {0, 0, 0}, {1, 1, 1}, 6),
+        wr({1, 2, 2}, {16, 16, 16}, {0, 0, 0}, {1, 1, 1}, 6);
+    for (size_t i = 0; i < a.size(); ++i)
+        for (Index z = 0; z < 16; ++z)
+            for (Index y = 0; y < 16; ++y)
+                for (Index x = 0; x < 16; ++x) {
+                    a[i](x, y, z) = (float)std::sin(0.2 * x + 0.3 * y - 0.1 * z + (double)i);
+                    w[i](x, y, z) = (float)(1.0 + 0.5 * std::tanh(0.7 * (x - 8.0) + 0.1 * y * z + (double)i));
+                }
+    a.upload(0);
+    w.upload(0);
+    const double nudt = 1e-4;
+    Device::diffusionStep(a, ar, nudt);
+    Device::weno5(w, wr);
+    ar.download(0);
+    std::vector<std::vector<float>> wref;
+    for (size_t c = 0; c < 6; ++c) {
+        wr.download(c);
+        std::vector<float> v;
+        for (size_t i = 0; i < wr.size(); ++i)
+            v.insert(v.end(), wr.field(c, i).getData(), wr.field(c, i).getData() + 16 * 16 * 16);
+        wref.push_back(v);
+    }
+    Device::Batch b({{&a, &ao, CNV_OP_DIFFUSION, 1, nudt}, {&w, &wo, CNV_OP_WENO5, 2, 0.0}});
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1)
+            b.autotune(2);
+        b.run();
+        ao.download(0);
+        for (size_t i = 0; i < a.size(); ++i)
+            for (Index z = 0; z < 16; ++z)
+                for (Index y = 0; y < 16; ++y)
+                    for (Index x = 0; x < 16; ++x)
+                        EXPECT(ao[i](x, y, z) == ar[i](x, y, z));
+        for (size_t c = 0; c < 6; ++c) {
+            wo.download(c);
+            size_t k = 0;
+            for (size_t i = 0; i < wo.size(); ++i)
+                for (size_t e = 0; e < 16 * 16 * 16; ++e, ++k)
+                    EXPECT(std::fabs(wo.field(c, i).getData()[e] - wref[c][k]) <= 1e-6f * (1.0f + std::fabs(wref[c][k])));
+        }
+    }
+}
+
 int main()
 {
     fieldlab_ghosts(false);
@@ -396,6 +519,8 @@ int main()
     device_allocator_wrap();
     hdf_read_back();
     device_field_ops();
+    jacobi_refined_vs_fp64();
+    batch_vs_single_grids();
     std::printf("%d checks, %d failed\n", g_checks, g_fail);
     return g_fail == 0 ? 0 : 1;
 }
