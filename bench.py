@@ -503,6 +503,23 @@ def bench_batch(args):
     wall = time.perf_counter() - t0
     kern_s = ev.elapsed_ms(e0, e1) / 1e3 / args.steps
     info = b.info()
+    # after the timed replay: every grid against its single-grid entry point
+    # (diffusion bit for bit, WENO5 states within 1e-6 of their largest value)
+    worst = 0.0
+    for g, o, s in grids:
+        ref = cnv.Cartesian(g.nblocks, g.block_cells, dtype=np.float32, ncomp=o.ncomp)
+        if s["op"] == "diffusion":
+            h = 1.0 / (g.nblocks[0] * g.block_cells[0])
+            cnv.diffusion_step(g, ref, 0.1 * h * h)
+        else:
+            cnv.weno5(g, ref)
+        for c in range(o.ncomp):
+            x, y = o.download(c).astype(np.float64), ref.download(c).astype(np.float64)
+            if s["op"] == "weno5":
+                worst = max(worst, float(np.max(np.abs(x - y))) / max(float(np.max(np.abs(y))), 1e-300))
+            elif not np.array_equal(x, y):
+                worst = 1.0
+        del ref
     out = {
         "metric": METRIC, "value": round(cells_total * args.steps / wall / 1e9, 3), "unit": "GCell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -516,6 +533,7 @@ def bench_batch(args):
                    "classes": [dict(zip(("op", "edge", "wg", "zdepth", "items"), c)) for c in b.classes()],
                    "persistent_wg": info["wg"], "persistent_lds_bytes": info["lds_bytes"]},
         "roofline": roofline(bytes_total, kern_s, measured_traffic("batch_96.md", "batch")),
+        "batch_vs_single_grid_max_rel": worst,
     }
     if not args.no_cpu:
         import oracle   # the CPU baseline leg only
@@ -601,6 +619,24 @@ def copy_ceiling():
     return 2 * nbytes / (ev.elapsed_ms(e0, e1) / 1e3 / 5) / 1e9
 
 
+def result_check(out):
+    """The tolerances each workload's line is quoted at: FP32 vs FP64 rel-L2
+    <= 1e-6 (diffusion after all steps, WENO5 on the worst state); the
+    refined Jacobi solve at 1e-12 and equal to FP64 Jacobi to 1e-9; the
+    batch equal to its single-grid entry points (diffusion bit for bit,
+    WENO5 within 1e-6).  Returns a message for the first one missed, or ""."""
+    if "fp32_rel_l2_vs_fp64" in out and not out["fp32_rel_l2_vs_fp64"] <= 1e-6:
+        return "diffusion FP32 vs FP64 rel-L2 %g" % out["fp32_rel_l2_vs_fp64"]
+    if "fp32_rel_l2_vs_fp64_worst_state" in out and not out["fp32_rel_l2_vs_fp64_worst_state"] <= 1e-6:
+        return "WENO5 FP32 vs FP64 rel-L2 %g" % out["fp32_rel_l2_vs_fp64_worst_state"]
+    if "batch_vs_single_grid_max_rel" in out and not out["batch_vs_single_grid_max_rel"] <= 1e-6:
+        return "batch vs single-grid max rel %g" % out["batch_vs_single_grid_max_rel"]
+    cv = out.get("convergence_32cube")
+    if cv and not (cv["refined_rel_residual"] <= 1e-12 and cv["refined_vs_fp64_rel_l2"] <= 1e-9):
+        return "refined Jacobi residual %g, vs FP64 %g" % (cv["refined_rel_residual"], cv["refined_vs_fp64_rel_l2"])
+    return ""
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -626,6 +662,11 @@ def main():
         sys.exit("--steps must be >= 1")
     out = {"weno5": bench_weno5, "batch": bench_batch, "jacobi": bench_jacobi,
            "copy": bench_copy}.get(args.workload, bench_diffusion)(args)
+    bad = result_check(out)
+    if bad:
+        # a fast line over wrong results is not a measurement
+        print(json.dumps(out), file=sys.stderr)
+        sys.exit("bench.py: results outside the stated tolerance: " + bad)
     if args.workload != "copy":
         c = copy_ceiling()
         out["hbm_ceiling"] = {"copy_1GiB_GB_s": round(c, 1), "frac_of_peak": round(c / HBM_PEAK_GBS, 4),
