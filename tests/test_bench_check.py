@@ -1,0 +1,23 @@
+"""bench.py prints a line only over results inside their stated tolerance
+(result_check is pure host logic: no GPU needed)."""
+import math
+
+import bench
+
+
+def test_result_check_passes_good_lines():
+    assert bench.result_check({"fp32_rel_l2_vs_fp64": 3.3e-8}) == ""
+    assert bench.result_check({"fp32_rel_l2_vs_fp64_worst_state": 3.7e-8}) == ""
+    assert bench.result_check({"batch_vs_single_grid_max_rel": 0.0}) == ""
+    assert bench.result_check({"convergence_32cube": {"refined_rel_residual": 9.96e-13,
+                                                      "refined_vs_fp64_rel_l2": 2.3e-16}}) == ""
+    assert bench.result_check({"metric": "copy"}) == ""
+
+
+def test_result_check_rejects_bad_or_nan_lines():
+    assert "diffusion" in bench.result_check({"fp32_rel_l2_vs_fp64": 3e-4})
+    assert "diffusion" in bench.result_check({"fp32_rel_l2_vs_fp64": math.nan})
+    assert "WENO5" in bench.result_check({"fp32_rel_l2_vs_fp64_worst_state": 2e-6})
+    assert "batch" in bench.result_check({"batch_vs_single_grid_max_rel": 1.0})
+    assert "Jacobi" in bench.result_check({"convergence_32cube": {"refined_rel_residual": 1e-11,
+                                                                  "refined_vs_fp64_rel_l2": 0.0}})
