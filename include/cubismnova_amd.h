@@ -220,10 +220,12 @@ int cnv_diffusion_run(cnv_grid u, cnv_grid tmp, int nsteps, double nu_dt, double
  * component c of `in`: out component 6c+2d   = u^-_{i+1/2} (linear weights
  * (1,6,3)/10), out component 6c+2d+1 = u^+_{i-1/2} (weights (3,6,1)/10), from
  * cells i-2..i+2.  eps = 1e-6 * m2 with m2 the mean squared value of the five
- * cells (+1e-30), applied as alpha_k = d_k / (1e-6 + beta_k/m2)^2.  FP64
- * grids are evaluated in IEEE order without contraction; FP32 grids use fused
- * multiply-adds and hardware reciprocals.  `out` has the same shape and
- * dtype and at least 6*ncomp(in) components. */
+ * cells (+1e-30), applied as alpha_k = d_k / (1e-6 + beta_k/m2)^2 up to
+ * factors common to the three weights.  Both precisions evaluate the same
+ * difference form with explicit fused multiply-adds: FP64 with IEEE
+ * quotients (bit-identical to the CPU restatement), FP32 with hardware
+ * reciprocals.  `out` has the same shape and dtype and at least
+ * 6*ncomp(in) components. */
 int cnv_weno5(cnv_grid in, cnv_grid out, cnv_stream s);
 
 /* FP64 sum and sum of squares of one component into device doubles out[0],
