@@ -68,3 +68,27 @@ def test_layout_roundtrip():
     assert s[0:5].tolist() == [0, 1, 2, 3, 4]
     assert s[44] == 5  # block (1,0,0) starts at x = 5
     assert np.array_equal(blocks_to_global(s, nb, n, 44), u)
+
+
+def test_tile_tables_host_side():
+    """The tile tables are host state: valid shapes are accepted, the
+    switched-off two-step tile (FP32, 32^3 blocks, 256 threads, 8 planes) and
+    shapes without a kernel are rejected, and nothing touches a GPU."""
+    import numpy as np
+    import cubismnova_amd as cnv
+    base = cnv.tune_diffusion_run(np.float32, 32)
+    try:
+        with pytest.raises(cnv.CnvError, match="disabled"):
+            cnv.set_tune_diffusion_run(np.float32, 32, 256, 8)
+        cnv.set_tune_diffusion_run(np.float32, 32, 512, 16)   # 8-byte vectors, 512 threads
+        assert cnv.tune_diffusion_run(np.float32, 32) == (512, 16)
+        cnv.set_tune_diffusion_run(np.float32, 16, 128, 8)    # 8-byte vectors on 16^3 blocks
+        with pytest.raises(cnv.CnvError):
+            cnv.set_tune_diffusion_run(np.float32, 32, 1024, 16)
+        with pytest.raises(cnv.CnvError):
+            cnv.set_tune_weno5(np.float32, 32, 512, 16)        # no 512-thread FP32 WENO5 tile
+        cnv.set_tune_weno5(np.float64, 32, 512, 16)
+    finally:
+        cnv.set_tune_diffusion_run(np.float32, 32, *base)
+        cnv.set_tune_diffusion_run(np.float32, 16, 64, 16)
+        cnv.set_tune_weno5(np.float64, 32, 256, 32)
