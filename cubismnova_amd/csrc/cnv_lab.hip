@@ -9,7 +9,7 @@
 // periodic neighbour's value, or unwritten (edges/corners of non-tensorial
 // labs; sides skipped by a non-periodic BC).  That is the same result as the
 // sequential passes because every BC pass reads only cells it does not write
-// (BC/Absorbing.h:180-188, BC/Symmetry.h:107-113).
+// (BC/Absorbing.h:94-102, BC/Symmetry.h:107-113).
 //
 // Node and face slabs: a block at the upper end of a "plus" dimension holds
 // one more element there, so each block's extent (and its lab range) is its

@@ -45,7 +45,7 @@ enum cnv_dtype { CNV_F32 = 0, CNV_F64 = 1 };
 /* BC kinds: include/Cubism/BC/{Base,Absorbing,Dirichlet,Symmetry}.h */
 enum cnv_bc_kind {
     CNV_BC_BASE = 0,      /* periodic, no-op apply (BC/Base.h:45-62) */
-    CNV_BC_ABSORBING = 1, /* zero-gradient copy (BC/Absorbing.h:147-189) */
+    CNV_BC_ABSORBING = 1, /* zero-gradient copy (BC/Absorbing.h:62-104) */
     CNV_BC_DIRICHLET = 2, /* ghost = value (BC/Dirichlet.h:75-113) */
     CNV_BC_SYMMETRY = 3   /* ghost = value * mirror (BC/Symmetry.h:69-114) */
 };

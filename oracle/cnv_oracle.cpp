@@ -13,7 +13,7 @@
 //   OLab::loadInner     include/Cubism/Block/FieldLabLoader.h:129-173
 //   OLab::loadGhosts    include/Cubism/Block/FieldLabLoader.h:175-287
 //   wrap_block          include/Cubism/Block/FieldLabLoader.h:405-411
-//   applyAbsorbing      include/Cubism/BC/Absorbing.h:147-189
+//   applyAbsorbing      include/Cubism/BC/Absorbing.h:62-104
 //   applyDirichlet      include/Cubism/BC/Dirichlet.h:75-113
 //   applySymmetry       include/Cubism/BC/Symmetry.h:69-114
 //   bcValidStencil      include/Cubism/BC/Base.h:73-78

@@ -197,3 +197,72 @@ def test_weno5_field_matches_points():
     for i, v in enumerate((vx, vy, vz)):
         um, up = oracle.weno5_point(v)
         assert O[2 * i][z, y, x] == um and O[2 * i + 1][z, y, x] == up
+
+
+# Known answers for the smoothness indicators, worked by hand from Jiang &
+# Shu (1996), eq. (3.2)-(3.4): beta_k = 13/12 t_k^2 + 1/4 s_k^2 with the
+# second differences t_k and the one-sided first differences s_k of the
+# three-cell stencils of (a, b, c, d, e) = cells i-2 .. i+2.
+#  * cell averages of x^3 on unit cells j = -2..2 (j^3 + j/4):
+#    (-17/2, -5/4, 0, 5/4, 17/2); t = (-6, 0, 6), s = (-7/2, -5/2, 7/2);
+#    beta = (36*13/12 + 49/16, 25/16, 36*13/12 + 49/16) = (673/16, 25/16, 673/16)
+#  * cell averages of (x+1)^3: (-5/4, 0, 5/4, 17/2, 111/4);
+#    t = (0, 6, 12), s = (5/2, -17/2, -5/2); beta = (25/16, 913/16, 2521/16)
+#  * a kink (0, 1, 3, 2, 0): t = (1, -3, -1), s = (5, -1, 1);
+#    beta = (13/12 + 25/4, 117/12 + 1/4, 13/12 + 1/4) = (22/3, 10, 4/3)
+from fractions import Fraction as Fr
+
+W5_KNOWN = [
+    ((-8.5, -1.25, 0.0, 1.25, 8.5), (Fr(673, 16), Fr(25, 16), Fr(673, 16))),
+    ((-1.25, 0.0, 1.25, 8.5, 27.75), (Fr(25, 16), Fr(913, 16), Fr(2521, 16))),
+    ((0.0, 1.0, 3.0, 2.0, 0.0), (Fr(22, 3), Fr(10), Fr(4, 3))),
+]
+
+
+def w5_states_from_betas(v, beta):
+    """Left/right states from given indicators, exactly: alpha_k =
+    d_k / (eps + beta_k)^2 with eps = 1e-6 * m2 (m2 = mean of the five
+    squares + 1e-30), normalised weights, and the third-order candidates of
+    Jiang & Shu eq. (2.11) / their mirror images."""
+    a, b, c, d, e = (Fr(x) for x in v)
+    m2 = (a * a + b * b + c * c + d * d + e * e) / 5 + Fr(1e-30)
+    eps = Fr(1e-6) * m2
+    qm = ((2 * a - 7 * b + 11 * c) / 6, (-b + 5 * c + 2 * d) / 6, (2 * c + 5 * d - e) / 6)
+    qp = ((-a + 5 * b + 2 * c) / 6, (2 * b + 5 * c - d) / 6, (11 * c - 7 * d + 2 * e) / 6)
+
+    def combine(dk, q):
+        al = [dk[k] / (eps + beta[k]) ** 2 for k in range(3)]
+        return sum(al[k] * q[k] for k in range(3)) / sum(al)
+
+    return (combine((Fr(1, 10), Fr(6, 10), Fr(3, 10)), qm), combine((Fr(3, 10), Fr(6, 10), Fr(1, 10)), qp))
+
+
+def w5_betas_with(v, k13, k14):
+    a, b, c, d, e = (Fr(x) for x in v)
+    t = (a - 2 * b + c, b - 2 * c + d, c - 2 * d + e)
+    s = (a - 4 * b + 3 * c, b - d, 3 * c - 4 * d + e)
+    return tuple(k13 * t[k] ** 2 + k14 * s[k] ** 2 for k in range(3))
+
+
+@pytest.mark.parametrize("case", range(len(W5_KNOWN)))
+@pytest.mark.parametrize("tb", [False, True], ids=["diff", "textbook"])
+def test_weno5_known_indicators(case, tb):
+    v, beta = W5_KNOWN[case]
+    um, up = w5_states_from_betas(v, beta)
+    got = oracle.weno5_point(np.array(v), tb)
+    for g, w in zip(got, (um, up)):
+        assert abs(Fr(g) - w) <= Fr(1e-14) * (1 + abs(w)), (v, got, float(um), float(up))
+
+
+@pytest.mark.parametrize("case", range(len(W5_KNOWN)))
+def test_weno5_known_answers_see_the_constants(case):
+    """The known answers above move by far more than their tolerance when
+    either indicator constant is changed (13/12 -> 13/11, 1/4 -> 1/3), so a
+    wrong constant shared by both oracle statements fails them."""
+    v, beta = W5_KNOWN[case]
+    assert w5_betas_with(v, Fr(13, 12), Fr(1, 4)) == beta
+    ref = w5_states_from_betas(v, beta)
+    for k13, k14 in ((Fr(13, 11), Fr(1, 4)), (Fr(13, 12), Fr(1, 3))):
+        bad = w5_states_from_betas(v, w5_betas_with(v, k13, k14))
+        moved = max(abs(bad[i] - ref[i]) / (1 + abs(ref[i])) for i in range(2))
+        assert moved > Fr(1e-8), (v, k13, k14, float(moved))
