@@ -65,16 +65,20 @@ struct X2Shape {
 };
 
 // Value held by lane i-1 (CTRL = row_shr:1) or lane i+1 (row_shl:1) within a
-// 16-lane DPP row; callers override the lanes at the ends of their rows.
+// 16-lane DPP row; the lane with no source in the row (lane 0 for row_shr,
+// lane 15 for row_shl) gets `old`, so every lane's result is defined.
 template <typename T, int CTRL>
-__device__ inline T dpp_shift(T v)
+__device__ inline T dpp_shift(T old, T v)
 {
     if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
     } else {
+        const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
         const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-        const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, false);
-        const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+        const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+        const int hi =
+            __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
         return __builtin_bit_cast(T, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
     }
 }
@@ -277,14 +281,15 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
             const int o = (y + 2) * PA + VEC + x0;
             const V ym = *reinterpret_cast<const V *>(Ac + o - PA);
             const V yp = *reinterpret_cast<const V *>(Ac + o + PA);
-            // x neighbours of the end cells from the adjacent lanes; the
-            // first / last lane of a row reads the halo
-            T xl = dpp_shift<T, kDppRowShr1>(c0c[r].v[VEC - 1]);
-            T xr = dpp_shift<T, kDppRowShl1>(c0c[r].v[0]);
-            if (tx == 0)
-                xl = Ac[o - 1];
-            if (tx == TX - 1)
-                xr = Ac[o + VEC];
+            // x neighbours of the end cells from the adjacent lanes; every
+            // lane of a row also reads the row's two halo values (one LDS
+            // address per row), and the first / last lane of the row selects
+            // them
+            const T hl = Ac[o - x0 - 1], hr = Ac[o - x0 + N];
+            const T sl = dpp_shift<T, kDppRowShr1>(hl, c0c[r].v[VEC - 1]);
+            const T sr = dpp_shift<T, kDppRowShl1>(hr, c0c[r].v[0]);
+            const T xl = tx == 0 ? hl : sl;
+            const T xr = tx == TX - 1 ? hr : sr;
             V res;
 #pragma unroll
             for (int v = 0; v < VEC; ++v) {
@@ -343,12 +348,11 @@ __global__ __launch_bounds__(WG) void k_stencil7x2(const X2Args a)
                 const int o = (y + 1) * PA + VEC + x0;
                 const V ym = *reinterpret_cast<const V *>(Br + o - PA);
                 const V yp = *reinterpret_cast<const V *>(Br + o + PA);
-                T xl = dpp_shift<T, kDppRowShr1>(c1[r].v[VEC - 1]);
-                T xr = dpp_shift<T, kDppRowShl1>(c1[r].v[0]);
-                if (tx == 0)
-                    xl = Br[o - 1];
-                if (tx == TX - 1)
-                    xr = Br[o + VEC];
+                const T hl = Br[o - x0 - 1], hr = Br[o - x0 + N];
+                const T sl = dpp_shift<T, kDppRowShr1>(hl, c1[r].v[VEC - 1]);
+                const T sr = dpp_shift<T, kDppRowShl1>(hr, c1[r].v[0]);
+                const T xl = tx == 0 ? hl : sl;
+                const T xr = tx == TX - 1 ? hr : sr;
                 V res;
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) {
@@ -424,15 +428,6 @@ X2Tune x2_tune(int dtype, int edge)
 
 // z-chunks compiled: 8, 16 and 32 planes (up to the block edge)
 inline bool x2_zchunk_ok(int zc) { return zc == 8 || zc == 16 || zc == 32; }
-
-// FP32 32^3 blocks in 256-thread tiles of 8 planes are switched off: on
-// 256^3 grids (2048 tiles, two rounds of workgroups) this instantiation
-// returned wrong cells in the last plane of some tiles, a few each run and
-// different from run to run; the cause is not found.
-inline bool x2_tile_disabled(int dtype, int edge, int wg, int zc)
-{
-    return dtype == CNV_F32 && edge == 32 && wg == 256 && zc == 8;
-}
 
 template <typename T, int N, int WG>
 bool x2_launch1(const X2Args &a, int zc, int ntiles, hipStream_t st)
@@ -544,8 +539,6 @@ int launch_diffusion_x2(cnv_grid_s *u, cnv_grid_s *out, double nu_dt, double *pa
         return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: zchunk must be 8, 16 or 32");
     if (zchunk > edge)
         return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: zchunk larger than the block edge");
-    if (x2_tile_disabled(u->desc.dtype, edge, wg, zchunk))
-        return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: FP32 edge-32 tile of 256 threads x 8 planes is disabled");
     a.nzc = edge / zchunk;
     a.tiles_per_comp = (int)(u->nblocks_total * a.nzc);
     for (int d = 0; d < 3; ++d)
@@ -577,8 +570,6 @@ int x2_set_tune(int dtype, int edge, int wg, int zchunk)
         return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: zchunk must be 8, 16 or 32, at most the edge");
     if (!x2_valid(dtype, edge, wg))
         return fail(CNV_ERR_UNSUPPORTED, "two-step tile shape not available for this block edge and dtype");
-    if (x2_tile_disabled(dtype, edge, wg, zchunk))
-        return fail(CNV_ERR_UNSUPPORTED, "two-step kernel: FP32 edge-32 tile of 256 threads x 8 planes is disabled");
     std::lock_guard<std::mutex> lk(g_x2_mtx);
     g_x2_tune[{dtype, edge}] = X2Tune{wg, zchunk};
     return CNV_OK;

@@ -417,10 +417,7 @@ int cnv_tune_set_stencil7(int dtype, int edge, int wg, int zchunk);
 int cnv_tune_get_stencil7(int dtype, int edge, int *wg, int *zchunk);
 /* The same for the two-step kernel that cnv_diffusion_run uses on periodic
  * grids (no BC list on any block) of 16^3 or 32^3 blocks: `tmp` is
- * overwritten; CNV_ERR_UNSUPPORTED for other grids.  The FP32 tile of 32^3
- * blocks, 256 threads and 8 planes is switched off (wrong cells at 256^3):
- * the autotuner skips it and cnv_tune_set_diffusion_run rejects it with
- * CNV_ERR_UNSUPPORTED. */
+ * overwritten; CNV_ERR_UNSUPPORTED for other grids. */
 int cnv_autotune_diffusion_run(cnv_grid u, cnv_grid tmp, int iters, int *wg, int *zchunk, double *results,
                                int max_results, int *nresults);
 int cnv_tune_set_diffusion_run(int dtype, int edge, int wg, int zchunk);

@@ -71,15 +71,14 @@ def test_layout_roundtrip():
 
 
 def test_tile_tables_host_side():
-    """The tile tables are host state: valid shapes are accepted, the
-    switched-off two-step tile (FP32, 32^3 blocks, 256 threads, 8 planes) and
-    shapes without a kernel are rejected, and nothing touches a GPU."""
+    """The tile tables are host state: valid shapes are accepted, shapes
+    without a kernel are rejected, and nothing touches a GPU."""
     import numpy as np
     import cubismnova_amd as cnv
     base = cnv.tune_diffusion_run(np.float32, 32)
     try:
-        with pytest.raises(cnv.CnvError, match="disabled"):
-            cnv.set_tune_diffusion_run(np.float32, 32, 256, 8)
+        cnv.set_tune_diffusion_run(np.float32, 32, 256, 8)
+        assert cnv.tune_diffusion_run(np.float32, 32) == (256, 8)
         cnv.set_tune_diffusion_run(np.float32, 32, 512, 16)   # 8-byte vectors, 512 threads
         assert cnv.tune_diffusion_run(np.float32, 32) == (512, 16)
         cnv.set_tune_diffusion_run(np.float32, 16, 128, 8)    # 8-byte vectors on 16^3 blocks

@@ -131,3 +131,69 @@ def test_batch_rejects_fp64_and_small_ghost(gpu):
     o32 = cnv.Cartesian(1, 8, dtype=np.float32, ncomp=6)
     with pytest.raises(cnv.CnvError):
         cnv.Batch([(g32, o32, "weno5", 1, 0.0)])
+
+
+def test_full_batch_96_grids(gpu):
+    """The full 96-grid workload (blocks 1x1x1 .. 8x8x4, edges {8,16,32},
+    1-3 components, ghost 1 or 3, per-face periodic / Dirichlet / absorbing):
+    after autotuning, a graph replay of the batch gives every grid's
+    single-grid result (diffusion bit-exact, also against the oracle; WENO5
+    within 1e-6 of the single-grid kernel's largest state and within 1e-6
+    rel-L2 of the oracle's FP64 states), on each of three replays."""
+    specs = workloads.batch_specs(96, k=4)
+    grids = [make_grids(s) for s in specs]
+    items = []
+    for s, (gi, go) in zip(specs, grids):
+        h = 1.0 / (s["nblocks"][0] * s["edge"])
+        items.append((gi, go, s["op"], s["ghost"], 0.1 * h * h))
+    b = cnv.Batch(items)
+    assert b.info()["items"] > 0
+    b.autotune(iters=2)
+    st = cnv.Stream()
+    b.run(stream=st)
+    st.synchronize()
+    gr = cnv.Graph(st).capture()
+    b.run(stream=st)
+    gr.end()
+    nt = oracle.max_threads()
+    want = []
+    for s, (gi, go), it in zip(specs, grids, items):
+        nb, e = s["nblocks"], s["edge"]
+        ref = cnv.Cartesian(nb, e, dtype=np.float32, ncomp=go.ncomp)
+        if s["op"] == "diffusion":
+            cnv.diffusion_step(gi, ref, it[4])
+        else:
+            cnv.weno5(gi, ref)
+        bcs = oracle_bcs(s)
+        per = []
+        for c in range(s["ncomp"]):
+            f = oracle.Field(gi.download(c), nb, e, gi.block_stride, bcs)
+            if s["op"] == "diffusion":
+                one = cells_of(ref.download(c), nb, (e,) * 3, gi.block_stride)
+                un, _ = oracle.diffusion_step(f, [1.0 / (nb[d] * e) for d in range(3)], it[4], nthreads=nt)
+                assert np.array_equal(one, cells_of(un, nb, (e,) * 3, gi.block_stride)), (s["edge"], nb, c)
+                per.append((one, None))
+            else:
+                o64 = oracle.weno5(f, prec=1, nthreads=nt)
+                for k in range(6):
+                    one = cells_of(ref.download(6 * c + k), nb, (e,) * 3, gi.block_stride)
+                    per.append((one, cells_of(o64[k], nb, (e,) * 3, gi.block_stride)))
+        want.append(per)
+        del ref
+    for rep in range(3):
+        for _, go in grids:            # clobber the outputs between replays
+            for c in range(go.ncomp):
+                cnv.call("cnv_fill", go.data_ptr(c), 0, -7.0, go.component_bytes // 4, None)
+        gr.launch()
+        st.synchronize()
+        for s, (gi, go), per in zip(specs, grids, want):
+            nb, e = s["nblocks"], s["edge"]
+            for k, (one, o64) in enumerate(per):
+                got = cells_of(go.download(k), nb, (e,) * 3, gi.block_stride)
+                if o64 is None:
+                    assert np.array_equal(got, one), (rep, s["edge"], nb, k)
+                else:
+                    g64, one64 = got.astype(np.float64), one.astype(np.float64)
+                    scale = np.max(np.abs(o64))
+                    assert np.max(np.abs(g64 - one64)) <= 1e-6 * scale, (rep, s["edge"], nb, k)
+                    assert np.linalg.norm(g64 - o64) <= 1e-6 * np.linalg.norm(o64), (rep, s["edge"], nb, k)

@@ -275,8 +275,6 @@ def test_run_argument_errors(gpu):
 
 TWO_STEP_SHAPES = {np.float32: [(16, 64), (16, 128), (32, 128), (32, 256), (32, 512)],
                    np.float64: [(16, 64), (16, 128), (32, 256), (32, 512)]}
-# switched off in the library: wrong cells on 256^3 grids, cause not found
-DISABLED_TWO_STEP = {(np.float32, 32, 256, 8)}
 # the library's built-in (workgroup size, z depth) per block edge
 BUILTIN_TWO_STEP = {np.float32: {16: (64, 16), 32: (256, 16)}, np.float64: {16: (128, 16), 32: (256, 32)}}
 
@@ -303,10 +301,6 @@ def test_two_step_kernel_matches_single_steps(gpu, dtype):
             if e2 != e:
                 continue
             for z in sorted({8, 16, e}):
-                if (dtype, e, wg, z) in DISABLED_TWO_STEP:
-                    with pytest.raises(cnv.CnvError, match="disabled"):
-                        cnv.set_tune_diffusion_run(dtype, e, wg, z)
-                    continue
                 cnv.set_tune_diffusion_run(dtype, e, wg, z)
                 for ns in (4, 5, nsteps):
                     u = cnv.Cartesian(nb, n, dtype=dtype)
@@ -325,17 +319,22 @@ def test_two_step_kernel_matches_single_steps(gpu, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_two_step_full_size_graph_every_tile(gpu, dtype):
-    """256^3, 100 steps through diffusion_run for every tile shape of 32^3
-    blocks, run directly and as a captured HIP graph replayed twice (the
-    benchmark's use): fields bit-identical to 100 single steps each time."""
-    nb, n, N = (8, 8, 8), (32, 32, 32), 256
+@pytest.mark.parametrize("edge,N,steps", [(32, 256, 100), (16, 256, 100), (32, 512, 20)],
+                         ids=["32^3-blocks-256", "16^3-blocks-256", "32^3-blocks-512"])
+def test_two_step_full_size_graph_every_tile(gpu, dtype, edge, N, steps):
+    """Full-size grids (256^3 of 32^3 or 16^3 blocks, and 512^3 of 32^3
+    blocks: several rounds of workgroups per launch for every tile), `steps`
+    steps through diffusion_run for every tile shape of the block edge, run
+    directly and as a captured HIP graph replayed twice (the benchmark's use):
+    fields bit-identical to single steps each time."""
+    n = (edge,) * 3
+    nb = (N // edge,) * 3
     u0 = workloads.diffusion_large(N).astype(dtype)
     dt = 0.1 / N / N
     a, b = cnv.Cartesian(nb, n, dtype=dtype), cnv.Cartesian(nb, n, dtype=dtype)
     a.upload_global(0, u0)
     src, dst = a, b
-    for k in range(100):
+    for k in range(steps):
         cnv.diffusion_step(src, dst, dt)
         src, dst = dst, src
     want = src.download(0)
@@ -344,25 +343,23 @@ def test_two_step_full_size_graph_every_tile(gpu, dtype):
     b.prepare()
     try:
         for e2, wg in TWO_STEP_SHAPES[dtype]:
-            if e2 != 32:
+            if e2 != edge:
                 continue
-            for z in (8, 16, 32):
-                if (dtype, 32, wg, z) in DISABLED_TWO_STEP:
-                    continue
-                cnv.set_tune_diffusion_run(dtype, 32, wg, z)
+            for z in sorted({8, 16, edge}):
+                cnv.set_tune_diffusion_run(dtype, edge, wg, z)
                 a.upload_global(0, u0)
-                res = cnv.diffusion_run(a, b, 100, dt)
-                assert np.array_equal(res.download(0), want), ("direct", wg, z)
+                res = cnv.diffusion_run(a, b, steps, dt)
+                assert_same(res.download(0), want, "direct wg %d z %d" % (wg, z))
                 g = cnv.Graph(st).capture()
-                res = cnv.diffusion_run(a, b, 100, dt, stream=st)
+                res = cnv.diffusion_run(a, b, steps, dt, stream=st)
                 g.end()
                 for rep in range(2):
                     a.upload_global(0, u0)
                     g.launch()
                     st.synchronize()
-                    assert np.array_equal(res.download(0), want), ("graph", wg, z, rep)
+                    assert_same(res.download(0), want, "graph wg %d z %d replay %d" % (wg, z, rep))
     finally:
-        cnv.set_tune_diffusion_run(dtype, 32, *BUILTIN_TWO_STEP[dtype][32])
+        cnv.set_tune_diffusion_run(dtype, edge, *BUILTIN_TWO_STEP[dtype][edge])
 
 
 def test_two_step_autotune_and_fallback(gpu):

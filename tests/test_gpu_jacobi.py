@@ -1,7 +1,8 @@
 """GPU parity of the stretched-z Jacobi kernels and the refined solver.
 
 Sweep and defect: bit-identical to the oracle in FP64 and FP32 (same
-operation order, no contraction).  Refined solve (FP32 sweeps, FP64 defect
+operation order, no contraction), on blocks of edge 16, 32 and 64 and on the
+full 512^3 grid of 64^3 blocks (tanh-stretched z, Dirichlet 0).  Refined solve (FP32 sweeps, FP64 defect
 and correction) to ||r||/||f|| <= 1e-12 against the direct solution of the
 independently assembled sparse matrix and against an all-FP64 Jacobi solve."""
 import numpy as np
@@ -28,7 +29,7 @@ def grid(nb, n, dtype, zf, ncomp=1):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("edge", [16, 32])
+@pytest.mark.parametrize("edge", [16, 32, 64])
 def test_sweep_and_defect_bitexact(gpu, dtype, edge):
     nb, n = (2, 2, 2), (edge,) * 3
     N = 2 * edge
@@ -105,3 +106,58 @@ def test_solver_rejects_bad_options(gpu):
         cnv.jacobi_solve(u, f, grid(nb, n, np.float64, zf, ncomp=3), refine=True)  # work must be FP32
     with pytest.raises(cnv.CnvError):
         cnv.jacobi_solve(u, f, grid(nb, n, np.float32, zf, ncomp=3), sweeps_per_cycle=3)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_full_size_sweep_512_bitexact(gpu, dtype):
+    """The 512^3 stretched-z Poisson grid of 8x8x8 blocks of 64^3 (Dirichlet 0
+    on every face): one sweep and one defect, every cell bit-exact against
+    the oracle run on the same bytes."""
+    from cubismnova_amd import workloads
+    N, nb, n = 512, (8, 8, 8), (64, 64, 64)
+    zf = stretch.tanh_faces(N, 2.0)
+    czm, czp, _, zc = stretch.metric(zf)
+    h = [1.0 / N] * 3
+    rng = np.random.Generator(np.random.PCG64(512))
+    u0 = rng.random((N, N, N), dtype=np.float32 if dtype == np.float32 else np.float64).astype(dtype)
+    f0 = workloads.poisson_source(N, zc).astype(dtype)
+    u, f, out = grid(nb, n, dtype, zf), grid(nb, n, dtype, zf), grid(nb, n, dtype, zf)
+    u.upload_global(0, u0)
+    f.upload_global(0, f0)
+    fo = oracle.Field(global_to_blocks(u0, nb, n), nb, n, bcs=face_bc_lists(nb, D0))
+    fs = global_to_blocks(f0, nb, n)
+    del u0, f0
+    nt = oracle.max_threads()
+    cnv.jacobi_sweep(u, f, out, 2.0 / 3.0)
+    assert_same(out.download(0), oracle.jacobi(fo, fs, czm, czp, h, 2.0 / 3.0, 0, nthreads=nt), "sweep 512^3")
+    s = cnv.DeviceBuffer(8)
+    cnv.jacobi_defect(u, f, out, sumsq=s)
+    r, sref = oracle.jacobi(fo, fs, czm, czp, h, 2.0 / 3.0, 1, nthreads=nt)
+    assert_same(out.download(0), r, "defect 512^3")
+    assert abs(s.download(np.float64, 1)[0] - sref) <= 1e-12 * sref
+
+
+def test_refined_solve_on_64_blocks(gpu):
+    """The refined solver on blocks of edge 64 (the edge of the 512^3
+    workload), one block of 64^3: FP32 sweeps with FP64 defect and
+    correction reach ||r||/||f|| <= 1e-12 and agree with all-FP64 Jacobi."""
+    from cubismnova_amd import workloads
+    nb, n, N = (1, 1, 1), (64, 64, 64), 64
+    zf = stretch.tanh_faces(N, 2.0)
+    _, _, _, zc = stretch.metric(zf)
+    src = workloads.poisson_source(N, zc)
+    u = grid(nb, n, np.float64, zf)
+    f = grid(nb, n, np.float64, zf)
+    u.upload_global(0, np.zeros((N, N, N)))
+    f.upload_global(0, src)
+    cyc, sw, rel, hist = cnv.jacobi_solve(u, f, grid(nb, n, np.float32, zf, ncomp=3), refine=True,
+                                          sweeps_per_cycle=50, max_cycles=4000, tol=1e-12)
+    assert rel <= 1e-12, (cyc, rel)
+    u2 = grid(nb, n, np.float64, zf)
+    u2.upload_global(0, np.zeros((N, N, N)))
+    cyc2, sw2, rel2, _ = cnv.jacobi_solve(u2, f, grid(nb, n, np.float64, zf, ncomp=2), refine=False,
+                                          sweeps_per_cycle=50, max_cycles=4000, tol=1e-12)
+    assert rel2 <= 1e-12, (cyc2, rel2)
+    a, b = u.download_global(0), u2.download_global(0)
+    assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b))
+    assert sw <= 1.2 * sw2 + 100
