@@ -1,18 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the block-structured stencil path on one MI355X.
 
-Default workload (one "step" = lab + 7-point Laplacian + forward Euler +
-FP64 L2 norm of the new field, over the whole grid): 8x8x8 blocks of 32^3
-cells (256^3), periodic, ghost width 1, seeded sum of 8 Fourier modes,
-dt = 0.1 h^2, 100 steps (one diffusion_run: two steps per launch on this
-periodic grid) run in FP32 and in FP64 from the same initial field.  `value`
-is the FP32 path in GCell-updates/s; the FP64 path, the FP32-vs-FP64 rel-L2
-error, the roofline of the dominant kernel (HIP events on the launch stream;
-algorithmic bytes = one read and one write of the field per launch) and the
-CPU baseline (the C++ oracle on the host cores) are reported in the same JSON
-line.
+One run measures every workload and prints one JSON line.  `value` is the
+headline: FP32 GCell-updates/s of the diffusion workload -- 8x8x8 blocks of
+32^3 cells (256^3), periodic, ghost width 1, seeded sum of 8 Fourier modes,
+dt = 0.1 h^2, 100 steps, one "step" = lab + 7-point Laplacian + forward Euler
++ FP64 L2 norm of the new field over the whole grid (one diffusion_run,
+two steps per launch on this periodic grid), in FP32 and in FP64 from the
+same initial field.  `workloads` holds one record each for the other
+workloads at their full sizes: WENO5 face states (512^3 x 5 components),
+the stretched-z Jacobi solve (512^3 of 64^3 blocks, FP32 sweeps with FP64
+refinement) and the batch of 96 uneven grids.  Every record carries its
+FP32-vs-FP64 error (or its result check), the roofline of its dominant
+kernel (HIP events on the launch stream; the bytes model is stated in the
+record) and a CPU baseline (the C++ oracle on the host cores and on one
+core, CPU model stated).
 
-  python bench.py [--steps K] [--warmup W] [--workload diffusion|weno5|jacobi|batch|copy]
+  python bench.py [--steps K] [--warmup W]
+                  [--workload all|diffusion|weno5|jacobi|batch|copy]
 """
 import argparse
 import ctypes
@@ -59,10 +64,74 @@ class HipEvents:
         assert self.hip.hipDeviceSynchronize() == 0
 
 
-def roofline(bytes_per_launch, avg_s, traffic=None):
+def roofline(bytes_per_launch, avg_s, traffic=None, bytes_model=None):
     ach = bytes_per_launch / avg_s / 1e9
-    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
+    r = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
+    if bytes_model:
+        r["bytes_model"] = bytes_model
+    return r
+
+
+# ------------------------------------------------------------------ CPU baseline
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Host threads for the all-core baseline: the job's CPU share
+    (OMP_NUM_THREADS when the environment sets it), else every core this
+    process may run on."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_timed(fn, budget_s, min_samples=10, max_samples=10000):
+    """One warm-up call, then samples until at least min_samples and the
+    budget are both reached (the protocol of ISPC/BaseBenchmark.cpp:65-165:
+    warm-up, repeated samples, mean / min / sdev); a sample that alone
+    exceeds the budget ends the run after three samples."""
+    fn()
+    times = []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+        el = time.perf_counter() - t_start
+        if len(times) >= max_samples:
+            break
+        if len(times) >= min_samples and el >= budget_s:
+            break
+        if len(times) >= 3 and el >= 3 * budget_s:
+            break
+    a = np.array(times)
+    return {"samples": int(a.size), "mean_s": float(a.mean()), "min_s": float(a.min()), "sdev_s": float(a.std())}
+
+
+def cpu_baseline(work_all, cells_all, sample_all, work_one, cells_one, sample_one, budget_s):
+    """CPU baseline record: work_all(nthreads) on the job's host cores and
+    work_one() on one core, each timed by cpu_timed; `cells_*` are the
+    cell-updates of one call.  value = cells / mean sample time."""
+    nt = cpu_threads()
+    a = cpu_timed(lambda: work_all(nt), budget_s)
+    o = cpu_timed(work_one, budget_s)
+    return {"value": round(cells_all / a["mean_s"] / 1e9, 5), "unit": "GCell-updates/s", "cores": nt,
+            "kind": "port", "sample": sample_all, "cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
+            "timing": a, "best_value": round(cells_all / a["min_s"] / 1e9, 5),
+            "one_core": {"value": round(cells_one / o["mean_s"] / 1e9, 5), "unit": "GCell-updates/s", "cores": 1,
+                         "sample": sample_one, "timing": o}}
 
 
 def measured_traffic(name, key):
@@ -156,26 +225,19 @@ def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup, autotune=True, tile=None, t
     }
 
 
-def cpu_baseline_diffusion(u0, budget_s=10.0):
-    import oracle
+def cpu_baseline_diffusion(u0, budget_s):
+    import oracle   # the CPU baseline leg only
     from cubismnova_amd.layout import global_to_blocks
     nb, n, N = (8, 8, 8), (32, 32, 32), 256
-    nt = max(1, min(16, os.cpu_count() or 1))
-    slab = global_to_blocks(u0, nb, n)
+    f = oracle.Field(global_to_blocks(u0, nb, n), nb, n)
     h = [1.0 / N] * 3
     dt = 0.1 / N / N
-    cur = slab
-    done = 0
-    t0 = time.perf_counter()
-    while True:
-        cur, _ = oracle.diffusion_step(oracle.Field(cur, nb, n), h, dt, nthreads=nt)
-        done += 1
-        el = time.perf_counter() - t0
-        if (el >= budget_s and done >= 2) or done >= 5000:
-            break
-    return {"value": round(N ** 3 * done / el / 1e9, 4), "unit": "GCell-updates/s", "cores": nt, "kind": "port",
-            "sample": "%d FP64 diffusion steps on the full 256^3 grid (C++ oracle: per-block lab + Laplacian + "
-                      "Euler, OpenMP over blocks with thread-local labs), %.1f s" % (done, el)}
+    return cpu_baseline(lambda nt: oracle.diffusion_step(f, h, dt, nthreads=nt), N ** 3,
+                        "one FP64 diffusion step of the full 256^3 grid per sample (C++ oracle: per-block lab + "
+                        "7-point Laplacian in the Laplacian.cpp order + two-rounding Euler + FP64 norm, OpenMP "
+                        "over blocks with thread-local labs)",
+                        lambda: oracle.diffusion_step(f, h, dt, nthreads=1), N ** 3,
+                        "the same step on one thread (the per-block Laplacian.cpp loop)", budget_s)
 
 
 def bench_diffusion(args):
@@ -190,6 +252,22 @@ def bench_diffusion(args):
     f32, f64 = r32["final"].astype(np.float64), r64["final"]
     rel = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
     nrel = float(np.max(np.abs(r32["norms"] - r64["norms"]) / r64["norms"]))
+
+    def rl(r, isz, key):
+        # one launch = two steps (one read of u^n, one write of u^{n+2}); the
+        # per-step model of 2*sizeof(T) per cell-update counts that pass once
+        # per step, so its rate is the launch rate times the steps per launch
+        out = roofline(r["bytes_per_launch"], r["avg_kernel_s"], measured_traffic("stencil7_diffusion_256.md", key),
+                       "%d B per cell per launch (u read once, u written once); a launch advances %.2f steps on "
+                       "average over the run (two-step launches, single steps for the remainder)"
+                       % (2 * isz, args.steps / r["launches"]))
+        per_step = 2 * isz * 256 ** 3 * args.steps / (r["avg_kernel_s"] * r["launches"]) / 1e9
+        out["per_step_model"] = {"bytes_per_cell_update": 2 * isz, "achieved": round(per_step, 1),
+                                 "frac": round(per_step / HBM_PEAK_GBS, 4),
+                                 "note": "counts one read and one write per step; above the peak only because "
+                                         "two steps share one pass over HBM"}
+        return out
+
     out = {
         "metric": METRIC, "value": round(r32["gcups"], 3), "unit": "GCell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 5),
@@ -201,12 +279,12 @@ def bench_diffusion(args):
                    "blocks": [8, 8, 8], "block_cells": [32, 32, 32], "steps_per_graph": args.steps,
                    "launches_per_graph": r32["launches"], "tile_f32": r32["tile"], "tile_f64": r64["tile"],
                    "tile2_f32": r32["tile2"], "tile2_f64": r64["tile2"]},
-        "roofline": roofline(r32["bytes_per_launch"], r32["avg_kernel_s"],
-                             measured_traffic("stencil7_diffusion_256.md", "f32")),
+        "roofline": rl(r32, 4, "f32"),
+        "kernel_ms_per_step": round(r32["event_ms"] / args.steps, 5),
         "fp64": {"value": round(r64["gcups"], 3), "unit": "GCell-updates/s",
                  "ms_per_step": round(r64["wall_s"] * 1e3 / args.steps, 5),
-                 "roofline": roofline(r64["bytes_per_launch"], r64["avg_kernel_s"],
-                                      measured_traffic("stencil7_diffusion_256.md", "f64"))},
+                 "kernel_ms_per_step": round(r64["event_ms"] / args.steps, 5),
+                 "roofline": rl(r64, 8, "f64")},
         "fp32_over_fp64": round(r32["gcups"] / r64["gcups"], 3),
         "fp32_rel_l2_vs_fp64": rel,
         "fp32_norm_history_max_rel_dev": nrel,
@@ -259,9 +337,8 @@ def weno_gpu(cnv, ev, dtype, steps, warmup, autotune, comps, tile=None):
             "tile": tune}
 
 
-def cpu_baseline_weno(grid64, budget_s=10.0):
-    import oracle
-    nt = max(1, min(16, os.cpu_count() or 1))
+def cpu_baseline_weno(grid64, budget_s):
+    import oracle   # the CPU baseline leg only
     nb, n = grid64.nblocks, grid64.block_cells
     cells_blk = int(np.prod(n))
     slab = grid64.download(0)
@@ -279,48 +356,48 @@ def cpu_baseline_weno(grid64, budget_s=10.0):
             lst.append((1, 2, 1, 0.0))
         bcl[b] = lst
     f = oracle.Field(slab, nb, n, grid64.block_stride, bcl)
-    done, t0 = 0, time.perf_counter()
-    chunk = 2 * nt
-    while True:
-        sel = [(done + j) % grid64.nblocks_total for j in range(chunk)]
-        oracle.weno5(f, prec=1, nthreads=nt, blocks=sel)
-        done += chunk
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-    # one cell-update = all 5 components: the sample ran one component
-    return {"value": round(done * cells_blk / 5 / el / 1e9, 5), "unit": "GCell-updates/s", "cores": nt,
-            "kind": "port",
-            "sample": "%d blocks of 32^3 of one component (x/y/z WENO5 states, FP64 C++ oracle with per-block "
-                      "labs, OpenMP over blocks) in %.1f s, scaled to 5 components" % (done, el)}
+    nt = cpu_threads()
+    # spread the sampled blocks over the grid (corner, face and interior blocks)
+    sel_all = [(j * 997) % grid64.nblocks_total for j in range(4 * nt)]
+    sel_one = sel_all[:4]
+    # one cell-update = all 5 components; a sample runs one component
+    return cpu_baseline(lambda t: oracle.weno5(f, prec=1, nthreads=t, blocks=sel_all), len(sel_all) * cells_blk / 5,
+                        "x/y/z WENO5 states of %d blocks of 32^3 of one component per sample (FP64 C++ oracle, "
+                        "per-block labs, OpenMP over blocks), counted as 1/5 of a cell-update per cell"
+                        % len(sel_all),
+                        lambda: oracle.weno5(f, prec=1, nthreads=1, blocks=sel_one), len(sel_one) * cells_blk / 5,
+                        "the same on 4 blocks, one thread", budget_s)
 
 
-def bench_weno5(args):
+def bench_weno5(args, steps, warmup):
     import cubismnova_amd as cnv
     from cubismnova_amd import workloads
     ev = HipEvents()
     comps = [workloads.weno_component(512, c) for c in range(5)]
-    r32 = weno_gpu(cnv, ev, np.float32, args.steps, args.warmup, not args.no_autotune, comps, args.tile_f32)
-    r64 = weno_gpu(cnv, ev, np.float64, args.steps, args.warmup, not args.no_autotune, comps, args.tile_f64)
+    r32 = weno_gpu(cnv, ev, np.float32, steps, warmup, not args.no_autotune, comps, args.tile_f32)
+    r64 = weno_gpu(cnv, ev, np.float64, steps, warmup, not args.no_autotune, comps, args.tile_f64)
     del comps
     worst = 0.0
     for k in range(30):
         d2, b2, _ = cnv.grid_compare(r32["out"], r64["out"], k, k)
         worst = max(worst, float(np.sqrt(d2 / b2)))
+    model = "%d B per cell per launch: 5 components read once, 30 face states (2 per direction and component) " \
+            "written once"
     out = {
-        "metric": METRIC, "value": round(r32["gcups"], 3), "unit": "GCell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (smooth compressible state + tanh pressure front, ratio 10, seeded)",
+        "metric": METRIC, "value": round(r32["gcups"], 3), "unit": "GCell-updates/s",
+        "steps": steps, "warmup": warmup, "ms_per_step": round(r32["wall_s"] * 1e3 / steps, 4),
+        "kernel_ms_per_step": round(r32["avg_kernel_s"] * 1e3, 4),
+        "dtype": "f32", "data": "synthetic (smooth compressible state + tanh pressure front, ratio 10, seeded)",
         "config": {"workload": "WENO5 face states in x, y, z: 16x16x16 blocks of 32^3 (512^3), 5 components SoA, "
                                "ghost 3, x periodic, y/z absorbing", "blocks": [16, 16, 16],
                    "block_cells": [32, 32, 32], "components": 5, "tile_f32": r32["tile"], "tile_f64": r64["tile"]},
-        "roofline": roofline(r32["bytes_per_launch"], r32["avg_kernel_s"],
-                             measured_traffic("weno5_512.md", "f32")),
+        "roofline": roofline(r32["bytes_per_launch"], r32["avg_kernel_s"], measured_traffic("weno5_512.md", "f32"),
+                             model % 140),
         "fp64": {"value": round(r64["gcups"], 3), "unit": "GCell-updates/s",
-                 "ms_per_step": round(r64["wall_s"] * 1e3 / args.steps, 4),
+                 "ms_per_step": round(r64["wall_s"] * 1e3 / steps, 4),
+                 "kernel_ms_per_step": round(r64["avg_kernel_s"] * 1e3, 4),
                  "roofline": roofline(r64["bytes_per_launch"], r64["avg_kernel_s"],
-                                      measured_traffic("weno5_512.md", "f64"))},
+                                      measured_traffic("weno5_512.md", "f64"), model % 280)},
         "fp32_over_fp64": round(r32["gcups"] / r64["gcups"], 3),
         "fp32_rel_l2_vs_fp64_worst_state": worst,
     }
@@ -341,7 +418,7 @@ def jacobi_grids(cnv, N, nb, n, dtype, zf, ncomp=1):
     return g
 
 
-def bench_jacobi(args):
+def bench_jacobi(args, steps, warmup):
     import cubismnova_amd as cnv
     from cubismnova_amd import stretch, workloads
     ev = HipEvents()
@@ -357,10 +434,10 @@ def bench_jacobi(args):
     S = 50
     # warm: one refined cycle, then `steps` cycles timed (each = 50 FP32 sweeps
     # + FP64 correction + FP64 defect, one graph replay per cycle)
-    cnv.jacobi_solve(u, f, work, refine=True, sweeps_per_cycle=S, max_cycles=max(1, args.warmup), tol=0.0)
+    cnv.jacobi_solve(u, f, work, refine=True, sweeps_per_cycle=S, max_cycles=max(1, warmup), tol=0.0)
     ev.device_sync()
     t0 = time.perf_counter()
-    cyc, sw, rel, hist = cnv.jacobi_solve(u, f, work, refine=True, sweeps_per_cycle=S, max_cycles=args.steps,
+    cyc, sw, rel, hist = cnv.jacobi_solve(u, f, work, refine=True, sweeps_per_cycle=S, max_cycles=steps,
                                           tol=0.0)
     ev.device_sync()
     wall = time.perf_counter() - t0
@@ -411,18 +488,22 @@ def bench_jacobi(args):
                                      sweeps_per_cycle=S, max_cycles=20000, tol=1e-12)
     d2, b2, mx = cnv.grid_compare(us, u64)
     out = {
-        "metric": METRIC, "value": round(cells / sweep_s / 1e9, 3), "unit": "GCell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (3 seeded Gaussian sources, sigma 0.05, mean removed)",
+        "metric": METRIC, "value": round(cells / sweep_s / 1e9, 3), "unit": "GCell-updates/s",
+        "steps": steps, "warmup": warmup, "ms_per_step": round(wall * 1e3 / steps, 4),
+        "dtype": "f32", "data": "synthetic (3 seeded Gaussian sources, sigma 0.05, mean removed)",
         "config": {"workload": "iteratively refined weighted Jacobi (omega 2/3): 8x8x8 blocks of 64^3 (512^3), "
                                "tanh-stretched z (factor 2), Dirichlet 0; a step = one cycle of 50 FP32 sweeps "
                                "+ FP64 correction + FP64 defect", "sweeps_per_cycle": S,
                    "value_is": "FP32 sweep kernel GCell-updates/s (cells per sweep / sweep time)"},
+        "kernel_ms_per_sweep": round(sweep_s * 1e3, 4),
         "sweeps_per_s": round(sw / wall, 1), "cycle_gcups": round(cells * sw / wall / 1e9, 3),
-        "roofline": roofline(cells * 12, sweep_s, measured_traffic("jacobi_sweep_512.md", "f32")),
+        "roofline": roofline(cells * 12, sweep_s, measured_traffic("jacobi_sweep_512.md", "f32"),
+                             "12 B per cell per FP32 sweep: u and f read once, u' written once (the z metric "
+                             "is two 1-D arrays)"),
         "fp64": {"value": round(cells / sweep64_s / 1e9, 3), "unit": "GCell-updates/s",
-                 "roofline": roofline(cells * 24, sweep64_s, measured_traffic("jacobi_sweep_512.md", "f64"))},
+                 "kernel_ms_per_sweep": round(sweep64_s * 1e3, 4),
+                 "roofline": roofline(cells * 24, sweep64_s, measured_traffic("jacobi_sweep_512.md", "f64"),
+                                      "24 B per cell per FP64 sweep")},
         "fp32_over_fp64": round(sweep64_s / sweep_s, 3),
         "convergence_32cube": {"refined_sweeps": int(s1), "refined_rel_residual": r1, "fp64_sweeps": int(s2),
                                "fp64_rel_residual": r2,
@@ -432,44 +513,54 @@ def bench_jacobi(args):
     }
     if not args.no_cpu:
         import oracle   # the CPU baseline leg only
-        nt = max(1, min(16, os.cpu_count() or 1))
-        lists = {}
-        for b in range(512):
-            bi = (b % 8, (b // 8) % 8, b // 64)
-            lst = [(2, d, s, 0.0) for d in range(3) for s in (0, 1) if bi[d] == (0 if s == 0 else 7)]
-            if lst:
-                lists[b] = lst
-        fo = oracle.Field(u.download(0), nb, n, u.block_stride, lists)
+
+        def dirichlet_lists(nbk):
+            lists = {}
+            for b in range(nbk[0] * nbk[1] * nbk[2]):
+                bi = (b % nbk[0], (b // nbk[0]) % nbk[1], b // (nbk[0] * nbk[1]))
+                lst = [(2, d, sd, 0.0) for d in range(3) for sd in (0, 1) if bi[d] == (0 if sd == 0 else nbk[d] - 1)]
+                if lst:
+                    lists[b] = lst
+            return lists
+
+        fo = oracle.Field(u.download(0), nb, n, u.block_stride, dirichlet_lists(nb))
         fsl = f.download(0)
-        done, t0 = 0, time.perf_counter()
-        while True:
-            oracle.jacobi(fo, fsl, czm, czp, [1.0 / N] * 3, 2.0 / 3.0, 0, nthreads=nt)
-            done += 1
-            el = time.perf_counter() - t0
-            if el >= args.cpu_seconds:
-                break
-        out["cpu_baseline"] = {"value": round(cells * done / el / 1e9, 4), "unit": "GCell-updates/s", "cores": nt,
-                               "kind": "port",
-                               "sample": "%d FP64 Jacobi sweeps of the full 512^3 grid (C++ oracle, per-block "
-                                         "labs, OpenMP over blocks) in %.1f s" % (done, el)}
+        # one-core sample: a 4x4x4-block (256^3) grid of the same blocks,
+        # sources and z metric
+        nb1 = (4, 4, 4)
+        bs = u.block_stride
+        fo1 = oracle.Field(np.zeros(64 * bs), nb1, n, bs, dirichlet_lists(nb1))
+        fs1 = np.ascontiguousarray(fsl[:64 * bs])
+        out["cpu_baseline"] = cpu_baseline(
+            lambda t: oracle.jacobi(fo, fsl, czm, czp, [1.0 / N] * 3, 2.0 / 3.0, 0, nthreads=t), cells,
+            "one FP64 weighted-Jacobi sweep of the full 512^3 stretched grid per sample (C++ oracle, per-block "
+            "labs, OpenMP over blocks)",
+            lambda: oracle.jacobi(fo1, fs1, czm[:256], czp[:256], [1.0 / N] * 3, 2.0 / 3.0, 0, nthreads=1),
+            256 ** 3, "one FP64 sweep of a 4x4x4-block (256^3) grid of the same 64^3 blocks, one thread",
+            args.cpu_seconds)
     return out
 
 
 # ------------------------------------------------------------------ batch
 
 
-def bench_batch(args):
+def bench_batch(args, steps, warmup):
+    import concurrent.futures
     import cubismnova_amd as cnv
     from cubismnova_amd import workloads
     ev = HipEvents()
     specs = workloads.batch_specs(96, k=4)
+    # seeded fields of every grid, generated on host threads
+    with concurrent.futures.ThreadPoolExecutor(max_workers=cpu_threads()) as ex:
+        fields = list(ex.map(lambda sc: [workloads.batch_field(sc[0], c).astype(np.float32)
+                                         for c in range(sc[0]["ncomp"])], [(sp,) for sp in specs]))
     grids, items = [], []
     cells_total, bytes_total = 0, 0
-    for s in specs:
+    for s, fl in zip(specs, fields):
         nb, e = s["nblocks"], s["edge"]
         g = cnv.Cartesian(nb, e, dtype=np.float32, ncomp=s["ncomp"])
         for c in range(s["ncomp"]):
-            g.upload_global(c, workloads.batch_field(s, c).astype(np.float32))
+            g.upload_global(c, fl[c])
         for kind, d, side, v in s["bcs"]:
             g.addFaceBC(cnv.Dirichlet(d, side, v) if kind == "dirichlet" else cnv.Absorbing(d, side))
         nout = 6 * s["ncomp"] if s["op"] == "weno5" else s["ncomp"]
@@ -485,12 +576,13 @@ def bench_batch(args):
     if table:
         print("autotune batch (op/edge/wg/depth): " + " ".join("%d/%d/%d/%d:%.0fus" % t for t in table),
               file=sys.stderr, flush=True)
+    del fields
     st = cnv.Stream()
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         b.run(stream=st)
     st.synchronize()
     gr = cnv.Graph(st).capture()
-    for _ in range(args.steps):
+    for _ in range(steps):
         b.run(stream=st)
     gr.end()
     e0, e1 = ev.new(), ev.new()
@@ -501,7 +593,7 @@ def bench_batch(args):
     ev.record(e1, st)
     st.synchronize()
     wall = time.perf_counter() - t0
-    kern_s = ev.elapsed_ms(e0, e1) / 1e3 / args.steps
+    kern_s = ev.elapsed_ms(e0, e1) / 1e3 / steps
     info = b.info()
     # after the timed replay: every grid against its single-grid entry point
     # (diffusion bit for bit, WENO5 states within 1e-6 of their largest value)
@@ -521,10 +613,10 @@ def bench_batch(args):
                 worst = 1.0
         del ref
     out = {
-        "metric": METRIC, "value": round(cells_total * args.steps / wall / 1e9, 3), "unit": "GCell-updates/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Fourier-mode fields per grid)",
+        "metric": METRIC, "value": round(cells_total * steps / wall / 1e9, 3), "unit": "GCell-updates/s",
+        "steps": steps, "warmup": warmup, "ms_per_step": round(wall * 1e3 / steps, 4),
+        "kernel_ms_per_step": round(kern_s * 1e3, 4),
+        "dtype": "f32", "data": "synthetic (seeded Fourier-mode fields per grid)",
         "config": {"workload": "96 independent grids (blocks 1x1x1..8x8x4, edges {8,16,32}, 1-3 components, "
                                "ghost 1 or 3, per-face periodic/Dirichlet/absorbing), each a diffusion step or "
                                "WENO5; per step one launch per (operator, edge) class of 16^3/32^3 blocks and one "
@@ -532,13 +624,13 @@ def bench_batch(args):
                    "grids": len(specs), "cells": cells_total, "work_items": info["items"],
                    "classes": [dict(zip(("op", "edge", "wg", "zdepth", "items"), c)) for c in b.classes()],
                    "persistent_wg": info["wg"], "persistent_lds_bytes": info["lds_bytes"]},
-        "roofline": roofline(bytes_total, kern_s, measured_traffic("batch_96.md", "batch")),
+        "roofline": roofline(bytes_total, kern_s, measured_traffic("batch_96.md", "batch"),
+                             "sum over grids of cells x components x (8 B for a diffusion step: one read, one "
+                             "write; 28 B for WENO5: one read, six states written)"),
         "batch_vs_single_grid_max_rel": worst,
     }
     if not args.no_cpu:
         import oracle   # the CPU baseline leg only
-        nt = max(1, min(16, os.cpu_count() or 1))
-        done_cells, t0 = 0, time.perf_counter()
         host = []   # the oracle's inputs, staged before the clock starts
         for g, o, s in grids:
             nb, e = s["nblocks"], s["edge"]
@@ -550,22 +642,24 @@ def bench_batch(args):
                         lists.setdefault(bb, []).append((2 if kind == "dirichlet" else 1, d, side, v))
             host.append((s, g.nblocks_total * e ** 3,
                          [oracle.Field(g.download(c), nb, e, g.block_stride, lists) for c in range(s["ncomp"])]))
-        done_cells, passes, t0 = 0, 0, time.perf_counter()
-        while time.perf_counter() - t0 < args.cpu_seconds:   # repeat the whole batch until the budget
-            for s, cells, fields in host:
+
+        def one_pass(sel, nt):
+            for s, cells, fl in sel:
                 nb, e = s["nblocks"], s["edge"]
-                for f in fields:
+                for fo in fl:
                     if s["op"] == "diffusion":
-                        oracle.diffusion_step(f, [1.0 / (nb[d] * e) for d in range(3)], 1e-6, nthreads=nt)
+                        oracle.diffusion_step(fo, [1.0 / (nb[d] * e) for d in range(3)], 1e-6, nthreads=nt)
                     else:
-                        oracle.weno5(f, prec=1, nthreads=nt)
-                done_cells += cells
-            passes += 1
-        el = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(done_cells / el / 1e9, 5), "unit": "GCell-updates/s", "cores": nt,
-                               "kind": "port",
-                               "sample": "%d full passes over the 96 grids (%d cells each pass), FP64 C++ oracle, "
-                                         "OpenMP over blocks, %.1f s" % (passes, cells_total, el)}
+                        oracle.weno5(fo, prec=1, nthreads=nt)
+
+        sub = host[::8]
+        out["cpu_baseline"] = cpu_baseline(
+            lambda t: one_pass(host, t), cells_total,
+            "one full pass over the 96 grids per sample (%d cells; FP64 C++ oracle, OpenMP over blocks)"
+            % cells_total,
+            lambda: one_pass(sub, 1), sum(c for _, c, _ in sub),
+            "a pass over every 8th grid (%d grids, %d cells), one thread" % (len(sub), sum(c for _, c, _ in sub)),
+            args.cpu_seconds)
     return out
 
 
@@ -601,20 +695,22 @@ def bench_copy(args):
 
 def copy_ceiling():
     """Practical streaming ceiling of this GPU: a 1 GiB device-to-device copy
-    (past the last-level cache), GB/s of bytes read + written."""
+    (past the last-level cache) by the library's non-temporal vector copy
+    kernel, GB/s of bytes read + written."""
     import cubismnova_amd as cnv
     from cubismnova_amd import _abi
     ev = HipEvents()
     nbytes = 1 << 30
     a, b = cnv.DeviceBuffer(nbytes), cnv.DeviceBuffer(nbytes)
+    a.fill(np.float32, 1.0)
     st = cnv.Stream()
     for _ in range(2):
-        _abi.call("cnv_memcpy_d2d", b.ptr, a.ptr, nbytes, st.handle)
+        _abi.call("cnv_memcpy_stream", b.ptr, a.ptr, nbytes, st.handle)
     st.synchronize()
     e0, e1 = ev.new(), ev.new()
     ev.record(e0, st)
     for _ in range(5):
-        _abi.call("cnv_memcpy_d2d", b.ptr, a.ptr, nbytes, st.handle)
+        _abi.call("cnv_memcpy_stream", b.ptr, a.ptr, nbytes, st.handle)
     ev.record(e1, st)
     return 2 * nbytes / (ev.elapsed_ms(e0, e1) / 1e3 / 5) / 1e9
 
@@ -634,7 +730,24 @@ def result_check(out):
     cv = out.get("convergence_32cube")
     if cv and not (cv["refined_rel_residual"] <= 1e-12 and cv["refined_vs_fp64_rel_l2"] <= 1e-9):
         return "refined Jacobi residual %g, vs FP64 %g" % (cv["refined_rel_residual"], cv["refined_vs_fp64_rel_l2"])
+    for name, rec in out.get("workloads", {}).items():
+        bad = result_check(rec)
+        if bad:
+            return name + ": " + bad
     return ""
+
+
+# steps and warmup of the workloads that run beside the headline diffusion
+SUB_WORKLOADS = (("weno5", 5, 1), ("jacobi", 3, 1), ("batch", 20, 2))
+
+
+def line_fields(rec, steps, warmup):
+    """The fields every printed line carries, for a workload run on its own."""
+    out = {"metric": rec["metric"], "value": rec["value"], "unit": rec["unit"], "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": rec["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None}
+    out.update({k: v for k, v in rec.items() if k not in out})
+    return out
 
 
 def main():
@@ -642,9 +755,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="diffusion", choices=["diffusion", "weno5", "batch", "jacobi", "copy"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--workload", default="all", choices=["all", "diffusion", "weno5", "batch", "jacobi", "copy"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--cpu-seconds", type=float, default=3.0,
+                    help="budget of each CPU baseline sample run (at least 10 samples are taken)")
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in tile table")
     tile = lambda v: tuple(int(x) for x in v.split(","))
     ap.add_argument("--tile-f32", type=tile, default=None, help="WG,ZCHUNK for the FP32 kernel of the diffusion or WENO5 workload (skips autotune)")
@@ -660,8 +774,20 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
-    out = {"weno5": bench_weno5, "batch": bench_batch, "jacobi": bench_jacobi,
-           "copy": bench_copy}.get(args.workload, bench_diffusion)(args)
+    fns = {"weno5": bench_weno5, "batch": bench_batch, "jacobi": bench_jacobi}
+    if args.workload == "copy":
+        out = bench_copy(args)
+    elif args.workload in fns:
+        out = line_fields(fns[args.workload](args, args.steps, args.warmup), args.steps, args.warmup)
+    else:
+        out = bench_diffusion(args)
+        print("diffusion done: %.1f GCUPS FP32" % out["value"], file=sys.stderr, flush=True)
+        if args.workload == "all":
+            out["workloads"] = {}
+            for name, k, w in SUB_WORKLOADS:
+                out["workloads"][name] = fns[name](args, k, w)
+                print("%s done: %.1f GCUPS FP32" % (name, out["workloads"][name]["value"]), file=sys.stderr,
+                      flush=True)
     bad = result_check(out)
     if bad:
         # a fast line over wrong results is not a measurement
@@ -669,8 +795,12 @@ def main():
         sys.exit("bench.py: results outside the stated tolerance: " + bad)
     if args.workload != "copy":
         c = copy_ceiling()
-        out["hbm_ceiling"] = {"copy_1GiB_GB_s": round(c, 1), "frac_of_peak": round(c / HBM_PEAK_GBS, 4),
-                              "kernel_frac_of_copy": round(out["roofline"]["achieved"] / c, 4)}
+        out["hbm_ceiling"] = {"copy_nt_1GiB_GB_s": round(c, 1), "frac_of_peak": round(c / HBM_PEAK_GBS, 4),
+                              "kernel_frac_of_copy": round(out["roofline"]["achieved"] / c, 4),
+                              "copy": "1 GiB device-to-device, non-temporal 16-byte loads and stores "
+                                      "(cnv_memcpy_stream)"}
+        for rec in out.get("workloads", {}).values():
+            rec["roofline"]["frac_of_copy"] = round(rec["roofline"]["achieved"] / c, 4)
     print(json.dumps(out), flush=True)
 
 

@@ -238,11 +238,52 @@ static int same_layout(const cnv_grid_s *x, const cnv_grid_s *y)
     return CNV_OK;
 }
 
+// Streaming copy of n16 16-byte words: non-temporal loads and stores, four
+// words in flight per thread.  The practical read + write ceiling of HBM
+// that the operator kernels are compared with.
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy_nt(const u4v *__restrict__ src, u4v *__restrict__ dst, size_t n16)
+{
+    const size_t stride = (size_t)gridDim.x * 1024;
+    for (size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x; base < n16; base += stride) {
+        u4v v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t i = base + (size_t)j * 256;
+            if (i < n16)
+                v[j] = __builtin_nontemporal_load(src + i);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t i = base + (size_t)j * 256;
+            if (i < n16)
+                __builtin_nontemporal_store(v[j], dst + i);
+        }
+    }
+}
+
 } // namespace cnv
 
 using namespace cnv;
 
 extern "C" {
+
+int cnv_memcpy_stream(void *dst, const void *src, size_t bytes, cnv_stream s)
+{
+    if (bytes == 0)
+        return CNV_OK;
+    if (!dst || !src)
+        return fail(CNV_ERR_INVALID, "copy: NULL pointer");
+    if (((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes) & 15) != 0)
+        return fail(CNV_ERR_INVALID, "copy: pointers and size must be multiples of 16 bytes");
+    const size_t n16 = bytes / 16;
+    const size_t want = (n16 + 1023) / 1024;
+    const unsigned nwg = (unsigned)(want < 8192 ? want : 8192);
+    hipLaunchKernelGGL(k_copy_nt, dim3(nwg), dim3(256), 0, (hipStream_t)s, static_cast<const u4v *>(src),
+                       static_cast<u4v *>(dst), n16);
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
 
 int cnv_field_op2(int op, int dtype, const void *a, const void *b, void *dst, size_t n, cnv_stream s)
 {

@@ -5,7 +5,9 @@ x = begin + (i + 1/2) h (Mesh/StructuredUniform.h:361-365); callers cast to
 the grid dtype and upload with Cartesian.upload_global.  The seed of
 workload k is 0x5EED0000 + k.
 """
+import concurrent.futures
 import math
+import os
 
 import numpy as np
 
@@ -178,17 +180,26 @@ def weno_component(N, comp, k=3):
     x = cell_centres(N)
     h = 1.0 / N
     out = np.empty((N, N, N), dtype=np.float64)
-    for iz in range(N):
-        zc = x[iz]
-        rho = 1.0 + _modes_plane(rho_m, x, zc, 0.2)
-        if comp == 0:
-            out[iz] = rho
-            continue
-        vel = [_modes_plane(vel_m[d], x, zc, 0.1) for d in range(3)]
-        if comp in (1, 2, 3):
-            out[iz] = rho * vel[comp - 1]
-            continue
-        s = nrm[0] * (x[None, :] - 0.5) + nrm[1] * (x[:, None] - 0.5) + nrm[2] * (zc - 0.5)
-        p = 1.0 + 9.0 * 0.5 * (1.0 + np.tanh(s / (2.0 * h)))
-        out[iz] = p / (GAMMA - 1.0) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+
+    def planes(iz0, iz1):
+        for iz in range(iz0, iz1):
+            zc = x[iz]
+            rho = 1.0 + _modes_plane(rho_m, x, zc, 0.2)
+            if comp == 0:
+                out[iz] = rho
+                continue
+            vel = [_modes_plane(vel_m[d], x, zc, 0.1) for d in range(3)]
+            if comp in (1, 2, 3):
+                out[iz] = rho * vel[comp - 1]
+                continue
+            s = nrm[0] * (x[None, :] - 0.5) + nrm[1] * (x[:, None] - 0.5) + nrm[2] * (zc - 0.5)
+            p = 1.0 + 9.0 * 0.5 * (1.0 + np.tanh(s / (2.0 * h)))
+            out[iz] = p / (GAMMA - 1.0) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+
+    # planes are independent: chunks on threads (numpy releases the GIL in
+    # the array arithmetic); every plane is computed the same way either way
+    nth = max(1, min(16, len(os.sched_getaffinity(0)), N // 16))
+    step = (N + nth - 1) // nth
+    with concurrent.futures.ThreadPoolExecutor(max_workers=nth) as ex:
+        list(ex.map(lambda i0: planes(i0, min(N, i0 + step)), range(0, N, step)))
     return out

@@ -110,6 +110,10 @@ int cnv_memcpy_h2d(void *dst, const void *src, size_t bytes, cnv_stream s);
 int cnv_memcpy_d2h(void *dst, const void *src, size_t bytes, cnv_stream s);
 int cnv_memcpy_d2d(void *dst, const void *src, size_t bytes, cnv_stream s);
 int cnv_fill(void *dst, int dtype, double value, size_t count, cnv_stream s);
+/* No counterpart in the reference: device-to-device copy with non-temporal
+ * 16-byte loads and stores (dst, src and bytes multiples of 16), the
+ * streaming read + write ceiling the stencil kernels are measured against. */
+int cnv_memcpy_stream(void *dst, const void *src, size_t bytes, cnv_stream s);
 int cnv_stream_create(cnv_stream *s);
 int cnv_stream_destroy(cnv_stream s);
 int cnv_stream_synchronize(cnv_stream s);
