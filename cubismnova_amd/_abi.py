@@ -103,6 +103,7 @@ SIGNATURES = {
     "cnv_lab_load_all": (i32, [vp, i32, ctypes.POINTER(cnv_stencil), vp, vp]),
     "cnv_laplacian": (i32, [vp, i32, vp, i32, vp]),
     "cnv_diffusion_step": (i32, [vp, vp, f64, vp, vp, vp]),
+    "cnv_diffusion_step_noalloc": (i32, [ctypes.POINTER(cnv_grid_desc), vp, vp, f64, vp, vp]),
     "cnv_diffusion_run": (i32, [vp, vp, i32, f64, vp, vp]),
     "cnv_grid_sums": (i32, [vp, i32, vp, vp]),
     "cnv_autotune_stencil7": (i32, [vp, vp, i32, c_int_p, c_int_p, vp, i32, c_int_p]),

@@ -121,6 +121,47 @@ static int check_cell(const cnv_grid_s *g)
     return CNV_OK;
 }
 
+// Checks a descriptor and fills the layout fields of g (desc, element size,
+// block and component bytes, block count, slab count, mesh spacing); no
+// allocation.
+int grid_layout(const cnv_grid_desc *desc, cnv_grid_s *g)
+{
+    if (!desc)
+        return fail(CNV_ERR_INVALID, "descriptor is NULL");
+    for (int d = 0; d < 3; ++d) {
+        if (desc->nblocks[d] < 1 || desc->block_cells[d] < 1)
+            return fail(CNV_ERR_INVALID, "nblocks and block_cells must be >= 1");
+        if (!(desc->end[d] > desc->begin[d]))
+            return fail(CNV_ERR_INVALID, "mesh end must exceed begin");
+    }
+    if (desc->ncomp < 1)
+        return fail(CNV_ERR_INVALID, "ncomp must be >= 1");
+    if (desc->dtype != CNV_F32 && desc->dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    if (desc->entity != CNV_CELL && desc->entity != CNV_NODE && desc->entity != CNV_FACE)
+        return fail(CNV_ERR_INVALID, "entity must be CNV_CELL, CNV_NODE or CNV_FACE");
+    const long long cells = (long long)desc->block_cells[0] * desc->block_cells[1] * desc->block_cells[2];
+    // Cartesian::alloc_ (Grid/Cartesian.h:490-503): (N+1)^3 elements per
+    // block for node and face data, 3 face slabs per component
+    const long long elements = desc->entity == CNV_CELL
+                                   ? cells
+                                   : (long long)(desc->block_cells[0] + 1) * (desc->block_cells[1] + 1) *
+                                         (desc->block_cells[2] + 1);
+    const long long nbt = (long long)desc->nblocks[0] * desc->nblocks[1] * desc->nblocks[2];
+    if (cells > (1LL << 30) || nbt > (1LL << 30))
+        return fail(CNV_ERR_INVALID, "grid too large");
+    g->desc = *desc;
+    g->elem = desc->dtype == CNV_F64 ? 8 : 4;
+    // Cartesian::alloc_ (Grid/Cartesian.h:490-529): 32-byte aligned blocks
+    g->block_bytes = ((size_t)elements * g->elem + 31) / 32 * 32;
+    g->nblocks_total = nbt;
+    g->nslabs = desc->entity == CNV_FACE ? 3 * desc->ncomp : desc->ncomp;
+    g->component_bytes = g->block_bytes * (size_t)nbt;
+    for (int d = 0; d < 3; ++d)
+        g->h[d] = (desc->end[d] - desc->begin[d]) / (double)(desc->nblocks[d] * desc->block_cells[d]);
+    return CNV_OK;
+}
+
 } // namespace cnv
 
 using namespace cnv;
@@ -265,33 +306,12 @@ static int grid_make(const cnv_grid_desc *desc, void *data, size_t data_bytes, c
     if (!desc || !out)
         return fail(CNV_ERR_INVALID, "descriptor or output handle is NULL");
     *out = nullptr;
-    for (int d = 0; d < 3; ++d) {
-        if (desc->nblocks[d] < 1 || desc->block_cells[d] < 1)
-            return fail(CNV_ERR_INVALID, "nblocks and block_cells must be >= 1");
-        if (!(desc->end[d] > desc->begin[d]))
-            return fail(CNV_ERR_INVALID, "mesh end must exceed begin");
-    }
-    if (desc->ncomp < 1)
-        return fail(CNV_ERR_INVALID, "ncomp must be >= 1");
-    if (desc->dtype != CNV_F32 && desc->dtype != CNV_F64)
-        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
-    if (desc->entity != CNV_CELL && desc->entity != CNV_NODE && desc->entity != CNV_FACE)
-        return fail(CNV_ERR_INVALID, "entity must be CNV_CELL, CNV_NODE or CNV_FACE");
+    cnv_grid_s lay;
+    CNV_CHECK(grid_layout(desc, &lay));
     const long long cells = (long long)desc->block_cells[0] * desc->block_cells[1] * desc->block_cells[2];
-    // Cartesian::alloc_ (Grid/Cartesian.h:490-503): (N+1)^3 elements per
-    // block for node and face data, 3 face slabs per component
-    const long long elements = desc->entity == CNV_CELL
-                                   ? cells
-                                   : (long long)(desc->block_cells[0] + 1) * (desc->block_cells[1] + 1) *
-                                         (desc->block_cells[2] + 1);
-    const int nslabs = desc->entity == CNV_FACE ? 3 * desc->ncomp : desc->ncomp;
-    const long long nbt = (long long)desc->nblocks[0] * desc->nblocks[1] * desc->nblocks[2];
-    if (cells > (1LL << 30) || nbt > (1LL << 30))
-        return fail(CNV_ERR_INVALID, "grid too large");
-    const size_t elem = desc->dtype == CNV_F64 ? 8 : 4;
-    // Cartesian::alloc_ (Grid/Cartesian.h:490-529): 32-byte aligned blocks
-    const size_t block_bytes = ((size_t)elements * elem + 31) / 32 * 32;
-    const size_t all = block_bytes * (size_t)nbt * (size_t)nslabs;
+    const long long nbt = lay.nblocks_total;
+    const int nslabs = lay.nslabs;
+    const size_t all = lay.component_bytes * (size_t)nslabs;
     if (data) {
         if ((reinterpret_cast<uintptr_t>(data) & 31) != 0)
             return fail(CNV_ERR_INVALID, "wrapped grid memory must be 32-byte aligned");
@@ -306,14 +326,7 @@ static int grid_make(const cnv_grid_desc *desc, void *data, size_t data_bytes, c
     }
 
     cnv_grid_s *g = new cnv_grid_s();
-    g->desc = *desc;
-    g->elem = elem;
-    g->block_bytes = block_bytes;
-    g->nblocks_total = nbt;
-    g->nslabs = nslabs;
-    g->component_bytes = block_bytes * (size_t)nbt;
-    for (int d = 0; d < 3; ++d)
-        g->h[d] = (desc->end[d] - desc->begin[d]) / (double)(desc->nblocks[d] * desc->block_cells[d]);
+    CNV_CHECK(grid_layout(desc, g));
     g->bcs.assign((size_t)nslabs, std::vector<std::vector<cnv_bc>>((size_t)nbt));
     g->tables.resize((size_t)nslabs);
     if (data) {
@@ -651,6 +664,20 @@ int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_ou
     }
     return launch_stencil7(u, u_new, lap_out, 0, 0, u->desc.ncomp, lap_out ? 2 : 1, nu_dt, sumsq,
                            (hipStream_t)s);
+    CNV_GUARD_END
+}
+
+int cnv_diffusion_step_noalloc(const cnv_grid_desc *desc, const void *u, void *u_new, double nu_dt, double *sumsq,
+                               cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    if (!u || !u_new)
+        return fail(CNV_ERR_INVALID, "u or u_new is NULL");
+    if (u == u_new)
+        return fail(CNV_ERR_INVALID, "u and u_new must be different memory");
+    if (((reinterpret_cast<uintptr_t>(u) | reinterpret_cast<uintptr_t>(u_new)) & 31) != 0)
+        return fail(CNV_ERR_INVALID, "grid memory must be 32-byte aligned");
+    return launch_diffusion_noalloc(desc, u, u_new, nu_dt, sumsq, (hipStream_t)s);
     CNV_GUARD_END
 }
 

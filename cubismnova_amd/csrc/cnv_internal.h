@@ -122,6 +122,8 @@ inline cnv::Geom geom_of(const cnv_grid_s *g)
     return G;
 }
 
+// Check a descriptor and fill the layout fields of g (no allocation).
+int grid_layout(const cnv_grid_desc *desc, cnv_grid_s *g);
 // Upload (if stale) the BC table of component comp; returns status.
 int ensure_bc_table(cnv_grid_s *g, int comp);
 // Make sure the reduction scratch holds at least n partials.
@@ -174,6 +176,11 @@ int launch_lab(const cnv_grid_s *g, int comp, const cnv_stencil &s, bool apply_b
 
 int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
                     int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st);
+
+// cnv_diffusion_step on caller memory of a periodic cell grid, norm
+// partials in the library's static device array (cnv_stencil7.hip)
+int launch_diffusion_noalloc(const cnv_grid_desc *desc, const void *u, void *u_new, double nu_dt, double *sumsq,
+                             hipStream_t st);
 
 int launch_sums(cnv_grid_s *g, int comp, double *out, hipStream_t st);
 int launch_diffusion_run(cnv_grid_s *u, cnv_grid_s *tmp, int nsteps, double nu_dt, double *norms, hipStream_t st);

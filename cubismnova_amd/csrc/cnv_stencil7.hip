@@ -285,9 +285,11 @@ static bool dispatch_spec(const S7Args &a, int edge, int wg, int ntiles, hipStre
     }
 }
 
+// cap: partials the buffer at a.partials holds (0: sized by the caller for
+// the largest launch)
 template <typename T, int MODE, bool SUMSQ>
 static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic, const S7Tune *over, bool finalize,
-                 int *nparts)
+                 int *nparts, long long cap = 0)
 {
     const int *n = a.G.n;
     const long long nbt = (long long)a.G.nb[0] * a.G.nb[1] * a.G.nb[2];
@@ -298,6 +300,8 @@ static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic, const 
         a.nzc = (n[2] + a.zchunk - 1) / a.zchunk;
         a.tiles_per_comp = (int)(nbt * a.nzc);
         const long long ntiles = (long long)a.tiles_per_comp * ncomp;
+        if (SUMSQ && cap > 0 && ntiles > cap)
+            return fail(CNV_ERR_SIZE, "grid needs more norm partials than the scratch array holds");
         if (ntiles < (1LL << 31)) {
             if (dispatch_spec<T, MODE, SUMSQ>(a, n[0], tu.wg, (int)ntiles, st)) {
                 CNV_HIP(hipGetLastError());
@@ -318,6 +322,8 @@ static int run_t(S7Args &a, int ncomp, hipStream_t st, int force_generic, const 
     const long long gy = nbt * ncomp;
     if (gy > 65535 || gx > 65535)
         return fail(CNV_ERR_UNSUPPORTED, "grid too large for the generic stencil kernel");
+    if (SUMSQ && cap > 0 && gx * gy > cap)
+        return fail(CNV_ERR_SIZE, "grid needs more norm partials than the scratch array holds");
     hipLaunchKernelGGL((k_stencil7_generic<T, MODE, SUMSQ>), dim3((unsigned)gx, (unsigned)gy), dim3(256),
                        0, st, a);
     CNV_HIP(hipGetLastError());
@@ -375,9 +381,15 @@ struct S7Run {
     size_t cap;                // capacity of each buffer half
 };
 
+// Partials of a launch that owns no grid scratch: `cap` doubles at `p`.
+struct S7Scratch {
+    double *p;
+    long long cap;
+};
+
 static int launch_stencil7_t(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
                              int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st,
-                             const S7Tune *over, const S7Run *run = nullptr)
+                             const S7Tune *over, const S7Run *run = nullptr, const S7Scratch *scr = nullptr)
 {
     if (ncomp > kMaxComp)
         return fail(CNV_ERR_UNSUPPORTED, "more than 8 components in one stencil call");
@@ -392,7 +404,12 @@ static int launch_stencil7_t(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, in
     a.comp_in = comp_in;
     a.comp_out = comp_out;
     a.ncomp = ncomp;
-    {
+    for (int c = 0; c < ncomp; ++c) {
+        a.has_bc[c] = 0;
+        a.heads[c] = nullptr;
+        a.entries[c] = nullptr;
+    }
+    if (!u->tables.empty()) { // a grid without tables is periodic on every face
         std::lock_guard<std::mutex> lk(u->mtx);
         for (int c = 0; c < ncomp; ++c) {
             int rc = ensure_bc_table(u, comp_in + c);
@@ -432,6 +449,8 @@ static int launch_stencil7_t(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, in
             if (need > (long long)run->cap)
                 return fail(CNV_ERR_SIZE, "partial buffer too small for this run");
             a.partials = run->partials;
+        } else if (scr) { // fixed scratch, checked against the launch's own tile count in run_t
+            a.partials = scr->p;
         } else {
             int rc = ensure_partials(u, (size_t)need);
             if (rc != CNV_OK)
@@ -443,6 +462,12 @@ static int launch_stencil7_t(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, in
     const bool fin = run == nullptr;
     int *np = run ? run->nparts : nullptr;
     const bool dbl = u->desc.dtype == CNV_F64;
+    if (scr) { // the one-step Euler update with its norm (the allocation-free entry point)
+        if (mode != 1 || !sumsq)
+            return fail(CNV_ERR_INVALID, "scratch partials serve the Euler step with a norm only");
+        return dbl ? run_t<double, 1, true>(a, ncomp, st, gen, over, true, nullptr, scr->cap)
+                   : run_t<float, 1, true>(a, ncomp, st, gen, over, true, nullptr, scr->cap);
+    }
     if (mode == 0)
         return dbl ? run_t<double, 0, false>(a, ncomp, st, gen, over, fin, np)
                    : run_t<float, 0, false>(a, ncomp, st, gen, over, fin, np);
@@ -464,6 +489,65 @@ int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in
                     int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st)
 {
     return launch_stencil7_t(u, out, lap, comp_in, comp_out, ncomp, mode, nu_dt, sumsq, st, nullptr);
+}
+
+// ------------------------------------------------ allocation-free Euler step
+//
+// The per-tile FP64 partials of the norm of cnv_diffusion_step_noalloc live
+// in `scratch`, a static device array (one copy per GPU, held until the
+// process ends).  Host code never passes the array's name to a kernel: on
+// the host it is a stand-in, not a device address.  The first call on a
+// device looks up that device's address of the array with
+// hipGetSymbolAddress, and every launch takes it from g_scratch_dev.  Each
+// launch writes every partial it reads before the finalize kernel reads it,
+// so nothing needs clearing between calls; calls that overlap on one GPU
+// share the array and race.
+constexpr long long kNoAllocPartials = 1LL << 17; // 1 MiB of doubles
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mtx;
+static double *g_scratch_dev[kMaxDevices] = {};
+
+static int scratch_partials(double **out)
+{
+    int dev = -1;
+    CNV_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(CNV_ERR_UNSUPPORTED, "device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mtx);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return CNV_OK;
+}
+
+int launch_diffusion_noalloc(const cnv_grid_desc *desc, const void *u, void *u_new, double nu_dt, double *sumsq,
+                             hipStream_t st)
+{
+    // grid objects on the stack over the caller's memory: layout fields
+    // only, no BC lists or tables (every face periodic), no partials
+    cnv_grid_s gu, go;
+    int rc = grid_layout(desc, &gu);
+    if (rc != CNV_OK)
+        return rc;
+    if (desc->entity != CNV_CELL)
+        return fail(CNV_ERR_UNSUPPORTED, "the stencil operators take cell grids only");
+    (void)grid_layout(desc, &go);
+    gu.data = const_cast<void *>(u);
+    go.data = u_new;
+    gu.owns_data = go.owns_data = false;
+    if (!sumsq)
+        return launch_stencil7_t(&gu, &go, nullptr, 0, 0, desc->ncomp, 1, nu_dt, nullptr, st, nullptr);
+    S7Scratch scr{nullptr, kNoAllocPartials};
+    rc = scratch_partials(&scr.p);
+    if (rc != CNV_OK)
+        return rc;
+    return launch_stencil7_t(&gu, &go, nullptr, 0, 0, desc->ncomp, 1, nu_dt, sumsq, st, nullptr, nullptr, &scr);
 }
 
 // nsteps Euler steps alternating u -> tmp -> u ...  On periodic grids of

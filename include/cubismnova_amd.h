@@ -206,6 +206,23 @@ int cnv_laplacian(cnv_grid in, int comp_in, cnv_grid out, int comp_out, cnv_stre
 int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_out, double *sumsq,
                        cnv_stream s);
 
+/* No counterpart in the reference: cnv_diffusion_step for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  u and u_new are caller-owned
+ * device memory (32-byte aligned, distinct, not overlapping) laid out as the
+ * cell grid `desc` describes, every face periodic (no BC lists); every
+ * component takes one Euler step, bit-identical to cnv_diffusion_step on a
+ * grid of that layout, and sumsq (device double, may be NULL) receives the
+ * same FP64 norm.  No grid handle is made and nothing is allocated: the
+ * norm's per-tile partials live in a static device array of 2^17 doubles
+ * (1 MiB, one per GPU, held until the process ends); CNV_ERR_SIZE when a
+ * launch needs more partials than that (one per tile: blocks x components x
+ * z-chunks of the tile shape in use, e.g. 16384 for 512^3 cells in 32^3
+ * blocks with the built-in 8-plane tiles).  Calls that overlap on one GPU
+ * share that array and race. */
+int cnv_diffusion_step_noalloc(const cnv_grid_desc *desc, const void *u, void *u_new, double nu_dt, double *sumsq,
+                               cnv_stream s);
+
 /* nsteps forward-Euler steps alternating u -> tmp -> u -> ... (the same
  * arithmetic, bit for bit, as nsteps calls of cnv_diffusion_step; the field
  * ends in u when nsteps is even, in tmp when odd).  tmp must carry the same
