@@ -69,6 +69,11 @@ int ensure_bc_table(cnv_grid_s *g, int comp)
             any = true;
         }
     }
+    if (t.heads || t.entries) {
+        // the table is rewritten (BC lists changed): kernels still queued on
+        // any stream may read it, so they finish first
+        CNV_HIP(hipDeviceSynchronize());
+    }
     if (!t.heads)
         CNV_HIP(hipMalloc((void **)&t.heads, sizeof(DevBCHead) * (size_t)nb));
     if ((int)entries.size() > t.nentries_cap) {
@@ -638,8 +643,8 @@ int cnv_laplacian(cnv_grid in, int comp_in, cnv_grid out, int comp_out, cnv_stre
     CNV_CHECK(check_comp(in, comp_in));
     CNV_CHECK(check_comp(out, comp_out));
     CNV_CHECK(same_shape(in, out));
-    if (in == out)
-        return fail(CNV_ERR_INVALID, "input and output grids must differ");
+    if (grids_overlap(in, out))
+        return fail(CNV_ERR_INVALID, "input and output grids must not share memory");
     return launch_stencil7(in, out, nullptr, comp_in, comp_out, 1, 0, 0.0, nullptr, (hipStream_t)s);
     CNV_GUARD_END
 }
@@ -653,13 +658,13 @@ int cnv_diffusion_step(cnv_grid u, cnv_grid u_new, double nu_dt, cnv_grid lap_ou
     CNV_CHECK(check_grid(u_new));
     CNV_CHECK(check_cell(u_new));
     CNV_CHECK(same_shape(u, u_new));
-    if (u == u_new)
-        return fail(CNV_ERR_INVALID, "u and u_new must be different grids");
+    if (grids_overlap(u, u_new))
+        return fail(CNV_ERR_INVALID, "u and u_new must not share memory");
     if (u_new->desc.ncomp < u->desc.ncomp)
         return fail(CNV_ERR_SIZE, "u_new has fewer components than u");
     if (lap_out) {
         CNV_CHECK(same_shape(u, lap_out));
-        if (lap_out->desc.ncomp < u->desc.ncomp || lap_out == u || lap_out == u_new)
+        if (lap_out->desc.ncomp < u->desc.ncomp || grids_overlap(lap_out, u) || grids_overlap(lap_out, u_new))
             return fail(CNV_ERR_INVALID, "lap_out must be a separate grid with enough components");
     }
     return launch_stencil7(u, u_new, lap_out, 0, 0, u->desc.ncomp, lap_out ? 2 : 1, nu_dt, sumsq,
@@ -689,8 +694,8 @@ int cnv_diffusion_run(cnv_grid u, cnv_grid tmp, int nsteps, double nu_dt, double
     CNV_CHECK(check_grid(tmp));
     CNV_CHECK(check_cell(tmp));
     CNV_CHECK(same_shape(u, tmp));
-    if (u == tmp)
-        return fail(CNV_ERR_INVALID, "u and tmp must be different grids");
+    if (grids_overlap(u, tmp))
+        return fail(CNV_ERR_INVALID, "u and tmp must not share memory");
     if (tmp->desc.ncomp < u->desc.ncomp)
         return fail(CNV_ERR_SIZE, "tmp has fewer components than u");
     if (nsteps < 0)
@@ -752,8 +757,8 @@ int cnv_weno5(cnv_grid in, cnv_grid out, cnv_stream s)
     CNV_CHECK(check_grid(out));
     CNV_CHECK(check_cell(out));
     CNV_CHECK(same_shape(in, out));
-    if (in == out)
-        return fail(CNV_ERR_INVALID, "input and output grids must differ");
+    if (grids_overlap(in, out))
+        return fail(CNV_ERR_INVALID, "input and output grids must not share memory");
     if (out->desc.ncomp < 6 * in->desc.ncomp)
         return fail(CNV_ERR_SIZE, "out needs 6 components per input component");
     return launch_weno5(in, out, (hipStream_t)s, nullptr);
@@ -769,7 +774,7 @@ int cnv_autotune_weno5(cnv_grid in, cnv_grid out, int iters, int *wg, int *zchun
     CNV_CHECK(check_grid(out));
     CNV_CHECK(check_cell(out));
     CNV_CHECK(same_shape(in, out));
-    if (in == out || out->desc.ncomp < 6 * in->desc.ncomp)
+    if (grids_overlap(in, out) || out->desc.ncomp < 6 * in->desc.ncomp)
         return fail(CNV_ERR_INVALID, "out must be a separate grid with 6 components per input component");
     return autotune_weno5(in, out, iters, wg, zchunk, results, max_results, nresults);
     CNV_GUARD_END
@@ -1044,8 +1049,8 @@ int cnv_autotune_stencil7(cnv_grid u, cnv_grid out, int iters, int *wg, int *zch
     CNV_CHECK(check_grid(out));
     CNV_CHECK(check_cell(out));
     CNV_CHECK(same_shape(u, out));
-    if (u == out)
-        return fail(CNV_ERR_INVALID, "autotune needs two different grids");
+    if (grids_overlap(u, out))
+        return fail(CNV_ERR_INVALID, "autotune needs two grids that do not share memory");
     if (out->desc.ncomp < u->desc.ncomp)
         return fail(CNV_ERR_SIZE, "out has fewer components than u");
     return autotune_stencil7(u, out, iters, wg, zchunk, results, max_results, nresults);
@@ -1061,8 +1066,8 @@ int cnv_autotune_diffusion_run(cnv_grid u, cnv_grid tmp, int iters, int *wg, int
     CNV_CHECK(check_grid(tmp));
     CNV_CHECK(check_cell(tmp));
     CNV_CHECK(same_shape(u, tmp));
-    if (u == tmp)
-        return fail(CNV_ERR_INVALID, "autotune needs two different grids");
+    if (grids_overlap(u, tmp))
+        return fail(CNV_ERR_INVALID, "autotune needs two grids that do not share memory");
     if (tmp->desc.ncomp < u->desc.ncomp)
         return fail(CNV_ERR_SIZE, "tmp has fewer components than u");
     if (iters < 1)

@@ -269,6 +269,8 @@ static void class_defaults(BClass &c)
 
 static int upload_items(BItem **d, size_t *cap, const std::vector<BItem> &v)
 {
+    if (*d) // a rebuild: launches still queued on any stream may read the old list
+        CNV_HIP(hipDeviceSynchronize());
     if (v.size() > *cap) {
         if (*d)
             CNV_HIP(hipFree(*d));
@@ -352,8 +354,10 @@ static int build_items(cnv_batch_s *b, const std::map<std::pair<int, int>, int> 
     std::stable_sort(b->items.begin(), b->items.end(),
                      [&](const BItem &x, const BItem &y) { return cost(x) > cost(y); });
     b->nitems = (int)b->items.size();
-    if (b->d_items)
+    if (b->d_items) { // a rebuild: launches still queued on any stream may read the old list
+        CNV_HIP(hipDeviceSynchronize());
         CNV_HIP(hipFree(b->d_items));
+    }
     b->d_items = nullptr;
     if (b->nitems > 0) {
         CNV_HIP(hipMalloc((void **)&b->d_items, sizeof(BItem) * b->items.size()));
@@ -436,8 +440,13 @@ static int build_grids(cnv_batch_s *b)
         d.nudt = (float)it.nu_dt;
         b->grids.push_back(d);
     }
-    if (!b->d_grids)
+    if (!b->d_grids) {
         CNV_HIP(hipMalloc((void **)&b->d_grids, sizeof(BGrid) * b->grids.size()));
+    } else {
+        // a rebuild (BC lists changed): launches still queued on the caller's
+        // or the batch's streams may read the old table
+        CNV_HIP(hipDeviceSynchronize());
+    }
     CNV_HIP(hipMemcpy(b->d_grids, b->grids.data(), sizeof(BGrid) * b->grids.size(), hipMemcpyHostToDevice));
     return CNV_OK;
 }
@@ -509,15 +518,16 @@ static int launch_batch(cnv_batch_s *b, hipStream_t st)
     return rc;
 }
 
-// Concurrent launches need every output grid distinct from every other
-// item's output and from every input.
+// Concurrent launches need the memory of every output grid apart from every
+// other item's output and from every input (handles wrapped over the same
+// memory count as one grid).
 static bool independent_items(const cnv_batch_s *b)
 {
     for (size_t i = 0; i < b->src.size(); ++i)
         for (size_t j = 0; j < b->src.size(); ++j) {
-            if (i != j && b->src[i].out == b->src[j].out)
+            if (i != j && grids_overlap(b->src[i].out, b->src[j].out))
                 return false;
-            if (b->src[i].out == b->src[j].in)
+            if (grids_overlap(b->src[i].out, b->src[j].in))
                 return false;
         }
     return true;
@@ -559,8 +569,8 @@ int cnv_batch_create(const cnv_batch_item *items, int n, cnv_batch *out)
         *out = nullptr;
         for (int i = 0; i < n; ++i) {
             const cnv_batch_item &it = items[i];
-            if (!it.in || !it.out || it.in == it.out)
-                return fail(CNV_ERR_INVALID, "batch item needs distinct input and output grids");
+            if (!it.in || !it.out || grids_overlap(it.in, it.out))
+                return fail(CNV_ERR_INVALID, "batch item needs input and output grids that do not share memory");
             if (it.in->desc.dtype != CNV_F32 || it.out->desc.dtype != CNV_F32)
                 return fail(CNV_ERR_UNSUPPORTED, "the batched kernel runs FP32 grids");
             if (it.in->desc.entity != CNV_CELL || it.out->desc.entity != CNV_CELL)

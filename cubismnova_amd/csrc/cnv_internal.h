@@ -110,6 +110,15 @@ inline int plus_mask(const cnv_grid_s *g, int slab)
     return 0;
 }
 
+// True when the device memory of two grids overlaps: handles wrapped over
+// the same (or overlapping) memory count as one grid.
+inline bool grids_overlap(const cnv_grid_s *a, const cnv_grid_s *b)
+{
+    const char *pa = static_cast<const char *>(a->data), *pb = static_cast<const char *>(b->data);
+    const size_t na = a->component_bytes * (size_t)a->nslabs, nb = b->component_bytes * (size_t)b->nslabs;
+    return pa < pb + nb && pb < pa + na;
+}
+
 inline cnv::Geom geom_of(const cnv_grid_s *g)
 {
     Geom G;

@@ -541,6 +541,8 @@ int launch_diffusion_noalloc(const cnv_grid_desc *desc, const void *u, void *u_n
     gu.data = const_cast<void *>(u);
     go.data = u_new;
     gu.owns_data = go.owns_data = false;
+    if (grids_overlap(&gu, &go))
+        return fail(CNV_ERR_INVALID, "u and u_new must not overlap");
     if (!sumsq)
         return launch_stencil7_t(&gu, &go, nullptr, 0, 0, desc->ncomp, 1, nu_dt, nullptr, st, nullptr);
     S7Scratch scr{nullptr, kNoAllocPartials};

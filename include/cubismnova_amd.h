@@ -459,7 +459,9 @@ int cnv_tune_get_weno5(int dtype, int edge, int *wg, int *zchunk);
  * finalize kernel, so no memset or counter node is ever recorded. */
 /* Does now what the first operator call on g would do synchronously (device
  * BC tables of every component, the z metric): call it before recording
- * calls on g into a graph. */
+ * calls on g into a graph.  A change of g's BC lists rebuilds its tables
+ * (after waiting for the device), which may move them: a graph recorded
+ * before the change must be recorded again. */
 int cnv_grid_prepare(cnv_grid g);
 
 typedef struct cnv_graph_s *cnv_graph;

@@ -99,3 +99,27 @@ def test_wrapped_memory_outlives_grid(gpu):
     assert (g.download(0) == 3.5).all()
     del g
     assert (buf.download(np.float32) == 3.5).all()
+
+
+def test_handles_over_shared_memory_count_as_one_grid(gpu):
+    """Two handles wrapped over the same (or overlapping) memory are one grid
+    for the operators' input/output checks and for a batch's items."""
+    nb, n = (2, 2, 2), (16, 16, 16)
+    own = cnv.Cartesian(nb, n, dtype=np.float32)
+    alias = cnv.Cartesian(nb, n, dtype=np.float32, device_data=own.data_ptr(0), device_bytes=own.component_bytes)
+    with pytest.raises(cnv.CnvError, match="share memory"):
+        cnv.diffusion_step(own, alias, 1e-4)
+    with pytest.raises(cnv.CnvError, match="share memory"):
+        cnv.laplacian(own, alias)
+    with pytest.raises(cnv.CnvError, match="share memory"):
+        cnv.Batch([(own, alias, "diffusion", 1, 1e-4)])
+    # a handle over the second half of another grid's memory overlaps it too
+    big = cnv.Cartesian((2, 2, 4), n, dtype=np.float32)
+    half = cnv.Cartesian(nb, n, dtype=np.float32, device_data=big.data_ptr(0).value + big.component_bytes // 2,
+                         device_bytes=big.component_bytes // 2)
+    with pytest.raises(cnv.CnvError, match="share memory"):
+        cnv.Batch([(half, big, "diffusion", 1, 1e-4)])
+    # apart: fine
+    other = cnv.Cartesian(nb, n, dtype=np.float32)
+    own.upload(0, random_slab(3, nb, n, np.float32))
+    cnv.diffusion_step(own, other, 1e-4)
