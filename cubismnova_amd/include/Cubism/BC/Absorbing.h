@@ -1,0 +1,5 @@
+// BC/Absorbing.h -- the drop-in's header under the reference's file name
+// (include/Cubism/BC/Absorbing.h): application code that includes it gets the
+// declarations of Cubism/AMD.h.
+#pragma once
+#include "Cubism/AMD.h"

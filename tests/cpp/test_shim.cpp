@@ -9,7 +9,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <limits>
+#include <string>
 
 using namespace Cubism;
 
@@ -244,6 +246,32 @@ static void laplacian_vs_host_loop()
                     EXPECT(res(x, y, z) == ddx + ddy + ddz);
                 }
     }
+    // the per-block loop moved no slab: the one explicit upload is all
+    EXPECT(u.uploadCount() == 1);
+    // a host edit is seen by the next lab load (one upload), and a device
+    // result by the next host read (no explicit transfer)
+    u[3](5, 6, 7) = 42.0;
+    u.loadLab(u[3], lab);
+    EXPECT(lab(5, 6, 7) == 42.0);
+    EXPECT(u.uploadCount() == 2);
+    u.loadLab(u[4], lab);
+    EXPECT(u.uploadCount() == 2);
+    Grid64 lap2({2, 2, 2}, {16, 16, 16});
+    Device::laplacian(u, lap2);
+    const Field64 &r3 = lap2[3];
+    EXPECT(r3(5, 6, 7) == ihx2 * (u[3](6, 6, 7) - 2 * 42.0 + u[3](4, 6, 7)) +
+                              ihy2 * (u[3](5, 7, 7) - 2 * 42.0 + u[3](5, 5, 7)) +
+                              ihz2 * (u[3](5, 6, 8) - 2 * 42.0 + u[3](5, 6, 6)));
+    // FieldLab::allocate keeps its buffers for a smaller request, takes new
+    // ones when forced or when the request grows (FieldLab.h:147-158), and
+    // reports the ranges of Block/FieldLab.h:403-456
+    lab.allocate(Stencil(-1, 2, false), MIndex{8, 8, 8});
+    EXPECT(lab.getMaximumRange().getExtent() == MIndex(8));
+    lab.allocate(Stencil(-3, 4, false), MIndex{16, 16, 16}, true);
+    u.loadLab(u[0], lab);
+    EXPECT(lab.getActiveRange().getExtent() == MIndex(16));
+    EXPECT(lab.getActiveLabRange().getBegin() == MIndex(-3) && lab.getActiveLabRange().getEnd() == MIndex(19));
+    EXPECT(lab(-3, 0, 0) == u[1](13, 0, 0) && lab(18, 0, 0) == u[1](2, 0, 0));
 }
 
 // Device::diffusionRun (two steps per launch on this periodic grid) against
@@ -508,8 +536,34 @@ static void batch_vs_single_grids()
     }
 }
 
-int main()
+// The grid, initial field and Laplacian of the reference's Laplacian
+// benchmark (test/sandbox/Benchmarks/Kernel/Laplacian/Laplacian.cpp, which
+// builds against this header unchanged) computed here with Device::laplacian
+// and written as laplacian_device.h5 (FP32), the file the benchmark writes
+// as laplacian.h5 from its per-block host loop over the labs.
+static int laplacian_device(const int nb)
 {
+    using Mesh = Mesh::StructuredUniform<double, 3>;
+    using ScalarGrid = Grid::Cartesian<double, Mesh, EntityType::Cell, 0>;
+    ScalarGrid u(Mesh::MultiIndex(nb), Mesh::MultiIndex(32));
+    ScalarGrid lap_u(Mesh::MultiIndex(nb), Mesh::MultiIndex(32));
+    for (auto bf : u) {
+        const Mesh &bm = *bf->getState().mesh;
+        for (auto &ci : bm[EntityType::Cell]) {
+            const Mesh::PointType x = bm.getCoordsCell(ci);
+            (*bf)[ci] = std::sin(2.0 * M_PI * x[0]) * std::cos(6.0 * M_PI * x[1]) * std::sin(4.0 * M_PI * x[2]);
+        }
+    }
+    Device::laplacian(u, lap_u);
+    IO::CartesianWriteHDF<float>("laplacian_device", "div(grad(U))", lap_u, 0);
+    std::printf("laplacian_device.h5 written (%d^3 blocks of 32^3)\n", nb);
+    return 0;
+}
+
+int main(int argc, char *argv[])
+{
+    if (argc == 3 && std::string(argv[1]) == "laplacian-device")
+        return laplacian_device(std::atoi(argv[2]));
     fieldlab_ghosts(false);
     fieldlab_ghosts(true);
     bc_tests();
