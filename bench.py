@@ -576,6 +576,10 @@ def bench_batch(args, steps, warmup):
     if table:
         print("autotune batch (op/edge/wg/depth): " + " ".join("%d/%d/%d/%d:%.0fus" % t for t in table),
               file=sys.stderr, flush=True)
+    # the two forms as the autotuner timed them (whole runs)
+    form_us = {("single_launch" if int(t[2]) == 1 else "class_launches"): round(t[4], 1) for t in table
+               if t[0] == -3}
+    one, nunits = b.form()
     del fields
     st = cnv.Stream()
     for _ in range(warmup):
@@ -619,8 +623,11 @@ def bench_batch(args, steps, warmup):
         "dtype": "f32", "data": "synthetic (seeded Fourier-mode fields per grid)",
         "config": {"workload": "96 independent grids (blocks 1x1x1..8x8x4, edges {8,16,32}, 1-3 components, "
                                "ghost 1 or 3, per-face periodic/Dirichlet/absorbing), each a diffusion step or "
-                               "WENO5; per step one launch per (operator, edge) class of 16^3/32^3 blocks and one "
-                               "persistent launch for the 8^3 blocks, HIP-graph replayed",
+                               "WENO5, HIP-graph replayed; the step runs in the form the autotuner timed faster: "
+                               "one persistent launch over a device list of work units, or one launch per "
+                               "(operator, edge) class of 16^3/32^3 blocks plus a persistent launch for the rest",
+                   "form": "single_launch" if one else "class_launches", "form_times_us": form_us,
+                   "units": nunits,
                    "grids": len(specs), "cells": cells_total, "work_items": info["items"],
                    "classes": [dict(zip(("op", "edge", "wg", "zdepth", "items"), c)) for c in b.classes()],
                    "persistent_wg": info["wg"], "persistent_lds_bytes": info["lds_bytes"]},

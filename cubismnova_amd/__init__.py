@@ -697,9 +697,10 @@ def autotune_weno5(u, out, iters=5):
 
 
 class Batch:
-    """Independent FP32 grids processed together: one launch per (operator,
-    edge) class of cubic 16^3 / 32^3 blocks, one persistent launch for the
-    rest.
+    """Independent FP32 grids processed together: one persistent launch over
+    a device list of work units (or, when the autotuner finds it faster, one
+    launch per (operator, edge) class of cubic 16^3 / 32^3 blocks plus one
+    persistent launch for the rest).
 
     items: list of (in_grid, out_grid, op, ghost, nu_dt) with op "diffusion"
     (out = in + nu_dt lap(in)) or "weno5" (6 states per component)."""
@@ -716,13 +717,20 @@ class Batch:
         self.handle = h
 
     def autotune(self, iters=5):
-        """Returns the trials as (op, edge, wg, depth, microseconds); op -1 is
-        the persistent launch."""
+        """Returns the trials as (op, edge, wg, depth, microseconds): op >= 0
+        a class launch, -1 the class form's persistent launch, -2 the single
+        launch, -3 each form as a whole (wg 1: single launch, 0: classes)."""
         res = np.zeros(5 * 512, dtype=np.float64)
         nr = ctypes.c_int()
         call("cnv_batch_autotune", self.handle, int(iters), res.ctypes.data_as(ctypes.c_void_p), 512,
              ctypes.byref(nr))
-        return [tuple(res[5 * i:5 * i + 5]) for i in range(nr.value)]
+        return [tuple(res[5 * i:5 * i + 5]) for i in range(min(nr.value, 512))]
+
+    def form(self):
+        """(one_launch, units): 1 when the batch runs as one persistent launch."""
+        one, nu = ctypes.c_int(), ctypes.c_int()
+        call("cnv_batch_form", self.handle, ctypes.byref(one), ctypes.byref(nu))
+        return one.value, nu.value
 
     def classes(self):
         """[(op, edge, wg, z depth, items)] of the per-class launches."""

@@ -1,24 +1,25 @@
 // cnv_batch.hip -- many independent grids of uneven block counts, block
 // sizes, component counts, ghost widths and BCs processed together.
 //
-// A work item is (grid, component, block, z-range).  Items are grouped by
-// class:
-//  * (operator, block edge) for cubic FP32 blocks of edge 16 or 32: one
-//    launch per class runs the same z-marching tile bodies as the single-grid
-//    kernels (s7_march, w5_march), one workgroup per item, with the
-//    workgroup size and z depth of that class chosen by cnv_batch_autotune;
-//  * everything else (other block shapes): one persistent launch that walks
-//    a device work list sorted by cost (largest first).  The workgroup stages
-//    the item's cells plus the face halo its operator reads (one layer for
-//    the 7-point Euler step, two for WENO5) in LDS, evaluating every ghost
-//    with the same lab rules as cnv_lab_load, then applies the operator.
+// A work item is (grid, component, block, z-range).  The batch runs in one
+// of two forms, chosen by cnv_batch_autotune from their measured times (the
+// single launch until then):
+//  * one persistent launch (k_batch_one) over a device list of work units,
+//    largest first: a unit is one item of a cubic 32^3 block run by the
+//    whole workgroup on the single-grid tile body of its operator (s7_march,
+//    w5_march), or up to WG/64 items of cubic 16^3 blocks, one per 64-thread
+//    slice on the same tile bodies, or one item of any other block shape,
+//    staged in LDS with its face halo (one layer for the 7-point Euler step,
+//    two for WENO5) by the generic body;
+//  * one launch per (operator, edge) class of 16^3/32^3 blocks (k_bmarch,
+//    one workgroup per item on the same tile bodies) plus the persistent
+//    generic launch (k_batch) for the other blocks, forked onto streams of
+//    the batch when the items are independent.
 // Operators: op 0: u_new = u + nu_dt * lap(u) in the Laplacian.cpp:125-129
 // order; op 1: WENO5 u^-_{i+1/2}, u^+_{i-1/2} in x, y, z (outputs 6c+2d,
 // 6c+2d+1).  Ghosts follow the grid's Stencil(-g, g+1) and BC lists.
-// FP32 grids only.  When no item's output is another item's input or output,
-// the launches fork from the caller's stream onto streams of the batch and
-// join back into it, so they overlap; either way a batch run is capturable
-// as one unit in a HIP graph on the caller's stream.
+// FP32 grids only.  Either form is capturable as one unit in a HIP graph on
+// the caller's stream.
 #include "cnv_march.h"
 
 #include <algorithm>
@@ -51,76 +52,168 @@ struct BItem {
     short z0, nz;
 };
 
+// One item of any block shape: its cells plus the face halo its operator
+// reads staged in LDS (every ghost by the lab rules), then the operator.
+template <int WG>
+__device__ __attribute__((always_inline)) inline void gen_item(const BGrid *__restrict__ grids, const BItem I, float *lds)
+{
+    const int tid = threadIdx.x;
+    const BGrid &Gd = grids[I.grid];
+    const int c = I.comp;
+    const int nx = Gd.G.n[0], ny = Gd.G.n[1], nzb = Gd.G.n[2];
+    const int gl = Gd.op == 0 ? 1 : 2; // layers read by the operator
+    const int Lx = nx + 2 * gl, Ly = ny + 2 * gl, Lz = I.nz + 2 * gl;
+    const float *cbase = Gd.in + (long long)c * Gd.G.cstride;
+    const float *ub = cbase + (long long)I.blk * Gd.G.bstride;
+    FaceL<float> F[6];
+    const int sb[3] = {-Gd.g, -Gd.g, -Gd.g}, se[3] = {Gd.g + 1, Gd.g + 1, Gd.g + 1};
+    make_faces<float>(Gd.G, Gd.heads[c], Gd.entries[c], Gd.has_bc[c], I.blk, cbase, sb, se, F);
+    const long long sx = nx, sxy = (long long)nx * ny;
+    // stage the tile and its face halo
+    const int ltot = Lx * Ly * Lz;
+    for (int l = tid; l < ltot; l += WG) {
+        const int ix = l % Lx, iy = (l / Lx) % Ly, iz = l / (Lx * Ly);
+        const int x = ix - gl, y = iy - gl, z = I.z0 + iz - gl;
+        const bool ox = x < 0 || x >= nx, oy = y < 0 || y >= ny, oz = z < 0 || z >= nzb;
+        if ((int)ox + (int)oy + (int)oz > 1)
+            continue; // edges and corners are never read
+        float v;
+        if (ox)
+            v = x < 0 ? face_layer(F[0], -x, 1, sx * y + sxy * z) : face_layer(F[1], x - nx + 1, 1, sx * y + sxy * z);
+        else if (oy)
+            v = y < 0 ? face_layer(F[2], -y, sx, x + sxy * z) : face_layer(F[3], y - ny + 1, sx, x + sxy * z);
+        else if (oz)
+            v = z < 0 ? face_layer(F[4], -z, sxy, x + sx * y) : face_layer(F[5], z - nzb + 1, sxy, x + sx * y);
+        else
+            v = ub[x + sx * y + sxy * z];
+        lds[l] = v;
+    }
+    __syncthreads();
+    const int ctot = nx * ny * I.nz;
+    float *ob = Gd.out + (long long)I.blk * Gd.G.bstride;
+    const int LXY = Lx * Ly;
+    for (int l = tid; l < ctot; l += WG) {
+        const int x = l % nx, y = (l / nx) % ny, zt = l / (nx * ny);
+        const int z = I.z0 + zt;
+        const int p = (x + gl) + Lx * (y + gl) + LXY * (zt + gl);
+        const long long o = x + sx * y + sxy * z;
+        const float cc = lds[p];
+        if (Gd.op == 0) {
+            const float ddx = Gd.ih2[0] * (fms2(cc, lds[p + 1]) + lds[p - 1]);
+            const float ddy = Gd.ih2[1] * (fms2(cc, lds[p + Lx]) + lds[p - Lx]);
+            const float ddz = Gd.ih2[2] * (fms2(cc, lds[p + LXY]) + lds[p - LXY]);
+            const float lap = ddx + ddy + ddz;
+            const float lt = lap * Gd.nudt;
+            ob[(long long)c * Gd.ocstride + o] = cc + lt;
+        } else {
+            float um, up;
+            float *oc = ob + (long long)(6 * c) * Gd.ocstride + o;
+            weno5<float>(lds[p - 2], lds[p - 1], cc, lds[p + 1], lds[p + 2], um, up);
+            oc[0] = um;
+            oc[Gd.ocstride] = up;
+            weno5<float>(lds[p - 2 * Lx], lds[p - Lx], cc, lds[p + Lx], lds[p + 2 * Lx], um, up);
+            oc[2 * Gd.ocstride] = um;
+            oc[3 * Gd.ocstride] = up;
+            weno5<float>(lds[p - 2 * LXY], lds[p - LXY], cc, lds[p + LXY], lds[p + 2 * LXY], um, up);
+            oc[4 * Gd.ocstride] = um;
+            oc[5 * Gd.ocstride] = up;
+        }
+    }
+}
+
 template <int WG>
 __global__ __launch_bounds__(WG) void k_batch(const BGrid *__restrict__ grids, const BItem *__restrict__ items,
                                               int nitems)
 {
     extern __shared__ float lds[];
-    const int tid = threadIdx.x;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const BItem I = items[it];
-        const BGrid &Gd = grids[I.grid];
-        const int c = I.comp;
-        const int nx = Gd.G.n[0], ny = Gd.G.n[1], nzb = Gd.G.n[2];
-        const int gl = Gd.op == 0 ? 1 : 2; // layers read by the operator
-        const int Lx = nx + 2 * gl, Ly = ny + 2 * gl, Lz = I.nz + 2 * gl;
-        const float *cbase = Gd.in + (long long)c * Gd.G.cstride;
-        const float *ub = cbase + (long long)I.blk * Gd.G.bstride;
-        FaceL<float> F[6];
-        const int sb[3] = {-Gd.g, -Gd.g, -Gd.g}, se[3] = {Gd.g + 1, Gd.g + 1, Gd.g + 1};
-        make_faces<float>(Gd.G, Gd.heads[c], Gd.entries[c], Gd.has_bc[c], I.blk, cbase, sb, se, F);
-        const long long sx = nx, sxy = (long long)nx * ny;
-        // stage the tile and its face halo
-        const int ltot = Lx * Ly * Lz;
-        for (int l = tid; l < ltot; l += WG) {
-            const int ix = l % Lx, iy = (l / Lx) % Ly, iz = l / (Lx * Ly);
-            const int x = ix - gl, y = iy - gl, z = I.z0 + iz - gl;
-            const bool ox = x < 0 || x >= nx, oy = y < 0 || y >= ny, oz = z < 0 || z >= nzb;
-            if ((int)ox + (int)oy + (int)oz > 1)
-                continue; // edges and corners are never read
-            float v;
-            if (ox)
-                v = x < 0 ? face_layer(F[0], -x, 1, sx * y + sxy * z) : face_layer(F[1], x - nx + 1, 1, sx * y + sxy * z);
-            else if (oy)
-                v = y < 0 ? face_layer(F[2], -y, sx, x + sxy * z) : face_layer(F[3], y - ny + 1, sx, x + sxy * z);
-            else if (oz)
-                v = z < 0 ? face_layer(F[4], -z, sxy, x + sx * y) : face_layer(F[5], z - nzb + 1, sxy, x + sx * y);
-            else
-                v = ub[x + sx * y + sxy * z];
-            lds[l] = v;
-        }
-        __syncthreads();
-        const int ctot = nx * ny * I.nz;
-        float *ob = Gd.out + (long long)I.blk * Gd.G.bstride;
-        const int LXY = Lx * Ly;
-        for (int l = tid; l < ctot; l += WG) {
-            const int x = l % nx, y = (l / nx) % ny, zt = l / (nx * ny);
-            const int z = I.z0 + zt;
-            const int p = (x + gl) + Lx * (y + gl) + LXY * (zt + gl);
-            const long long o = x + sx * y + sxy * z;
-            const float cc = lds[p];
-            if (Gd.op == 0) {
-                const float ddx = Gd.ih2[0] * (fms2(cc, lds[p + 1]) + lds[p - 1]);
-                const float ddy = Gd.ih2[1] * (fms2(cc, lds[p + Lx]) + lds[p - Lx]);
-                const float ddz = Gd.ih2[2] * (fms2(cc, lds[p + LXY]) + lds[p - LXY]);
-                const float lap = ddx + ddy + ddz;
-                const float lt = lap * Gd.nudt;
-                ob[(long long)c * Gd.ocstride + o] = cc + lt;
-            } else {
-                float um, up;
-                float *oc = ob + (long long)(6 * c) * Gd.ocstride + o;
-                weno5<float>(lds[p - 2], lds[p - 1], cc, lds[p + 1], lds[p + 2], um, up);
-                oc[0] = um;
-                oc[Gd.ocstride] = up;
-                weno5<float>(lds[p - 2 * Lx], lds[p - Lx], cc, lds[p + Lx], lds[p + 2 * Lx], um, up);
-                oc[2 * Gd.ocstride] = um;
-                oc[3 * Gd.ocstride] = up;
-                weno5<float>(lds[p - 2 * LXY], lds[p - LXY], cc, lds[p + LXY], lds[p + 2 * LXY], um, up);
-                oc[4 * Gd.ocstride] = um;
-                oc[5 * Gd.ocstride] = up;
-            }
-        }
+        gen_item<WG>(grids, items[it], lds);
         __syncthreads(); // the next item overwrites the tile
+    }
+}
+
+// The 7-point Euler tile body of one item on cubic blocks of edge NX (the
+// single-grid tile), run by WG threads or by one WG-thread slice (SUB).
+template <int NX, int WG, int SUB = 0>
+__device__ __attribute__((always_inline)) inline void s7_item(const BGrid *__restrict__ grids, const BItem I, float *lds)
+{
+    const BGrid &Gd = grids[I.grid];
+    const int c = I.comp;
+    const float *cbase = Gd.in + (long long)c * Gd.G.cstride;
+    const float *ub = cbase + (long long)I.blk * Gd.G.bstride;
+    float *ob = Gd.out + (long long)I.blk * Gd.G.bstride;
+    FaceSrc<float> F[6];
+    block_faces<float>(Gd.G, Gd.heads[c], Gd.entries[c], Gd.has_bc[c], I.blk, cbase, F);
+    (void)s7_march<float, NX, NX, WG, 1, false, SUB>(ub, ob + (long long)c * Gd.ocstride, nullptr, F, NX, I.z0,
+                                                      I.z0 + I.nz, Gd.ih2[0], Gd.ih2[1], Gd.ih2[2], Gd.nudt, lds);
+}
+
+// The WENO5 tile body of one item on cubic blocks of edge NX.
+template <int NX, int WG, int SUB = 0>
+__device__ __attribute__((always_inline)) inline void w5_item(const BGrid *__restrict__ grids, const BItem I, float *lds)
+{
+    const BGrid &Gd = grids[I.grid];
+    const int c = I.comp;
+    const float *cbase = Gd.in + (long long)c * Gd.G.cstride;
+    const float *ub = cbase + (long long)I.blk * Gd.G.bstride;
+    float *ob = Gd.out + (long long)I.blk * Gd.G.bstride;
+    FaceL<float> F[6];
+    const int sb[3] = {-Gd.g, -Gd.g, -Gd.g}, se[3] = {Gd.g + 1, Gd.g + 1, Gd.g + 1};
+    make_faces<float>(Gd.G, Gd.heads[c], Gd.entries[c], Gd.has_bc[c], I.blk, cbase, sb, se, F);
+    w5_march<float, NX, NX, WG, SUB>(ub, ob + (long long)(6 * c) * Gd.ocstride, Gd.ocstride, F, NX, I.z0,
+                                     I.z0 + I.nz, lds);
+}
+
+// A unit of the single persistent launch.
+enum BKind { BK_S7_32 = 0, BK_W5_32 = 1, BK_S7_16 = 2, BK_W5_16 = 3, BK_GEN = 4 };
+struct BUnit {
+    int kind;
+    int n;     // items of the unit: one, or one per 64-thread slice (16^3 kinds)
+    int item0; // first item
+    int pad;
+};
+
+// LDS floats of the tile bodies of k_batch_one<WG> (the generic body's tile
+// comes on top as dynamic size): a 16^3 slice gets 2 planes of its tile
+template <int WG>
+struct OneLds {
+    static constexpr int S16 = 2 * S7Shape<float, 16, 16, 64>::PLANE;
+    static constexpr int W16 = 2 * W5Shape<float, 16, 16, 64>::PLANE;
+    static constexpr int S32 = 2 * S7Shape<float, 32, 32, WG>::PLANE;
+    static constexpr int W32 = 2 * W5Shape<float, 32, 32, WG>::PLANE;
+    static constexpr int a = S32 > W32 ? S32 : W32;
+    static constexpr int b = (WG / 64) * (S16 > W16 ? S16 : W16);
+    static constexpr int floats = a > b ? a : b;
+};
+
+// The whole batch in one persistent launch: workgroup w runs units w,
+// w + gridDim.x, ... of a list sorted by cost.  In a 16^3 unit every slice
+// runs an item (slices past the unit's items repeat its last item, writing
+// the same values again), so the tile bodies' barriers match.
+template <int WG>
+__global__ __launch_bounds__(WG) void k_batch_one(const BGrid *__restrict__ grids, const BItem *__restrict__ items,
+                                                  const BUnit *__restrict__ units, int nunits)
+{
+    extern __shared__ __align__(16) float lds[];
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const BUnit U = units[u];
+        if (U.kind == BK_GEN) {
+            gen_item<WG>(grids, items[U.item0], lds);
+        } else if (U.kind == BK_S7_32) {
+            s7_item<32, WG>(grids, items[U.item0], lds);
+        } else if (U.kind == BK_W5_32) {
+            w5_item<32, WG>(grids, items[U.item0], lds);
+        } else {
+            // the slice index is uniform in a wave: scalar loads of the
+            // item and its grid descriptor
+            const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+            const BItem I = items[U.item0 + (w < U.n ? w : U.n - 1)];
+            if (U.kind == BK_S7_16)
+                s7_item<16, 64, 64>(grids, I, lds + w * OneLds<WG>::S16);
+            else
+                w5_item<16, 64, 64>(grids, I, lds + w * OneLds<WG>::W16);
+        }
+        __syncthreads(); // the next unit overwrites the tiles
     }
 }
 
@@ -144,23 +237,10 @@ __global__ __launch_bounds__(WG) void k_bmarch(const BGrid *__restrict__ grids, 
 {
     __shared__ alignas(16) float lds[2 * bm_plane<OP, NX, WG>()];
     const BItem I = items[xcd_remap(blockIdx.x, gridDim.x)];
-    const BGrid &Gd = grids[I.grid];
-    const int c = I.comp;
-    const float *cbase = Gd.in + (long long)c * Gd.G.cstride;
-    const float *ub = cbase + (long long)I.blk * Gd.G.bstride;
-    float *ob = Gd.out + (long long)I.blk * Gd.G.bstride;
-    if constexpr (OP == 0) {
-        FaceSrc<float> F[6];
-        block_faces<float>(Gd.G, Gd.heads[c], Gd.entries[c], Gd.has_bc[c], I.blk, cbase, F);
-        (void)s7_march<float, NX, NX, WG, 1, false>(ub, ob + (long long)c * Gd.ocstride, nullptr, F, NX, I.z0,
-                                                     I.z0 + I.nz, Gd.ih2[0], Gd.ih2[1], Gd.ih2[2], Gd.nudt, lds);
-    } else {
-        FaceL<float> F[6];
-        const int sb[3] = {-Gd.g, -Gd.g, -Gd.g}, se[3] = {Gd.g + 1, Gd.g + 1, Gd.g + 1};
-        make_faces<float>(Gd.G, Gd.heads[c], Gd.entries[c], Gd.has_bc[c], I.blk, cbase, sb, se, F);
-        w5_march<float, NX, NX, WG>(ub, ob + (long long)(6 * c) * Gd.ocstride, Gd.ocstride, F, NX, I.z0,
-                                    I.z0 + I.nz, lds);
-    }
+    if constexpr (OP == 0)
+        s7_item<NX, WG>(grids, I, lds);
+    else
+        w5_item<NX, WG>(grids, I, lds);
 }
 
 } // namespace cnv
@@ -201,6 +281,20 @@ struct cnv_batch_s {
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> joins;
     hipEvent_t fork = nullptr;
+    // the single persistent launch (k_batch_one): its items, grouped in
+    // units and sorted by cost, planes per item per (block edge, ghost
+    // width), workgroup size
+    bool one = true;
+    int one_wg = 256;
+    std::map<std::pair<int, int>, int> zmap;
+    std::vector<BItem> one_items;
+    std::vector<BUnit> units;
+    BItem *d_one_items = nullptr;
+    size_t one_items_cap = 0;
+    BUnit *d_units = nullptr;
+    size_t units_cap = 0;
+    size_t one_lds = 0;
+    int one_per_cu = 0; // resident workgroups per CU for (one_wg, one_lds)
 };
 
 static int class_key_edge(const cnv_grid_s *g) { return g->desc.block_cells[0]; }
@@ -471,6 +565,143 @@ static int launch_generic(cnv_batch_s *b, hipStream_t st)
     return CNV_OK;
 }
 
+// planes per item of the single launch for a grid (per block edge and ghost
+// width; the single-grid kernels' depths by default)
+static int one_zdepth(const cnv_batch_s *b, const cnv_batch_item &it)
+{
+    const cnv_grid_s *g = it.in;
+    const std::pair<int, int> key(class_key_edge(g), it.ghost);
+    int nz = b->zmap.count(key) ? b->zmap.at(key) : (key.first >= 32 ? 32 : 8);
+    return std::max(1, std::min(nz, g->desc.block_cells[2]));
+}
+
+static int upload_units(BUnit **d, size_t *cap, const std::vector<BUnit> &v)
+{
+    if (*d) // a rebuild: launches still queued on any stream may read the old list
+        CNV_HIP(hipDeviceSynchronize());
+    if (v.size() > *cap) {
+        if (*d)
+            CNV_HIP(hipFree(*d));
+        *d = nullptr;
+        *cap = 0;
+        CNV_HIP(hipMalloc((void **)d, sizeof(BUnit) * v.size()));
+        *cap = v.size();
+    }
+    if (!v.empty())
+        CNV_HIP(hipMemcpy(*d, v.data(), sizeof(BUnit) * v.size(), hipMemcpyHostToDevice));
+    return CNV_OK;
+}
+
+// Items and units of the single launch for the current depths and workgroup
+// size: a 32^3 item or an item of another shape is a unit of its own; the
+// items of cubic 16^3 blocks of one kind and depth go WG/64 to a unit.
+// Units are sorted by cost (cells x operator weight), largest first.
+static int build_one(cnv_batch_s *b)
+{
+    b->one_items.clear();
+    b->units.clear();
+    size_t gen_lds = 0;
+    struct Tmp {
+        int kind, nz;
+        BItem item;
+    };
+    std::vector<Tmp> all;
+    for (size_t gi = 0; gi < b->src.size(); ++gi) {
+        const cnv_batch_item &it = b->src[gi];
+        const cnv_grid_s *g = it.in;
+        const int edge = g->desc.block_cells[0];
+        int kind = BK_GEN;
+        if (marchable(it))
+            kind = edge == 32 ? (it.op == 0 ? BK_S7_32 : BK_W5_32) : (it.op == 0 ? BK_S7_16 : BK_W5_16);
+        int nz = one_zdepth(b, it);
+        if (kind == BK_GEN) {
+            while (nz > 1 && tile_bytes(g, it.op, nz) > kMaxLds)
+                nz /= 2;
+            if (tile_bytes(g, it.op, nz) > kMaxLds)
+                return fail(CNV_ERR_UNSUPPORTED, "block too large for the batched kernel");
+            gen_lds = std::max(gen_lds, tile_bytes(g, it.op, nz));
+        }
+        for (int c = 0; c < g->desc.ncomp; ++c)
+            for (long long blk = 0; blk < g->nblocks_total; ++blk)
+                for (int z0 = 0; z0 < g->desc.block_cells[2]; z0 += nz) {
+                    BItem I;
+                    I.grid = (int)gi;
+                    I.comp = c;
+                    I.blk = (int)blk;
+                    I.z0 = (short)z0;
+                    I.nz = (short)std::min(nz, g->desc.block_cells[2] - z0);
+                    all.push_back(Tmp{kind, I.nz, I});
+                }
+    }
+    // 16^3 items: grouped by (kind, depth) so that the slices of a unit march
+    // through the same number of planes
+    std::stable_sort(all.begin(), all.end(), [](const Tmp &x, const Tmp &y) {
+        return x.kind != y.kind ? x.kind < y.kind : x.nz < y.nz;
+    });
+    const int slices = b->one_wg / 64;
+    std::vector<std::pair<long long, BUnit>> units;
+    for (size_t i = 0; i < all.size();) {
+        const Tmp &t = all[i];
+        const bool sliced = t.kind == BK_S7_16 || t.kind == BK_W5_16;
+        size_t n = 1;
+        if (sliced)
+            while (n < (size_t)slices && i + n < all.size() && all[i + n].kind == t.kind && all[i + n].nz == t.nz)
+                ++n;
+        BUnit u;
+        u.kind = t.kind;
+        u.n = (int)n;
+        u.item0 = (int)b->one_items.size();
+        u.pad = 0;
+        const cnv_grid_s *g = b->src[t.item.grid].in;
+        const long long cost = (long long)g->desc.block_cells[0] * g->desc.block_cells[1] * t.nz *
+                               (b->src[t.item.grid].op == 1 ? 8 : 1);
+        for (size_t k = 0; k < n; ++k)
+            b->one_items.push_back(all[i + k].item);
+        units.push_back({cost, u});
+        i += n;
+    }
+    std::stable_sort(units.begin(), units.end(),
+                     [](const std::pair<long long, BUnit> &x, const std::pair<long long, BUnit> &y) {
+                         return x.first > y.first;
+                     });
+    for (const auto &u : units)
+        b->units.push_back(u.second);
+    const size_t fixed = sizeof(float) * (size_t)(b->one_wg == 128 ? OneLds<128>::floats : OneLds<256>::floats);
+    b->one_lds = std::max(fixed, gen_lds);
+    b->one_per_cu = 0;
+    int rc = upload_items(&b->d_one_items, &b->one_items_cap, b->one_items);
+    if (rc != CNV_OK)
+        return rc;
+    return upload_units(&b->d_units, &b->units_cap, b->units);
+}
+
+static int launch_one(cnv_batch_s *b, hipStream_t st)
+{
+    if (b->units.empty())
+        return CNV_OK;
+    int dev = 0, ncu = 0;
+    CNV_HIP(hipGetDevice(&dev));
+    CNV_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (b->one_per_cu <= 0) {
+        int n = 0;
+        if (b->one_wg == 128)
+            CNV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_one<128>, 128, b->one_lds));
+        else
+            CNV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_one<256>, 256, b->one_lds));
+        b->one_per_cu = std::max(1, n);
+    }
+    const long long want = (long long)ncu * b->one_per_cu;
+    const int nwg = (int)std::min<long long>((long long)b->units.size(), want);
+    if (b->one_wg == 128)
+        hipLaunchKernelGGL(k_batch_one<128>, dim3(nwg), dim3(128), b->one_lds, st, b->d_grids, b->d_one_items,
+                           b->d_units, (int)b->units.size());
+    else
+        hipLaunchKernelGGL(k_batch_one<256>, dim3(nwg), dim3(256), b->one_lds, st, b->d_grids, b->d_one_items,
+                           b->d_units, (int)b->units.size());
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
+
 static int launch_class(cnv_batch_s *b, const BClass &c, hipStream_t st)
 {
     if (c.items.empty())
@@ -483,6 +714,8 @@ static int launch_class(cnv_batch_s *b, const BClass &c, hipStream_t st)
 
 static int launch_batch(cnv_batch_s *b, hipStream_t st)
 {
+    if (b->one)
+        return launch_one(b, st);
     // the non-empty launches: classes in order, then the persistent one (-1)
     std::vector<int> parts;
     for (size_t k = 0; k < b->classes.size(); ++k)
@@ -602,6 +835,8 @@ int cnv_batch_create(const cnv_batch_item *items, int n, cnv_batch *out)
         if (rc == CNV_OK)
             rc = build_items(b, nullptr);
         if (rc == CNV_OK)
+            rc = build_one(b);
+        if (rc == CNV_OK)
             rc = make_side_streams(b);
         if (rc != CNV_OK) {
             cnv_batch_destroy(b);
@@ -622,6 +857,10 @@ int cnv_batch_destroy(cnv_batch b)
         (void)hipFree(b->d_grids);
     if (b->d_items)
         (void)hipFree(b->d_items);
+    if (b->d_one_items)
+        (void)hipFree(b->d_one_items);
+    if (b->d_units)
+        (void)hipFree(b->d_units);
     for (auto &c : b->classes)
         if (c.d_items)
             (void)hipFree(c.d_items);
@@ -660,12 +899,23 @@ int cnv_batch_info(cnv_batch b, int *nitems, int *wg, size_t *lds_bytes)
         long long n = b->nitems;
         for (const auto &c : b->classes)
             n += (long long)c.items.size();
-        *nitems = (int)n;
+        *nitems = (int)(b->one ? (long long)b->one_items.size() : n);
     }
     if (wg)
-        *wg = b->wg;
+        *wg = b->one ? b->one_wg : b->wg;
     if (lds_bytes)
-        *lds_bytes = b->lds_bytes;
+        *lds_bytes = b->one ? b->one_lds : b->lds_bytes;
+    return CNV_OK;
+}
+
+int cnv_batch_form(cnv_batch b, int *one_launch, int *nunits)
+{
+    if (!b)
+        return fail(CNV_ERR_INVALID, "batch is NULL");
+    if (one_launch)
+        *one_launch = b->one ? 1 : 0;
+    if (nunits)
+        *nunits = (int)b->units.size();
     return CNV_OK;
 }
 
@@ -812,6 +1062,70 @@ int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results,
         }
         b->wg = bestwg;
     }
+    // the class form as a whole (its launches forked and joined as in a run)
+    float t_class = 1e30f;
+    if (rc == CNV_OK) {
+        b->one = false;
+        rc = time_it([&]() { return launch_batch(b, st); }, t_class);
+        if (rc == CNV_OK)
+            record(-3, -1, 0, 0, t_class);
+    }
+    // the single launch: per (block edge, ghost width) the planes per item,
+    // then the workgroup size, each timed on the whole launch
+    float t_one = 1e30f;
+    if (rc == CNV_OK) {
+        b->one = true;
+        std::map<std::pair<int, int>, int> okeys;
+        for (const auto &it : b->src)
+            okeys[{class_key_edge(it.in), it.ghost}] = 1;
+        for (const auto &kv : okeys) {
+            const auto key = kv.first;
+            const int edge = key.first;
+            float best = 1e30f;
+            int bestz = one_zdepth(b, b->src[0]);
+            for (int z = edge >= 16 ? 2 : 1; z <= edge && rc == CNV_OK; z *= 2) {
+                b->zmap[key] = z;
+                rc = build_one(b);
+                if (rc != CNV_OK)
+                    break;
+                float us = 0.f;
+                rc = time_it([&]() { return launch_one(b, st); }, us);
+                if (rc != CNV_OK)
+                    break;
+                record(-2, edge, key.second, z, us);
+                if (us < best) {
+                    best = us;
+                    bestz = z;
+                }
+            }
+            b->zmap[key] = bestz;
+        }
+        int bestwg = b->one_wg;
+        for (int wg : {128, 256}) {
+            if (rc != CNV_OK)
+                break;
+            b->one_wg = wg;
+            rc = build_one(b);
+            if (rc != CNV_OK)
+                break;
+            float us = 0.f;
+            rc = time_it([&]() { return launch_one(b, st); }, us);
+            if (rc != CNV_OK)
+                break;
+            record(-2, -1, wg, 0, us);
+            if (us < t_one) {
+                t_one = us;
+                bestwg = wg;
+            }
+        }
+        b->one_wg = bestwg;
+        if (rc == CNV_OK)
+            rc = build_one(b);
+        if (rc == CNV_OK)
+            record(-3, -1, 1, 0, t_one);
+    }
+    // the faster form runs from now on
+    b->one = t_one <= t_class;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(st);

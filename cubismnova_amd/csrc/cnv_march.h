@@ -35,8 +35,11 @@ struct S7Shape {
 // One tile of the fused 7-point kernel: planes [z0, z1) of the block at `ub`
 // (NZ planes), one-layer faces F, output at `ob` (and the Laplacian at `lb`
 // for MODE 2).  `lds` holds two planes (2 * PLANE elements).  Returns this
-// thread's FP64 sum of u_new^2 (SUMSQ).
-template <typename T, int NX, int NY, int WG, int MODE, bool SUMSQ>
+// thread's FP64 sum of u_new^2 (SUMSQ).  SUB = 0: the tile is the whole
+// workgroup of WG threads; SUB = WG: the tile is one WG-thread slice of a
+// larger workgroup (threads threadIdx.x / WG * WG ...), every slice running
+// a tile of the same z-range so that the workgroup's barriers match.
+template <typename T, int NX, int NY, int WG, int MODE, bool SUMSQ, int SUB = 0>
 __device__ inline double s7_march(const T *__restrict__ ub, T *__restrict__ ob, T *__restrict__ lb,
                                   const FaceSrc<T> F[6], int NZ, int z0, int z1, T ihx2, T ihy2, T ihz2, T nudt,
                                   T *lds)
@@ -44,8 +47,9 @@ __device__ inline double s7_march(const T *__restrict__ ub, T *__restrict__ ob, 
     using S = S7Shape<T, NX, NY, WG>;
     constexpr int VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, P = S::P, PLANE = S::PLANE;
     static_assert(S::valid, "tile shape");
+    static_assert(SUB == 0 || SUB == WG, "a slice is one tile");
     using V = VecT<T, VEC>;
-    const int tid = threadIdx.x;
+    const int tid = SUB ? (int)(threadIdx.x % SUB) : (int)threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const int x0 = tx * VEC;
     constexpr long long SXY = (long long)NX * NY;
@@ -307,8 +311,9 @@ struct W5Shape {
 
 // One tile of WENO5: planes [z0, z1) of the block at `ub` (NZ planes), faces
 // F resolved for the lab's stencil, the six state components at ob + k *
-// ocstride.  `lds` holds two planes (2 * PLANE elements).
-template <typename T, int NX, int NY, int WG>
+// ocstride.  `lds` holds two planes (2 * PLANE elements).  SUB as in
+// s7_march.
+template <typename T, int NX, int NY, int WG, int SUB = 0>
 __device__ inline void w5_march(const T *__restrict__ ub, T *__restrict__ ob, long long ocstride,
                                 const FaceL<T> F[6], int NZ, int z0, int z1, T *lds)
 {
@@ -316,8 +321,9 @@ __device__ inline void w5_march(const T *__restrict__ ub, T *__restrict__ ob, lo
     constexpr int G = S::G, VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, P = S::P, PLANE = S::PLANE;
     constexpr int NHX = S::NHX, NHY = S::NHY, HX = S::HX, HY = S::HY;
     static_assert(S::valid, "tile shape");
+    static_assert(SUB == 0 || SUB == WG, "a slice is one tile");
     using V = VecT<T, VEC>;
-    const int tid = threadIdx.x;
+    const int tid = SUB ? (int)(threadIdx.x % SUB) : (int)threadIdx.x;
     const FaceL<T> FX0 = F[0], FX1 = F[1], FY0 = F[2], FY1 = F[3], FZ0 = F[4], FZ1 = F[5];
 
     const int tx = tid % TX, ty = tid / TX;

@@ -385,16 +385,19 @@ int cnv_grid_read_hdf(cnv_grid g, const char *fname, int file_dtype, int face_di
  * blocks on the host, Laplacian.cpp:110-131).  Many independent FP32 grids
  * of uneven block counts, block sizes, component counts, ghost widths and
  * BCs, each with its own operator, processed together over a device work
- * list of (grid, component, block, z-range) items.  Items of cubic blocks of
- * edge 16 or 32 are grouped per (operator, edge) class, one launch per class
- * running the single-grid tile bodies; all other items go to one persistent
- * launch that walks its list largest first.  Ghosts follow the lab rules of
- * cnv_lab_load for the grid's Stencil(-ghost, ghost+1) and BC lists.  A run
- * is ordered on the caller's stream: when no item's output grid is another
- * item's output or any item's input, its launches fork from that stream onto
- * streams of the batch and join back into it, so they overlap; otherwise
- * they run one after another.  Either way it is capturable in a HIP graph
- * (cnv_graph_begin/end) after one plain run. */
+ * list of (grid, component, block, z-range) items.  A run is one persistent
+ * launch over a list of work units sorted by cost: an item of a cubic 32^3
+ * block (the single-grid tile body of its operator, whole workgroup), up to
+ * WG/64 items of cubic 16^3 blocks (one per 64-thread slice, same tile
+ * bodies), or an item of any other block shape (staged in LDS by a generic
+ * body).  The other form, one launch per (operator, edge) class of 16^3/32^3
+ * blocks plus a persistent launch for the rest (forked onto streams of the
+ * batch when no item's output grid is another item's output or any item's
+ * input), stays available: cnv_batch_autotune times both and keeps the
+ * faster.  Ghosts follow the lab rules of cnv_lab_load for the grid's
+ * Stencil(-ghost, ghost+1) and BC lists.  A run is ordered on the caller's
+ * stream and capturable in a HIP graph (cnv_graph_begin/end) after one plain
+ * run. */
 enum cnv_op {
     CNV_OP_DIFFUSION = 0, /* out = in + nu_dt * laplacian(in), as cnv_diffusion_step */
     CNV_OP_WENO5 = 1      /* out = WENO5 states of in, as cnv_weno5 */
@@ -410,19 +413,29 @@ typedef struct {
 
 typedef struct cnv_batch_s *cnv_batch;
 int cnv_batch_create(const cnv_batch_item *items, int n, cnv_batch *out);
-/* Autotuner: per (operator, edge) class the workgroup size and z depth of
- * its launch; for the persistent launch, per (block edge, ghost width) the z
- * depth, then its workgroup size.  results: up to max_results records of 5
- * doubles {op, edge, wg, depth, microseconds}; trials of the persistent
- * launch have op = -1 (and edge = -1 for its workgroup trials, wg then being
- * the workgroup size and depth 0; for its depth trials wg holds the ghost
- * width). */
+/* Autotuner.  The single launch: per (block edge, ghost width) the planes per
+ * item, then the workgroup size (128 or 256), each timed on the whole
+ * launch.  The class form: per (operator, edge) class the workgroup size and
+ * z depth of its launch, for the other items per (block edge, ghost width)
+ * the z depth, then the workgroup size of their launch.  Then both forms as
+ * a whole; the faster one runs from then on.  results: up to max_results
+ * records of 5 doubles {op, edge, wg, depth, microseconds}: class trials
+ * carry the class's op; trials of the class form's persistent launch op = -1
+ * (edge = -1 for its workgroup trials, wg then the workgroup size and depth 0;
+ * for its depth trials wg holds the ghost width); single-launch trials
+ * op = -2 (depth trials: edge, wg = ghost width, depth; workgroup trials:
+ * edge = -1, wg, depth 0); the two forms op = -3 with wg = 1 for the single
+ * launch and 0 for the class form. */
 int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results, int *nresults);
 int cnv_batch_run(cnv_batch b, cnv_stream s);
 /* nitems: all work items; wg, lds_bytes: of the persistent launch. */
 int cnv_batch_info(cnv_batch b, int *nitems, int *wg, size_t *lds_bytes);
-/* Per class 5 ints {op, edge, wg, z depth, items}, up to max_classes. */
+/* Per class 5 ints {op, edge, wg, z depth, items}, up to max_classes (the
+ * class form's launches). */
 int cnv_batch_class_info(cnv_batch b, int max_classes, int *out, int *nclasses);
+/* Which form runs (1: the single persistent launch, 0: the class form) and
+ * the number of work units of the single launch. */
+int cnv_batch_form(cnv_batch b, int *one_launch, int *nunits);
 int cnv_batch_destroy(cnv_batch b);
 
 /* ------------------------------------------------------------ autotuner */

@@ -197,3 +197,37 @@ def test_full_batch_96_grids(gpu):
                     scale = np.max(np.abs(o64))
                     assert np.max(np.abs(g64 - one64)) <= 1e-6 * scale, (rep, s["edge"], nb, k)
                     assert np.linalg.norm(g64 - o64) <= 1e-6 * np.linalg.norm(o64), (rep, s["edge"], nb, k)
+
+
+def test_batch_single_launch_form(gpu, batch_setup):
+    """Before tuning a batch runs as one persistent launch over its work
+    units, with every grid's single-grid result; the autotuner times both
+    forms and the one it keeps gives the same bits."""
+    specs, grids, items = batch_setup
+    b = cnv.Batch(items)
+    one, nunits = b.form()
+    assert one == 1 and nunits > 0
+    b.run()
+    first = [[go.download(c) for c in range(go.ncomp)] for _, go in grids]
+    for s, (gi, go), it in zip(specs, grids, items):
+        ref = cnv.Cartesian(s["nblocks"], s["edge"], dtype=np.float32, ncomp=go.ncomp)
+        if s["op"] == "diffusion":
+            cnv.diffusion_step(gi, ref, it[4])
+        else:
+            cnv.weno5(gi, ref)
+        n = (s["edge"],) * 3
+        for c in range(go.ncomp):
+            got = cells_of(go.download(c), s["nblocks"], n, gi.block_stride)
+            want = cells_of(ref.download(c), s["nblocks"], n, gi.block_stride)
+            if s["op"] == "diffusion":
+                assert np.array_equal(got, want), (s["edge"], c)
+            else:
+                assert np.max(np.abs(got.astype(np.float64) - want)) <= 1e-6 * np.max(np.abs(want)), (s["edge"], c)
+    table = b.autotune(iters=2)
+    forms = {int(t[2]): t[4] for t in table if t[0] == -3}
+    assert set(forms) == {0, 1} and all(v > 0 for v in forms.values())
+    assert b.form()[0] == (1 if forms[1] <= forms[0] else 0)
+    b.run()
+    for (_, go), ref in zip(grids, first):
+        for c in range(go.ncomp):
+            assert np.array_equal(go.download(c), ref[c]), c
