@@ -722,6 +722,28 @@ def copy_ceiling():
     return 2 * nbytes / (ev.elapsed_ms(e0, e1) / 1e3 / 5) / 1e9
 
 
+def fanout_ceiling(fanout=6, nbytes=512 << 20):
+    """Streaming ceiling of a one-read, `fanout`-write pattern (the WENO5
+    operator's: one input component, six state components): GB/s of bytes
+    read + written by the library's non-temporal fanout copy kernel, 512 MiB
+    read (past the last-level cache)."""
+    import cubismnova_amd as cnv
+    from cubismnova_amd import _abi
+    ev = HipEvents()
+    a, b = cnv.DeviceBuffer(nbytes), cnv.DeviceBuffer(nbytes * fanout)
+    a.fill(np.float32, 1.0)
+    st = cnv.Stream()
+    for _ in range(2):
+        _abi.call("cnv_memcpy_fanout", b.ptr, nbytes, a.ptr, nbytes, fanout, st.handle)
+    st.synchronize()
+    e0, e1 = ev.new(), ev.new()
+    ev.record(e0, st)
+    for _ in range(5):
+        _abi.call("cnv_memcpy_fanout", b.ptr, nbytes, a.ptr, nbytes, fanout, st.handle)
+    ev.record(e1, st)
+    return (1 + fanout) * nbytes / (ev.elapsed_ms(e0, e1) / 1e3 / 5) / 1e9
+
+
 def result_check(out):
     """The tolerances each workload's line is quoted at: FP32 vs FP64 rel-L2
     <= 1e-6 (diffusion after all steps, WENO5 on the worst state); the
@@ -808,6 +830,11 @@ def main():
                                       "(cnv_memcpy_stream)"}
         for rec in out.get("workloads", {}).values():
             rec["roofline"]["frac_of_copy"] = round(rec["roofline"]["achieved"] / c, 4)
+        w = out["workloads"].get("weno5") if "workloads" in out else (out if args.workload == "weno5" else None)
+        if w is not None:
+            f = fanout_ceiling()
+            w["roofline"]["fanout6_ceiling_GB_s"] = round(f, 1)
+            w["roofline"]["frac_of_fanout6"] = round(w["roofline"]["achieved"] / f, 4)
     print(json.dumps(out), flush=True)
 
 

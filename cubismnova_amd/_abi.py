@@ -81,6 +81,7 @@ SIGNATURES = {
     "cnv_memcpy_d2d": (i32, [vp, vp, sz, vp]),
     "cnv_fill": (i32, [vp, i32, f64, sz, vp]),
     "cnv_memcpy_stream": (i32, [vp, vp, sz, vp]),
+    "cnv_memcpy_fanout": (i32, [vp, sz, vp, sz, i32, vp]),
     "cnv_stream_create": (i32, [ctypes.POINTER(vp)]),
     "cnv_stream_destroy": (i32, [vp]),
     "cnv_stream_synchronize": (i32, [vp]),

@@ -262,11 +262,56 @@ __global__ __launch_bounds__(256) void k_copy_nt(const u4v *__restrict__ src, u4
     }
 }
 
+// One read, `fanout` writes: every 16-byte word of src goes to the same
+// offset of `fanout` destination slabs, dst_stride16 words apart, with
+// non-temporal loads and stores (the memory pattern of the WENO5 operator,
+// one input component to six state components, without its arithmetic).
+__global__ __launch_bounds__(256) void k_fanout_nt(const u4v *__restrict__ src, u4v *__restrict__ dst, size_t n16,
+                                                   int fanout, size_t dst_stride16)
+{
+    const size_t stride = (size_t)gridDim.x * 1024;
+    for (size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x; base < n16; base += stride) {
+        u4v v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t i = base + (size_t)j * 256;
+            if (i < n16)
+                v[j] = __builtin_nontemporal_load(src + i);
+        }
+        for (int k = 0; k < fanout; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const size_t i = base + (size_t)j * 256;
+                if (i < n16)
+                    __builtin_nontemporal_store(v[j], dst + (size_t)k * dst_stride16 + i);
+            }
+    }
+}
+
 } // namespace cnv
 
 using namespace cnv;
 
 extern "C" {
+
+int cnv_memcpy_fanout(void *dst, size_t dst_stride, const void *src, size_t bytes, int fanout, cnv_stream s)
+{
+    if (bytes == 0 || fanout == 0)
+        return CNV_OK;
+    if (!dst || !src || fanout < 0)
+        return fail(CNV_ERR_INVALID, "fanout copy: NULL pointer or negative fanout");
+    if (((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes | dst_stride) & 15) != 0)
+        return fail(CNV_ERR_INVALID, "fanout copy: pointers, size and stride must be multiples of 16 bytes");
+    if (fanout > 1 && dst_stride < bytes)
+        return fail(CNV_ERR_INVALID, "fanout copy: destination slabs overlap");
+    const size_t n16 = bytes / 16;
+    const size_t want = (n16 + 1023) / 1024;
+    const unsigned nwg = (unsigned)(want < 8192 ? want : 8192);
+    hipLaunchKernelGGL(k_fanout_nt, dim3(nwg), dim3(256), 0, (hipStream_t)s, static_cast<const u4v *>(src),
+                       static_cast<u4v *>(dst), n16, fanout, dst_stride / 16);
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
 
 int cnv_memcpy_stream(void *dst, const void *src, size_t bytes, cnv_stream s)
 {

@@ -114,6 +114,12 @@ int cnv_fill(void *dst, int dtype, double value, size_t count, cnv_stream s);
  * 16-byte loads and stores (dst, src and bytes multiples of 16), the
  * streaming read + write ceiling the stencil kernels are measured against. */
 int cnv_memcpy_stream(void *dst, const void *src, size_t bytes, cnv_stream s);
+/* No counterpart in the reference: `bytes` of src copied to `fanout` slabs
+ * of dst, dst_stride bytes apart (every value read once, written fanout
+ * times; non-temporal 16-byte accesses; all sizes multiples of 16): the
+ * memory pattern of the WENO5 operator (fanout 6) without its arithmetic,
+ * the ceiling that kernel is measured against. */
+int cnv_memcpy_fanout(void *dst, size_t dst_stride, const void *src, size_t bytes, int fanout, cnv_stream s);
 int cnv_stream_create(cnv_stream *s);
 int cnv_stream_destroy(cnv_stream s);
 int cnv_stream_synchronize(cnv_stream s);
