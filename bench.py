@@ -579,6 +579,8 @@ def bench_batch(args, steps, warmup):
     # the two forms as the autotuner timed them (whole runs)
     form_us = {("single_launch" if int(t[2]) == 1 else "class_launches"): round(t[4], 1) for t in table
                if t[0] == -3}
+    if args.batch_form != "auto":
+        b.set_form(args.batch_form == "one")
     one, nunits = b.form()
     del fields
     st = cnv.Stream()
@@ -789,6 +791,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=3.0,
                     help="budget of each CPU baseline sample run (at least 10 samples are taken)")
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in tile table")
+    ap.add_argument("--batch-form", default="auto", choices=["auto", "one", "classes"],
+                    help="batch workload: the autotuner's form, or pin the single launch or the class launches")
     tile = lambda v: tuple(int(x) for x in v.split(","))
     ap.add_argument("--tile-f32", type=tile, default=None, help="WG,ZCHUNK for the FP32 kernel of the diffusion or WENO5 workload (skips autotune)")
     ap.add_argument("--tile2-f32", type=tile, default=None,

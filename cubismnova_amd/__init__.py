@@ -726,6 +726,10 @@ class Batch:
              ctypes.byref(nr))
         return [tuple(res[5 * i:5 * i + 5]) for i in range(min(nr.value, 512))]
 
+    def set_form(self, one_launch):
+        """Pin the form (True: single persistent launch) until the next autotune."""
+        call("cnv_batch_set_form", self.handle, 1 if one_launch else 0)
+
     def form(self):
         """(one_launch, units): 1 when the batch runs as one persistent launch."""
         one, nu = ctypes.c_int(), ctypes.c_int()

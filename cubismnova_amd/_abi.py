@@ -129,6 +129,7 @@ SIGNATURES = {
     "cnv_batch_info": (i32, [vp, c_int_p, c_int_p, ctypes.POINTER(sz)]),
     "cnv_batch_class_info": (i32, [vp, i32, c_int_p, c_int_p]),
     "cnv_batch_form": (i32, [vp, c_int_p, c_int_p]),
+    "cnv_batch_set_form": (i32, [vp, i32]),
     "cnv_batch_destroy": (i32, [vp]),
     "cnv_field_op2": (i32, [i32, i32, vp, vp, vp, sz, vp]),
     "cnv_field_op1": (i32, [i32, i32, vp, f64, vp, sz, vp]),

@@ -186,16 +186,23 @@ struct OneLds {
     static constexpr int floats = a > b ? a : b;
 };
 
-// The whole batch in one persistent launch: workgroup w runs units w,
-// w + gridDim.x, ... of a list sorted by cost.  In a 16^3 unit every slice
-// runs an item (slices past the unit's items repeat its last item, writing
-// the same values again), so the tile bodies' barriers match.
+// The whole batch in one persistent launch over a list of units sorted by
+// cost, in rounds of gridDim.x units: in round k workgroup w runs unit
+// k * gridDim.x + xcd_remap(w), so that each XCD takes a contiguous run of
+// the round's units (neighbouring blocks of one grid, whose halos then meet
+// in that XCD's L2).  In a 16^3 unit every slice runs an item (slices past
+// the unit's items repeat its last item, writing the same values again), so
+// the tile bodies' barriers match.
 template <int WG>
 __global__ __launch_bounds__(WG) void k_batch_one(const BGrid *__restrict__ grids, const BItem *__restrict__ items,
                                                   const BUnit *__restrict__ units, int nunits)
 {
     extern __shared__ __align__(16) float lds[];
-    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const int mine = xcd_remap(blockIdx.x, gridDim.x);
+    for (int u0 = 0; u0 < nunits; u0 += gridDim.x) {
+        const int u = u0 + mine;
+        if (u >= nunits)
+            break;
         const BUnit U = units[u];
         if (U.kind == BK_GEN) {
             gen_item<WG>(grids, items[U.item0], lds);
@@ -908,6 +915,14 @@ int cnv_batch_info(cnv_batch b, int *nitems, int *wg, size_t *lds_bytes)
     return CNV_OK;
 }
 
+int cnv_batch_set_form(cnv_batch b, int one_launch)
+{
+    if (!b)
+        return fail(CNV_ERR_INVALID, "batch is NULL");
+    b->one = one_launch != 0;
+    return CNV_OK;
+}
+
 int cnv_batch_form(cnv_batch b, int *one_launch, int *nunits)
 {
     if (!b)
@@ -968,21 +983,22 @@ int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results,
         }
         ++cnt;
     };
-    auto time_it = [&](auto &&launch, float &us) -> int {
+    auto time_n = [&](auto &&launch, float &us, int n) -> int {
         int r = launch(); // warm
         if (r != CNV_OK)
             return r;
         (void)hipEventRecord(e0, st);
-        for (int i = 0; i < iters && r == CNV_OK; ++i)
+        for (int i = 0; i < n && r == CNV_OK; ++i)
             r = launch();
         (void)hipEventRecord(e1, st);
         if (hipEventSynchronize(e1) != hipSuccess)
             return fail(CNV_ERR_HIP, "autotune: batch kernel failed");
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        us = ms * 1000.f / iters;
+        us = ms * 1000.f / n;
         return r;
     };
+    auto time_it = [&](auto &&launch, float &us) -> int { return time_n(launch, us, iters); };
     for (auto &c : b->classes) {
         if (rc != CNV_OK)
             break;
@@ -1066,7 +1082,7 @@ int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results,
     float t_class = 1e30f;
     if (rc == CNV_OK) {
         b->one = false;
-        rc = time_it([&]() { return launch_batch(b, st); }, t_class);
+        rc = time_n([&]() { return launch_batch(b, st); }, t_class, 4 * iters);
         if (rc == CNV_OK)
             record(-3, -1, 0, 0, t_class);
     }
@@ -1121,6 +1137,9 @@ int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results,
         b->one_wg = bestwg;
         if (rc == CNV_OK)
             rc = build_one(b);
+        // the whole form timed as long as the class form
+        if (rc == CNV_OK)
+            rc = time_n([&]() { return launch_one(b, st); }, t_one, 4 * iters);
         if (rc == CNV_OK)
             record(-3, -1, 1, 0, t_one);
     }

@@ -442,6 +442,9 @@ int cnv_batch_class_info(cnv_batch b, int max_classes, int *out, int *nclasses);
 /* Which form runs (1: the single persistent launch, 0: the class form) and
  * the number of work units of the single launch. */
 int cnv_batch_form(cnv_batch b, int *one_launch, int *nunits);
+/* Pins the form (1: the single persistent launch, 0: the class form) until
+ * the next cnv_batch_autotune. */
+int cnv_batch_set_form(cnv_batch b, int one_launch);
 int cnv_batch_destroy(cnv_batch b);
 
 /* ------------------------------------------------------------ autotuner */
