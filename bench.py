@@ -633,7 +633,8 @@ def bench_batch(args, steps, warmup):
                    "grids": len(specs), "cells": cells_total, "work_items": info["items"],
                    "classes": [dict(zip(("op", "edge", "wg", "zdepth", "items"), c)) for c in b.classes()],
                    "persistent_wg": info["wg"], "persistent_lds_bytes": info["lds_bytes"]},
-        "roofline": roofline(bytes_total, kern_s, measured_traffic("batch_96.md", "batch"),
+        "roofline": roofline(bytes_total, kern_s,
+                             measured_traffic("batch_96.md", "batch_single_launch" if one else "batch_class_launches"),
                              "sum over grids of cells x components x (8 B for a diffusion step: one read, one "
                              "write; 28 B for WENO5: one read, six states written)"),
         "batch_vs_single_grid_max_rel": worst,

@@ -41,3 +41,21 @@ def test_cpu_timed_protocol():
     assert r["samples"] >= 10 and len(calls) == r["samples"] + 1   # one warm-up, then the samples
     assert 0.0 <= r["min_s"] <= r["mean_s"] and r["sdev_s"] >= 0.0
     assert bench.cpu_threads() >= 1
+
+
+def test_profiles_hold_every_traffic_figure_the_line_reads():
+    """Each roofline `traffic` the bench line quotes comes from a committed
+    profile summary; every (summary, key) it looks up is there and is at
+    least the record's algorithmic bytes per launch."""
+    algorithmic = {("stencil7_diffusion_256.md", "f32"): 256 ** 3 * 8,
+                   ("stencil7_diffusion_256.md", "f64"): 256 ** 3 * 16,
+                   ("weno5_512.md", "f32"): 512 ** 3 * 140,
+                   ("weno5_512.md", "f64"): 512 ** 3 * 280,
+                   ("jacobi_sweep_512.md", "f32"): 512 ** 3 * 12,
+                   ("jacobi_sweep_512.md", "f64"): 512 ** 3 * 24,
+                   ("batch_96.md", "batch_single_launch"): 2264686592,
+                   ("batch_96.md", "batch_class_launches"): 2264686592}
+    for (name, key), alg in algorithmic.items():
+        t = bench.measured_traffic(name, key)
+        assert t is not None, (name, key)
+        assert alg <= t < 1.5 * alg, (name, key, t, alg)
