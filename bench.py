@@ -432,6 +432,23 @@ def bench_jacobi(args, steps, warmup):
     f.upload_global(0, src)
     work = jacobi_grids(cnv, N, nb, n, np.float32, zf, ncomp=3)
     S = 50
+    # sweep tiles: timed on the workload's own grids per dtype (or the built-in table)
+    tiles = {}
+    for dt in (np.float32, np.float64):
+        key = np.dtype(dt).name
+        pin = args.tile_f32 if dt == np.float32 else args.tile_f64
+        if pin:
+            cnv.set_tune_jacobi(dt, 64, pin[0], pin[1], pin[2] if len(pin) > 2 else 0)
+        elif not args.no_autotune:
+            ga, gb, gf = (jacobi_grids(cnv, N, nb, n, dt, zf) for _ in range(3))
+            ga.upload_global(0, np.zeros((N, N, N), dtype=dt))
+            gf.upload_global(0, src.astype(dt))
+            tw, tz, tp, table = cnv.autotune_jacobi(ga, gf, gb, iters=5)
+            print("autotune Jacobi sweep %s: best wg=%d zchunk=%d prefetch=%d; %s" % (
+                key, tw, tz, tp, " ".join("%d/%d/%d:%.1fus" % t for t in table)), file=sys.stderr, flush=True)
+            del ga, gb, gf
+        tw, tz, tp = cnv.tune_jacobi(dt, 64)
+        tiles[key] = {"wg": tw, "zchunk": tz, "f_prefetch": tp}
     # warm: one refined cycle, then `steps` cycles timed (each = 50 FP32 sweeps
     # + FP64 correction + FP64 defect, one graph replay per cycle)
     cnv.jacobi_solve(u, f, work, refine=True, sweeps_per_cycle=S, max_cycles=max(1, warmup), tol=0.0)
@@ -494,7 +511,8 @@ def bench_jacobi(args, steps, warmup):
         "config": {"workload": "iteratively refined weighted Jacobi (omega 2/3): 8x8x8 blocks of 64^3 (512^3), "
                                "tanh-stretched z (factor 2), Dirichlet 0; a step = one cycle of 50 FP32 sweeps "
                                "+ FP64 correction + FP64 defect", "sweeps_per_cycle": S,
-                   "value_is": "FP32 sweep kernel GCell-updates/s (cells per sweep / sweep time)"},
+                   "value_is": "FP32 sweep kernel GCell-updates/s (cells per sweep / sweep time)",
+                   "tile_f32": tiles["float32"], "tile_f64": tiles["float64"]},
         "kernel_ms_per_sweep": round(sweep_s * 1e3, 4),
         "sweeps_per_s": round(sw / wall, 1), "cycle_gcups": round(cells * sw / wall / 1e9, 3),
         "roofline": roofline(cells * 12, sweep_s, measured_traffic("jacobi_sweep_512.md", "f32"),
@@ -795,12 +813,12 @@ def main():
     ap.add_argument("--batch-form", default="auto", choices=["auto", "one", "classes"],
                     help="batch workload: the autotuner's form, or pin the single launch or the class launches")
     tile = lambda v: tuple(int(x) for x in v.split(","))
-    ap.add_argument("--tile-f32", type=tile, default=None, help="WG,ZCHUNK for the FP32 kernel of the diffusion or WENO5 workload (skips autotune)")
+    ap.add_argument("--tile-f32", type=tile, default=None, help="WG,ZCHUNK for the FP32 kernel of the diffusion or WENO5 workload, WG,ZCHUNK[,PREFETCH] for the Jacobi sweep (skips autotune)")
     ap.add_argument("--tile2-f32", type=tile, default=None,
                     help="WG,ZCHUNK for the FP32 two-step diffusion kernel (skips its autotune)")
     ap.add_argument("--tile2-f64", type=tile, default=None,
                     help="WG,ZCHUNK for the FP64 two-step diffusion kernel (skips its autotune)")
-    ap.add_argument("--tile-f64", type=tile, default=None, help="WG,ZCHUNK for the FP64 kernel of the diffusion or WENO5 workload (skips autotune)")
+    ap.add_argument("--tile-f64", type=tile, default=None, help="WG,ZCHUNK for the FP64 kernel of the diffusion or WENO5 workload, WG,ZCHUNK[,PREFETCH] for the Jacobi sweep (skips autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path (got --gpus %d)" % args.gpus,

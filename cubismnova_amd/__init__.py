@@ -803,6 +803,32 @@ def set_tune_weno5(dtype, edge, wg, zchunk):
          int(wg), int(zchunk))
 
 
+def autotune_jacobi(u, f, out, iters=5):
+    """Times the Jacobi sweep tiles (workgroup size, z-planes per tile, f
+    prefetch) on u -> out (component 0; out is overwritten) and keeps the
+    fastest for (dtype, x/y block edge); returns (wg, zchunk, prefetch,
+    [(wg, zchunk, prefetch, us per sweep), ...])."""
+    wg, zc, pf, nr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    res = np.zeros(4 * 64, dtype=np.float64)
+    call("cnv_autotune_jacobi", u.handle, f.handle, out.handle, int(iters), ctypes.byref(wg), ctypes.byref(zc),
+         ctypes.byref(pf), res.ctypes.data_as(ctypes.c_void_p), 64, ctypes.byref(nr))
+    table = [(int(res[4 * i]), int(res[4 * i + 1]), int(res[4 * i + 2]), float(res[4 * i + 3]))
+             for i in range(nr.value)]
+    return wg.value, zc.value, pf.value, table
+
+
+def set_tune_jacobi(dtype, edge, wg, zchunk, prefetch):
+    call("cnv_tune_set_jacobi", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
+         int(wg), int(zchunk), int(prefetch))
+
+
+def tune_jacobi(dtype, edge):
+    wg, zc, pf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    call("cnv_tune_get_jacobi", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
+         ctypes.byref(wg), ctypes.byref(zc), ctypes.byref(pf))
+    return wg.value, zc.value, pf.value
+
+
 def tune_weno5(dtype, edge):
     wg, zc = ctypes.c_int(), ctypes.c_int()
     call("cnv_tune_get_weno5", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),

@@ -118,6 +118,9 @@ SIGNATURES = {
     "cnv_jacobi_sweep": (i32, [vp, i32, vp, i32, vp, i32, f64, vp]),
     "cnv_jacobi_defect": (i32, [vp, i32, vp, i32, vp, i32, vp, vp]),
     "cnv_add_correction": (i32, [vp, i32, vp, i32, vp]),
+    "cnv_autotune_jacobi": (i32, [vp, vp, vp, i32, c_int_p, c_int_p, c_int_p, vp, i32, c_int_p]),
+    "cnv_tune_set_jacobi": (i32, [i32, i32, i32, i32, i32]),
+    "cnv_tune_get_jacobi": (i32, [i32, i32, c_int_p, c_int_p, c_int_p]),
     "cnv_jacobi_solve": (i32, [vp, vp, vp, vp, vp, vp, i32, vp]),
     "cnv_weno5": (i32, [vp, vp, vp]),
     "cnv_autotune_weno5": (i32, [vp, vp, i32, c_int_p, c_int_p, vp, i32, c_int_p]),

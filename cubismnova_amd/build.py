@@ -36,13 +36,37 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
+def scratch_kernels(remarks):
+    """Kernels whose per-lane scratch (private) memory is not zero, from the
+    compiler's kernel-resource-usage remarks: [(kernel, bytes per lane)]."""
+    import re
+    out, name = [], None
+    for line in remarks.splitlines():
+        m = re.search(r"remark:\s+Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            continue
+        m = re.search(r"remark:\s+ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and name and int(m.group(1)) > 0:
+            out.append((name, int(m.group(1))))
+    return out
+
+
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC] + COMMON + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + COMMON + ["-Rpass-analysis=kernel-resource-usage", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr))
-    return obj, r.stderr
+    # no kernel of the library spills registers to scratch memory: a tile
+    # shape that would is excluded by its shape rule, and the build says so
+    # if one ever does
+    spills = scratch_kernels(r.stderr)
+    if spills:
+        raise RuntimeError("%s: kernels using scratch memory (bytes per lane): %s"
+                           % (src, ", ".join("%s %d" % k for k in spills)))
+    err = "\n".join(l for l in r.stderr.splitlines() if "remark:" not in l)
+    return obj, err
 
 
 def build(verbose=False, jobs=None):

@@ -827,8 +827,8 @@ static int check_jacobi_grids(const cnv_grid_s *u, int cu, const cnv_grid_s *f, 
         if (u->desc.nblocks[d] != f->desc.nblocks[d] || u->desc.block_cells[d] != f->desc.block_cells[d] ||
             u->desc.nblocks[d] != o->desc.nblocks[d] || u->desc.block_cells[d] != o->desc.block_cells[d])
             return fail(CNV_ERR_SIZE, "grids differ in block count or block size");
-    if (u == o && cu == co)
-        return fail(CNV_ERR_INVALID, "output must not alias the input component");
+    if (slabs_overlap(u, cu, o, co) || slabs_overlap(f, cf, o, co))
+        return fail(CNV_ERR_INVALID, "output must not alias an input component");
     return CNV_OK;
 }
 
@@ -846,6 +846,31 @@ int cnv_jacobi_defect(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid r, int cr
     CNV_CHECK(check_jacobi_grids(u, cu, f, cf, r, cr));
     return launch_jacobi(u, cu, f, cf, r, cr, 1, 0.0, sumsq, (hipStream_t)s);
     CNV_GUARD_END
+}
+
+int cnv_autotune_jacobi(cnv_grid u, cnv_grid f, cnv_grid out, int iters, int *wg, int *zchunk, int *prefetch,
+                        double *results, int max_results, int *nresults)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_jacobi_grids(u, 0, f, 0, out, 0));
+    if (f->desc.dtype != u->desc.dtype || out->desc.dtype != u->desc.dtype)
+        return fail(CNV_ERR_SIZE, "u, f and out must share a dtype");
+    if (slabs_overlap(u, 0, f, 0))
+        return fail(CNV_ERR_INVALID, "autotune needs u and f in separate memory");
+    return autotune_jacobi(u, f, out, iters, wg, zchunk, prefetch, results, max_results, nresults);
+    CNV_GUARD_END
+}
+
+int cnv_tune_set_jacobi(int dtype, int edge, int wg, int zchunk, int prefetch)
+{
+    return jacobi_set_tune(dtype, edge, wg, zchunk, prefetch);
+}
+
+int cnv_tune_get_jacobi(int dtype, int edge, int *wg, int *zchunk, int *prefetch)
+{
+    if (!wg || !zchunk || !prefetch)
+        return fail(CNV_ERR_INVALID, "output pointer is NULL");
+    return jacobi_get_tune(dtype, edge, wg, zchunk, prefetch);
 }
 
 int cnv_add_correction(cnv_grid u, int cu, cnv_grid e, int ce, cnv_stream s)

@@ -119,6 +119,14 @@ inline bool grids_overlap(const cnv_grid_s *a, const cnv_grid_s *b)
     return pa < pb + nb && pb < pa + na;
 }
 
+// True when component (slab) ca of a and cb of b share device memory.
+inline bool slabs_overlap(const cnv_grid_s *a, int ca, const cnv_grid_s *b, int cb)
+{
+    const char *pa = static_cast<const char *>(a->data) + (size_t)ca * a->component_bytes;
+    const char *pb = static_cast<const char *>(b->data) + (size_t)cb * b->component_bytes;
+    return pa < pb + b->component_bytes && pb < pa + a->component_bytes;
+}
+
 inline cnv::Geom geom_of(const cnv_grid_s *g)
 {
     Geom G;
@@ -212,8 +220,13 @@ int get_tune_stencil7(int dtype, int edge, int *wg, int *zchunk);
 int launch_compare(const cnv_grid_s *a, int ca, const cnv_grid_s *b, int cb, double *out, hipStream_t st);
 
 int set_zfaces(cnv_grid_s *g, const double *zf, int n);
+// tile (may be null: the table's choice) = {workgroup size, z-planes per tile, f prefetch}
 int launch_jacobi(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out, int co, int mode, double omega,
-                  double *sumsq, hipStream_t st);
+                  double *sumsq, hipStream_t st, const int *tile = nullptr);
+int autotune_jacobi(cnv_grid_s *u, cnv_grid_s *f, cnv_grid_s *out, int iters, int *best_wg, int *best_z,
+                    int *best_pf, double *results, int max_results, int *nresults);
+int jacobi_set_tune(int dtype, int edge, int wg, int zchunk, int prefetch);
+int jacobi_get_tune(int dtype, int edge, int *wg, int *zchunk, int *prefetch);
 int launch_correct(cnv_grid_s *u, int cu, cnv_grid_s *e, int ce, hipStream_t st);
 int jacobi_prepare(cnv_grid_s *g, int comp);
 

@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <map>
+#include <mutex>
 
 namespace cnv
 {
@@ -37,7 +38,9 @@ struct JArgs {
     double *partials;
 };
 
-template <typename T, typename TF, typename TO, int NX, int WG, int MODE>
+// PF: f of plane z + 1 is loaded while plane z is evaluated (one plane of
+// latency cover for the streamed operand, at 4 * R more registers)
+template <typename T, typename TF, typename TO, int NX, int WG, int MODE, bool PF>
 __global__ __launch_bounds__(WG) void k_jacobi(const JArgs a)
 {
     constexpr int NY = NX;
@@ -138,14 +141,30 @@ __global__ __launch_bounds__(WG) void k_jacobi(const JArgs a)
     load_plane(z0 - 1, false, pm);
     load_plane(z0, true, pc);
     load_plane(z0 + 1, z0 + 1 < z1, pn);
-    for (int z = z0; z < z1; ++z) {
-        prio_ramp(z - z0, z1 - z0);
-        if (z + 2 <= z1)
-            load_plane(z + 2, z + 2 < z1, pnn);
-        VF fr[R];
+    VF fr[R], fn[R];
+    if (PF)
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            fr[r] = load_nt<TF, VEC>(fb + x0 + NX * (ty + r * ROWS) + SXY * z);
+            fn[r] = load_nt<TF, VEC>(fb + x0 + NX * (ty + r * ROWS) + SXY * z0);
+    for (int z = z0; z < z1; ++z) {
+        prio_ramp(z - z0, z1 - z0);
+        if (PF) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                fr[r] = fn[r];
+        }
+        if (z + 2 <= z1)
+            load_plane(z + 2, z + 2 < z1, pnn);
+        if (PF) {
+            if (z + 1 < z1)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    fn[r] = load_nt<TF, VEC>(fb + x0 + NX * (ty + r * ROWS) + SXY * (z + 1));
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                fr[r] = load_nt<TF, VEC>(fb + x0 + NX * (ty + r * ROWS) + SXY * z);
+        }
         T *L = lds[(z - z0) & 1];
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -251,7 +270,7 @@ __global__ __launch_bounds__(256) void k_correct(double *u, const TE *e, long lo
 // ---------------------------------------------------------------- host side
 
 struct JTune {
-    int wg, zchunk;
+    int wg, zchunk, pf;
 };
 
 template <typename T, int NX, int WG>
@@ -259,33 +278,41 @@ constexpr bool j_valid()
 {
     constexpr int VEC = 16 / (int)sizeof(T);
     constexpr int TX = NX / VEC;
+    // more than eight rows per thread spill registers to scratch memory
     return NX % VEC == 0 && WG % TX == 0 && (WG / TX) <= NX && NX % (WG / TX) == 0 && WG >= 2 * NX &&
-           WG >= 2 * TX;
+           WG >= 2 * TX && NX / (WG / TX) <= 8;
 }
 
+// the prefetch variant exists for sweeps (MODE 0) only
 template <typename T, typename TF, typename TO, int NX, int WG, int MODE>
-static bool j_launch(const JArgs &a, int ntiles, hipStream_t st)
+static bool j_launch(const JArgs &a, int ntiles, bool pf, hipStream_t st)
 {
     if constexpr (j_valid<T, NX, WG>()) {
-        hipLaunchKernelGGL((k_jacobi<T, TF, TO, NX, WG, MODE>), dim3(ntiles), dim3(WG), 0, st, a);
+        if constexpr (MODE == 0) {
+            if (pf) {
+                hipLaunchKernelGGL((k_jacobi<T, TF, TO, NX, WG, MODE, true>), dim3(ntiles), dim3(WG), 0, st, a);
+                return true;
+            }
+        }
+        hipLaunchKernelGGL((k_jacobi<T, TF, TO, NX, WG, MODE, false>), dim3(ntiles), dim3(WG), 0, st, a);
         return true;
     }
     return false;
 }
 
 template <typename T, typename TF, typename TO, int MODE>
-static bool j_dispatch(const JArgs &a, int edge, int wg, int ntiles, hipStream_t st)
+static bool j_dispatch(const JArgs &a, int edge, int wg, bool pf, int ntiles, hipStream_t st)
 {
 #define J_WG(NX)                                                                                        \
     switch (wg) {                                                                                       \
     case 64:                                                                                            \
-        return j_launch<T, TF, TO, NX, 64, MODE>(a, ntiles, st);                                        \
+        return j_launch<T, TF, TO, NX, 64, MODE>(a, ntiles, pf, st);                                    \
     case 128:                                                                                           \
-        return j_launch<T, TF, TO, NX, 128, MODE>(a, ntiles, st);                                       \
+        return j_launch<T, TF, TO, NX, 128, MODE>(a, ntiles, pf, st);                                   \
     case 256:                                                                                           \
-        return j_launch<T, TF, TO, NX, 256, MODE>(a, ntiles, st);                                       \
+        return j_launch<T, TF, TO, NX, 256, MODE>(a, ntiles, pf, st);                                   \
     case 512:                                                                                           \
-        return j_launch<T, TF, TO, NX, 512, MODE>(a, ntiles, st);                                       \
+        return j_launch<T, TF, TO, NX, 512, MODE>(a, ntiles, pf, st);                                   \
     default:                                                                                            \
         return false;                                                                                   \
     }
@@ -302,13 +329,72 @@ static bool j_dispatch(const JArgs &a, int edge, int wg, int ntiles, hipStream_t
 #undef J_WG
 }
 
-static JTune j_tune(int dtype, int edge)
+static JTune j_default(int dtype, int edge)
 {
     if (edge >= 64)
-        return {512, 16};
+        return {512, 16, dtype == CNV_F64 ? 1 : 0};
     if (edge >= 32)
-        return {256, 16};
-    return {dtype == CNV_F64 ? 128 : 64, 16};
+        return {256, 16, 0};
+    return {dtype == CNV_F64 ? 128 : 64, 16, 0};
+}
+
+static std::mutex g_j_mtx;
+static std::map<std::pair<int, int>, JTune> g_j_tune; // (dtype, x/y block edge) -> choice
+
+static JTune j_tune(int dtype, int edge)
+{
+    std::lock_guard<std::mutex> lk(g_j_mtx);
+    auto it = g_j_tune.find({dtype, edge});
+    return it == g_j_tune.end() ? j_default(dtype, edge) : it->second;
+}
+
+// does a Jacobi tile of `wg` threads exist for this dtype and x/y edge
+static bool j_tile_ok(int dtype, int edge, int wg)
+{
+    auto chk = [&](auto tag) {
+        using T = decltype(tag);
+        switch (edge) {
+        case 16:
+            return wg == 64 ? j_valid<T, 16, 64>() : wg == 128 ? j_valid<T, 16, 128>()
+                   : wg == 256 ? j_valid<T, 16, 256>() : wg == 512 ? j_valid<T, 16, 512>() : false;
+        case 32:
+            return wg == 64 ? j_valid<T, 32, 64>() : wg == 128 ? j_valid<T, 32, 128>()
+                   : wg == 256 ? j_valid<T, 32, 256>() : wg == 512 ? j_valid<T, 32, 512>() : false;
+        case 64:
+            return wg == 64 ? j_valid<T, 64, 64>() : wg == 128 ? j_valid<T, 64, 128>()
+                   : wg == 256 ? j_valid<T, 64, 256>() : wg == 512 ? j_valid<T, 64, 512>() : false;
+        default:
+            return false;
+        }
+    };
+    return dtype == CNV_F64 ? chk(double()) : chk(float());
+}
+
+int jacobi_set_tune(int dtype, int edge, int wg, int zchunk, int prefetch)
+{
+    if (dtype != CNV_F32 && dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    if (zchunk < 1)
+        return fail(CNV_ERR_INVALID, "zchunk must be >= 1");
+    if (!j_tile_ok(dtype, edge, wg))
+        return fail(CNV_ERR_UNSUPPORTED, "no Jacobi tile of this workgroup size for this dtype and block edge");
+    std::lock_guard<std::mutex> lk(g_j_mtx);
+    g_j_tune[{dtype, edge}] = JTune{wg, zchunk, prefetch ? 1 : 0};
+    return CNV_OK;
+}
+
+int jacobi_get_tune(int dtype, int edge, int *wg, int *zchunk, int *prefetch)
+{
+    if (dtype != CNV_F32 && dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    const JTune t = j_tune(dtype, edge);
+    if (wg)
+        *wg = t.wg;
+    if (zchunk)
+        *zchunk = t.zchunk;
+    if (prefetch)
+        *prefetch = t.pf;
+    return CNV_OK;
 }
 
 // metric of the grid's z direction: uniform (1/h^2) unless faces were set
@@ -354,7 +440,7 @@ int set_zfaces(cnv_grid_s *g, const double *zf, int n)
 // mode 0: sweep out = u + omega (f - Au)/diag (u, f, out of one dtype)
 // mode 1: defect out = f - A u (u, f FP64 or both FP32; out any dtype) + sumsq
 int launch_jacobi(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out, int co, int mode, double omega,
-                  double *sumsq, hipStream_t st)
+                  double *sumsq, hipStream_t st, const int *tile)
 {
     const int *n = u->desc.block_cells;
     if (!(n[0] == n[1] && (n[0] == 16 || n[0] == 32 || n[0] == 64)))
@@ -409,7 +495,7 @@ int launch_jacobi(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out,
     a.ihy2 = 1. / (u->h[1] * u->h[1]);
     a.omega = omega;
     a.partials = u->partials;
-    const JTune tu = j_tune(u->desc.dtype, n[0]);
+    const JTune tu = tile ? JTune{tile[0], tile[1], tile[2]} : j_tune(u->desc.dtype, n[0]);
     a.zchunk = tu.zchunk < n[2] ? tu.zchunk : n[2];
     a.nzc = (n[2] + a.zchunk - 1) / a.zchunk;
     const long long ntiles = u->nblocks_total * a.nzc;
@@ -417,15 +503,16 @@ int launch_jacobi(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out,
         return fail(CNV_ERR_UNSUPPORTED, "grid too large for the Jacobi kernels");
     bool ok;
     const bool dbl = u->desc.dtype == CNV_F64;
+    const bool pf = tu.pf != 0;
     if (mode == 0)
-        ok = dbl ? j_dispatch<double, double, double, 0>(a, n[0], tu.wg, (int)ntiles, st)
-                 : j_dispatch<float, float, float, 0>(a, n[0], tu.wg, (int)ntiles, st);
+        ok = dbl ? j_dispatch<double, double, double, 0>(a, n[0], tu.wg, pf, (int)ntiles, st)
+                 : j_dispatch<float, float, float, 0>(a, n[0], tu.wg, pf, (int)ntiles, st);
     else if (dbl)
-        ok = out->desc.dtype == CNV_F64 ? j_dispatch<double, double, double, 1>(a, n[0], tu.wg, (int)ntiles, st)
-                                        : j_dispatch<double, double, float, 1>(a, n[0], tu.wg, (int)ntiles, st);
+        ok = out->desc.dtype == CNV_F64 ? j_dispatch<double, double, double, 1>(a, n[0], tu.wg, pf, (int)ntiles, st)
+                                        : j_dispatch<double, double, float, 1>(a, n[0], tu.wg, pf, (int)ntiles, st);
     else
-        ok = out->desc.dtype == CNV_F64 ? j_dispatch<float, float, double, 1>(a, n[0], tu.wg, (int)ntiles, st)
-                                        : j_dispatch<float, float, float, 1>(a, n[0], tu.wg, (int)ntiles, st);
+        ok = out->desc.dtype == CNV_F64 ? j_dispatch<float, float, double, 1>(a, n[0], tu.wg, pf, (int)ntiles, st)
+                                        : j_dispatch<float, float, float, 1>(a, n[0], tu.wg, pf, (int)ntiles, st);
     if (!ok)
         return fail(CNV_ERR_UNSUPPORTED, "no Jacobi tile for this block edge");
     CNV_HIP(hipGetLastError());
@@ -433,6 +520,86 @@ int launch_jacobi(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out,
         hipLaunchKernelGGL(k_jfinalize, dim3(1), dim3(1024), 0, st, (const double *)u->partials, (int)ntiles, sumsq);
         CNV_HIP(hipGetLastError());
     }
+    return CNV_OK;
+}
+
+// Times every (workgroup size, z-planes per tile, f prefetch) candidate of
+// the sweep u -> out (iters launches each), keeps the fastest for (dtype of
+// u, x/y block edge).
+int autotune_jacobi(cnv_grid_s *u, cnv_grid_s *f, cnv_grid_s *out, int iters, int *best_wg, int *best_z,
+                    int *best_pf, double *results, int max_results, int *nresults)
+{
+    const int *n = u->desc.block_cells;
+    if (!(n[0] == n[1] && (n[0] == 16 || n[0] == 32 || n[0] == 64)))
+        return fail(CNV_ERR_UNSUPPORTED, "autotuning needs x/y block edges of 16, 32 or 64");
+    if (iters < 1)
+        return fail(CNV_ERR_INVALID, "iters must be >= 1");
+    hipStream_t st;
+    CNV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    CNV_HIP(hipEventCreate(&e0));
+    CNV_HIP(hipEventCreate(&e1));
+    const int wgs[4] = {64, 128, 256, 512};
+    const int zcs[4] = {8, 16, 32, 64};
+    float best = 1e30f;
+    int bw = 0, bz = 0, bp = 0, cnt = 0, rc = CNV_OK;
+    for (int wi = 0; wi < 4 && rc == CNV_OK; ++wi)
+        for (int zi = 0; zi < 4 && rc == CNV_OK; ++zi)
+            for (int pf = 0; pf < 2; ++pf) {
+                if (zcs[zi] > n[2] || !j_tile_ok(u->desc.dtype, n[0], wgs[wi]))
+                    continue;
+                const int tile[3] = {wgs[wi], zcs[zi], pf};
+                int r = launch_jacobi(u, 0, f, 0, out, 0, 0, 2.0 / 3.0, nullptr, st, tile);
+                if (r == CNV_ERR_UNSUPPORTED)
+                    continue;
+                if (r != CNV_OK) {
+                    rc = r;
+                    break;
+                }
+                (void)hipEventRecord(e0, st);
+                for (int i = 0; i < iters; ++i)
+                    (void)launch_jacobi(u, 0, f, 0, out, 0, 0, 2.0 / 3.0, nullptr, st, tile);
+                (void)hipEventRecord(e1, st);
+                if (hipEventSynchronize(e1) != hipSuccess) {
+                    rc = fail(CNV_ERR_HIP, "autotune: kernel failed");
+                    break;
+                }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                const float us = ms * 1000.f / iters;
+                if (results && cnt < max_results) {
+                    results[4 * cnt] = tile[0];
+                    results[4 * cnt + 1] = tile[1];
+                    results[4 * cnt + 2] = tile[2];
+                    results[4 * cnt + 3] = us;
+                }
+                ++cnt;
+                if (us < best) {
+                    best = us;
+                    bw = tile[0];
+                    bz = tile[1];
+                    bp = tile[2];
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    if (rc != CNV_OK)
+        return rc;
+    if (bw == 0)
+        return fail(CNV_ERR_UNSUPPORTED, "no tile candidate ran");
+    {
+        std::lock_guard<std::mutex> lk(g_j_mtx);
+        g_j_tune[{u->desc.dtype, n[0]}] = JTune{bw, bz, bp};
+    }
+    if (best_wg)
+        *best_wg = bw;
+    if (best_z)
+        *best_z = bz;
+    if (best_pf)
+        *best_pf = bp;
+    if (nresults)
+        *nresults = cnt < max_results ? cnt : max_results;
     return CNV_OK;
 }
 

@@ -306,7 +306,9 @@ struct W5Shape {
     static constexpr int NHY = 2 * G * TX; // y-ghost vectors per plane
     static constexpr int HX = (NHX + WG - 1) / WG;
     static constexpr int HY = (NHY + WG - 1) / WG;
-    static constexpr bool valid = NX % VEC == 0 && WG % TX == 0 && ROWS <= NY && NY % ROWS == 0 && VEC >= G;
+    // more than eight rows per thread spill registers to scratch memory
+    static constexpr bool valid =
+        NX % VEC == 0 && WG % TX == 0 && ROWS <= NY && NY % ROWS == 0 && VEC >= G && R <= 8;
 };
 
 // One tile of WENO5: planes [z0, z1) of the block at `ub` (NZ planes), faces

@@ -285,6 +285,17 @@ int cnv_jacobi_sweep(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int c
 int cnv_jacobi_defect(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid r, int cr, double *sumsq, cnv_stream s);
 /* u (FP64) += e (FP32 or FP64), elementwise */
 int cnv_add_correction(cnv_grid u, int cu, cnv_grid e, int ce, cnv_stream s);
+/* No counterpart in the reference.  Times every (workgroup size, z-planes
+ * per tile, f prefetch) candidate of the sweep u -> out (component 0 of
+ * each, `iters` launches each; `out` is overwritten), keeps the fastest in
+ * the process-wide table the sweeps and defects use for (dtype, x/y block
+ * edge) and returns it.  Prefetch (sweeps only) loads f one plane ahead.
+ * results (may be NULL) receives up to max_results quadruples {wg, zchunk,
+ * prefetch, microseconds per sweep}. */
+int cnv_autotune_jacobi(cnv_grid u, cnv_grid f, cnv_grid out, int iters, int *wg, int *zchunk, int *prefetch,
+                        double *results, int max_results, int *nresults);
+int cnv_tune_set_jacobi(int dtype, int edge, int wg, int zchunk, int prefetch);
+int cnv_tune_get_jacobi(int dtype, int edge, int *wg, int *zchunk, int *prefetch);
 
 typedef struct {
     int refine;           /* 1: FP32 sweeps on A e = r, FP64 defect and correction */

@@ -76,6 +76,8 @@ def test_tile_tables_host_side():
     import numpy as np
     import cubismnova_amd as cnv
     base = cnv.tune_diffusion_run(np.float32, 32)
+    w64 = cnv.tune_weno5(np.float64, 64)
+    j64, j32 = cnv.tune_jacobi(np.float64, 64), cnv.tune_jacobi(np.float32, 64)
     try:
         cnv.set_tune_diffusion_run(np.float32, 32, 256, 8)
         assert cnv.tune_diffusion_run(np.float32, 32) == (256, 8)
@@ -87,7 +89,37 @@ def test_tile_tables_host_side():
         with pytest.raises(cnv.CnvError):
             cnv.set_tune_weno5(np.float32, 32, 512, 16)        # no 512-thread FP32 WENO5 tile
         cnv.set_tune_weno5(np.float64, 32, 512, 16)
+        # shapes of more than eight rows per thread (they would spill
+        # registers to scratch memory) have no kernel
+        for dt, wg in ((np.float32, 64), (np.float64, 64), (np.float64, 128)):
+            with pytest.raises(cnv.CnvError):
+                cnv.set_tune_weno5(dt, 64, wg, 16)
+        cnv.set_tune_weno5(np.float64, 64, 256, 16)
+        with pytest.raises(cnv.CnvError):
+            cnv.set_tune_jacobi(np.float64, 64, 128, 16, 0)
+        cnv.set_tune_jacobi(np.float64, 64, 256, 16, 1)
+        assert cnv.tune_jacobi(np.float64, 64) == (256, 16, 1)
+        cnv.set_tune_jacobi(np.float32, 64, 128, 8, 0)
+        assert cnv.tune_jacobi(np.float32, 64) == (128, 8, 0)
     finally:
         cnv.set_tune_diffusion_run(np.float32, 32, *base)
         cnv.set_tune_diffusion_run(np.float32, 16, 64, 16)
         cnv.set_tune_weno5(np.float64, 32, 256, 32)
+        cnv.set_tune_weno5(np.float64, 64, *w64)
+        cnv.set_tune_jacobi(np.float64, 64, *j64)
+        cnv.set_tune_jacobi(np.float32, 64, *j32)
+
+
+def test_build_rejects_kernels_with_scratch_memory():
+    """build.py reads the compiler's kernel-resource-usage remarks and fails
+    on any kernel with per-lane scratch memory (register spills)."""
+    from cubismnova_amd import build
+    remarks = "\n".join([
+        "x.hip:45:1: remark: Function Name: _Z1aPf [-Rpass-analysis=kernel-resource-usage]",
+        "x.hip:45:1: remark:     VGPRs: 256 [-Rpass-analysis=kernel-resource-usage]",
+        "x.hip:45:1: remark:     ScratchSize [bytes/lane]: 132 [-Rpass-analysis=kernel-resource-usage]",
+        "x.hip:60:1: remark: Function Name: _Z1bPf [-Rpass-analysis=kernel-resource-usage]",
+        "x.hip:60:1: remark:     ScratchSize [bytes/lane]: 0 [-Rpass-analysis=kernel-resource-usage]",
+    ])
+    assert build.scratch_kernels(remarks) == [("_Z1aPf", 132)]
+    assert build.scratch_kernels(remarks.replace(": 132 ", ": 0 ")) == []

@@ -129,12 +129,85 @@ def test_full_size_sweep_512_bitexact(gpu, dtype):
     del u0, f0
     nt = oracle.max_threads()
     cnv.jacobi_sweep(u, f, out, 2.0 / 3.0)
-    assert_same(out.download(0), oracle.jacobi(fo, fs, czm, czp, h, 2.0 / 3.0, 0, nthreads=nt), "sweep 512^3")
+    ref = out.download(0)
+    assert_same(ref, oracle.jacobi(fo, fs, czm, czp, h, 2.0 / 3.0, 0, nthreads=nt), "sweep 512^3")
     s = cnv.DeviceBuffer(8)
     cnv.jacobi_defect(u, f, out, sumsq=s)
     r, sref = oracle.jacobi(fo, fs, czm, czp, h, 2.0 / 3.0, 1, nthreads=nt)
     assert_same(out.download(0), r, "defect 512^3")
     assert abs(s.download(np.float64, 1)[0] - sref) <= 1e-12 * sref
+    # every sweep tile the autotuner can pick, at full size (several rounds
+    # of workgroups): bit-identical to the sweep checked above
+    saved = cnv.tune_jacobi(dtype, 64)
+    try:
+        ran = 0
+        for wg, zc, pf in sweep_tiles(64):
+            try:
+                cnv.set_tune_jacobi(dtype, 64, wg, zc, pf)
+            except cnv.CnvError:
+                continue
+            clear(out)
+            cnv.jacobi_sweep(u, f, out, 2.0 / 3.0)
+            assert_same(out.download(0), ref, "sweep 512^3, tile %d x %d, prefetch %d" % (wg, zc, pf))
+            ran += 1
+        assert ran >= 12
+    finally:
+        cnv.set_tune_jacobi(dtype, 64, *saved)
+
+
+def clear(g):
+    """component 0 of g set to zero (a sweep that wrote nothing would fail)"""
+    code = 1 if g.dtype == np.float64 else 0
+    cnv.call("cnv_fill", g.data_ptr(0), code, 0.0, g.component_bytes // g.dtype.itemsize, None)
+
+
+def sweep_tiles(edge):
+    """the autotuner's candidates for blocks of this edge (z chunks up to the edge)"""
+    return [(wg, zc, pf) for wg in (64, 128, 256, 512) for zc in (8, 16, 32, 64) if zc <= edge for pf in (0, 1)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("edge", [16, 32, 64])
+def test_every_sweep_tile_bitexact(gpu, dtype, edge):
+    """Every (workgroup size, z-planes per tile, f prefetch) sweep tile of
+    the autotuner bit-exact against the oracle on 2x2x2 blocks, then the
+    autotuner itself: it picks one of them and leaves the sweep exact."""
+    nb, n = (2, 2, 2), (edge,) * 3
+    N = 2 * edge
+    zf = stretch.tanh_faces(N, 2.0)
+    czm, czp, _, _ = stretch.metric(zf)
+    h = [1.0 / N] * 3
+    rng = np.random.Generator(np.random.PCG64(7 * edge))
+    u0 = rng.standard_normal((N, N, N)).astype(dtype)
+    f0 = rng.standard_normal((N, N, N)).astype(dtype)
+    u, f, out = grid(nb, n, dtype, zf), grid(nb, n, dtype, zf), grid(nb, n, dtype, zf)
+    u.upload_global(0, u0)
+    f.upload_global(0, f0)
+    fo = oracle.Field(global_to_blocks(u0, nb, n), nb, n, bcs=face_bc_lists(nb, D0))
+    ref = oracle.jacobi(fo, global_to_blocks(f0, nb, n), czm, czp, h, 2.0 / 3.0, 0)
+    saved = cnv.tune_jacobi(dtype, edge)
+    try:
+        ran = 0
+        for wg, zc, pf in sweep_tiles(edge):
+            try:
+                cnv.set_tune_jacobi(dtype, edge, wg, zc, pf)
+            except cnv.CnvError:
+                continue
+            clear(out)
+            cnv.jacobi_sweep(u, f, out, 2.0 / 3.0)
+            assert_same(out.download(0), ref, "tile %d x %d, prefetch %d" % (wg, zc, pf))
+            ran += 1
+        assert ran >= 4
+        wg, zc, pf, table = cnv.autotune_jacobi(u, f, out, iters=2)
+        assert (wg, zc, pf) in [t[:3] for t in table] and cnv.tune_jacobi(dtype, edge) == (wg, zc, pf)
+        assert len(table) == ran and all(t[3] > 0 for t in table)
+        clear(out)
+        cnv.jacobi_sweep(u, f, out, 2.0 / 3.0)
+        assert_same(out.download(0), ref, "autotuned tile")
+    finally:
+        cnv.set_tune_jacobi(dtype, edge, *saved)
+    with pytest.raises(cnv.CnvError):
+        cnv.set_tune_jacobi(dtype, edge, 96, 16, 0)
 
 
 def test_refined_solve_on_64_blocks(gpu):
