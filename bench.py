@@ -594,12 +594,13 @@ def bench_batch(args, steps, warmup):
     if table:
         print("autotune batch (op/edge/wg/depth): " + " ".join("%d/%d/%d/%d:%.0fus" % t for t in table),
               file=sys.stderr, flush=True)
-    # the two forms as the autotuner timed them (whole runs)
-    form_us = {("single_launch" if int(t[2]) == 1 else "class_launches"): round(t[4], 1) for t in table
-               if t[0] == -3}
+    # the forms as the autotuner timed them (whole runs)
+    form_us = {cnv.Batch.FORM_NAMES[int(t[2])]: round(t[4], 1) for t in table if t[0] == -3}
     if args.batch_form != "auto":
-        b.set_form(args.batch_form == "one")
-    one, nunits = b.form()
+        b.set_form({"one": cnv.Batch.SINGLE, "classes": cnv.Batch.CLASSES,
+                    "serial": cnv.Batch.CLASSES_SERIAL}[args.batch_form])
+    form, nunits = b.form()
+    one = form == cnv.Batch.SINGLE
     del fields
     st = cnv.Stream()
     for _ in range(warmup):
@@ -646,7 +647,7 @@ def bench_batch(args, steps, warmup):
                                "WENO5, HIP-graph replayed; the step runs in the form the autotuner timed faster: "
                                "one persistent launch over a device list of work units, or one launch per "
                                "(operator, edge) class of 16^3/32^3 blocks plus a persistent launch for the rest",
-                   "form": "single_launch" if one else "class_launches", "form_times_us": form_us,
+                   "form": cnv.Batch.FORM_NAMES[form], "form_times_us": form_us,
                    "units": nunits,
                    "grids": len(specs), "cells": cells_total, "work_items": info["items"],
                    "classes": [dict(zip(("op", "edge", "wg", "zdepth", "items"), c)) for c in b.classes()],
@@ -810,8 +811,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=3.0,
                     help="budget of each CPU baseline sample run (at least 10 samples are taken)")
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in tile table")
-    ap.add_argument("--batch-form", default="auto", choices=["auto", "one", "classes"],
-                    help="batch workload: the autotuner's form, or pin the single launch or the class launches")
+    ap.add_argument("--batch-form", default="auto", choices=["auto", "one", "classes", "serial"],
+                    help="batch workload: the autotuner's form, or pin the single launch, the class launches "
+                         "(overlapping) or the class launches one after another")
     tile = lambda v: tuple(int(x) for x in v.split(","))
     ap.add_argument("--tile-f32", type=tile, default=None, help="WG,ZCHUNK for the FP32 kernel of the diffusion or WENO5 workload, WG,ZCHUNK[,PREFETCH] for the Jacobi sweep (skips autotune)")
     ap.add_argument("--tile2-f32", type=tile, default=None,

@@ -719,22 +719,28 @@ class Batch:
     def autotune(self, iters=5):
         """Returns the trials as (op, edge, wg, depth, microseconds): op >= 0
         a class launch, -1 the class form's persistent launch, -2 the single
-        launch, -3 each form as a whole (wg 1: single launch, 0: classes)."""
+        launch, -3 each form as a whole (wg: the form, as in form())."""
         res = np.zeros(5 * 512, dtype=np.float64)
         nr = ctypes.c_int()
         call("cnv_batch_autotune", self.handle, int(iters), res.ctypes.data_as(ctypes.c_void_p), 512,
              ctypes.byref(nr))
         return [tuple(res[5 * i:5 * i + 5]) for i in range(min(nr.value, 512))]
 
-    def set_form(self, one_launch):
-        """Pin the form (True: single persistent launch) until the next autotune."""
-        call("cnv_batch_set_form", self.handle, 1 if one_launch else 0)
+    # the forms of a step: class launches overlapping on the batch's streams,
+    # one persistent launch, class launches one after another
+    CLASSES, SINGLE, CLASSES_SERIAL = 0, 1, 2
+    FORM_NAMES = {0: "class_launches", 1: "single_launch", 2: "class_launches_serial"}
+
+    def set_form(self, form):
+        """Pin the form (Batch.CLASSES, SINGLE or CLASSES_SERIAL) until the
+        next autotune."""
+        call("cnv_batch_set_form", self.handle, int(form))
 
     def form(self):
-        """(one_launch, units): 1 when the batch runs as one persistent launch."""
-        one, nu = ctypes.c_int(), ctypes.c_int()
-        call("cnv_batch_form", self.handle, ctypes.byref(one), ctypes.byref(nu))
-        return one.value, nu.value
+        """(form, units): the form that runs and the single launch's work units."""
+        f, nu = ctypes.c_int(), ctypes.c_int()
+        call("cnv_batch_form", self.handle, ctypes.byref(f), ctypes.byref(nu))
+        return f.value, nu.value
 
     def classes(self):
         """[(op, edge, wg, z depth, items)] of the per-class launches."""

@@ -285,6 +285,7 @@ struct cnv_batch_s {
     // when no item writes what another item reads or writes: forked from
     // the caller's stream onto `side` and joined back (capturable)
     bool concurrent = false;
+    bool serial = false; // class launches one after another even when they could overlap
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> joins;
     hipEvent_t fork = nullptr;
@@ -731,7 +732,7 @@ static int launch_batch(cnv_batch_s *b, hipStream_t st)
     if (b->nitems > 0)
         parts.push_back(-1);
     auto run = [&](int k, hipStream_t s) { return k < 0 ? launch_generic(b, s) : launch_class(b, b->classes[k], s); };
-    if (!b->concurrent || parts.size() < 2 || b->side.size() + 1 < parts.size()) {
+    if (!b->concurrent || b->serial || parts.size() < 2 || b->side.size() + 1 < parts.size()) {
         for (int k : parts) {
             const int rc = run(k, st);
             if (rc != CNV_OK)
@@ -915,20 +916,24 @@ int cnv_batch_info(cnv_batch b, int *nitems, int *wg, size_t *lds_bytes)
     return CNV_OK;
 }
 
-int cnv_batch_set_form(cnv_batch b, int one_launch)
+int cnv_batch_set_form(cnv_batch b, int form)
 {
     if (!b)
         return fail(CNV_ERR_INVALID, "batch is NULL");
-    b->one = one_launch != 0;
+    if (form < 0 || form > 2)
+        return fail(CNV_ERR_INVALID, "form must be 0 (class launches), 1 (single launch) or 2 (class launches "
+                                     "one after another)");
+    b->one = form == 1;
+    b->serial = form == 2;
     return CNV_OK;
 }
 
-int cnv_batch_form(cnv_batch b, int *one_launch, int *nunits)
+int cnv_batch_form(cnv_batch b, int *form, int *nunits)
 {
     if (!b)
         return fail(CNV_ERR_INVALID, "batch is NULL");
-    if (one_launch)
-        *one_launch = b->one ? 1 : 0;
+    if (form)
+        *form = b->one ? 1 : (b->serial ? 2 : 0);
     if (nunits)
         *nunits = (int)b->units.size();
     return CNV_OK;
@@ -1078,14 +1083,23 @@ int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results,
         }
         b->wg = bestwg;
     }
-    // the class form as a whole (its launches forked and joined as in a run)
-    float t_class = 1e30f;
+    // the class form as a whole: its launches forked and joined as in a run
+    // (when the items allow it), and one after another on one stream
+    float t_class = 1e30f, t_serial = 1e30f;
     if (rc == CNV_OK) {
         b->one = false;
+        b->serial = false;
         rc = time_n([&]() { return launch_batch(b, st); }, t_class, 4 * iters);
         if (rc == CNV_OK)
             record(-3, -1, 0, 0, t_class);
     }
+    if (rc == CNV_OK && b->concurrent) {
+        b->serial = true;
+        rc = time_n([&]() { return launch_batch(b, st); }, t_serial, 4 * iters);
+        if (rc == CNV_OK)
+            record(-3, -1, 2, 0, t_serial);
+    }
+    b->serial = t_serial < t_class;
     // the single launch: per (block edge, ghost width) the planes per item,
     // then the workgroup size, each timed on the whole launch
     float t_one = 1e30f;
@@ -1143,8 +1157,8 @@ int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results,
         if (rc == CNV_OK)
             record(-3, -1, 1, 0, t_one);
     }
-    // the faster form runs from now on
-    b->one = t_one <= t_class;
+    // the fastest form runs from now on
+    b->one = t_one <= std::min(t_class, t_serial);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(st);

@@ -434,15 +434,18 @@ int cnv_batch_create(const cnv_batch_item *items, int n, cnv_batch *out);
  * item, then the workgroup size (128 or 256), each timed on the whole
  * launch.  The class form: per (operator, edge) class the workgroup size and
  * z depth of its launch, for the other items per (block edge, ghost width)
- * the z depth, then the workgroup size of their launch.  Then both forms as
- * a whole; the faster one runs from then on.  results: up to max_results
+ * the z depth, then the workgroup size of their launch.  Then the forms as
+ * a whole (the class form with and without overlapping launches); the
+ * fastest runs from then on.  results: up to max_results
  * records of 5 doubles {op, edge, wg, depth, microseconds}: class trials
  * carry the class's op; trials of the class form's persistent launch op = -1
  * (edge = -1 for its workgroup trials, wg then the workgroup size and depth 0;
  * for its depth trials wg holds the ghost width); single-launch trials
  * op = -2 (depth trials: edge, wg = ghost width, depth; workgroup trials:
- * edge = -1, wg, depth 0); the two forms op = -3 with wg = 1 for the single
- * launch and 0 for the class form. */
+ * edge = -1, wg, depth 0); the forms as a whole op = -3 with wg = 1 for the
+ * single launch, 0 for the class form with its launches overlapping on the
+ * batch's streams and 2 for the class form with its launches one after
+ * another on the caller's stream. */
 int cnv_batch_autotune(cnv_batch b, int iters, double *results, int max_results, int *nresults);
 int cnv_batch_run(cnv_batch b, cnv_stream s);
 /* nitems: all work items; wg, lds_bytes: of the persistent launch. */
@@ -450,12 +453,14 @@ int cnv_batch_info(cnv_batch b, int *nitems, int *wg, size_t *lds_bytes);
 /* Per class 5 ints {op, edge, wg, z depth, items}, up to max_classes (the
  * class form's launches). */
 int cnv_batch_class_info(cnv_batch b, int max_classes, int *out, int *nclasses);
-/* Which form runs (1: the single persistent launch, 0: the class form) and
- * the number of work units of the single launch. */
-int cnv_batch_form(cnv_batch b, int *one_launch, int *nunits);
-/* Pins the form (1: the single persistent launch, 0: the class form) until
- * the next cnv_batch_autotune. */
-int cnv_batch_set_form(cnv_batch b, int one_launch);
+/* Which form runs (1: the single persistent launch; 0: the class form, its
+ * launches overlapping on the batch's streams when no item's output is
+ * another's input or output; 2: the class form, its launches one after
+ * another on the caller's stream) and the number of work units of the
+ * single launch. */
+int cnv_batch_form(cnv_batch b, int *form, int *nunits);
+/* Pins the form (0, 1 or 2 as above) until the next cnv_batch_autotune. */
+int cnv_batch_set_form(cnv_batch b, int form);
 int cnv_batch_destroy(cnv_batch b);
 
 /* ------------------------------------------------------------ autotuner */

@@ -225,9 +225,19 @@ def test_batch_single_launch_form(gpu, batch_setup):
                 assert np.max(np.abs(got.astype(np.float64) - want)) <= 1e-6 * np.max(np.abs(want)), (s["edge"], c)
     table = b.autotune(iters=2)
     forms = {int(t[2]): t[4] for t in table if t[0] == -3}
-    assert set(forms) == {0, 1} and all(v > 0 for v in forms.values())
-    assert b.form()[0] == (1 if forms[1] <= forms[0] else 0)
+    assert set(forms) == {0, 1, 2} and all(v > 0 for v in forms.values())
+    assert b.form()[0] == min(forms, key=lambda k: (forms[k], k != 1))
     b.run()
     for (_, go), ref in zip(grids, first):
         for c in range(go.ncomp):
             assert np.array_equal(go.download(c), ref[c]), c
+    # every form, pinned, gives the same bits
+    for f in (cnv.Batch.CLASSES, cnv.Batch.CLASSES_SERIAL, cnv.Batch.SINGLE):
+        b.set_form(f)
+        assert b.form()[0] == f
+        b.run()
+        for (_, go), ref in zip(grids, first):
+            for c in range(go.ncomp):
+                assert np.array_equal(go.download(c), ref[c]), (f, c)
+    with pytest.raises(cnv.CnvError):
+        b.set_form(3)
