@@ -2,8 +2,8 @@
 // sizes, component counts, ghost widths and BCs processed together.
 //
 // A work item is (grid, component, block, z-range).  The batch runs in one
-// of two forms, chosen by cnv_batch_autotune from their measured times (the
-// single launch until then):
+// of three forms, chosen by cnv_batch_autotune from their measured times
+// (the single launch until then):
 //  * one persistent launch (k_batch_one) over a device list of work units,
 //    largest first: a unit is one item of a cubic 32^3 block run by the
 //    whole workgroup on the single-grid tile body of its operator (s7_march,
@@ -14,7 +14,8 @@
 //  * one launch per (operator, edge) class of 16^3/32^3 blocks (k_bmarch,
 //    one workgroup per item on the same tile bodies) plus the persistent
 //    generic launch (k_batch) for the other blocks, forked onto streams of
-//    the batch when the items are independent.
+//    the batch when the items are independent, or one after another on the
+//    caller's stream.
 // Operators: op 0: u_new = u + nu_dt * lap(u) in the Laplacian.cpp:125-129
 // order; op 1: WENO5 u^-_{i+1/2}, u^+_{i-1/2} in x, y, z (outputs 6c+2d,
 // 6c+2d+1).  Ghosts follow the grid's Stencil(-g, g+1) and BC lists.
