@@ -53,6 +53,36 @@ struct BItem {
     short z0, nz;
 };
 
+// (ix, iy, iz) of a flat index over a box of nx x ny x ... cells, x fastest,
+// advanced by a fixed step without divisions (the step's digits are found
+// once): the generic item visits its cells in strides of the workgroup size.
+struct Walk3 {
+    int ix, iy, iz, dx, dy, dz, nx, ny;
+    __device__ Walk3(int start, int step, int nx_, int ny_) : nx(nx_), ny(ny_)
+    {
+        ix = start % nx;
+        iy = (start / nx) % ny;
+        iz = start / (nx * ny);
+        dx = step % nx;
+        dy = (step / nx) % ny;
+        dz = step / (nx * ny);
+    }
+    __device__ void next()
+    {
+        ix += dx;
+        iy += dy;
+        iz += dz;
+        if (ix >= nx) { // dx < nx: one carry at most
+            ix -= nx;
+            ++iy;
+        }
+        if (iy >= ny) { // dy + 1 <= ny: one carry at most
+            iy -= ny;
+            ++iz;
+        }
+    }
+};
+
 // One item of any block shape: its cells plus the face halo its operator
 // reads staged in LDS (every ghost by the lab rules), then the operator.
 template <int WG>
@@ -70,31 +100,49 @@ __device__ __attribute__((always_inline)) inline void gen_item(const BGrid *__re
     const int sb[3] = {-Gd.g, -Gd.g, -Gd.g}, se[3] = {Gd.g + 1, Gd.g + 1, Gd.g + 1};
     make_faces<float>(Gd.G, Gd.heads[c], Gd.entries[c], Gd.has_bc[c], I.blk, cbase, sb, se, F);
     const long long sx = nx, sxy = (long long)nx * ny;
-    // stage the tile and its face halo
+    // stage the tile and its face halo, U cells per thread and pass: their
+    // loads are all issued before the first LDS store, so they are in
+    // flight together
+    constexpr int U = 4;
     const int ltot = Lx * Ly * Lz;
-    for (int l = tid; l < ltot; l += WG) {
-        const int ix = l % Lx, iy = (l / Lx) % Ly, iz = l / (Lx * Ly);
-        const int x = ix - gl, y = iy - gl, z = I.z0 + iz - gl;
-        const bool ox = x < 0 || x >= nx, oy = y < 0 || y >= ny, oz = z < 0 || z >= nzb;
-        if ((int)ox + (int)oy + (int)oz > 1)
-            continue; // edges and corners are never read
-        float v;
-        if (ox)
-            v = x < 0 ? face_layer(F[0], -x, 1, sx * y + sxy * z) : face_layer(F[1], x - nx + 1, 1, sx * y + sxy * z);
-        else if (oy)
-            v = y < 0 ? face_layer(F[2], -y, sx, x + sxy * z) : face_layer(F[3], y - ny + 1, sx, x + sxy * z);
-        else if (oz)
-            v = z < 0 ? face_layer(F[4], -z, sxy, x + sx * y) : face_layer(F[5], z - nzb + 1, sxy, x + sx * y);
-        else
-            v = ub[x + sx * y + sxy * z];
-        lds[l] = v;
+    Walk3 ws(tid, WG, Lx, Ly);
+    for (int l0 = tid; l0 < ltot; l0 += U * WG) {
+        float v[U];
+        bool w[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int l = l0 + k * WG;
+            const int x = ws.ix - gl, y = ws.iy - gl, z = I.z0 + ws.iz - gl;
+            ws.next();
+            const bool ox = x < 0 || x >= nx, oy = y < 0 || y >= ny, oz = z < 0 || z >= nzb;
+            // edges and corners are never read
+            w[k] = l < ltot && (int)ox + (int)oy + (int)oz <= 1;
+            v[k] = 0.f;
+            if (!w[k])
+                continue;
+            if (ox)
+                v[k] = x < 0 ? face_layer(F[0], -x, 1, sx * y + sxy * z)
+                             : face_layer(F[1], x - nx + 1, 1, sx * y + sxy * z);
+            else if (oy)
+                v[k] = y < 0 ? face_layer(F[2], -y, sx, x + sxy * z) : face_layer(F[3], y - ny + 1, sx, x + sxy * z);
+            else if (oz)
+                v[k] = z < 0 ? face_layer(F[4], -z, sxy, x + sx * y)
+                             : face_layer(F[5], z - nzb + 1, sxy, x + sx * y);
+            else
+                v[k] = ub[x + sx * y + sxy * z];
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if (w[k])
+                lds[l0 + k * WG] = v[k];
     }
     __syncthreads();
     const int ctot = nx * ny * I.nz;
     float *ob = Gd.out + (long long)I.blk * Gd.G.bstride;
     const int LXY = Lx * Ly;
-    for (int l = tid; l < ctot; l += WG) {
-        const int x = l % nx, y = (l / nx) % ny, zt = l / (nx * ny);
+    Walk3 wc(tid, WG, nx, ny);
+    for (int l = tid; l < ctot; l += WG, wc.next()) {
+        const int x = wc.ix, y = wc.iy, zt = wc.iz;
         const int z = I.z0 + zt;
         const int p = (x + gl) + Lx * (y + gl) + LXY * (zt + gl);
         const long long o = x + sx * y + sxy * z;
