@@ -202,18 +202,16 @@ def diffusion_gpu(cnv, ev, dtype, u0, steps, warmup, autotune=True, tile=None, t
     g = cnv.Graph(st).capture()
     s2 = cnv.diffusion_run(src, dst, steps, dt, norms=norms.at(8 * warmup), stream=st)
     g.end()
-    # the wall clock: one replay of the `steps` steps between two device
-    # synchronisations; the launches' own time: HIP events around a second
-    # replay (the steps it adds are the same in both precisions)
+    # one replay of the `steps` steps: the wall clock between two device
+    # synchronisations, the launches' own time from HIP events on the stream
+    e0, e1 = ev.new(), ev.new()
     ev.device_sync()
     t0 = time.perf_counter()
-    g.launch()
-    ev.device_sync()
-    wall = time.perf_counter() - t0
-    e0, e1 = ev.new(), ev.new()
     ev.record(e0, st)
     g.launch()
     ev.record(e1, st)
+    ev.device_sync()
+    wall = time.perf_counter() - t0
     kern_ms = ev.elapsed_ms(e0, e1)
     final = s2.download(0)
     hist = norms.download(np.float64, steps + warmup)
