@@ -1,6 +1,7 @@
 """CPU checks of the C ABI: the library loads, exports every entry point the
 header declares, and its host-side argument checks answer without a GPU."""
 import ctypes
+import os
 import subprocess
 
 import pytest
@@ -123,3 +124,35 @@ def test_build_rejects_kernels_with_scratch_memory():
     ])
     assert build.scratch_kernels(remarks) == [("_Z1aPf", 132)]
     assert build.scratch_kernels(remarks.replace(": 132 ", ": 0 ")) == []
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every struct of include/cubismnova_amd.h has the size and field
+    offsets that the ctypes binding declares (a C program compiled with gcc
+    against the header prints them)."""
+    import ctypes
+    import subprocess
+    from cubismnova_amd import _abi
+    structs = {"cnv_bc": _abi.cnv_bc, "cnv_stencil": _abi.cnv_stencil, "cnv_grid_desc": _abi.cnv_grid_desc,
+               "cnv_solve_opts": _abi.cnv_solve_opts, "cnv_solve_info": _abi.cnv_solve_info,
+               "cnv_batch_item": _abi.cnv_batch_item}
+    # ctypes names that differ from the C member names (Python keywords)
+    cname = {("cnv_batch_item", "grid_in"): "in", ("cnv_batch_item", "grid_out"): "out"}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cubismnova_amd.h"', 'int main(void) {']
+    for name, cls in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (name, name))
+        for f in cls._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));'
+                         % (name, f[0], name, cname.get((name, f[0]), f[0])))
+    lines += ['return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    r = subprocess.run(["gcc", "-std=c99", "-I" + inc, str(src), "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines())
+    for name, cls in structs.items():
+        assert int(got[name]) == ctypes.sizeof(cls), name
+        for f in cls._fields_:
+            assert int(got[name + "." + f[0]]) == getattr(cls, f[0]).offset, (name, f[0])
