@@ -989,8 +989,9 @@ int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts
         rel = std::sqrt(hst[2]) / fn;
         if (history && history_cap > 0)
             history[0] = rel;
-        // one cycle, recorded once and replayed
-        CNV_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        // one cycle, recorded once and replayed (relaxed capture: see
+        // cnv_graph_begin)
+        CNV_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
         int crc = CNV_OK;
         if (refine) {
             crc = launch_fill((char *)work->data, CNV_F32, 0.0, work->component_bytes / 4, cs);
@@ -1151,7 +1152,13 @@ int cnv_graph_begin(cnv_stream s)
 {
     if (!s)
         return fail(CNV_ERR_INVALID, "graph capture needs a non-default stream");
-    CNV_HIP(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeThreadLocal));
+    // Relaxed: with the thread-local mode, a capture begun after grids were
+    // destroyed earlier in the process failed at its first kernel launch
+    // ("operation failed due to a previous error during capture") while the
+    // same calls captured in a fresh process did not; the entry points make
+    // no allocating or synchronising call while capturing once their grids
+    // are prepared, so the relaxed mode records the same work.
+    CNV_HIP(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeRelaxed));
     return CNV_OK;
 }
 

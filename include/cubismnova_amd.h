@@ -494,7 +494,10 @@ int cnv_tune_get_weno5(int dtype, int edge, int *wg, int *zchunk);
  * graph, replayed with cnv_graph_launch.  Every entry point above is
  * capturable after it has run once on the same grids (first calls upload
  * BC tables synchronously); reductions use plain stores and a fixed-order
- * finalize kernel, so no memset or counter node is ever recorded. */
+ * finalize kernel, so no memset or counter node is ever recorded.  The
+ * capture runs in HIP's relaxed mode: in the thread-local mode a capture
+ * begun after grids had been destroyed earlier in the process failed at its
+ * first kernel launch. */
 /* Does now what the first operator call on g would do synchronously (device
  * BC tables of every component, the z metric): call it before recording
  * calls on g into a graph.  A change of g's BC lists rebuilds its tables

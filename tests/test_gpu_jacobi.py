@@ -234,3 +234,42 @@ def test_refined_solve_on_64_blocks(gpu):
     a, b = u.download_global(0), u2.download_global(0)
     assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b))
     assert sw <= 1.2 * sw2 + 100
+
+
+def test_capture_after_grids_were_freed(gpu):
+    """Sweeps recorded into a HIP graph in turn, each time on grids made after
+    the previous grids, graph and stream were destroyed in the same process:
+    every capture succeeds and every replay equals the same sweeps run
+    directly."""
+    nb, n, N = (2, 2, 2), (64, 64, 64), 128
+    zf = stretch.tanh_faces(N, 2.0)
+    rng = np.random.Generator(np.random.PCG64(11))
+    u0 = rng.standard_normal((N, N, N)).astype(np.float32)
+    f0 = rng.standard_normal((N, N, N)).astype(np.float32)
+    saved = cnv.tune_jacobi(np.float32, 64)
+    try:
+        for rnd, tile in enumerate([(256, 64, 0), (512, 64, 0), (256, 64, 0)]):
+            cnv.set_tune_jacobi(np.float32, 64, *tile)
+            a, b, f = (grid(nb, n, np.float32, zf) for _ in range(3))
+            f.upload_global(0, f0)
+            for g in (a, b, f):
+                g.prepare()
+            a.upload_global(0, u0)
+            for k in range(4):
+                x, y = (a, b) if k % 2 == 0 else (b, a)
+                cnv.jacobi_sweep(x, f, y, 2.0 / 3.0)
+            want = a.download(0)
+            st = cnv.Stream()
+            gr = cnv.Graph(st).capture()
+            for k in range(4):
+                x, y = (a, b) if k % 2 == 0 else (b, a)
+                cnv.jacobi_sweep(x, f, y, 2.0 / 3.0, stream=st)
+            gr.end()
+            for rep in range(2):
+                a.upload_global(0, u0)
+                gr.launch()
+                st.synchronize()
+                assert_same(a.download(0), want, "round %d tile %s replay %d" % (rnd, tile, rep))
+            del gr, a, b, f, st
+    finally:
+        cnv.set_tune_jacobi(np.float32, 64, *saved)
