@@ -581,7 +581,7 @@ int cnv_lab_load(cnv_grid g, int comp, const int block[3], const cnv_stencil *s,
     for (int d = 0; d < 3; ++d)
         if (block[d] < 0 || block[d] >= g->desc.nblocks[d])
             return fail(CNV_ERR_INVALID, "block index out of range");
-    const std::vector<cnv_bc> *lst;
+    const std::vector<cnv_bc> *lst = nullptr;
     std::vector<cnv_bc> ext;
     if (n_extern >= 0) {
         if (n_extern > 0 && !extern_bcs)
@@ -590,17 +590,26 @@ int cnv_lab_load(cnv_grid g, int comp, const int block[3], const cnv_stencil *s,
             CNV_CHECK(check_bc(extern_bcs[k]));
         ext.assign(extern_bcs, extern_bcs + n_extern);
         lst = &ext;
-    } else {
-        const long long b = block[0] + (long long)g->desc.nblocks[0] * (block[1] + (long long)g->desc.nblocks[1] * block[2]);
-        lst = &g->bcs[comp][(size_t)b];
     }
-    if (lst->size() > 16)
-        return fail(CNV_ERR_UNSUPPORTED, "more than 16 BCs on one block");
     DevBC list[16];
-    for (size_t k = 0; k < lst->size(); ++k)
-        list[k] = {(*lst)[k].kind, (*lst)[k].dir, (*lst)[k].side, 0, (*lst)[k].value};
-    return launch_lab(g, comp, *s, apply_bc != 0, true, list, (int)lst->size(), block, out,
-                      (hipStream_t)st);
+    int nl = 0;
+    {
+        // the block's own list is read under the grid's lock (other threads
+        // may set BC lists meanwhile)
+        std::unique_lock<std::mutex> lk(g->mtx, std::defer_lock);
+        if (!lst) {
+            lk.lock();
+            const long long b =
+                block[0] + (long long)g->desc.nblocks[0] * (block[1] + (long long)g->desc.nblocks[1] * block[2]);
+            lst = &g->bcs[comp][(size_t)b];
+        }
+        if (lst->size() > 16)
+            return fail(CNV_ERR_UNSUPPORTED, "more than 16 BCs on one block");
+        for (size_t k = 0; k < lst->size(); ++k)
+            list[k] = {(*lst)[k].kind, (*lst)[k].dir, (*lst)[k].side, 0, (*lst)[k].value};
+        nl = (int)lst->size();
+    }
+    return launch_lab(g, comp, *s, apply_bc != 0, true, list, nl, block, out, (hipStream_t)st);
     CNV_GUARD_END
 }
 

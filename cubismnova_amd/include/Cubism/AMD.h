@@ -36,6 +36,7 @@
 #include <functional>
 #include <initializer_list>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -603,8 +604,13 @@ public:
             check(cnv_malloc(&p, n * sizeof(DataType)), "cnv_malloc");
             dev_ = p;
             cap_ = n;
-            data_.assign(n, DataType(0));
-            check(cnv_memcpy_h2d(dev_, data_.data(), n * sizeof(DataType), nullptr), "cnv_memcpy_h2d");
+            // host buffer as Block::Data allocates it (Block/Data.h:89-521):
+            // 32-byte aligned, its bytes rounded up to a multiple of 32
+            size_t bytes = n * sizeof(DataType);
+            data_.reset(AlignedBlockAllocator<DataType>().allocate(bytes));
+            bytes_ = bytes;
+            std::fill_n(data_.get(), bytes / sizeof(DataType), DataType(0));
+            check(cnv_memcpy_h2d(dev_, data_.get(), n * sizeof(DataType), nullptr), "cnv_memcpy_h2d");
         }
         allocated_ = true;
     }
@@ -643,7 +649,7 @@ public:
         check(cnv_lab_load(g->handle(), (int)field_->component(), bi, &cs, apply_bc ? 1 : 0,
                            extern_bc ? ext_bc.data() : nullptr, extern_bc ? (int)ext_bc.size() : -1, dev_, nullptr),
               "cnv_lab_load");
-        check(cnv_memcpy_d2h(data_.data(), dev_, n * sizeof(DataType), nullptr), "cnv_memcpy_d2h");
+        check(cnv_memcpy_d2h(data_.get(), dev_, n * sizeof(DataType), nullptr), "cnv_memcpy_d2h");
     }
     template <typename Functor>
     void loadData(const MIndex &fid, Functor &id2field, const BCVector &boundaries, const bool apply_bc = true)
@@ -675,6 +681,11 @@ public:
         return *field_;
     }
     DataType *getInnerData() { return &at(0, 0, 0); }
+    // Block::Data::getBlockPtr / getBlockBytes (Block/Data.h:220-225): the
+    // lab's host buffer and its size; kept by a later allocate that fits
+    void *getBlockPtr() { return data_.get(); }
+    const void *getBlockPtr() const { return data_.get(); }
+    size_t getBlockBytes() const { return bytes_; }
 
 private:
     StencilType stencil_;
@@ -682,7 +693,11 @@ private:
     MIndex N_{0, 0, 0}, L_{0, 0, 0};
     bool allocated_ = false;
     FieldT *field_ = nullptr;
-    std::vector<DataType> data_;
+    struct FreeBlock {
+        void operator()(DataType *p) const { AlignedBlockAllocator<DataType>().deallocate(p); }
+    };
+    std::unique_ptr<DataType[], FreeBlock> data_;
+    size_t bytes_ = 0;
     void *dev_ = nullptr;
     size_t cap_ = 0;
 
@@ -875,14 +890,18 @@ public:
 
     // GridLink
     cnv_grid handle() const override { return g_; }
+    // serialised: threads with labs of their own may load them from one grid
+    // at once (CartesianTest.cpp:283-316)
     void pushComp(const size_t c) override
     {
+        std::lock_guard<std::mutex> lk(push_mtx_);
         if (sync_[c].host_dirty)
             upload(c);
         syncBCs(c);
     }
     void pullComp(const size_t c) override
     {
+        std::lock_guard<std::mutex> lk(push_mtx_);
         if (sync_[c].device_newer)
             download(c);
     }
@@ -917,6 +936,7 @@ private:
     std::vector<std::vector<std::vector<cnv_bc>>> pushed_; // BC lists on the device, per component and block
     std::array<double, 3> h_;
     size_t uploads_ = 0;
+    std::mutex push_mtx_;
 };
 } // namespace Grid
 

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <limits>
 #include <string>
+#include <thread>
 
 using namespace Cubism;
 
@@ -177,6 +178,71 @@ static void bc_tests()
     }
 }
 
+// FieldLabTest.cpp:117-152 (Reuse): a lab allocated for a larger stencil
+// keeps its buffer for a smaller one and gets a new one for a much larger
+// one; the buffer is 32-byte aligned every time
+static void fieldlab_reuse()
+{
+    Lab64 lab;
+    const MIndex range{16, 16, 16};
+    lab.allocate(Stencil(-3, 4), range);
+    const void *p0 = lab.getBlockPtr();
+    const size_t b0 = lab.getBlockBytes();
+    EXPECT(reinterpret_cast<size_t>(p0) % 32 == 0);
+    lab.allocate(Stencil(-1, 2), range);
+    const void *p1 = lab.getBlockPtr();
+    const size_t b1 = lab.getBlockBytes();
+    EXPECT(reinterpret_cast<size_t>(p1) % 32 == 0);
+    EXPECT(p0 == p1);
+    EXPECT(b0 == b1);
+    lab.allocate(Stencil(-128, 129), range);
+    const void *p2 = lab.getBlockPtr();
+    const size_t b2 = lab.getBlockBytes();
+    EXPECT(reinterpret_cast<size_t>(p2) % 32 == 0);
+    EXPECT(p2 != p1);
+    EXPECT(b2 != b1);
+}
+
+// FieldLabTest.cpp:157-221 (PeriodicIndexer), cell grids: block indices
+// -3 .. 5 map to the block of index mod 3, for every component's functor
+static void periodic_indexer()
+{
+    for (size_t ncomp : {size_t(1), size_t(3)}) {
+        Grid64 grid({3, 3, 3}, {8, 8, 8}, Grid64::PointType(0), Grid64::PointType(1), ncomp);
+        for (size_t c = 0; c < ncomp; ++c) {
+            auto fields = grid.getIndexFunctor(c);
+            for (Index z = -3; z < 6; ++z)
+                for (Index y = -3; y < 6; ++y)
+                    for (Index x = -3; x < 6; ++x) {
+                        const MIndex q{x, y, z};
+                        const MIndex s{(x + 3) % 3, (y + 3) % 3, (z + 3) % 3};
+                        const Field64 &f = fields(q);
+                        EXPECT(f.getState().block_index == s);
+                        EXPECT(f.component() == c);
+                    }
+        }
+    }
+}
+
+// CartesianTest.cpp:141-170 (BlockAccess) on a 2 x 2 x 1 grid: the periodic
+// block functor at sums and differences of block indices
+static void block_access()
+{
+    Grid64 grid({2, 2, 1}, {8, 8, 8});
+    auto fields = grid.getIndexFunctor();
+    const MIndex f00{0, 0, 0}, f01{0, 1, 0}, f10{1, 0, 0}, f11{1, 1, 0};
+    EXPECT(fields(f00 - f10).getState().block_index == f10);
+    EXPECT(fields(f10 - f10).getState().block_index == f00);
+    EXPECT(fields(f00 + f10).getState().block_index == f10);
+    EXPECT(fields(f10 + f10).getState().block_index == f00);
+    EXPECT(fields(f10 - f01).getState().block_index == f11);
+    EXPECT(fields(f11 - f01).getState().block_index == f10);
+    EXPECT(fields(f11 + f01).getState().block_index == f10);
+    EXPECT(fields(f10 + f01).getState().block_index == f11);
+    EXPECT(fields(f11 + f11).getState().block_index == f00);
+    EXPECT(fields(f00 - f11).getState().block_index == f11);
+}
+
 // CartesianTest.cpp:172-329 (3-D cells): every lab cell of every block equals
 // the analytic field at its coordinate to 8 eps
 static void cartesian_lab_analytic()
@@ -210,6 +276,30 @@ static void cartesian_lab_analytic()
                     EXPECT(std::fabs(lab(x, y, z) - v) <= 8 * eps);
                 }
     }
+    // CartesianTest.cpp:283-316: two threads, each with a lab of its own,
+    // load the blocks of the same grid at the same time
+    int bad[2] = {0, 0};
+    auto work = [&](int t) {
+        Lab64 tlab;
+        tlab.allocate(Stencil(-2, 3, true), {8, 8, 8});
+        for (size_t i = (size_t)t; i < grid.size(); i += 2) {
+            Field64 &f = grid[i];
+            grid.loadLab(f, tlab);
+            const MIndex bi = f.getState().block_index;
+            for (Index z = -2; z < 10; ++z)
+                for (Index y = -2; y < 10; ++y)
+                    for (Index x = -2; x < 10; ++x) {
+                        const double v = fexact((bi[0] * 8 + x + 0.5) * h, (bi[1] * 8 + y + 0.5) * h,
+                                                (bi[2] * 8 + z + 0.5) * h);
+                        bad[t] += !(std::fabs(tlab(x, y, z) - v) <= 8 * eps);
+                    }
+        }
+    };
+    std::thread t0(work, 0), t1(work, 1);
+    t0.join();
+    t1.join();
+    EXPECT(bad[0] == 0);
+    EXPECT(bad[1] == 0);
 }
 
 // Laplacian.cpp:110-131 on the host through the lab vs the device operator
@@ -567,6 +657,9 @@ int main(int argc, char *argv[])
     fieldlab_ghosts(false);
     fieldlab_ghosts(true);
     bc_tests();
+    fieldlab_reuse();
+    periodic_indexer();
+    block_access();
     cartesian_lab_analytic();
     laplacian_vs_host_loop();
     diffusion_run_vs_steps();

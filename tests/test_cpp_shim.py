@@ -19,7 +19,8 @@ REF_SRC = "/root/reference/test/sandbox/Benchmarks/Kernel/Laplacian/Laplacian.cp
 
 
 def build(out):
-    cmd = ["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-Werror", "-I" + os.path.join(PKG, "include"),
+    cmd = ["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-pthread", "-Wall", "-Werror",
+           "-I" + os.path.join(PKG, "include"),
            os.path.join(ROOT, "tests", "cpp", "test_shim.cpp"), "-o", out, "-L" + PKG, "-lcubismnova_amd",
            "-Wl,-rpath," + PKG]
     r = subprocess.run(cmd, capture_output=True, text=True)
