@@ -55,6 +55,11 @@ inline void check(int rc, const char *what)
 
 // Cubism::EntityType (include/Cubism/Common.h)
 enum class EntityType { Cell = 0, Node, Face, Undefined };
+// Cubism::Dir, Vector, Tensor (include/Cubism/Common.h, three dimensions):
+// direction and tensor-component indices, cast to size_t where they are used
+enum class Dir { X = 0, Y, Z };
+using Vector = Dir;
+enum class Tensor { XX = 0, XY, YX, YY, XZ, YZ, ZX, ZY, ZZ };
 
 namespace Core
 {
@@ -826,9 +831,14 @@ public:
     const BaseType &operator[](const MultiIndex &p) const { return *fields_[flat(p, nblocks_)]; }
     BaseType &field(const size_t comp, const size_t i) { return *fields_[comp * size() + i]; }
 
-    // Grid::Cartesian::getIndexFunctor (Cartesian.h:383-390): periodic block map
-    std::function<BaseType &(const MultiIndex &)> getIndexFunctor(const size_t comp = 0)
+    // Grid::Cartesian::getIndexFunctor (Cartesian.h:383-390): periodic block
+    // map of component c (any type cast to size_t, e.g. Vector::X); the face
+    // direction d has no meaning for cell data and is not used
+    template <typename Comp = size_t, typename DirT = size_t>
+    std::function<BaseType &(const MultiIndex &)> getIndexFunctor(const Comp c = 0, const DirT d = 0)
     {
+        (void)d;
+        const size_t comp = static_cast<size_t>(c);
         return [this, comp](const MultiIndex &p) -> BaseType & {
             MultiIndex q;
             for (int d = 0; d < 3; ++d)
@@ -836,12 +846,14 @@ public:
             return field(comp, flat(q, nblocks_));
         };
     }
-    // Grid::Cartesian::loadLab (Cartesian.h:411-427)
-    void loadLab(const BaseType &f, FieldLabType &lab, const size_t comp = 0)
+    // Grid::Cartesian::loadLab (Cartesian.h:411-427), component c and face
+    // direction d as in getIndexFunctor
+    template <typename Comp = size_t, typename DirT = size_t>
+    void loadLab(const BaseType &f, FieldLabType &lab, const Comp c = 0, const DirT d = 0)
     {
         if (!lab.isAllocated())
             throw std::runtime_error("FieldLab: can not load lab data when not allocated first");
-        auto fn = getIndexFunctor(comp);
+        auto fn = getIndexFunctor(c, d);
         lab.loadData(f.getState().block_index, fn);
     }
 

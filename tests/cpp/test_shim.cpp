@@ -284,7 +284,7 @@ static void cartesian_lab_analytic()
         tlab.allocate(Stencil(-2, 3, true), {8, 8, 8});
         for (size_t i = (size_t)t; i < grid.size(); i += 2) {
             Field64 &f = grid[i];
-            grid.loadLab(f, tlab);
+            grid.loadLab(f, tlab, Vector::X, Dir::X); // the reference's call form
             const MIndex bi = f.getState().block_index;
             for (Index z = -2; z < 10; ++z)
                 for (Index y = -2; y < 10; ++y)
