@@ -755,10 +755,23 @@ public:
     static constexpr size_t Rank = RANK;
     static constexpr EntityType EntityType_ = Entity;
 
+    // the reference's main constructor (Cartesian.h:132-143): gbegin / gend
+    // are the global domain, which differs from [begin, end] only for a
+    // sub-domain of a distributed grid; the mesh spacing comes from
+    // [begin, end] in both
     Cartesian(const MultiIndex &nblocks, const MultiIndex &block_cells, const PointType &begin = PointType(0),
-              const PointType &end = PointType(1), const size_t ncomp = 1)
+              const PointType &end = PointType(1), const PointType &gbegin = PointType(0),
+              const PointType &gend = PointType(1))
+        : Cartesian(nblocks, block_cells, begin, end, size_t(1))
+    {
+        gbegin_ = gbegin;
+        gend_ = gend;
+    }
+    // several scalar components in one grid (consecutive component slabs)
+    Cartesian(const MultiIndex &nblocks, const MultiIndex &block_cells, const PointType &begin,
+              const PointType &end, const size_t ncomp)
         : nblocks_(nblocks), cells_(block_cells), ncomp_(ncomp),
-          mesh_(begin, end, MultiIndex(nblocks * block_cells))
+          mesh_(begin, end, MultiIndex(nblocks * block_cells)), gbegin_(begin), gend_(end)
     {
         cnv_grid_desc d{}; // entity: cell
         for (int k = 0; k < 3; ++k) {
@@ -819,6 +832,11 @@ public:
 
     size_t size() const { return (size_t)nblocks_[0] * nblocks_[1] * nblocks_[2]; }
     MultiIndex getSize() const { return nblocks_; }
+    // Cartesian::getGlobalSize (Cartesian.h:397): the blocks of the whole grid
+    MultiIndex getGlobalSize() const { return nblocks_; }
+    // the global domain given at construction ([begin, end] by default)
+    const PointType &getGlobalBegin() const { return gbegin_; }
+    const PointType &getGlobalEnd() const { return gend_; }
     MultiIndex getBlockCells() const { return cells_; }
     IndexRangeType getBlockRange() const { return IndexRangeType(nblocks_); }
     const MeshType &getMesh() const { return mesh_; }
@@ -939,6 +957,7 @@ private:
     MultiIndex nblocks_, cells_;
     size_t ncomp_;
     MeshType mesh_;
+    PointType gbegin_, gend_;
     cnv_grid g_ = nullptr;
     size_t block_bytes_ = 0, comp_bytes_ = 0;
     std::vector<T *> host_;

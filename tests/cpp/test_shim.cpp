@@ -243,6 +243,21 @@ static void block_access()
     EXPECT(fields(f00 - f11).getState().block_index == f11);
 }
 
+// Cartesian.h:132-143: the reference's constructor with the global domain
+// (gbegin, gend); the spacing comes from [begin, end], and an integer fifth
+// argument is the drop-in's component count
+static void cartesian_constructors()
+{
+    using P = Grid64::PointType;
+    Grid64 g({2, 2, 2}, {8, 8, 8}, P(0), P(2), P(-1), P(3));
+    EXPECT(g.getMesh().getCellSize(0)[0] == 2.0 / 16);
+    EXPECT(g.getGlobalBegin()[0] == -1.0 && g.getGlobalEnd()[2] == 3.0);
+    EXPECT(g.getGlobalSize() == MIndex({2, 2, 2}));
+    Grid64 c({2, 2, 2}, {8, 8, 8}, P(0), P(1), 3);
+    EXPECT(c.getNumComponents() == 3);
+    EXPECT(c.getGlobalBegin()[1] == 0.0 && c.getGlobalEnd()[1] == 1.0);
+}
+
 // CartesianTest.cpp:172-329 (3-D cells): every lab cell of every block equals
 // the analytic field at its coordinate to 8 eps
 static void cartesian_lab_analytic()
@@ -660,6 +675,7 @@ int main(int argc, char *argv[])
     fieldlab_reuse();
     periodic_indexer();
     block_access();
+    cartesian_constructors();
     cartesian_lab_analytic();
     laplacian_vs_host_loop();
     diffusion_run_vs_steps();
