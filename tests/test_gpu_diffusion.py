@@ -405,3 +405,40 @@ def test_two_step_kernel_multicomponent(gpu, dtype):
         assert_same(res.download(c), refs[c], "component %d" % c)
     gn = norms.download(np.float64, nsteps)
     assert np.all(np.abs(gn - tot) <= 1e-13 * tot), (gn, tot)
+
+
+def test_two_gib_component_slab(gpu):
+    """A grid whose component slab is 2^31 bytes (1024 x 1024 x 512 FP32
+    cells in 32^3 blocks): the two-step kernel, which addresses a slab with
+    32-bit byte offsets, is not used and diffusion_run takes single steps.
+    The field is a separable Fourier mode, an eigenfunction of the discrete
+    7-point Laplacian on the periodic grid, so one step multiplies it by
+    1 + dt * lambda; every cell, the highest addresses included, is checked
+    against that after one step and after a two-step run."""
+    nb, n = (32, 32, 16), (32, 32, 32)
+    NX, NY, NZ = nb[0] * n[0], nb[1] * n[1], nb[2] * n[2]
+    k = (4, 4, 2)
+    h = (1.0 / NX, 1.0 / NY, 1.0 / NZ)
+    lam = sum((2 * np.cos(2 * np.pi * k[d] * h[d]) - 2) / h[d] ** 2 for d in range(3))
+    dt = 0.1 * h[0] ** 2
+    rows = [np.cos(2 * np.pi * k[d] * (np.arange(N) + 0.5) * h[d]) for d, N in enumerate((NX, NY, NZ))]
+    u0 = (rows[2][:, None, None].astype(np.float32) * rows[1][None, :, None].astype(np.float32)
+          * rows[0][None, None, :].astype(np.float32))
+    a = cnv.Cartesian(nb, n, dtype=np.float32)
+    b = cnv.Cartesian(nb, n, dtype=np.float32)
+    assert a.component_bytes == 1 << 31
+    a.upload_global(0, u0)
+
+    def check(res, steps):
+        got = res.download_global(0)
+        f = (1.0 + dt * lam) ** steps
+        for z0 in range(0, NZ, 64):
+            d = np.abs(got[z0:z0 + 64].astype(np.float64) - f * u0[z0:z0 + 64].astype(np.float64))
+            assert float(d.max()) <= 1e-6, (steps, z0, float(d.max()))
+        # the step changed the field by much more than the tolerance
+        assert abs(1.0 - f) * float(np.abs(u0).max()) > 100e-6
+
+    cnv.diffusion_step(a, b, dt)
+    check(b, 1)
+    res = cnv.diffusion_run(a, b, 2, dt)
+    check(res, 2)
