@@ -2,7 +2,9 @@
 
 Sweep and defect: bit-identical to the oracle in FP64 and FP32 (same
 operation order, no contraction), on blocks of edge 16, 32 and 64 and on the
-full 512^3 grid of 64^3 blocks (tanh-stretched z, Dirichlet 0).  Refined solve (FP32 sweeps, FP64 defect
+full 512^3 grid of 64^3 blocks (tanh-stretched z, Dirichlet 0), and with
+periodic, zero-gradient, mirrored and Dirichlet-constant faces on blocks
+whose spacings differ per direction.  Refined solve (FP32 sweeps, FP64 defect
 and correction) to ||r||/||f|| <= 1e-12 against the direct solution of the
 independently assembled sparse matrix and against an all-FP64 Jacobi solve."""
 import numpy as np
@@ -273,3 +275,51 @@ def test_capture_after_grids_were_freed(gpu):
             del gr, a, b, f, st
     finally:
         cnv.set_tune_jacobi(np.float32, 64, *saved)
+
+
+# face BC sets as (kind, dir, side, value): 1 absorbing, 2 Dirichlet, 3 symmetry (value = sign)
+BC_SETS = {
+    "periodic": [],
+    "mixed": [(1, 1, 0, 0.0), (1, 1, 1, 0.0), (2, 2, 0, 0.5), (3, 2, 1, -1.0)],
+    "symmetric": [(3, d, s, 1.0) for d in range(3) for s in (0, 1)],
+    "dirichlet": [(2, d, s, 0.25 * (1 + d) * (1 if s else -1)) for d in range(3) for s in (0, 1)],
+}
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("bcs", list(BC_SETS), ids=list(BC_SETS))
+def test_sweep_and_defect_other_bcs(gpu, dtype, bcs):
+    """Sweep and defect bit-exact against the oracle with face BCs other than
+    Dirichlet 0 (periodic, zero-gradient, symmetric and antisymmetric
+    mirrors, Dirichlet constants) on 3 x 2 x 2 blocks of 16^3, whose x, y
+    and z spacings differ."""
+    nb, n = (3, 2, 2), (16, 16, 16)
+    NX, NY, NZ = 48, 32, 32
+    zf = stretch.tanh_faces(NZ, 2.0)
+    czm, czp, _, _ = stretch.metric(zf)
+    h = [1.0 / NX, 1.0 / NY, 1.0 / NZ]
+    make = {1: lambda d, s, v: cnv.Absorbing(d, s), 2: lambda d, s, v: cnv.Dirichlet(d, s, v),
+            3: lambda d, s, v: cnv.Symmetry(d, s, v)}
+
+    def bgrid():
+        g = cnv.Cartesian(nb, n, dtype=dtype)
+        for (k, d, s, v) in BC_SETS[bcs]:
+            g.addFaceBC(make[k](d, s, v))
+        cnv.set_zfaces(g, zf)
+        return g
+
+    rng = np.random.Generator(np.random.PCG64(len(bcs) + 31))
+    u0 = rng.standard_normal((NZ, NY, NX)).astype(dtype)
+    f0 = rng.standard_normal((NZ, NY, NX)).astype(dtype)
+    u, f, out = bgrid(), bgrid(), bgrid()
+    u.upload_global(0, u0)
+    f.upload_global(0, f0)
+    fo = oracle.Field(global_to_blocks(u0, nb, n), nb, n, bcs=face_bc_lists(nb, BC_SETS[bcs]))
+    fs = global_to_blocks(f0, nb, n)
+    cnv.jacobi_sweep(u, f, out, 2.0 / 3.0)
+    assert_same(out.download(0), oracle.jacobi(fo, fs, czm, czp, h, 2.0 / 3.0, 0), "sweep")
+    s = cnv.DeviceBuffer(8)
+    cnv.jacobi_defect(u, f, out, sumsq=s)
+    r, sref = oracle.jacobi(fo, fs, czm, czp, h, 2.0 / 3.0, 1)
+    assert_same(out.download(0), r, "defect")
+    assert abs(s.download(np.float64, 1)[0] - sref) <= 1e-12 * sref
