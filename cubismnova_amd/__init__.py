@@ -775,6 +775,24 @@ def jacobi_sweep(u, f, out, omega=2.0 / 3.0, cu=0, cf=0, co=0, stream=None):
     call("cnv_jacobi_sweep", u.handle, cu, f.handle, cf, out.handle, co, ctypes.c_double(omega), _st(stream))
 
 
+def jacobi_sweep2(u, f, out, omega=2.0 / 3.0, cu=0, cf=0, co=0, stream=None):
+    """Two sweeps u -> out in one launch (periodic or Dirichlet-constant
+    domain faces only); equal to two jacobi_sweep calls."""
+    call("cnv_jacobi_sweep2", u.handle, cu, f.handle, cf, out.handle, co, ctypes.c_double(omega), _st(stream))
+
+
+def set_tune_jacobi2(dtype, edge, wg, zchunk):
+    call("cnv_tune_set_jacobi2", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
+         int(wg), int(zchunk))
+
+
+def get_tune_jacobi2(dtype, edge):
+    wg, zc = ctypes.c_int(), ctypes.c_int()
+    call("cnv_tune_get_jacobi2", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
+         ctypes.byref(wg), ctypes.byref(zc))
+    return wg.value, zc.value
+
+
 def jacobi_defect(u, f, r, sumsq=None, cu=0, cf=0, cr=0, stream=None):
     p = sumsq.ptr if isinstance(sumsq, DeviceBuffer) else sumsq
     call("cnv_jacobi_defect", u.handle, cu, f.handle, cf, r.handle, cr, p, _st(stream))

@@ -882,6 +882,26 @@ int cnv_tune_get_jacobi(int dtype, int edge, int *wg, int *zchunk, int *prefetch
     return jacobi_get_tune(dtype, edge, wg, zchunk, prefetch);
 }
 
+int cnv_jacobi_sweep2(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int co, double omega, cnv_stream s)
+{
+    CNV_GUARD_BEGIN
+    CNV_CHECK(check_jacobi_grids(u, cu, f, cf, out, co));
+    return launch_jacobi2(u, cu, f, cf, out, co, omega, (hipStream_t)s, nullptr);
+    CNV_GUARD_END
+}
+
+int cnv_tune_set_jacobi2(int dtype, int edge, int wg, int zchunk)
+{
+    return jacobi2_set_tune(dtype, edge, wg, zchunk);
+}
+
+int cnv_tune_get_jacobi2(int dtype, int edge, int *wg, int *zchunk)
+{
+    if (!wg || !zchunk)
+        return fail(CNV_ERR_INVALID, "output pointer is NULL");
+    return jacobi2_get_tune(dtype, edge, wg, zchunk);
+}
+
 int cnv_add_correction(cnv_grid u, int cu, cnv_grid e, int ce, cnv_stream s)
 {
     CNV_GUARD_BEGIN
@@ -1001,17 +1021,26 @@ int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts
         // one cycle, recorded once and replayed (relaxed capture: see
         // cnv_graph_begin)
         CNV_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+        // sweep pairs in one launch each where the faces allow it (the same
+        // values as single sweeps, u and f read once per pair)
         int crc = CNV_OK;
         if (refine) {
+            const bool two = jacobi2_applies(work, 0, work, work) && jacobi2_applies(work, 1, work, work);
             crc = launch_fill((char *)work->data, CNV_F32, 0.0, work->component_bytes / 4, cs);
-            for (int k = 0; k < S && crc == CNV_OK; ++k)
-                crc = launch_jacobi(work, k % 2, work, 2, work, 1 - k % 2, 0, o->omega, nullptr, cs);
+            const int L = two ? S / 2 : S;
+            for (int k = 0; k < L && crc == CNV_OK; ++k)
+                crc = two ? launch_jacobi2(work, k % 2, work, 2, work, 1 - k % 2, o->omega, cs, nullptr)
+                          : launch_jacobi(work, k % 2, work, 2, work, 1 - k % 2, 0, o->omega, nullptr, cs);
             if (crc == CNV_OK)
-                crc = launch_correct(u, 0, work, S % 2, cs);
+                crc = launch_correct(u, 0, work, L % 2, cs);
         } else {
-            for (int k = 0; k < S && crc == CNV_OK; ++k)
-                crc = (k % 2 == 0) ? launch_jacobi(u, 0, f, 0, work, 0, 0, o->omega, nullptr, cs)
-                                   : launch_jacobi(work, 0, f, 0, u, 0, 0, o->omega, nullptr, cs);
+            const bool two = S % 4 == 0 && jacobi2_applies(u, 0, f, work) && jacobi2_applies(work, 0, f, u);
+            const int L = two ? S / 2 : S;
+            for (int k = 0; k < L && crc == CNV_OK; ++k) {
+                cnv_grid_s *a = k % 2 == 0 ? u : work, *b = k % 2 == 0 ? work : u;
+                crc = two ? launch_jacobi2(a, 0, f, 0, b, 0, o->omega, cs, nullptr)
+                          : launch_jacobi(a, 0, f, 0, b, 0, 0, o->omega, nullptr, cs);
+            }
         }
         if (crc == CNV_OK)
             crc = defect(cs);

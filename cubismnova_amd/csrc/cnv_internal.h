@@ -191,6 +191,17 @@ __host__ __device__ inline void resolve_faces(const DevBC *list, int n, const in
 int launch_lab(const cnv_grid_s *g, int comp, const cnv_stencil &s, bool apply_bc, bool use_list,
                const DevBC *list, int nlist, const int *block, void *out, hipStream_t st);
 
+// z metric of u (uniform unless set), uploaded on first use; caller holds
+// the grid's mutex (cnv_jacobi.hip)
+int ensure_zmetric(cnv_grid_s *g);
+// two Jacobi sweeps per launch on grids whose axes are periodic or carry one
+// Dirichlet constant per domain face (cnv_jacobi2.hip)
+bool jacobi2_applies(cnv_grid_s *u, int cu, cnv_grid_s *f, cnv_grid_s *out);
+int launch_jacobi2(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out, int co, double omega,
+                   hipStream_t st, const int *tile);
+int jacobi2_set_tune(int dtype, int edge, int wg, int zchunk);
+int jacobi2_get_tune(int dtype, int edge, int *wg, int *zchunk);
+
 int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
                     int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st);
 

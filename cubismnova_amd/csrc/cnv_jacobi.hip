@@ -398,7 +398,7 @@ int jacobi_get_tune(int dtype, int edge, int *wg, int *zchunk, int *prefetch)
 }
 
 // metric of the grid's z direction: uniform (1/h^2) unless faces were set
-static int ensure_zmetric(cnv_grid_s *g)
+int ensure_zmetric(cnv_grid_s *g)
 {
     if (g->czm_dev)
         return CNV_OK;

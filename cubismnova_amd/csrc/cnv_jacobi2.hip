@@ -1,0 +1,625 @@
+// cnv_jacobi2.hip -- two weighted-Jacobi sweeps per launch, u -> u' -> u'',
+// on the stretched-z operator of cnv_jacobi.hip, for grids whose every axis
+// is either periodic or carries a Dirichlet constant on both domain faces
+// (the same constant on every block of a face).  On such a grid every value
+// the two sweeps read is a grid cell (periodic axes wrapped) or a face
+// constant, so a tile reads u and f once, with a two-cell x/y halo of u, and
+// writes u'' once: about half the HBM traffic of two single sweeps.
+//
+// Per z step (plane p, one barrier): u'(p) on the block's cells and, for
+// planes of the tile, on the one-cell x/y ring around the block (the ring
+// cells are the neighbour blocks' edge cells, which the second sweep reads);
+// then u''(p - 1) on the block's cells from u'(p - 2), u'(p - 1), u'(p).
+// Every value is the single sweep's expression on the same operands
+// (cnv_jacobi.hip), so u'' is bit-identical to two calls of the single sweep.
+#include "cnv_device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <type_traits>
+
+namespace cnv
+{
+
+constexpr int kJ2ZMax = 64; // planes per tile at most
+
+struct J2Args {
+    const char *u;  // grid base of u (component cu)
+    const char *f;  // grid base of f (component cf)
+    char *out;      // grid base of u'' (component co)
+    Geom G;         // geometry (the three grids share it)
+    long long cs_u, cs_f, cs_o; // component strides (elements)
+    int cu, cf, co;
+    int nzc, zchunk;
+    int per[3];      // axis periodic (1) or with Dirichlet faces (0)
+    double dv[3][2]; // Dirichlet value of each domain face
+    const double *czm, *czp;
+    double ihx2, ihy2, omega;
+};
+
+template <typename T, int NX, int WG>
+struct J2Shape {
+    static constexpr int VEC = 16 / (int)sizeof(T);
+    static constexpr int TX = NX / VEC;
+    static constexpr int ROWS = WG / TX;
+    static constexpr int R = ROWS > 0 ? NX / ROWS : 1;
+    static constexpr int PA = NX + 2 * VEC;       // row pitch of both LDS plane kinds
+    static constexpr int PLA = (NX + 4) * PA;     // u: two halo layers in x and y
+    static constexpr int PLB = (NX + 2) * PA;     // u': the one-cell ring
+    static constexpr int NR = 4 * NX;             // ring cells per plane
+    static constexpr int NH = 8 * NX + 4;         // halo cells of a u plane
+    static constexpr int HS = (NH + WG - 1) / WG; // halo cells per thread
+    static constexpr size_t lds = sizeof(T) * (size_t)(2 * PLA + 2 * PLB);
+    // (FP64 tiles of more than 512 threads spill registers to scratch)
+    static constexpr bool valid = NX % VEC == 0 && WG % TX == 0 && ROWS >= 1 && ROWS <= NX && NX % ROWS == 0 &&
+                                  R <= 8 && WG >= NR && (sizeof(T) == 4 || WG <= 512) && lds + 2 * (kJ2ZMax + 2) * sizeof(double) <= 160 * 1024;
+};
+
+template <typename T, int NX, int WG>
+__global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
+{
+    using S = J2Shape<T, NX, WG>;
+    constexpr int VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, PA = S::PA, PLA = S::PLA, PLB = S::PLB;
+    constexpr int NR = S::NR, NH = S::NH, HS = S::HS;
+    static_assert(S::valid, "tile shape");
+    using V = VecT<T, VEC>;
+    constexpr long long SXY = (long long)NX * NX;
+
+    __shared__ alignas(16) T sA[2 * PLA]; // u planes p (two slots)
+    __shared__ alignas(16) T sB[2 * PLB]; // u' planes (two slots)
+    __shared__ double czl[2][kJ2ZMax + 2];
+
+    const int tid = threadIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const long long blk = tile / a.nzc;
+    const int zc = tile - (int)blk * a.nzc;
+    const int NZ = a.G.n[2];
+    const int nbx = a.G.nb[0], nby = a.G.nb[1], nbz = a.G.nb[2];
+    const int bx = (int)(blk % nbx), by = (int)((blk / nbx) % nby), bz = (int)(blk / ((long long)nbx * nby));
+    const int z0 = zc * a.zchunk;
+    const int z1 = min(NZ, z0 + a.zchunk);
+    const int nzt = z1 - z0;
+    const int NGX = nbx * NX, NGY = nby * NX, NGZ = nbz * NZ;
+    const int per0 = a.per[0], per1 = a.per[1], per2 = a.per[2];
+    const T *ubase = reinterpret_cast<const T *>(a.u) + (long long)a.cu * a.cs_u;
+    const T *fbase = reinterpret_cast<const T *>(a.f) + (long long)a.cf * a.cs_f;
+    T *obase = reinterpret_cast<T *>(a.out) + (long long)a.co * a.cs_o;
+    const long long bstride = a.G.bstride;
+    const T dx0 = (T)a.dv[0][0], dx1 = (T)a.dv[0][1], dy0 = (T)a.dv[1][0], dy1 = (T)a.dv[1][1];
+    const T dz0 = (T)a.dv[2][0], dz1 = (T)a.dv[2][1];
+
+    // value at global cell (gx, gy, gz), each at most two cells past the
+    // grid: periodic axes wrap, a cell past a Dirichlet face reads the face's
+    // value; the load itself is unconditional (a valid address either way)
+    auto fetch = [&](const T *base, int gx, int gy, int gz) -> T {
+        bool in = true;
+        T cval = (T)0;
+        if (gx < 0 || gx >= NGX) {
+            if (per0) {
+                gx = gx < 0 ? gx + NGX : gx - NGX;
+            } else {
+                in = false;
+                cval = gx < 0 ? dx0 : dx1;
+                gx = 0;
+            }
+        }
+        if (gy < 0 || gy >= NGY) {
+            if (per1) {
+                gy = gy < 0 ? gy + NGY : gy - NGY;
+            } else {
+                if (in)
+                    cval = gy < 0 ? dy0 : dy1;
+                in = false;
+                gy = 0;
+            }
+        }
+        if (gz < 0 || gz >= NGZ) {
+            if (per2) {
+                gz = gz < 0 ? gz + NGZ : gz - NGZ;
+            } else {
+                if (in)
+                    cval = gz < 0 ? dz0 : dz1;
+                in = false;
+                gz = 0;
+            }
+        }
+        const int qx = gx / NX, lx = gx - qx * NX;
+        const int qy = gy / NX, ly = gy - qy * NX;
+        const int qz = gz / NZ, lz = gz - qz * NZ;
+        const long long o = ((long long)qx + (long long)nbx * (qy + (long long)nby * qz)) * bstride + lx +
+                            (long long)NX * (ly + (long long)NX * lz);
+        const T v = base[o];
+        return in ? v : cval;
+    };
+    // base of block-local plane p of this block's column (the plane may lie
+    // in the block below or above); nullptr past a Dirichlet z face
+    auto plane_of = [&](const T *base, int p) -> const T * {
+        int gz = bz * NZ + p;
+        if (gz < 0 || gz >= NGZ) {
+            if (!per2)
+                return nullptr;
+            gz = gz < 0 ? gz + NGZ : gz - NGZ;
+        }
+        const int qz = gz / NZ, lz = gz - qz * NZ;
+        return base + ((long long)bx + (long long)nbx * (by + (long long)nby * qz)) * bstride + SXY * lz;
+    };
+    const int tx = tid % TX, ty = tid / TX;
+    const int x0 = tx * VEC;
+    auto load_cells = [&](const T *base, int p, V (&Q)[R]) {
+        const T *pl = plane_of(base, p);
+        if (pl) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                Q[r] = *reinterpret_cast<const V *>(pl + x0 + NX * (ty + r * ROWS));
+        } else {
+            const T c = bz * NZ + p < 0 ? dz0 : dz1;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v)
+                    Q[r].v[v] = c;
+        }
+    };
+    auto load_f = [&](int p, V (&Q)[R]) {
+        const T *pl = plane_of(fbase, p);
+        if (pl) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                Q[r] = load_nt<T, VEC>(pl + x0 + NX * (ty + r * ROWS));
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v)
+                    Q[r].v[v] = (T)0;
+        }
+    };
+
+    // halo cell s of a u plane: layer 1 (the ring positions, s < NR), layer 2
+    // (s < 2 NR), the four diagonal cells (s < NH)
+    int hx[HS], hy[HS];
+#pragma unroll
+    for (int h = 0; h < HS; ++h) {
+        const int s = tid + h * WG;
+        int x = 0, y = 0;
+        if (s < NR) {
+            const int k = s / NX, t = s - k * NX;
+            x = k == 0 ? -1 : k == 1 ? NX : t;
+            y = k == 0 || k == 1 ? t : (k == 2 ? -1 : NX);
+        } else if (s < 2 * NR) {
+            const int k = (s - NR) / NX, t = (s - NR) - k * NX;
+            x = k == 0 ? -2 : k == 1 ? NX + 1 : t;
+            y = k == 0 || k == 1 ? t : (k == 2 ? -2 : NX + 1);
+        } else {
+            const int d = s - 2 * NR;
+            x = (d & 1) ? NX : -1;
+            y = (d & 2) ? NX : -1;
+        }
+        hx[h] = x;
+        hy[h] = y;
+    }
+    auto gather = [&](int p, T (&H)[HS]) {
+#pragma unroll
+        for (int h = 0; h < HS; ++h)
+            H[h] = tid + h * WG < NH ? fetch(ubase, bx * NX + hx[h], by * NX + hy[h], bz * NZ + p) : (T)0;
+    };
+    // the ring cell of thread tid < NR is its halo cell 0; past a Dirichlet
+    // face it is that face's constant for u' as well
+    const bool ring_thread = tid < NR;
+    const int rx = hx[0], ry = hy[0];
+    bool ring_out = false;
+    T ring_c = (T)0;
+    if (ring_thread) {
+        if (!per0 && ((rx < 0 && bx == 0) || (rx >= NX && bx == nbx - 1))) {
+            ring_out = true;
+            ring_c = rx < 0 ? dx0 : dx1;
+        }
+        if (!per1 && ((ry < 0 && by == 0) || (ry >= NX && by == nby - 1))) {
+            ring_out = true;
+            ring_c = ry < 0 ? dy0 : dy1;
+        }
+    }
+    auto fetch_fring = [&](int p) -> T {
+        return ring_thread ? fetch(fbase, bx * NX + rx, by * NX + ry, bz * NZ + p) : (T)0;
+    };
+
+    // z metric of planes z0 - 1 .. z1 (step j holds plane z0 - 1 + j)
+    for (int k = tid; k < 2 * (nzt + 2); k += WG) {
+        const int j = k >> 1;
+        int gz = bz * NZ + z0 - 1 + j;
+        double v = 0.0;
+        if (gz >= 0 && gz < NGZ) {
+            v = (k & 1) ? a.czp[gz] : a.czm[gz];
+        } else if (per2) {
+            gz = gz < 0 ? gz + NGZ : gz - NGZ;
+            v = (k & 1) ? a.czp[gz] : a.czm[gz];
+        }
+        czl[k & 1][j] = v;
+    }
+
+    const T ihx2 = (T)a.ihx2, ihy2 = (T)a.ihy2, w = (T)a.omega;
+    // single-sweep update of one cell (cnv_jacobi.hip, MODE 0)
+    auto upd = [&](T c, T xm, T xp, T ym, T yp, T zlo, T zhi, T fv, T cm, T cp, T wd) -> T {
+        const T ax = ihx2 * (fms2(c, xp) + xm);
+        const T ay = ihy2 * (fms2(c, yp) + ym);
+        const T az = cp * (zhi - c) - cm * (c - zlo);
+        const T au = ax + ay + az;
+        const T rr = fv - au;
+        return c + rr * wd;
+    };
+
+    V U0[R], U1[R], U2[R], U3[R]; // u at planes p - 1, p, p + 1, p + 2
+    V P0[R], P1[R], P2[R];        // u' at planes p - 2, p - 1, p
+    V F1[R], FN[R], FQ[R];        // f at planes p, p + 1, p - 1
+    T H0[HS], H1[HS], H2[HS];     // halos of u at planes p, p + 1, p + 2
+    T rzm = (T)0;                 // u at the ring cell, plane p - 1
+    T fr = (T)0, frn = (T)0;      // f at the ring cell, planes p, p + 1
+    load_cells(ubase, z0 - 2, U0);
+    load_cells(ubase, z0 - 1, U1);
+    load_cells(ubase, z0, U2);
+    gather(z0 - 1, H0);
+    gather(z0, H1);
+    load_f(z0 - 1, F1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        P0[r] = U0[r];
+        P1[r] = U0[r];
+        FQ[r] = F1[r];
+    }
+
+    for (int j = 0; j < nzt + 2; ++j) {
+        const int p = z0 - 1 + j;
+        prio_ramp(j, nzt + 2);
+        T *A = sA + (j & 1) * PLA;
+        T *B = sB + (j & 1) * PLB;
+        const T *Bq = sB + ((j + 1) & 1) * PLB; // plane p - 1, written in the step before
+        // u at plane p into LDS: the block's cells and the halo
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            *reinterpret_cast<V *>(A + (ty + r * ROWS + 2) * PA + VEC + x0) = U1[r];
+#pragma unroll
+        for (int h = 0; h < HS; ++h)
+            if (tid + h * WG < NH)
+                A[(hy[h] + 2) * PA + VEC + hx[h]] = H0[h];
+        // the next planes' loads (f first: the nearest use); planes past
+        // those the tile reads (z0 - 2 .. z1 + 1) are not loaded
+        if (j <= nzt) {
+            load_f(p + 1, FN);
+            if (j < nzt) {
+                frn = fetch_fring(p + 1);
+                gather(p + 2, H2);
+            }
+            load_cells(ubase, p + 2, U3);
+        }
+        __syncthreads();
+
+        // first sweep at plane p
+        const int gzp = bz * NZ + p;
+        const bool ghost = !per2 && (gzp < 0 || gzp >= NGZ);
+        if (ghost) {
+            const T c = gzp < 0 ? dz0 : dz1;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v)
+                    P2[r].v[v] = c;
+        } else {
+            const T cm = (T)czl[0][j], cp = (T)czl[1][j];
+            const T diag = -(2 * ihx2 + 2 * ihy2 + cm + cp);
+            const T wd = w / diag;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int y = ty + r * ROWS;
+                const T *row = A + (y + 2) * PA + VEC;
+                const V ym = *reinterpret_cast<const V *>(row - PA + x0);
+                const V yp = *reinterpret_cast<const V *>(row + PA + x0);
+                const T xl = row[x0 - 1];
+                const T xr = row[x0 + VEC];
+                V res;
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    const T c = U1[r].v[v];
+                    const T xm = v == 0 ? xl : U1[r].v[v - 1];
+                    const T xp = v == VEC - 1 ? xr : U1[r].v[v + 1];
+                    res.v[v] = upd(c, xm, xp, ym.v[v], yp.v[v], U0[r].v[v], U2[r].v[v], F1[r].v[v], cm, cp, wd);
+                }
+                P2[r] = res;
+                *reinterpret_cast<V *>(B + (y + 1) * PA + VEC + x0) = res;
+            }
+            // the ring, for planes of the tile
+            if (j >= 1 && j <= nzt && ring_thread) {
+                T val = ring_c;
+                if (!ring_out) {
+                    const T *o = A + (ry + 2) * PA + VEC + rx;
+                    val = upd(H0[0], o[-1], o[1], o[-PA], o[PA], rzm, H1[0], fr, cm, cp, wd);
+                }
+                B[(ry + 1) * PA + VEC + rx] = val;
+            }
+        }
+
+        // second sweep at plane p - 1 (a plane of the tile)
+        if (j >= 2) {
+            const T cm = (T)czl[0][j - 1], cp = (T)czl[1][j - 1];
+            const T diag = -(2 * ihx2 + 2 * ihy2 + cm + cp);
+            const T wd = w / diag;
+            const int q = p - 1;
+            T *oq = obase + blk * bstride + SXY * q;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int y = ty + r * ROWS;
+                const T *row = Bq + (y + 1) * PA + VEC;
+                const V ym = *reinterpret_cast<const V *>(row - PA + x0);
+                const V yp = *reinterpret_cast<const V *>(row + PA + x0);
+                const T xl = row[x0 - 1];
+                const T xr = row[x0 + VEC];
+                V res;
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    const T c = P1[r].v[v];
+                    const T xm = v == 0 ? xl : P1[r].v[v - 1];
+                    const T xp = v == VEC - 1 ? xr : P1[r].v[v + 1];
+                    res.v[v] = upd(c, xm, xp, ym.v[v], yp.v[v], P0[r].v[v], P2[r].v[v], FQ[r].v[v], cm, cp, wd);
+                }
+                store_nt<T, VEC>(oq + x0 + NX * y, res);
+            }
+        }
+
+        // shift the z windows by one plane
+        rzm = H0[0];
+#pragma unroll
+        for (int h = 0; h < HS; ++h) {
+            H0[h] = H1[h];
+            H1[h] = H2[h];
+        }
+        fr = frn;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            U0[r] = U1[r];
+            U1[r] = U2[r];
+            U2[r] = U3[r];
+            P0[r] = P1[r];
+            P1[r] = P2[r];
+            FQ[r] = F1[r];
+            F1[r] = FN[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+struct J2Tune {
+    int wg;
+    int zchunk;
+};
+
+template <typename T, int NX, int WG>
+constexpr bool j2_valid()
+{
+    return J2Shape<T, NX, WG>::valid;
+}
+
+template <typename T, int NX, int WG>
+static bool j2_launch(const J2Args &a, int ntiles, hipStream_t st)
+{
+    if constexpr (j2_valid<T, NX, WG>()) {
+        hipLaunchKernelGGL((k_jacobi2<T, NX, WG>), dim3(ntiles), dim3(WG), 0, st, a);
+        return true;
+    }
+    return false;
+}
+
+template <typename T>
+static bool j2_dispatch(const J2Args &a, int edge, int wg, int ntiles, hipStream_t st)
+{
+#define J2_WG(NX)                                                                                       \
+    switch (wg) {                                                                                       \
+    case 64:                                                                                            \
+        return j2_launch<T, NX, 64>(a, ntiles, st);                                                     \
+    case 128:                                                                                           \
+        return j2_launch<T, NX, 128>(a, ntiles, st);                                                    \
+    case 256:                                                                                           \
+        return j2_launch<T, NX, 256>(a, ntiles, st);                                                    \
+    case 512:                                                                                           \
+        return j2_launch<T, NX, 512>(a, ntiles, st);                                                    \
+    case 1024:                                                                                          \
+        return j2_launch<T, NX, 1024>(a, ntiles, st);                                                   \
+    default:                                                                                            \
+        return false;                                                                                   \
+    }
+    switch (edge) {
+    case 16:
+        J2_WG(16)
+    case 32:
+        J2_WG(32)
+    case 64:
+        J2_WG(64)
+    default:
+        return false;
+    }
+#undef J2_WG
+}
+
+static bool j2_tile_ok(int dtype, int edge, int wg)
+{
+    auto chk = [&](auto tag) {
+        using T = decltype(tag);
+        auto w = [&](auto nx) {
+            constexpr int NX = decltype(nx)::value;
+            return wg == 64 ? j2_valid<T, NX, 64>() : wg == 128 ? j2_valid<T, NX, 128>() : wg == 256 ? j2_valid<T, NX, 256>()
+                   : wg == 512 ? j2_valid<T, NX, 512>() : wg == 1024 ? j2_valid<T, NX, 1024>() : false;
+        };
+        return edge == 16 ? w(std::integral_constant<int, 16>{}) : edge == 32 ? w(std::integral_constant<int, 32>{})
+               : edge == 64 ? w(std::integral_constant<int, 64>{}) : false;
+    };
+    return dtype == CNV_F64 ? chk(double()) : chk(float());
+}
+
+static J2Tune j2_default(int dtype, int edge)
+{
+    if (edge >= 64)
+        return {dtype == CNV_F64 ? 512 : 1024, 64};
+    if (edge >= 32)
+        return {256, 32};
+    return {dtype == CNV_F64 ? 128 : 64, 16};
+}
+
+static std::mutex g_j2_mtx;
+static std::map<std::pair<int, int>, J2Tune> g_j2_tune;
+
+static J2Tune j2_tune(int dtype, int edge)
+{
+    std::lock_guard<std::mutex> lk(g_j2_mtx);
+    auto it = g_j2_tune.find({dtype, edge});
+    return it == g_j2_tune.end() ? j2_default(dtype, edge) : it->second;
+}
+
+int jacobi2_set_tune(int dtype, int edge, int wg, int zchunk)
+{
+    if (dtype != CNV_F32 && dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    if (zchunk < 1)
+        return fail(CNV_ERR_INVALID, "zchunk must be >= 1");
+    if (!j2_tile_ok(dtype, edge, wg))
+        return fail(CNV_ERR_UNSUPPORTED, "no two-sweep Jacobi tile of this workgroup size for this dtype and block edge");
+    std::lock_guard<std::mutex> lk(g_j2_mtx);
+    g_j2_tune[{dtype, edge}] = J2Tune{wg, zchunk};
+    return CNV_OK;
+}
+
+int jacobi2_get_tune(int dtype, int edge, int *wg, int *zchunk)
+{
+    if (dtype != CNV_F32 && dtype != CNV_F64)
+        return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
+    const J2Tune t = j2_tune(dtype, edge);
+    if (wg)
+        *wg = t.wg;
+    if (zchunk)
+        *zchunk = t.zchunk;
+    return CNV_OK;
+}
+
+// The grid's faces as the two-sweep kernel needs them (component comp):
+// every axis either periodic or with a Dirichlet constant on both faces, the
+// same on every block of a face, and no other BC on any block.  Fills per
+// and dv; false otherwise.
+bool jacobi2_faces(cnv_grid_s *g, int comp, int per[3], double dv[3][2])
+{
+    const int *nb = g->desc.nblocks;
+    std::lock_guard<std::mutex> lk(g->mtx);
+    const auto &lists = g->bcs[comp];
+    const int sb[3] = {-1, -1, -1}, se[3] = {2, 2, 2};
+    bool set[3][2] = {{false, false}, {false, false}, {false, false}};
+    int kind0[3][2];
+    for (int d = 0; d < 3; ++d)
+        for (int s = 0; s < 2; ++s)
+            kind0[d][s] = -1;
+    for (long long b = 0; b < g->nblocks_total; ++b) {
+        const int bi[3] = {(int)(b % nb[0]), (int)((b / nb[0]) % nb[1]), (int)(b / ((long long)nb[0] * nb[1]))};
+        std::vector<DevBC> tmp;
+        for (const auto &x : lists[(size_t)b]) {
+            if (x.kind != CNV_BC_DIRICHLET && x.kind != CNV_BC_BASE)
+                return false;
+            tmp.push_back({x.kind, x.dir, x.side, 0, x.value});
+        }
+        int kind[6];
+        double val[6];
+        resolve_faces(tmp.data(), (int)tmp.size(), sb, se, kind, val);
+        for (int f = 0; f < 6; ++f) {
+            const int d = f >> 1, s = f & 1;
+            const bool edge = bi[d] == (s == 0 ? 0 : nb[d] - 1);
+            if (!edge) {
+                if (kind[f] != FACE_NBR)
+                    return false;
+                continue;
+            }
+            if (kind[f] != FACE_NBR && kind[f] != FACE_DIRICHLET)
+                return false;
+            if (!set[d][s]) {
+                set[d][s] = true;
+                kind0[d][s] = kind[f];
+                dv[d][s] = kind[f] == FACE_DIRICHLET ? val[f] : 0.0;
+            } else if (kind[f] != kind0[d][s] || (kind[f] == FACE_DIRICHLET && val[f] != dv[d][s])) {
+                return false;
+            }
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        const bool p0 = kind0[d][0] == FACE_NBR, p1 = kind0[d][1] == FACE_NBR;
+        if (p0 != p1)
+            return false;
+        per[d] = p0 ? 1 : 0;
+        if (per[d])
+            dv[d][0] = dv[d][1] = 0.0;
+    }
+    // a periodic z wraps planes two cells past the grid with one addition
+    if (per[2] && nb[2] * g->desc.block_cells[2] < 2)
+        return false;
+    return true;
+}
+
+// Whether the two-sweep kernel runs u -> out on these grids (u, f, out of
+// one dtype and shape, x/y block edge 16, 32 or 64, faces as above).
+bool jacobi2_applies(cnv_grid_s *u, int cu, cnv_grid_s *f, cnv_grid_s *out)
+{
+    const int *n = u->desc.block_cells;
+    if (!(n[0] == n[1] && (n[0] == 16 || n[0] == 32 || n[0] == 64)))
+        return false;
+    if (f->desc.dtype != u->desc.dtype || out->desc.dtype != u->desc.dtype)
+        return false;
+    if (u->desc.entity != CNV_CELL)
+        return false;
+    int per[3];
+    double dv[3][2];
+    return jacobi2_faces(u, cu, per, dv);
+}
+
+int launch_jacobi2(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out, int co, double omega,
+                   hipStream_t st, const int *tile)
+{
+    const int *n = u->desc.block_cells;
+    if (!(n[0] == n[1] && (n[0] == 16 || n[0] == 32 || n[0] == 64)))
+        return fail(CNV_ERR_UNSUPPORTED, "two-sweep Jacobi needs x/y block edges of 16, 32 or 64");
+    if (f->desc.dtype != u->desc.dtype || out->desc.dtype != u->desc.dtype)
+        return fail(CNV_ERR_SIZE, "u, f and out must share a dtype");
+    J2Args a;
+    if (!jacobi2_faces(u, cu, a.per, a.dv))
+        return fail(CNV_ERR_UNSUPPORTED,
+                    "two-sweep Jacobi needs every axis periodic or with one Dirichlet constant per domain face");
+    {
+        std::lock_guard<std::mutex> lk(u->mtx);
+        int rc = ensure_zmetric(u);
+        if (rc != CNV_OK)
+            return rc;
+    }
+    a.u = static_cast<const char *>(u->data);
+    a.f = static_cast<const char *>(f->data);
+    a.out = static_cast<char *>(out->data);
+    a.G = geom_of(u);
+    a.cs_u = (long long)(u->component_bytes / u->elem);
+    a.cs_f = (long long)(f->component_bytes / f->elem);
+    a.cs_o = (long long)(out->component_bytes / out->elem);
+    a.cu = cu;
+    a.cf = cf;
+    a.co = co;
+    a.czm = u->czm_dev;
+    a.czp = u->czp_dev;
+    a.ihx2 = 1. / (u->h[0] * u->h[0]);
+    a.ihy2 = 1. / (u->h[1] * u->h[1]);
+    a.omega = omega;
+    const J2Tune tu = tile ? J2Tune{tile[0], tile[1]} : j2_tune(u->desc.dtype, n[0]);
+    a.zchunk = std::min(std::min(tu.zchunk, n[2]), kJ2ZMax);
+    a.nzc = (n[2] + a.zchunk - 1) / a.zchunk;
+    const long long ntiles = u->nblocks_total * a.nzc;
+    if (ntiles >= (1LL << 31))
+        return fail(CNV_ERR_UNSUPPORTED, "grid too large for the two-sweep Jacobi kernel");
+    const bool ok = u->desc.dtype == CNV_F64 ? j2_dispatch<double>(a, n[0], tu.wg, (int)ntiles, st)
+                                             : j2_dispatch<float>(a, n[0], tu.wg, (int)ntiles, st);
+    if (!ok)
+        return fail(CNV_ERR_UNSUPPORTED, "no two-sweep Jacobi tile for this block edge and workgroup size");
+    CNV_HIP(hipGetLastError());
+    return CNV_OK;
+}
+
+} // namespace cnv

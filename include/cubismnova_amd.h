@@ -283,6 +283,16 @@ int cnv_jacobi_sweep(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int c
 /* defect r = f - A u in u's dtype, stored in r's dtype; if sumsq is
  * non-NULL the FP64 sum of r^2 (before r is rounded) is written there */
 int cnv_jacobi_defect(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid r, int cr, double *sumsq, cnv_stream s);
+/* No counterpart in the reference (see cnv_jacobi_sweep).  Two sweeps in
+ * one launch: out = S(S(u)), S the sweep above, bit-identical to two calls
+ * of cnv_jacobi_sweep, reading u and f once.  Needs every axis of u's BC
+ * lists (component cu) periodic or with one Dirichlet constant per domain
+ * face (the same on every block of the face, no other BC), x/y block edges
+ * of 16, 32 or 64, and u, f, out of one dtype; CNV_ERR_UNSUPPORTED
+ * otherwise.  out must not alias u or f. */
+int cnv_jacobi_sweep2(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int co, double omega, cnv_stream s);
+int cnv_tune_set_jacobi2(int dtype, int edge, int wg, int zchunk);
+int cnv_tune_get_jacobi2(int dtype, int edge, int *wg, int *zchunk);
 /* u (FP64) += e (FP32 or FP64), elementwise */
 int cnv_add_correction(cnv_grid u, int cu, cnv_grid e, int ce, cnv_stream s);
 /* No counterpart in the reference.  Times every (workgroup size, z-planes
