@@ -90,71 +90,72 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     const T dx0 = (T)a.dv[0][0], dx1 = (T)a.dv[0][1], dy0 = (T)a.dv[1][0], dy1 = (T)a.dv[1][1];
     const T dz0 = (T)a.dv[2][0], dz1 = (T)a.dv[2][1];
 
-    // value at global cell (gx, gy, gz), each at most two cells past the
-    // grid: periodic axes wrap, a cell past a Dirichlet face reads the face's
-    // value; the load itself is unconditional (a valid address either way)
-    auto fetch = [&](const T *base, int gx, int gy, int gz) -> T {
-        bool in = true;
-        T cval = (T)0;
-        if (gx < 0 || gx >= NGX) {
-            if (per0) {
-                gx = gx < 0 ? gx + NGX : gx - NGX;
-            } else {
-                in = false;
-                cval = gx < 0 ? dx0 : dx1;
-                gx = 0;
-            }
-        }
-        if (gy < 0 || gy >= NGY) {
-            if (per1) {
-                gy = gy < 0 ? gy + NGY : gy - NGY;
-            } else {
-                if (in)
-                    cval = gy < 0 ? dy0 : dy1;
-                in = false;
-                gy = 0;
-            }
-        }
+    // global plane p of this block's column (at most two planes past the
+    // grid) as the offset of its cells from a grid base: a periodic z wraps;
+    // past a Dirichlet face zin is false and every cell of the plane reads
+    // that face's value zc (uniform over the workgroup: one division per
+    // plane, in scalar registers)
+    struct ZPart {
+        long long off;
+        int g;  // bz * NZ + p (before wrapping)
+        int lz; // plane within its block
+        bool in;
+        T c;
+    };
+    const long long zblk = (long long)nbx * nby * bstride;
+    auto zpart = [&](int p) -> ZPart {
+        int gz = bz * NZ + p;
+        ZPart r;
+        r.off = 0;
+        r.g = gz;
+        r.lz = 0;
+        r.in = true;
+        r.c = (T)0;
         if (gz < 0 || gz >= NGZ) {
             if (per2) {
                 gz = gz < 0 ? gz + NGZ : gz - NGZ;
             } else {
-                if (in)
-                    cval = gz < 0 ? dz0 : dz1;
-                in = false;
-                gz = 0;
+                r.in = false;
+                r.c = gz < 0 ? dz0 : dz1;
+                return r;
             }
         }
-        const int qx = gx / NX, lx = gx - qx * NX;
-        const int qy = gy / NX, ly = gy - qy * NX;
         const int qz = gz / NZ, lz = gz - qz * NZ;
-        const long long o = ((long long)qx + (long long)nbx * (qy + (long long)nby * qz)) * bstride + lx +
-                            (long long)NX * (ly + (long long)NX * lz);
-        const T v = base[o];
-        return in ? v : cval;
+        r.off = qz * zblk + SXY * lz;
+        r.lz = lz;
+        return r;
     };
+    // the plane after z, without a division where it stays inside the grid
+    auto zadvance = [&](const ZPart &z) -> ZPart {
+        const int g = z.g + 1;
+        if (!z.in || g <= 0 || g >= NGZ)
+            return zpart(g - bz * NZ);
+        ZPart r = z;
+        r.g = g;
+        r.lz = z.lz + 1;
+        r.off = z.off + SXY;
+        if (r.lz == NZ) {
+            r.lz = 0;
+            r.off += zblk - NZ * SXY;
+        }
+        return r;
+    };
+    const long long colxy = ((long long)bx + (long long)nbx * by) * bstride;
     // base of block-local plane p of this block's column (the plane may lie
     // in the block below or above); nullptr past a Dirichlet z face
-    auto plane_of = [&](const T *base, int p) -> const T * {
-        int gz = bz * NZ + p;
-        if (gz < 0 || gz >= NGZ) {
-            if (!per2)
-                return nullptr;
-            gz = gz < 0 ? gz + NGZ : gz - NGZ;
-        }
-        const int qz = gz / NZ, lz = gz - qz * NZ;
-        return base + ((long long)bx + (long long)nbx * (by + (long long)nby * qz)) * bstride + SXY * lz;
+    auto plane_of = [&](const T *base, const ZPart &z) -> const T * {
+        return z.in ? base + colxy + z.off : nullptr;
     };
     const int tx = tid % TX, ty = tid / TX;
     const int x0 = tx * VEC;
-    auto load_cells = [&](const T *base, int p, V (&Q)[R]) {
-        const T *pl = plane_of(base, p);
+    auto load_cells = [&](const T *base, const ZPart &z, V (&Q)[R]) {
+        const T *pl = plane_of(base, z);
         if (pl) {
 #pragma unroll
             for (int r = 0; r < R; ++r)
                 Q[r] = *reinterpret_cast<const V *>(pl + x0 + NX * (ty + r * ROWS));
         } else {
-            const T c = bz * NZ + p < 0 ? dz0 : dz1;
+            const T c = z.c;
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -162,8 +163,8 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
                     Q[r].v[v] = c;
         }
     };
-    auto load_f = [&](int p, V (&Q)[R]) {
-        const T *pl = plane_of(fbase, p);
+    auto load_f = [&](const ZPart &z, V (&Q)[R]) {
+        const T *pl = plane_of(fbase, z);
         if (pl) {
 #pragma unroll
             for (int r = 0; r < R; ++r)
@@ -178,8 +179,14 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     };
 
     // halo cell s of a u plane: layer 1 (the ring positions, s < NR), layer 2
-    // (s < 2 NR), the four diagonal cells (s < NH)
+    // (s < 2 NR), the four diagonal cells (s < NH); per halo cell of this
+    // thread its position (hx, hy), the x/y part of its offset in the grid
+    // (hoff: periodic axes wrapped) and, for a cell past a Dirichlet x or y
+    // face, that face's value (hin false, hcv)
     int hx[HS], hy[HS];
+    long long hoff[HS];
+    bool hin[HS];
+    T hcv[HS];
 #pragma unroll
     for (int h = 0; h < HS; ++h) {
         const int s = tid + h * WG;
@@ -199,30 +206,52 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         }
         hx[h] = x;
         hy[h] = y;
+        int gx = bx * NX + x, gy = by * NX + y;
+        bool in = true;
+        T cv = (T)0;
+        if (gx < 0 || gx >= NGX) {
+            if (per0) {
+                gx = gx < 0 ? gx + NGX : gx - NGX;
+            } else {
+                in = false;
+                cv = gx < 0 ? dx0 : dx1;
+                gx = 0;
+            }
+        }
+        if (gy < 0 || gy >= NGY) {
+            if (per1) {
+                gy = gy < 0 ? gy + NGY : gy - NGY;
+            } else {
+                if (in)
+                    cv = gy < 0 ? dy0 : dy1;
+                in = false;
+                gy = 0;
+            }
+        }
+        const int qx = gx / NX, lx = gx - qx * NX;
+        const int qy = gy / NX, ly = gy - qy * NX;
+        hoff[h] = ((long long)qx + (long long)nbx * qy) * bstride + lx + (long long)NX * ly;
+        hin[h] = in;
+        hcv[h] = cv;
     }
-    auto gather = [&](int p, T (&H)[HS]) {
+    // the halo of plane p; the loads are unconditional (valid addresses)
+    auto gather = [&](const ZPart &z, T (&H)[HS]) {
 #pragma unroll
-        for (int h = 0; h < HS; ++h)
-            H[h] = tid + h * WG < NH ? fetch(ubase, bx * NX + hx[h], by * NX + hy[h], bz * NZ + p) : (T)0;
+        for (int h = 0; h < HS; ++h) {
+            const T v = ubase[hoff[h] + z.off];
+            H[h] = tid + h * WG < NH ? (hin[h] ? (z.in ? v : z.c) : hcv[h]) : (T)0;
+        }
     };
     // the ring cell of thread tid < NR is its halo cell 0; past a Dirichlet
     // face it is that face's constant for u' as well
     const bool ring_thread = tid < NR;
     const int rx = hx[0], ry = hy[0];
-    bool ring_out = false;
-    T ring_c = (T)0;
-    if (ring_thread) {
-        if (!per0 && ((rx < 0 && bx == 0) || (rx >= NX && bx == nbx - 1))) {
-            ring_out = true;
-            ring_c = rx < 0 ? dx0 : dx1;
-        }
-        if (!per1 && ((ry < 0 && by == 0) || (ry >= NX && by == nby - 1))) {
-            ring_out = true;
-            ring_c = ry < 0 ? dy0 : dy1;
-        }
-    }
-    auto fetch_fring = [&](int p) -> T {
-        return ring_thread ? fetch(fbase, bx * NX + rx, by * NX + ry, bz * NZ + p) : (T)0;
+    const bool ring_out = ring_thread && !hin[0];
+    const T ring_c = hcv[0];
+    // f at the ring cell (planes of the tile only: z is inside the grid)
+    auto fetch_fring = [&](const ZPart &z) -> T {
+        const T v = fbase[hoff[0] + z.off];
+        return ring_thread ? v : (T)0;
     };
 
     // z metric of planes z0 - 1 .. z1 (step j holds plane z0 - 1 + j)
@@ -256,12 +285,17 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     T H0[HS], H1[HS], H2[HS];     // halos of u at planes p, p + 1, p + 2
     T rzm = (T)0;                 // u at the ring cell, plane p - 1
     T fr = (T)0, frn = (T)0;      // f at the ring cell, planes p, p + 1
-    load_cells(ubase, z0 - 2, U0);
-    load_cells(ubase, z0 - 1, U1);
-    load_cells(ubase, z0, U2);
-    gather(z0 - 1, H0);
-    gather(z0, H1);
-    load_f(z0 - 1, F1);
+    {
+        const ZPart zm2 = zpart(z0 - 2), zm1 = zpart(z0 - 1), zc0 = zpart(z0);
+        load_cells(ubase, zm2, U0);
+        load_cells(ubase, zm1, U1);
+        load_cells(ubase, zc0, U2);
+        gather(zm1, H0);
+        gather(zc0, H1);
+        load_f(zm1, F1);
+    }
+    // planes p + 1 and p + 2 of the step
+    ZPart za = zpart(z0), zb = zpart(z0 + 1);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         P0[r] = U0[r];
@@ -286,12 +320,12 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         // the next planes' loads (f first: the nearest use); planes past
         // those the tile reads (z0 - 2 .. z1 + 1) are not loaded
         if (j <= nzt) {
-            load_f(p + 1, FN);
+            load_f(za, FN);
             if (j < nzt) {
-                frn = fetch_fring(p + 1);
-                gather(p + 2, H2);
+                frn = fetch_fring(za);
+                gather(zb, H2);
             }
-            load_cells(ubase, p + 2, U3);
+            load_cells(ubase, zb, U3);
         }
         __syncthreads();
 
@@ -367,6 +401,8 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         }
 
         // shift the z windows by one plane
+        za = zb;
+        zb = zadvance(zb);
         rzm = H0[0];
 #pragma unroll
         for (int h = 0; h < HS; ++h) {
@@ -459,7 +495,7 @@ static bool j2_tile_ok(int dtype, int edge, int wg)
 static J2Tune j2_default(int dtype, int edge)
 {
     if (edge >= 64)
-        return {dtype == CNV_F64 ? 512 : 1024, 64};
+        return {dtype == CNV_F64 ? 512 : 256, 64};
     if (edge >= 32)
         return {256, 32};
     return {dtype == CNV_F64 ? 128 : 64, 16};
