@@ -1034,12 +1034,14 @@ int cnv_jacobi_solve(cnv_grid u, cnv_grid f, cnv_grid work, const cnv_solve_opts
             if (crc == CNV_OK)
                 crc = launch_correct(u, 0, work, L % 2, cs);
         } else {
-            const bool two = S % 4 == 0 && jacobi2_applies(u, 0, f, work) && jacobi2_applies(work, 0, f, u);
-            const int L = two ? S / 2 : S;
-            for (int k = 0; k < L && crc == CNV_OK; ++k) {
+            // an even number of pair launches (u -> work -> u ...), then the
+            // remaining two sweeps, if any, one at a time: the cycle ends in u
+            const bool two = jacobi2_applies(u, 0, f, work) && jacobi2_applies(work, 0, f, u);
+            const int pairs = two ? (S / 2) & ~1 : 0;
+            for (int k = 0; k < pairs + (S - 2 * pairs) && crc == CNV_OK; ++k) {
                 cnv_grid_s *a = k % 2 == 0 ? u : work, *b = k % 2 == 0 ? work : u;
-                crc = two ? launch_jacobi2(a, 0, f, 0, b, 0, o->omega, cs, nullptr)
-                          : launch_jacobi(a, 0, f, 0, b, 0, 0, o->omega, nullptr, cs);
+                crc = k < pairs ? launch_jacobi2(a, 0, f, 0, b, 0, o->omega, cs, nullptr)
+                                : launch_jacobi(a, 0, f, 0, b, 0, 0, o->omega, nullptr, cs);
             }
         }
         if (crc == CNV_OK)
