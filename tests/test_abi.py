@@ -156,3 +156,26 @@ def test_ctypes_structs_match_the_header(tmp_path):
         assert int(got[name]) == ctypes.sizeof(cls), name
         for f in cls._fields_:
             assert int(got[name + "." + f[0]]) == getattr(cls, f[0]).offset, (name, f[0])
+
+
+def test_two_sweep_jacobi_tiles_host_side():
+    """The two-sweep Jacobi tile table: defaults of whole 64-plane blocks,
+    shapes without a kernel (FP64 tiles of 1024 threads spill; a 64^3 block
+    needs at least 256 threads for its ring; more rows than the block is
+    wide) refused, valid shapes kept, no GPU touched."""
+    import numpy as np
+    import cubismnova_amd as cnv
+    f32, f64 = cnv.get_tune_jacobi2(np.float32, 64), cnv.get_tune_jacobi2(np.float64, 64)
+    assert f32 == (256, 64) and f64 == (512, 64)
+    try:
+        for dt, edge, wg in ((np.float64, 64, 1024), (np.float32, 64, 128), (np.float32, 16, 128),
+                             (np.float32, 64, 96)):
+            with pytest.raises(cnv.CnvError):
+                cnv.set_tune_jacobi2(dt, edge, wg, 16)
+        cnv.set_tune_jacobi2(np.float32, 64, 1024, 32)
+        assert cnv.get_tune_jacobi2(np.float32, 64) == (1024, 32)
+        with pytest.raises(cnv.CnvError):
+            cnv.set_tune_jacobi2(np.float32, 64, 256, 0)
+    finally:
+        cnv.set_tune_jacobi2(np.float32, 64, *f32)
+        cnv.set_tune_jacobi2(np.float64, 64, *f64)
