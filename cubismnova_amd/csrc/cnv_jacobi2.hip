@@ -52,9 +52,11 @@ struct J2Shape {
     static constexpr int NH = 8 * NX + 4;         // halo cells of a u plane
     static constexpr int HS = (NH + WG - 1) / WG; // halo cells per thread
     static constexpr size_t lds = sizeof(T) * (size_t)(2 * PLA + 2 * PLB);
-    // (FP64 tiles of more than 512 threads spill registers to scratch)
+    // a ring cell per thread at most; FP64 tiles of more than 512 threads
+    // spill registers to scratch memory; LDS within one compute unit's
     static constexpr bool valid = NX % VEC == 0 && WG % TX == 0 && ROWS >= 1 && ROWS <= NX && NX % ROWS == 0 &&
-                                  R <= 8 && WG >= NR && (sizeof(T) == 4 || WG <= 512) && lds + 2 * (kJ2ZMax + 2) * sizeof(double) <= 160 * 1024;
+                                  R <= 8 && WG >= NR && (sizeof(T) == 4 || WG <= 512) &&
+                                  lds + 2 * (kJ2ZMax + 2) * sizeof(double) <= 160 * 1024;
 };
 
 template <typename T, int NX, int WG>
