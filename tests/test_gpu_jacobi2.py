@@ -5,8 +5,9 @@ against two oracle sweeps (periodic, Dirichlet-constant and mixed periodic /
 Dirichlet faces on 3 x 2 x 2 blocks whose spacings differ per direction, a
 tanh-stretched z), against two GPU single sweeps for every workgroup size and
 z-planes-per-tile of each block edge (tiles split in z, ragged last tile), on
-blocks taller than wide, and on the 512^3 grid of 64^3 blocks.  BC lists the
-launch does not cover are refused."""
+blocks taller than wide, on axes of one block and blocks of 1 to 3 planes, and
+on the 512^3 grid of 64^3 blocks.  BC lists the launch does not cover are
+refused."""
 import numpy as np
 import pytest
 
@@ -168,3 +169,49 @@ def test_sweep2_refuses_other_bcs(gpu):
     u = g(lambda x: None)
     with pytest.raises(cnv.CnvError):
         cnv.jacobi_sweep2(u, u, u, 2.0 / 3.0)  # output aliases the input
+
+
+THIN = [
+    ((1, 1, 1), (16, 16, 16), "periodic"),
+    ((1, 1, 1), (16, 16, 16), "dirichlet"),
+    ((1, 3, 2), (16, 16, 8), "x_periodic"),
+    ((2, 2, 1), (16, 16, 2), "periodic"),
+    ((2, 2, 1), (16, 16, 2), "dirichlet"),
+    ((2, 1, 1), (16, 16, 1), "dirichlet"),
+    ((1, 2, 3), (32, 32, 3), "z_periodic"),
+]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nb,n,bcs", THIN, ids=["%s-%s-%s" % ("x".join(map(str, a)), "x".join(map(str, b)), c)
+                                               for a, b, c in THIN])
+def test_sweep2_one_block_axes_and_thin_blocks(gpu, dtype, nb, n, bcs):
+    """Axes of one block (periodic neighbours are the block itself) and
+    blocks of 1, 2 and 3 planes (the two-plane z halo spans several blocks
+    or wraps): one two-sweep launch equals two single sweeps, and two oracle
+    sweeps on the single-block grids."""
+    u, f, out, mid, u0, f0, zf = setup(nb, n, dtype, BC2[bcs], 3 + n[2])
+    ref = bgrid(nb, n, dtype, BC2[bcs], zf)
+    cnv.jacobi_sweep(u, f, mid, 2.0 / 3.0)
+    cnv.jacobi_sweep(mid, f, ref, 2.0 / 3.0)
+    cnv.jacobi_sweep2(u, f, out, 2.0 / 3.0)
+    assert_same(out.download(0), ref.download(0), "two sweeps in one launch")
+    if nb == (1, 1, 1):
+        N = [nb[d] * n[d] for d in range(3)]
+        czm, czp, _, _ = stretch.metric(zf)
+        h = [1.0 / N[0], 1.0 / N[1], 1.0 / N[2]]
+        lists = face_bc_lists(nb, BC2[bcs])
+        fs = global_to_blocks(f0, nb, n)
+        r1 = oracle.jacobi(oracle.Field(global_to_blocks(u0, nb, n), nb, n, bcs=lists), fs, czm, czp, h,
+                           2.0 / 3.0, 0)
+        r2 = oracle.jacobi(oracle.Field(np.asarray(r1), nb, n, bcs=lists), fs, czm, czp, h, 2.0 / 3.0, 0)
+        assert_same(out.download(0), r2, "two oracle sweeps")
+
+
+def test_sweep2_refuses_single_plane_periodic_z(gpu):
+    """a periodic z of one plane in all would wrap the two-plane halo twice"""
+    nb, n = (2, 1, 1), (16, 16, 1)
+    zf = stretch.uniform_faces(1)
+    u, f, out = (bgrid(nb, n, np.float32, [], zf) for _ in range(3))
+    with pytest.raises(cnv.CnvError):
+        cnv.jacobi_sweep2(u, f, out, 2.0 / 3.0)
