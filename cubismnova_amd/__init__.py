@@ -781,16 +781,19 @@ def jacobi_sweep2(u, f, out, omega=2.0 / 3.0, cu=0, cf=0, co=0, stream=None):
     call("cnv_jacobi_sweep2", u.handle, cu, f.handle, cf, out.handle, co, ctypes.c_double(omega), _st(stream))
 
 
-def set_tune_jacobi2(dtype, edge, wg, zchunk):
+def set_tune_jacobi2(dtype, edge, wg, zchunk, ysplit=1):
+    """Pins the two-sweep tile: workgroup size, z planes per tile, and tiles
+    per block plane (1 whole, 2 halves in y)."""
     call("cnv_tune_set_jacobi2", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
-         int(wg), int(zchunk))
+         int(wg), int(zchunk), int(ysplit))
 
 
 def get_tune_jacobi2(dtype, edge):
-    wg, zc = ctypes.c_int(), ctypes.c_int()
+    """(wg, zchunk, ysplit) of the two-sweep tile for (dtype, block edge)."""
+    wg, zc, ys = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     call("cnv_tune_get_jacobi2", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
-         ctypes.byref(wg), ctypes.byref(zc))
-    return wg.value, zc.value
+         ctypes.byref(wg), ctypes.byref(zc), ctypes.byref(ys))
+    return wg.value, zc.value, ys.value
 
 
 def jacobi_defect(u, f, r, sumsq=None, cu=0, cf=0, cr=0, stream=None):

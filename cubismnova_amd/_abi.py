@@ -118,8 +118,8 @@ SIGNATURES = {
     "cnv_jacobi_sweep": (i32, [vp, i32, vp, i32, vp, i32, f64, vp]),
     "cnv_jacobi_defect": (i32, [vp, i32, vp, i32, vp, i32, vp, vp]),
     "cnv_jacobi_sweep2": (i32, [vp, i32, vp, i32, vp, i32, f64, vp]),
-    "cnv_tune_set_jacobi2": (i32, [i32, i32, i32, i32]),
-    "cnv_tune_get_jacobi2": (i32, [i32, i32, c_int_p, c_int_p]),
+    "cnv_tune_set_jacobi2": (i32, [i32, i32, i32, i32, i32]),
+    "cnv_tune_get_jacobi2": (i32, [i32, i32, c_int_p, c_int_p, c_int_p]),
     "cnv_add_correction": (i32, [vp, i32, vp, i32, vp]),
     "cnv_autotune_jacobi": (i32, [vp, vp, vp, i32, c_int_p, c_int_p, c_int_p, vp, i32, c_int_p]),
     "cnv_tune_set_jacobi": (i32, [i32, i32, i32, i32, i32]),

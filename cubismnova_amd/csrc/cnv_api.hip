@@ -890,16 +890,16 @@ int cnv_jacobi_sweep2(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int 
     CNV_GUARD_END
 }
 
-int cnv_tune_set_jacobi2(int dtype, int edge, int wg, int zchunk)
+int cnv_tune_set_jacobi2(int dtype, int edge, int wg, int zchunk, int ysplit)
 {
-    return jacobi2_set_tune(dtype, edge, wg, zchunk);
+    return jacobi2_set_tune(dtype, edge, wg, zchunk, ysplit);
 }
 
-int cnv_tune_get_jacobi2(int dtype, int edge, int *wg, int *zchunk)
+int cnv_tune_get_jacobi2(int dtype, int edge, int *wg, int *zchunk, int *ysplit)
 {
-    if (!wg || !zchunk)
+    if (!wg || !zchunk || !ysplit)
         return fail(CNV_ERR_INVALID, "output pointer is NULL");
-    return jacobi2_get_tune(dtype, edge, wg, zchunk);
+    return jacobi2_get_tune(dtype, edge, wg, zchunk, ysplit);
 }
 
 int cnv_add_correction(cnv_grid u, int cu, cnv_grid e, int ce, cnv_stream s)

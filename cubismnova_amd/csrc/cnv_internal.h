@@ -199,8 +199,8 @@ int ensure_zmetric(cnv_grid_s *g);
 bool jacobi2_applies(cnv_grid_s *u, int cu, cnv_grid_s *f, cnv_grid_s *out);
 int launch_jacobi2(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out, int co, double omega,
                    hipStream_t st, const int *tile);
-int jacobi2_set_tune(int dtype, int edge, int wg, int zchunk);
-int jacobi2_get_tune(int dtype, int edge, int *wg, int *zchunk);
+int jacobi2_set_tune(int dtype, int edge, int wg, int zchunk, int split);
+int jacobi2_get_tune(int dtype, int edge, int *wg, int *zchunk, int *split);
 
 int launch_stencil7(cnv_grid_s *u, cnv_grid_s *out, cnv_grid_s *lap, int comp_in, int comp_out,
                     int ncomp, int mode, double nu_dt, double *sumsq, hipStream_t st);

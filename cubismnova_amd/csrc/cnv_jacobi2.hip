@@ -39,30 +39,34 @@ struct J2Args {
     double ihx2, ihy2, omega;
 };
 
-template <typename T, int NX, int WG>
+// A tile is NX x TY cells of a block's plane (TY = NX: the whole plane; TY =
+// NX / 2: half of it, for half the LDS) times a chunk of z planes.
+template <typename T, int NX, int TY, int WG>
 struct J2Shape {
     static constexpr int VEC = 16 / (int)sizeof(T);
     static constexpr int TX = NX / VEC;
     static constexpr int ROWS = WG / TX;
-    static constexpr int R = ROWS > 0 ? NX / ROWS : 1;
+    static constexpr int R = ROWS > 0 ? TY / ROWS : 1;
     static constexpr int PA = NX + 2 * VEC;       // row pitch of both LDS plane kinds
-    static constexpr int PLA = (NX + 4) * PA;     // u: two halo layers in x and y
-    static constexpr int PLB = (NX + 2) * PA;     // u': the one-cell ring
-    static constexpr int NR = 4 * NX;             // ring cells per plane
-    static constexpr int NH = 8 * NX + 4;         // halo cells of a u plane
+    static constexpr int PLA = (TY + 4) * PA;     // u: two halo layers in x and y
+    static constexpr int PLB = (TY + 2) * PA;     // u': the one-cell ring
+    static constexpr int NR = 2 * NX + 2 * TY;    // ring cells per plane
+    static constexpr int NH = 2 * NR + 4;         // halo cells of a u plane
     static constexpr int HS = (NH + WG - 1) / WG; // halo cells per thread
     static constexpr size_t lds = sizeof(T) * (size_t)(2 * PLA + 2 * PLB);
-    // a ring cell per thread at most; FP64 tiles of more than 512 threads
-    // spill registers to scratch memory; LDS within one compute unit's
-    static constexpr bool valid = NX % VEC == 0 && WG % TX == 0 && ROWS >= 1 && ROWS <= NX && NX % ROWS == 0 &&
-                                  R <= 8 && WG >= NR && (sizeof(T) == 4 || WG <= 512) &&
+    // a ring cell per thread at most; FP64 whole-plane tiles of more than 512
+    // threads spill registers to scratch memory; LDS within one compute unit's
+    static constexpr bool valid = (TY == NX || 2 * TY == NX) && NX % VEC == 0 && WG % TX == 0 && ROWS >= 1 &&
+                                  ROWS <= TY && TY % ROWS == 0 &&
+                                  R <= 8 && WG >= NR && (sizeof(T) == 4 || WG <= 512 || TY < NX) &&
                                   lds + 2 * (kJ2ZMax + 2) * sizeof(double) <= 160 * 1024;
 };
 
-template <typename T, int NX, int WG>
+template <typename T, int NX, int TY, int WG>
 __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
 {
-    using S = J2Shape<T, NX, WG>;
+    using S = J2Shape<T, NX, TY, WG>;
+    constexpr int NSPLIT = NX / TY; // tiles per block plane
     constexpr int VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, PA = S::PA, PLA = S::PLA, PLB = S::PLB;
     constexpr int NR = S::NR, NH = S::NH, HS = S::HS;
     static_assert(S::valid, "tile shape");
@@ -74,9 +78,13 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     __shared__ double czl[2][kJ2ZMax + 2];
 
     const int tid = threadIdx.x;
+    // tiles in the order (block, z chunk, y part): the parts of one chunk
+    // are neighbours in the launch and share their halo rows in L2
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const long long blk = tile / a.nzc;
-    const int zc = tile - (int)blk * a.nzc;
+    const long long blk = tile / (a.nzc * NSPLIT);
+    const int rem = tile - (int)(blk * (a.nzc * NSPLIT));
+    const int zc = rem / NSPLIT;
+    const int y0 = (rem - zc * NSPLIT) * TY; // first block row of the tile
     const int NZ = a.G.n[2];
     const int nbx = a.G.nb[0], nby = a.G.nb[1], nbz = a.G.nb[2];
     const int bx = (int)(blk % nbx), by = (int)((blk / nbx) % nby), bz = (int)(blk / ((long long)nbx * nby));
@@ -155,7 +163,7 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         if (pl) {
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                Q[r] = *reinterpret_cast<const V *>(pl + x0 + NX * (ty + r * ROWS));
+                Q[r] = *reinterpret_cast<const V *>(pl + x0 + NX * (y0 + ty + r * ROWS));
         } else {
             const T c = z.c;
 #pragma unroll
@@ -170,7 +178,7 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         if (pl) {
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                Q[r] = load_nt<T, VEC>(pl + x0 + NX * (ty + r * ROWS));
+                Q[r] = load_nt<T, VEC>(pl + x0 + NX * (y0 + ty + r * ROWS));
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r)
@@ -180,7 +188,8 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         }
     };
 
-    // halo cell s of a u plane: layer 1 (the ring positions, s < NR), layer 2
+    // halo cell s of a u plane, at (x, y) from the tile's first cell: layer 1
+    // (the ring positions, s < NR: the x sides, then the y sides), layer 2
     // (s < 2 NR), the four diagonal cells (s < NH); per halo cell of this
     // thread its position (hx, hy), the x/y part of its offset in the grid
     // (hoff: periodic axes wrapped) and, for a cell past a Dirichlet x or y
@@ -193,22 +202,26 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     for (int h = 0; h < HS; ++h) {
         const int s = tid + h * WG;
         int x = 0, y = 0;
-        if (s < NR) {
-            const int k = s / NX, t = s - k * NX;
-            x = k == 0 ? -1 : k == 1 ? NX : t;
-            y = k == 0 || k == 1 ? t : (k == 2 ? -1 : NX);
-        } else if (s < 2 * NR) {
-            const int k = (s - NR) / NX, t = (s - NR) - k * NX;
-            x = k == 0 ? -2 : k == 1 ? NX + 1 : t;
-            y = k == 0 || k == 1 ? t : (k == 2 ? -2 : NX + 1);
+        if (s < 2 * NR) {
+            const int layer = s < NR ? 1 : 2;
+            const int i = s < NR ? s : s - NR;
+            if (i < 2 * TY) {
+                const int k = i / TY;
+                x = k == 0 ? -layer : NX - 1 + layer;
+                y = i - k * TY;
+            } else {
+                const int k = (i - 2 * TY) / NX;
+                x = (i - 2 * TY) - k * NX;
+                y = k == 0 ? -layer : TY - 1 + layer;
+            }
         } else {
             const int d = s - 2 * NR;
             x = (d & 1) ? NX : -1;
-            y = (d & 2) ? NX : -1;
+            y = (d & 2) ? TY : -1;
         }
         hx[h] = x;
         hy[h] = y;
-        int gx = bx * NX + x, gy = by * NX + y;
+        int gx = bx * NX + x, gy = by * NX + y0 + y;
         bool in = true;
         T cv = (T)0;
         if (gx < 0 || gx >= NGX) {
@@ -398,7 +411,7 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
                     const T xp = v == VEC - 1 ? xr : P1[r].v[v + 1];
                     res.v[v] = upd(c, xm, xp, ym.v[v], yp.v[v], P0[r].v[v], P2[r].v[v], FQ[r].v[v], cm, cp, wd);
                 }
-                store_nt<T, VEC>(oq + x0 + NX * y, res);
+                store_nt<T, VEC>(oq + x0 + NX * (y0 + y), res);
             }
         }
 
@@ -430,77 +443,89 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
 struct J2Tune {
     int wg;
     int zchunk;
+    int split; // tiles per block plane: 1 (whole planes) or 2 (halves in y)
 };
 
-template <typename T, int NX, int WG>
+template <typename T, int NX, int TY, int WG>
 constexpr bool j2_valid()
 {
-    return J2Shape<T, NX, WG>::valid;
+    return J2Shape<T, NX, TY, WG>::valid;
 }
 
-template <typename T, int NX, int WG>
+template <typename T, int NX, int TY, int WG>
 static bool j2_launch(const J2Args &a, int ntiles, hipStream_t st)
 {
-    if constexpr (j2_valid<T, NX, WG>()) {
-        hipLaunchKernelGGL((k_jacobi2<T, NX, WG>), dim3(ntiles), dim3(WG), 0, st, a);
+    if constexpr (j2_valid<T, NX, TY, WG>()) {
+        hipLaunchKernelGGL((k_jacobi2<T, NX, TY, WG>), dim3(ntiles), dim3(WG), 0, st, a);
         return true;
     }
     return false;
 }
 
-template <typename T>
-static bool j2_dispatch(const J2Args &a, int edge, int wg, int ntiles, hipStream_t st)
+// Calls f(integral_constant<NX>, integral_constant<TY>, integral_constant<WG>)
+// for the tile (edge, split, wg); false when there is none.
+template <typename F>
+static bool j2_with_tile(int edge, int split, int wg, F &&f)
 {
-#define J2_WG(NX)                                                                                       \
-    switch (wg) {                                                                                       \
-    case 64:                                                                                            \
-        return j2_launch<T, NX, 64>(a, ntiles, st);                                                     \
-    case 128:                                                                                           \
-        return j2_launch<T, NX, 128>(a, ntiles, st);                                                    \
-    case 256:                                                                                           \
-        return j2_launch<T, NX, 256>(a, ntiles, st);                                                    \
-    case 512:                                                                                           \
-        return j2_launch<T, NX, 512>(a, ntiles, st);                                                    \
-    case 1024:                                                                                          \
-        return j2_launch<T, NX, 1024>(a, ntiles, st);                                                   \
-    default:                                                                                            \
-        return false;                                                                                   \
-    }
+    auto by_wg = [&](auto nx, auto ty) -> bool {
+        switch (wg) {
+        case 64:
+            return f(nx, ty, std::integral_constant<int, 64>{});
+        case 128:
+            return f(nx, ty, std::integral_constant<int, 128>{});
+        case 256:
+            return f(nx, ty, std::integral_constant<int, 256>{});
+        case 512:
+            return f(nx, ty, std::integral_constant<int, 512>{});
+        case 1024:
+            return f(nx, ty, std::integral_constant<int, 1024>{});
+        default:
+            return false;
+        }
+    };
+    auto by_split = [&](auto nx) -> bool {
+        constexpr int NX = decltype(nx)::value;
+        if (split == 1)
+            return by_wg(nx, std::integral_constant<int, NX>{});
+        if (split == 2)
+            return by_wg(nx, std::integral_constant<int, NX / 2>{});
+        return false;
+    };
     switch (edge) {
     case 16:
-        J2_WG(16)
+        return by_split(std::integral_constant<int, 16>{});
     case 32:
-        J2_WG(32)
+        return by_split(std::integral_constant<int, 32>{});
     case 64:
-        J2_WG(64)
+        return by_split(std::integral_constant<int, 64>{});
     default:
         return false;
     }
-#undef J2_WG
 }
 
-static bool j2_tile_ok(int dtype, int edge, int wg)
+template <typename T>
+static bool j2_dispatch(const J2Args &a, int edge, int split, int wg, int ntiles, hipStream_t st)
 {
-    auto chk = [&](auto tag) {
-        using T = decltype(tag);
-        auto w = [&](auto nx) {
-            constexpr int NX = decltype(nx)::value;
-            return wg == 64 ? j2_valid<T, NX, 64>() : wg == 128 ? j2_valid<T, NX, 128>() : wg == 256 ? j2_valid<T, NX, 256>()
-                   : wg == 512 ? j2_valid<T, NX, 512>() : wg == 1024 ? j2_valid<T, NX, 1024>() : false;
-        };
-        return edge == 16 ? w(std::integral_constant<int, 16>{}) : edge == 32 ? w(std::integral_constant<int, 32>{})
-               : edge == 64 ? w(std::integral_constant<int, 64>{}) : false;
-    };
-    return dtype == CNV_F64 ? chk(double()) : chk(float());
+    return j2_with_tile(edge, split, wg, [&](auto nx, auto ty, auto w) {
+        return j2_launch<T, decltype(nx)::value, decltype(ty)::value, decltype(w)::value>(a, ntiles, st);
+    });
+}
+
+static bool j2_tile_ok(int dtype, int edge, int split, int wg)
+{
+    return j2_with_tile(edge, split, wg, [&](auto nx, auto ty, auto w) {
+        constexpr int NX = decltype(nx)::value, TY = decltype(ty)::value, WG = decltype(w)::value;
+        return dtype == CNV_F64 ? j2_valid<double, NX, TY, WG>() : j2_valid<float, NX, TY, WG>();
+    });
 }
 
 static J2Tune j2_default(int dtype, int edge)
 {
     if (edge >= 64)
-        return {dtype == CNV_F64 ? 512 : 256, 64};
+        return dtype == CNV_F64 ? J2Tune{256, 64, 2} : J2Tune{256, 64, 1};
     if (edge >= 32)
-        return {256, 32};
-    return {dtype == CNV_F64 ? 128 : 64, 16};
+        return {256, 32, 1};
+    return {dtype == CNV_F64 ? 128 : 64, 16, 1};
 }
 
 static std::mutex g_j2_mtx;
@@ -513,20 +538,21 @@ static J2Tune j2_tune(int dtype, int edge)
     return it == g_j2_tune.end() ? j2_default(dtype, edge) : it->second;
 }
 
-int jacobi2_set_tune(int dtype, int edge, int wg, int zchunk)
+int jacobi2_set_tune(int dtype, int edge, int wg, int zchunk, int split)
 {
     if (dtype != CNV_F32 && dtype != CNV_F64)
         return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
     if (zchunk < 1)
         return fail(CNV_ERR_INVALID, "zchunk must be >= 1");
-    if (!j2_tile_ok(dtype, edge, wg))
-        return fail(CNV_ERR_UNSUPPORTED, "no two-sweep Jacobi tile of this workgroup size for this dtype and block edge");
+    if (!j2_tile_ok(dtype, edge, split, wg))
+        return fail(CNV_ERR_UNSUPPORTED,
+                    "no two-sweep Jacobi tile of this workgroup size and split for this dtype and block edge");
     std::lock_guard<std::mutex> lk(g_j2_mtx);
-    g_j2_tune[{dtype, edge}] = J2Tune{wg, zchunk};
+    g_j2_tune[{dtype, edge}] = J2Tune{wg, zchunk, split};
     return CNV_OK;
 }
 
-int jacobi2_get_tune(int dtype, int edge, int *wg, int *zchunk)
+int jacobi2_get_tune(int dtype, int edge, int *wg, int *zchunk, int *split)
 {
     if (dtype != CNV_F32 && dtype != CNV_F64)
         return fail(CNV_ERR_INVALID, "dtype must be CNV_F32 or CNV_F64");
@@ -535,6 +561,8 @@ int jacobi2_get_tune(int dtype, int edge, int *wg, int *zchunk)
         *wg = t.wg;
     if (zchunk)
         *zchunk = t.zchunk;
+    if (split)
+        *split = t.split;
     return CNV_OK;
 }
 
@@ -646,14 +674,14 @@ int launch_jacobi2(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out
     a.ihx2 = 1. / (u->h[0] * u->h[0]);
     a.ihy2 = 1. / (u->h[1] * u->h[1]);
     a.omega = omega;
-    const J2Tune tu = tile ? J2Tune{tile[0], tile[1]} : j2_tune(u->desc.dtype, n[0]);
+    const J2Tune tu = tile ? J2Tune{tile[0], tile[1], tile[2]} : j2_tune(u->desc.dtype, n[0]);
     a.zchunk = std::min(std::min(tu.zchunk, n[2]), kJ2ZMax);
     a.nzc = (n[2] + a.zchunk - 1) / a.zchunk;
-    const long long ntiles = u->nblocks_total * a.nzc;
+    const long long ntiles = u->nblocks_total * a.nzc * tu.split;
     if (ntiles >= (1LL << 31))
         return fail(CNV_ERR_UNSUPPORTED, "grid too large for the two-sweep Jacobi kernel");
-    const bool ok = u->desc.dtype == CNV_F64 ? j2_dispatch<double>(a, n[0], tu.wg, (int)ntiles, st)
-                                             : j2_dispatch<float>(a, n[0], tu.wg, (int)ntiles, st);
+    const bool ok = u->desc.dtype == CNV_F64 ? j2_dispatch<double>(a, n[0], tu.split, tu.wg, (int)ntiles, st)
+                                             : j2_dispatch<float>(a, n[0], tu.split, tu.wg, (int)ntiles, st);
     if (!ok)
         return fail(CNV_ERR_UNSUPPORTED, "no two-sweep Jacobi tile for this block edge and workgroup size");
     CNV_HIP(hipGetLastError());

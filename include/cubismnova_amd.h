@@ -291,8 +291,11 @@ int cnv_jacobi_defect(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid r, int cr
  * of 16, 32 or 64, and u, f, out of one dtype; CNV_ERR_UNSUPPORTED
  * otherwise.  out must not alias u or f. */
 int cnv_jacobi_sweep2(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int co, double omega, cnv_stream s);
-int cnv_tune_set_jacobi2(int dtype, int edge, int wg, int zchunk);
-int cnv_tune_get_jacobi2(int dtype, int edge, int *wg, int *zchunk);
+/* Tile of the two-sweep launch per (dtype, x/y block edge): workgroup size,
+ * z planes per tile and ysplit, the tiles per block plane (1: whole planes;
+ * 2: halves in y, for half the LDS per workgroup). */
+int cnv_tune_set_jacobi2(int dtype, int edge, int wg, int zchunk, int ysplit);
+int cnv_tune_get_jacobi2(int dtype, int edge, int *wg, int *zchunk, int *ysplit);
 /* u (FP64) += e (FP32 or FP64), elementwise */
 int cnv_add_correction(cnv_grid u, int cu, cnv_grid e, int ce, cnv_stream s);
 /* No counterpart in the reference.  Times every (workgroup size, z-planes

@@ -166,14 +166,18 @@ def test_two_sweep_jacobi_tiles_host_side():
     import numpy as np
     import cubismnova_amd as cnv
     f32, f64 = cnv.get_tune_jacobi2(np.float32, 64), cnv.get_tune_jacobi2(np.float64, 64)
-    assert f32 == (256, 64) and f64 == (512, 64)
+    assert f32 == (256, 64, 1) and f64 == (256, 64, 2)
     try:
         for dt, edge, wg in ((np.float64, 64, 1024), (np.float32, 64, 128), (np.float32, 16, 128),
                              (np.float32, 64, 96)):
             with pytest.raises(cnv.CnvError):
                 cnv.set_tune_jacobi2(dt, edge, wg, 16)
         cnv.set_tune_jacobi2(np.float32, 64, 1024, 32)
-        assert cnv.get_tune_jacobi2(np.float32, 64) == (1024, 32)
+        assert cnv.get_tune_jacobi2(np.float32, 64) == (1024, 32, 1)
+        cnv.set_tune_jacobi2(np.float64, 64, 1024, 64, 2)     # half planes: no spill at 1024 threads
+        assert cnv.get_tune_jacobi2(np.float64, 64) == (1024, 64, 2)
+        with pytest.raises(cnv.CnvError):
+            cnv.set_tune_jacobi2(np.float64, 64, 512, 64, 4)
         with pytest.raises(cnv.CnvError):
             cnv.set_tune_jacobi2(np.float32, 64, 256, 0)
     finally:

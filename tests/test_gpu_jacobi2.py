@@ -73,7 +73,7 @@ def test_sweep2_matches_two_oracle_sweeps(gpu, dtype, bcs):
 
 
 def tiles2():
-    return [(wg, zc) for wg in (64, 128, 256, 512, 1024) for zc in (8, 16, 24, 32, 64)]
+    return [(wg, zc, ys) for wg in (64, 128, 256, 512, 1024) for zc in (8, 16, 24, 32, 64) for ys in (1, 2)]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -89,20 +89,25 @@ def test_every_sweep2_tile_matches_two_sweeps(gpu, dtype, edge):
     saved = cnv.get_tune_jacobi2(dtype, edge)
     try:
         ran = 0
-        for wg, zc in tiles2():
+        splits = set()
+        for wg, zc, ys in tiles2():
             try:
-                cnv.set_tune_jacobi2(dtype, edge, wg, zc)
+                cnv.set_tune_jacobi2(dtype, edge, wg, zc, ys)
             except cnv.CnvError:
                 continue
             clear(out)
             cnv.jacobi_sweep2(u, f, out, 2.0 / 3.0)
-            assert_same(out.download(0), want, "tile %d x %d" % (wg, zc))
+            assert_same(out.download(0), want, "tile %d x %d, %d per plane" % (wg, zc, ys))
             ran += 1
-        assert ran >= 4
+            splits.add(ys)
+        # (FP32 blocks of 16^2 have no half-plane tile: 64 threads span 16 rows)
+        assert ran >= 4 and (splits == {1, 2} or (dtype == np.float32 and edge == 16))
     finally:
         cnv.set_tune_jacobi2(dtype, edge, *saved)
     with pytest.raises(cnv.CnvError):
         cnv.set_tune_jacobi2(dtype, edge, 96, 16)
+    with pytest.raises(cnv.CnvError):
+        cnv.set_tune_jacobi2(dtype, edge, 256, 16, 3)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
