@@ -783,7 +783,7 @@ def jacobi_sweep2(u, f, out, omega=2.0 / 3.0, cu=0, cf=0, co=0, stream=None):
 
 def set_tune_jacobi2(dtype, edge, wg, zchunk, ysplit=1):
     """Pins the two-sweep tile: workgroup size, z planes per tile, and tiles
-    per block plane (1 whole, 2 halves in y)."""
+    per block plane (1 whole, 2 or 4 parts in y)."""
     call("cnv_tune_set_jacobi2", _abi.CNV_F64 if np.dtype(dtype) == np.float64 else _abi.CNV_F32, int(edge),
          int(wg), int(zchunk), int(ysplit))
 

@@ -40,7 +40,8 @@ struct J2Args {
 };
 
 // A tile is NX x TY cells of a block's plane (TY = NX: the whole plane; TY =
-// NX / 2: half of it, for half the LDS) times a chunk of z planes.
+// NX / 2 or NX / 4: a half or a quarter of it, for less LDS per workgroup)
+// times a chunk of z planes.
 template <typename T, int NX, int TY, int WG>
 struct J2Shape {
     static constexpr int VEC = 16 / (int)sizeof(T);
@@ -56,7 +57,7 @@ struct J2Shape {
     static constexpr size_t lds = sizeof(T) * (size_t)(2 * PLA + 2 * PLB);
     // a ring cell per thread at most; FP64 whole-plane tiles of more than 512
     // threads spill registers to scratch memory; LDS within one compute unit's
-    static constexpr bool valid = (TY == NX || 2 * TY == NX) && NX % VEC == 0 && WG % TX == 0 && ROWS >= 1 &&
+    static constexpr bool valid = (TY == NX || 2 * TY == NX || 4 * TY == NX) && NX % VEC == 0 && WG % TX == 0 && ROWS >= 1 &&
                                   ROWS <= TY && TY % ROWS == 0 &&
                                   R <= 8 && WG >= NR && (sizeof(T) == 4 || WG <= 512 || TY < NX) &&
                                   lds + 2 * (kJ2ZMax + 2) * sizeof(double) <= 160 * 1024;
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
 struct J2Tune {
     int wg;
     int zchunk;
-    int split; // tiles per block plane: 1 (whole planes) or 2 (halves in y)
+    int split; // tiles per block plane: 1 (whole planes), 2 or 4 (parts in y)
 };
 
 template <typename T, int NX, int TY, int WG>
@@ -489,6 +490,10 @@ static bool j2_with_tile(int edge, int split, int wg, F &&f)
             return by_wg(nx, std::integral_constant<int, NX>{});
         if (split == 2)
             return by_wg(nx, std::integral_constant<int, NX / 2>{});
+        if constexpr (NX >= 32) {
+            if (split == 4)
+                return by_wg(nx, std::integral_constant<int, NX / 4>{});
+        }
         return false;
     };
     switch (edge) {

@@ -293,7 +293,7 @@ int cnv_jacobi_defect(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid r, int cr
 int cnv_jacobi_sweep2(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int co, double omega, cnv_stream s);
 /* Tile of the two-sweep launch per (dtype, x/y block edge): workgroup size,
  * z planes per tile and ysplit, the tiles per block plane (1: whole planes;
- * 2: halves in y, for half the LDS per workgroup). */
+ * 2 or 4: parts in y, for less LDS per workgroup). */
 int cnv_tune_set_jacobi2(int dtype, int edge, int wg, int zchunk, int ysplit);
 int cnv_tune_get_jacobi2(int dtype, int edge, int *wg, int *zchunk, int *ysplit);
 /* u (FP64) += e (FP32 or FP64), elementwise */

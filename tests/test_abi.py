@@ -176,8 +176,9 @@ def test_two_sweep_jacobi_tiles_host_side():
         assert cnv.get_tune_jacobi2(np.float32, 64) == (1024, 32, 1)
         cnv.set_tune_jacobi2(np.float64, 64, 1024, 64, 2)     # half planes: no spill at 1024 threads
         assert cnv.get_tune_jacobi2(np.float64, 64) == (1024, 64, 2)
+        cnv.set_tune_jacobi2(np.float64, 64, 256, 64, 4)      # quarter planes
         with pytest.raises(cnv.CnvError):
-            cnv.set_tune_jacobi2(np.float64, 64, 512, 64, 4)
+            cnv.set_tune_jacobi2(np.float64, 64, 512, 64, 8)
         with pytest.raises(cnv.CnvError):
             cnv.set_tune_jacobi2(np.float32, 64, 256, 0)
     finally:

@@ -73,7 +73,7 @@ def test_sweep2_matches_two_oracle_sweeps(gpu, dtype, bcs):
 
 
 def tiles2():
-    return [(wg, zc, ys) for wg in (64, 128, 256, 512, 1024) for zc in (8, 16, 24, 32, 64) for ys in (1, 2)]
+    return [(wg, zc, ys) for wg in (64, 128, 256, 512, 1024) for zc in (8, 16, 24, 32, 64) for ys in (1, 2, 4)]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -100,14 +100,20 @@ def test_every_sweep2_tile_matches_two_sweeps(gpu, dtype, edge):
             assert_same(out.download(0), want, "tile %d x %d, %d per plane" % (wg, zc, ys))
             ran += 1
             splits.add(ys)
-        # (FP32 blocks of 16^2 have no half-plane tile: 64 threads span 16 rows)
-        assert ran >= 4 and (splits == {1, 2} or (dtype == np.float32 and edge == 16))
+        # the parts of a plane with a tile: a workgroup needs a thread per ring
+        # cell and at most as many thread rows as the part has rows
+        want_splits = {(np.float32, 16): {1}, (np.float32, 32): {1, 2}, (np.float64, 16): {1, 2}}.get(
+            (dtype, edge), {1, 2, 4})
+        assert ran >= 4 and splits == want_splits
     finally:
         cnv.set_tune_jacobi2(dtype, edge, *saved)
     with pytest.raises(cnv.CnvError):
         cnv.set_tune_jacobi2(dtype, edge, 96, 16)
     with pytest.raises(cnv.CnvError):
         cnv.set_tune_jacobi2(dtype, edge, 256, 16, 3)
+    if edge == 16:
+        with pytest.raises(cnv.CnvError):
+            cnv.set_tune_jacobi2(dtype, edge, 64, 16, 4)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
