@@ -51,15 +51,13 @@ struct J2Shape {
     static constexpr int PA = NX + 2 * VEC;       // row pitch of both LDS plane kinds
     static constexpr int PLA = (TY + 4) * PA;     // u: two halo layers in x and y
     static constexpr int PLB = (TY + 2) * PA;     // u': the one-cell ring
-    static constexpr int NR = 2 * NX + 2 * TY;    // ring cells per plane
-    static constexpr int NH = 2 * NR + 4;         // halo cells of a u plane
-    static constexpr int HS = (NH + WG - 1) / WG; // halo cells per thread
+    static constexpr int NHT = 2 * TY + 4 * TX + 4; // threads holding halo cells
     static constexpr size_t lds = sizeof(T) * (size_t)(2 * PLA + 2 * PLB);
-    // a ring cell per thread at most; FP64 whole-plane tiles of more than 512
+    // a halo role per thread at most; FP64 whole-plane tiles of more than 512
     // threads spill registers to scratch memory; LDS within one compute unit's
     static constexpr bool valid = (TY == NX || 2 * TY == NX || 4 * TY == NX) && NX % VEC == 0 && WG % TX == 0 && ROWS >= 1 &&
                                   ROWS <= TY && TY % ROWS == 0 &&
-                                  R <= 8 && WG >= NR && (sizeof(T) == 4 || WG <= 512 || TY < NX) &&
+                                  R <= 8 && WG >= NHT && (sizeof(T) == 4 || WG <= 512 || TY < NX) &&
                                   lds + 2 * (kJ2ZMax + 2) * sizeof(double) <= 160 * 1024;
 };
 
@@ -69,7 +67,6 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     using S = J2Shape<T, NX, TY, WG>;
     constexpr int NSPLIT = NX / TY; // tiles per block plane
     constexpr int VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, PA = S::PA, PLA = S::PLA, PLB = S::PLB;
-    constexpr int NR = S::NR, NH = S::NH, HS = S::HS;
     static_assert(S::valid, "tile shape");
     using V = VecT<T, VEC>;
     constexpr long long SXY = (long long)NX * NX;
@@ -189,48 +186,50 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         }
     };
 
-    // halo cell s of a u plane, at (x, y) from the tile's first cell: layer 1
-    // (the ring positions, s < NR: the x sides, then the y sides), layer 2
-    // (s < 2 NR), the four diagonal cells (s < NH); per halo cell of this
-    // thread its position (hx, hy), the x/y part of its offset in the grid
-    // (hoff: periodic axes wrapped) and, for a cell past a Dirichlet x or y
-    // face, that face's value (hin false, hcv)
-    int hx[HS], hy[HS];
-    long long hoff[HS];
-    bool hin[HS];
-    T hcv[HS];
-#pragma unroll
-    for (int h = 0; h < HS; ++h) {
-        const int s = tid + h * WG;
-        int x = 0, y = 0;
-        if (s < 2 * NR) {
-            const int layer = s < NR ? 1 : 2;
-            const int i = s < NR ? s : s - NR;
-            if (i < 2 * TY) {
-                const int k = i / TY;
-                x = k == 0 ? -layer : NX - 1 + layer;
-                y = i - k * TY;
-            } else {
-                const int k = (i - 2 * TY) / NX;
-                x = (i - 2 * TY) - k * NX;
-                y = k == 0 ? -layer : TY - 1 + layer;
-            }
-        } else {
-            const int d = s - 2 * NR;
-            x = (d & 1) ? NX : -1;
-            y = (d & 2) ? TY : -1;
-        }
-        hx[h] = x;
-        hy[h] = y;
-        int gx = bx * NX + x, gy = by * NX + y0 + y;
-        bool in = true;
-        T cv = (T)0;
+    // The halo of a u plane by thread: threads tid < 2 TY hold a pair of
+    // cells of an x side (x = -2, -1, or NX, NX + 1, at row tid % TY), the
+    // next 4 TX threads VEC cells of a row y = -2, -1, TY or TY + 1, the next
+    // four a corner cell (x = -1 or NX, y = -1 or TY); coordinates from the
+    // tile's first cell.  The cells next to the tile (x = -1, NX; y = -1,
+    // TY) are the ring: their holders compute u' there.  Per thread: the
+    // first cell (hx, hy), the x/y part of its offset in the grid (hoff:
+    // periodic axes wrapped; the held cells are contiguous in one block) and,
+    // for cells past a Dirichlet x or y face, that face's value (hin false,
+    // hcv).
+    constexpr int NXS = 2 * TY, NYS = 4 * TX;
+    int role = 0; // 0 none, 1 x-side pair, 2 row vector, 3 corner
+    int hx = 0, hy = 0;
+    int nring = 0, ringv = 0; // ring cells held and the first one's index
+    if (tid < NXS) {
+        role = 1;
+        const int k = tid / TY;
+        hx = k == 0 ? -2 : NX;
+        hy = tid - k * TY;
+        nring = 1;
+        ringv = k == 0 ? 1 : 0;
+    } else if (tid < NXS + NYS) {
+        role = 2;
+        const int i = tid - NXS, k = i / TX;
+        hx = (i - k * TX) * VEC;
+        hy = k == 0 ? -2 : k == 1 ? -1 : k == 2 ? TY : TY + 1;
+        nring = (k == 1 || k == 2) ? VEC : 0;
+    } else if (tid < NXS + NYS + 4) {
+        role = 3;
+        const int d = tid - NXS - NYS;
+        hx = (d & 1) ? NX : -1;
+        hy = (d & 2) ? TY : -1;
+    }
+    long long hoff;
+    bool hin = true;
+    T hcv = (T)0;
+    {
+        int gx = bx * NX + hx, gy = by * NX + y0 + hy;
         if (gx < 0 || gx >= NGX) {
             if (per0) {
                 gx = gx < 0 ? gx + NGX : gx - NGX;
             } else {
-                in = false;
-                cv = gx < 0 ? dx0 : dx1;
+                hin = false;
+                hcv = gx < 0 ? dx0 : dx1;
                 gx = 0;
             }
         }
@@ -238,36 +237,54 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
             if (per1) {
                 gy = gy < 0 ? gy + NGY : gy - NGY;
             } else {
-                if (in)
-                    cv = gy < 0 ? dy0 : dy1;
-                in = false;
+                if (hin)
+                    hcv = gy < 0 ? dy0 : dy1;
+                hin = false;
                 gy = 0;
             }
         }
         const int qx = gx / NX, lx = gx - qx * NX;
         const int qy = gy / NX, ly = gy - qy * NX;
-        hoff[h] = ((long long)qx + (long long)nbx * qy) * bstride + lx + (long long)NX * ly;
-        hin[h] = in;
-        hcv[h] = cv;
+        hoff = ((long long)qx + (long long)nbx * qy) * bstride + lx + (long long)NX * ly;
     }
-    // the halo of plane p; the loads are unconditional (valid addresses)
-    auto gather = [&](const ZPart &z, T (&H)[HS]) {
+    // the held halo cells of plane p; the loads read valid addresses for
+    // every role, also past a Dirichlet face
+    auto gather = [&](const ZPart &z, V &H) {
+        const T *q = ubase + hoff + z.off;
+        if (role == 1) {
+            const VecT<T, 2> t = load_vec<T, 2>(q);
+            H.v[0] = t.v[0];
+            H.v[1] = t.v[1];
+        } else if (role == 2) {
+            H = load_vec<T, VEC>(q);
+        } else if (role == 3) {
+            H.v[0] = *q;
+        }
 #pragma unroll
-        for (int h = 0; h < HS; ++h) {
-            const T v = ubase[hoff[h] + z.off];
-            H[h] = tid + h * WG < NH ? (hin[h] ? (z.in ? v : z.c) : hcv[h]) : (T)0;
+        for (int v = 0; v < VEC; ++v)
+            H.v[v] = hin ? (z.in ? H.v[v] : z.c) : hcv;
+    };
+    auto put_halo = [&](T *A, const V &H) {
+        T *o = A + (hy + 2) * PA + VEC + hx;
+        if (role == 1) {
+            o[0] = H.v[0];
+            o[1] = H.v[1];
+        } else if (role == 2) {
+            *reinterpret_cast<V *>(o) = H;
+        } else if (role == 3) {
+            o[0] = H.v[0];
         }
     };
-    // the ring cell of thread tid < NR is its halo cell 0; past a Dirichlet
-    // face it is that face's constant for u' as well
-    const bool ring_thread = tid < NR;
-    const int rx = hx[0], ry = hy[0];
-    const bool ring_out = ring_thread && !hin[0];
-    const T ring_c = hcv[0];
-    // f at the ring cell (planes of the tile only: z is inside the grid)
-    auto fetch_fring = [&](const ZPart &z) -> T {
-        const T v = fbase[hoff[0] + z.off];
-        return ring_thread ? v : (T)0;
+    // past a Dirichlet face the ring's u' is that face's constant
+    const bool ring_out = !hin;
+    // f at the held ring cells (planes of the tile only: z inside the grid)
+    auto fetch_fring = [&](const ZPart &z, V &Fr) {
+        const T *q = fbase + hoff + z.off;
+        if (role == 1) {
+            Fr.v[0] = q[ringv];
+        } else if (nring > 0) {
+            Fr = load_vec<T, VEC>(q);
+        }
     };
 
     // z metric of planes z0 - 1 .. z1 (step j holds plane z0 - 1 + j)
@@ -298,9 +315,12 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     V U0[R], U1[R], U2[R], U3[R]; // u at planes p - 1, p, p + 1, p + 2
     V P0[R], P1[R], P2[R];        // u' at planes p - 2, p - 1, p
     V F1[R], FN[R], FQ[R];        // f at planes p, p + 1, p - 1
-    T H0[HS], H1[HS], H2[HS];     // halos of u at planes p, p + 1, p + 2
-    T rzm = (T)0;                 // u at the ring cell, plane p - 1
-    T fr = (T)0, frn = (T)0;      // f at the ring cell, planes p, p + 1
+    V H0, H1, H2;                 // held halo cells of u at planes p, p + 1, p + 2
+    V RZ;                         // held halo cells of u at plane p - 1
+    V FR, FRN;                    // f at the held ring cells, planes p, p + 1
+#pragma unroll
+    for (int v = 0; v < VEC; ++v)
+        H0.v[v] = H1.v[v] = H2.v[v] = RZ.v[v] = FR.v[v] = FRN.v[v] = (T)0;
     {
         const ZPart zm2 = zpart(z0 - 2), zm1 = zpart(z0 - 1), zc0 = zpart(z0);
         load_cells(ubase, zm2, U0);
@@ -329,16 +349,13 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
 #pragma unroll
         for (int r = 0; r < R; ++r)
             *reinterpret_cast<V *>(A + (ty + r * ROWS + 2) * PA + VEC + x0) = U1[r];
-#pragma unroll
-        for (int h = 0; h < HS; ++h)
-            if (tid + h * WG < NH)
-                A[(hy[h] + 2) * PA + VEC + hx[h]] = H0[h];
+        put_halo(A, H0);
         // the next planes' loads (f first: the nearest use); planes past
         // those the tile reads (z0 - 2 .. z1 + 1) are not loaded
         if (j <= nzt) {
             load_f(za, FN);
             if (j < nzt) {
-                frn = fetch_fring(za);
+                fetch_fring(za, FRN);
                 gather(zb, H2);
             }
             load_cells(ubase, zb, U3);
@@ -379,13 +396,27 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
                 *reinterpret_cast<V *>(B + (y + 1) * PA + VEC + x0) = res;
             }
             // the ring, for planes of the tile
-            if (j >= 1 && j <= nzt && ring_thread) {
-                T val = ring_c;
-                if (!ring_out) {
-                    const T *o = A + (ry + 2) * PA + VEC + rx;
-                    val = upd(H0[0], o[-1], o[1], o[-PA], o[PA], rzm, H1[0], fr, cm, cp, wd);
+            if (j >= 1 && j <= nzt && nring > 0) {
+                const T *o = A + (hy + 2) * PA + VEC + hx;
+                T *ob = B + (hy + 1) * PA + VEC + hx;
+                if (role == 1) {
+                    // (element ringv of the pair, selected by value: a
+                    // runtime index into registers would go through scratch)
+                    const int v = ringv;
+                    const T c = v ? H0.v[1] : H0.v[0];
+                    const T zlo = v ? RZ.v[1] : RZ.v[0];
+                    const T zhi = v ? H1.v[1] : H1.v[0];
+                    ob[v] = ring_out ? hcv
+                                     : upd(c, o[v - 1], o[v + 1], o[v - PA], o[v + PA], zlo, zhi, FR.v[0], cm, cp, wd);
+                } else {
+                    V res;
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v)
+                        res.v[v] = ring_out ? hcv
+                                            : upd(H0.v[v], o[v - 1], o[v + 1], o[v - PA], o[v + PA], RZ.v[v],
+                                                  H1.v[v], FR.v[v], cm, cp, wd);
+                    *reinterpret_cast<V *>(ob) = res;
                 }
-                B[(ry + 1) * PA + VEC + rx] = val;
             }
         }
 
@@ -419,13 +450,10 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         // shift the z windows by one plane
         za = zb;
         zb = zadvance(zb);
-        rzm = H0[0];
-#pragma unroll
-        for (int h = 0; h < HS; ++h) {
-            H0[h] = H1[h];
-            H1[h] = H2[h];
-        }
-        fr = frn;
+        RZ = H0;
+        H0 = H1;
+        H1 = H2;
+        FR = FRN;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             U0[r] = U1[r];

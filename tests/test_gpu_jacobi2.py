@@ -100,10 +100,10 @@ def test_every_sweep2_tile_matches_two_sweeps(gpu, dtype, edge):
             assert_same(out.download(0), want, "tile %d x %d, %d per plane" % (wg, zc, ys))
             ran += 1
             splits.add(ys)
-        # the parts of a plane with a tile: a workgroup needs a thread per ring
-        # cell and at most as many thread rows as the part has rows
-        want_splits = {(np.float32, 16): {1}, (np.float32, 32): {1, 2}, (np.float64, 16): {1, 2}}.get(
-            (dtype, edge), {1, 2, 4})
+        # the parts of a plane with a tile: a workgroup needs a thread per
+        # halo role and at most as many thread rows as the part has rows;
+        # blocks of 16^2 are not cut in quarters
+        want_splits = {(np.float32, 16): {1}, (np.float64, 16): {1, 2}}.get((dtype, edge), {1, 2, 4})
         assert ran >= 4 and splits == want_splits
     finally:
         cnv.set_tune_jacobi2(dtype, edge, *saved)
