@@ -51,13 +51,20 @@ struct J2Shape {
     static constexpr int PA = NX + 2 * VEC;       // row pitch of both LDS plane kinds
     static constexpr int PLA = (TY + 4) * PA;     // u: two halo layers in x and y
     static constexpr int PLB = (TY + 2) * PA;     // u': the one-cell ring
-    static constexpr int NHT = 2 * TY + 4 * TX + 4; // threads holding halo cells
+    static constexpr int NHT = 2 * TY + 4 * TX + 4; // threads holding halo cells by role
+    // blocks of 16 cells per edge: one scalar halo cell per thread and slot
+    // instead (one or two waves: the roles would diverge within a wave)
+    static constexpr bool SCALAR = NX == 16;
+    static constexpr int NR1 = 2 * NX + 2 * TY;        // ring cells per plane
+    static constexpr int NH1 = 2 * NR1 + 4;            // halo cells of a u plane
+    static constexpr int HS1 = (NH1 + WG - 1) / WG;    // scalar halo cells per thread
+    static constexpr int HN = SCALAR ? HS1 : VEC;      // held halo values per thread
     static constexpr size_t lds = sizeof(T) * (size_t)(2 * PLA + 2 * PLB);
     // a halo role per thread at most; FP64 whole-plane tiles of more than 512
     // threads spill registers to scratch memory; LDS within one compute unit's
     static constexpr bool valid = (TY == NX || 2 * TY == NX || 4 * TY == NX) && NX % VEC == 0 && WG % TX == 0 && ROWS >= 1 &&
                                   ROWS <= TY && TY % ROWS == 0 &&
-                                  R <= 8 && WG >= NHT && (sizeof(T) == 4 || WG <= 512 || TY < NX) &&
+                                  R <= 8 && WG >= (SCALAR ? NR1 : NHT) && (sizeof(T) == 4 || WG <= 512 || TY < NX) &&
                                   lds + 2 * (kJ2ZMax + 2) * sizeof(double) <= 160 * 1024;
 };
 
@@ -69,6 +76,9 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     constexpr int VEC = S::VEC, TX = S::TX, ROWS = S::ROWS, R = S::R, PA = S::PA, PLA = S::PLA, PLB = S::PLB;
     static_assert(S::valid, "tile shape");
     using V = VecT<T, VEC>;
+    constexpr bool SCALAR = S::SCALAR;
+    constexpr int NR1 = S::NR1, NH1 = S::NH1, HS1 = S::HS1;
+    using HV = VecT<T, S::HN>; // the held halo values of one plane
     constexpr long long SXY = (long long)NX * NX;
 
     __shared__ alignas(16) T sA[2 * PLA]; // u planes p (two slots)
@@ -219,17 +229,25 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         hx = (d & 1) ? NX : -1;
         hy = (d & 2) ? TY : -1;
     }
-    long long hoff;
-    bool hin = true;
-    T hcv = (T)0;
-    {
-        int gx = bx * NX + hx, gy = by * NX + y0 + hy;
+    // scalar halo (blocks of 16 cells per edge): slot s of the plane's halo,
+    // layer 1 (the ring positions, s < NR1: the x sides, then the y sides),
+    // layer 2 (s < 2 NR1), the four corner cells (s < NH1)
+    int shx[HS1], shy[HS1];
+    long long shoff[HS1];
+    bool shin[HS1];
+    T shcv[HS1];
+    // x/y part of the grid offset of tile cell (x, y) (periodic axes
+    // wrapped); past a Dirichlet x or y face, in = false and cv = its value
+    auto xy_offset = [&](int x, int y, bool &in, T &cv) -> long long {
+        int gx = bx * NX + x, gy = by * NX + y0 + y;
+        in = true;
+        cv = (T)0;
         if (gx < 0 || gx >= NGX) {
             if (per0) {
                 gx = gx < 0 ? gx + NGX : gx - NGX;
             } else {
-                hin = false;
-                hcv = gx < 0 ? dx0 : dx1;
+                in = false;
+                cv = gx < 0 ? dx0 : dx1;
                 gx = 0;
             }
         }
@@ -237,53 +255,103 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
             if (per1) {
                 gy = gy < 0 ? gy + NGY : gy - NGY;
             } else {
-                if (hin)
-                    hcv = gy < 0 ? dy0 : dy1;
-                hin = false;
+                if (in)
+                    cv = gy < 0 ? dy0 : dy1;
+                in = false;
                 gy = 0;
             }
         }
         const int qx = gx / NX, lx = gx - qx * NX;
         const int qy = gy / NX, ly = gy - qy * NX;
-        hoff = ((long long)qx + (long long)nbx * qy) * bstride + lx + (long long)NX * ly;
+        return ((long long)qx + (long long)nbx * qy) * bstride + lx + (long long)NX * ly;
+    };
+    if constexpr (SCALAR) {
+#pragma unroll
+        for (int h = 0; h < HS1; ++h) {
+            const int s = tid + h * WG;
+            int x = 0, y = 0;
+            if (s < 2 * NR1) {
+                const int layer = s < NR1 ? 1 : 2;
+                const int i = s < NR1 ? s : s - NR1;
+                if (i < 2 * TY) {
+                    const int k = i / TY;
+                    x = k == 0 ? -layer : NX - 1 + layer;
+                    y = i - k * TY;
+                } else {
+                    const int k = (i - 2 * TY) / NX;
+                    x = (i - 2 * TY) - k * NX;
+                    y = k == 0 ? -layer : TY - 1 + layer;
+                }
+            } else {
+                const int d = s - 2 * NR1;
+                x = (d & 1) ? NX : -1;
+                y = (d & 2) ? TY : -1;
+            }
+            shx[h] = x;
+            shy[h] = y;
+            shoff[h] = xy_offset(x, y, shin[h], shcv[h]);
+        }
     }
+    bool hin;
+    T hcv;
+    const long long hoff = xy_offset(hx, hy, hin, hcv);
     // the held halo cells of plane p; the loads read valid addresses for
     // every role, also past a Dirichlet face
-    auto gather = [&](const ZPart &z, V &H) {
-        const T *q = ubase + hoff + z.off;
-        if (role == 1) {
-            const VecT<T, 2> t = load_vec<T, 2>(q);
-            H.v[0] = t.v[0];
-            H.v[1] = t.v[1];
-        } else if (role == 2) {
-            H = load_vec<T, VEC>(q);
-        } else if (role == 3) {
-            H.v[0] = *q;
-        }
+    auto gather = [&](const ZPart &z, HV &H) {
+        if constexpr (SCALAR) {
 #pragma unroll
-        for (int v = 0; v < VEC; ++v)
-            H.v[v] = hin ? (z.in ? H.v[v] : z.c) : hcv;
+            for (int h = 0; h < HS1; ++h) {
+                const T v = ubase[shoff[h] + z.off];
+                H.v[h] = tid + h * WG < NH1 ? (shin[h] ? (z.in ? v : z.c) : shcv[h]) : (T)0;
+            }
+        } else {
+            const T *q = ubase + hoff + z.off;
+            if (role == 1) {
+                const VecT<T, 2> t = load_vec<T, 2>(q);
+                H.v[0] = t.v[0];
+                H.v[1] = t.v[1];
+            } else if (role == 2) {
+                H = load_vec<T, VEC>(q);
+            } else if (role == 3) {
+                H.v[0] = *q;
+            }
+#pragma unroll
+            for (int v = 0; v < VEC; ++v)
+                H.v[v] = hin ? (z.in ? H.v[v] : z.c) : hcv;
+        }
     };
-    auto put_halo = [&](T *A, const V &H) {
-        T *o = A + (hy + 2) * PA + VEC + hx;
-        if (role == 1) {
-            o[0] = H.v[0];
-            o[1] = H.v[1];
-        } else if (role == 2) {
-            *reinterpret_cast<V *>(o) = H;
-        } else if (role == 3) {
-            o[0] = H.v[0];
+    auto put_halo = [&](T *A, const HV &H) {
+        if constexpr (SCALAR) {
+#pragma unroll
+            for (int h = 0; h < HS1; ++h)
+                if (tid + h * WG < NH1)
+                    A[(shy[h] + 2) * PA + VEC + shx[h]] = H.v[h];
+        } else {
+            T *o = A + (hy + 2) * PA + VEC + hx;
+            if (role == 1) {
+                o[0] = H.v[0];
+                o[1] = H.v[1];
+            } else if (role == 2) {
+                *reinterpret_cast<V *>(o) = H;
+            } else if (role == 3) {
+                o[0] = H.v[0];
+            }
         }
     };
     // past a Dirichlet face the ring's u' is that face's constant
-    const bool ring_out = !hin;
+    const bool ring_out = SCALAR ? !shin[0] : !hin;
     // f at the held ring cells (planes of the tile only: z inside the grid)
     auto fetch_fring = [&](const ZPart &z, V &Fr) {
-        const T *q = fbase + hoff + z.off;
-        if (role == 1) {
-            Fr.v[0] = q[ringv];
-        } else if (nring > 0) {
-            Fr = load_vec<T, VEC>(q);
+        if constexpr (SCALAR) {
+            const T v = fbase[shoff[0] + z.off];
+            Fr.v[0] = tid < NR1 ? v : (T)0;
+        } else {
+            const T *q = fbase + hoff + z.off;
+            if (role == 1) {
+                Fr.v[0] = q[ringv];
+            } else if (nring > 0) {
+                Fr = load_vec<T, VEC>(q);
+            }
         }
     };
 
@@ -315,12 +383,15 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     V U0[R], U1[R], U2[R], U3[R]; // u at planes p - 1, p, p + 1, p + 2
     V P0[R], P1[R], P2[R];        // u' at planes p - 2, p - 1, p
     V F1[R], FN[R], FQ[R];        // f at planes p, p + 1, p - 1
-    V H0, H1, H2;                 // held halo cells of u at planes p, p + 1, p + 2
-    V RZ;                         // held halo cells of u at plane p - 1
+    HV H0, H1, H2;                // held halo cells of u at planes p, p + 1, p + 2
+    HV RZ;                        // held halo cells of u at plane p - 1
     V FR, FRN;                    // f at the held ring cells, planes p, p + 1
 #pragma unroll
+    for (int v = 0; v < S::HN; ++v)
+        H0.v[v] = H1.v[v] = H2.v[v] = RZ.v[v] = (T)0;
+#pragma unroll
     for (int v = 0; v < VEC; ++v)
-        H0.v[v] = H1.v[v] = H2.v[v] = RZ.v[v] = FR.v[v] = FRN.v[v] = (T)0;
+        FR.v[v] = FRN.v[v] = (T)0;
     {
         const ZPart zm2 = zpart(z0 - 2), zm1 = zpart(z0 - 1), zc0 = zpart(z0);
         load_cells(ubase, zm2, U0);
@@ -396,7 +467,16 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
                 *reinterpret_cast<V *>(B + (y + 1) * PA + VEC + x0) = res;
             }
             // the ring, for planes of the tile
-            if (j >= 1 && j <= nzt && nring > 0) {
+            if constexpr (SCALAR) {
+                // the ring cell of thread tid < NR1 is its halo slot 0
+                if (j >= 1 && j <= nzt && tid < NR1) {
+                    const int rx = shx[0], ry = shy[0];
+                    const T *o = A + (ry + 2) * PA + VEC + rx;
+                    B[(ry + 1) * PA + VEC + rx] =
+                        ring_out ? shcv[0]
+                                 : upd(H0.v[0], o[-1], o[1], o[-PA], o[PA], RZ.v[0], H1.v[0], FR.v[0], cm, cp, wd);
+                }
+            } else if (j >= 1 && j <= nzt && nring > 0) {
                 const T *o = A + (hy + 2) * PA + VEC + hx;
                 T *ob = B + (hy + 1) * PA + VEC + hx;
                 if (role == 1) {
