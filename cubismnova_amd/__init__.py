@@ -776,8 +776,9 @@ def jacobi_sweep(u, f, out, omega=2.0 / 3.0, cu=0, cf=0, co=0, stream=None):
 
 
 def jacobi_sweep2(u, f, out, omega=2.0 / 3.0, cu=0, cf=0, co=0, stream=None):
-    """Two sweeps u -> out in one launch (periodic or Dirichlet-constant
-    domain faces only); equal to two jacobi_sweep calls."""
+    """Two sweeps u -> out in one launch, equal to two jacobi_sweep calls
+    (each domain face one rule on all its blocks: periodic, Dirichlet,
+    Symmetry or Absorbing)."""
     call("cnv_jacobi_sweep2", u.handle, cu, f.handle, cf, out.handle, co, ctypes.c_double(omega), _st(stream))
 
 

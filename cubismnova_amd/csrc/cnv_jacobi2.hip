@@ -1,10 +1,12 @@
 // cnv_jacobi2.hip -- two weighted-Jacobi sweeps per launch, u -> u' -> u'',
-// on the stretched-z operator of cnv_jacobi.hip, for grids whose every axis
-// is either periodic or carries a Dirichlet constant on both domain faces
-// (the same constant on every block of a face).  On such a grid every value
-// the two sweeps read is a grid cell (periodic axes wrapped) or a face
-// constant, so a tile reads u and f once, with a two-cell x/y halo of u, and
-// writes u'' once: about half the HBM traffic of two single sweeps.
+// on the stretched-z operator of cnv_jacobi.hip, for grids whose BC lists
+// give every domain face one rule on all its blocks (periodic on both sides
+// of an axis, or per side a Dirichlet constant, a Symmetry mirror or an
+// Absorbing edge copy) and neighbour copies inside the domain.  On such a
+// grid every value the two sweeps read is a grid cell (wrapped or mirrored,
+// times the mirror's sign) or a face constant, so a tile reads u and f once,
+// with a two-cell x/y halo of u, and writes u'' once: about half the HBM
+// traffic of two single sweeps.
 //
 // Per z step (plane p, one barrier): u'(p) on the block's cells and, for
 // planes of the tile, on the one-cell x/y ring around the block (the ring
@@ -25,6 +27,14 @@ namespace cnv
 
 constexpr int kJ2ZMax = 64; // planes per tile at most
 
+// Rule of a domain face as the two-sweep kernel applies it to the ghost
+// cells the sweeps read past it: the periodic neighbour, a constant
+// (Dirichlet, BC/Dirichlet.h:75-113), sign times the mirror cell (Symmetry,
+// BC/Symmetry.h:69-114: ghost layer l reads interior cell l - 1 from the
+// face), or the face's own cell (Absorbing, BC/Absorbing.h:66-105: every
+// layer reads interior cell 0 from the face).
+enum { FM_PERIODIC = 0, FM_CONST = 1, FM_MIRROR = 2, FM_EDGE = 3 };
+
 struct J2Args {
     const char *u;  // grid base of u (component cu)
     const char *f;  // grid base of f (component cf)
@@ -33,8 +43,8 @@ struct J2Args {
     long long cs_u, cs_f, cs_o; // component strides (elements)
     int cu, cf, co;
     int nzc, zchunk;
-    int per[3];      // axis periodic (1) or with Dirichlet faces (0)
-    double dv[3][2]; // Dirichlet value of each domain face
+    int fm[3][2];    // domain face rule: FM_PERIODIC, FM_CONST, FM_MIRROR, FM_EDGE
+    double fv[3][2]; // its constant (FM_CONST) or sign (FM_MIRROR)
     const double *czm, *czp;
     double ihx2, ihy2, omega;
 };
@@ -68,7 +78,9 @@ struct J2Shape {
                                   lds + 2 * (kJ2ZMax + 2) * sizeof(double) <= 160 * 1024;
 };
 
-template <typename T, int NX, int TY, int WG>
+// MIR: some domain face is a mirror or edge face (the code for those rules
+// is compiled into this variant only)
+template <typename T, int NX, int TY, int WG, bool MIR>
 __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
 {
     using S = J2Shape<T, NX, TY, WG>;
@@ -100,25 +112,49 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     const int z1 = min(NZ, z0 + a.zchunk);
     const int nzt = z1 - z0;
     const int NGX = nbx * NX, NGY = nby * NX, NGZ = nbz * NZ;
-    const int per0 = a.per[0], per1 = a.per[1], per2 = a.per[2];
+    const int per2 = a.fm[2][0] == FM_PERIODIC;
     const T *ubase = reinterpret_cast<const T *>(a.u) + (long long)a.cu * a.cs_u;
     const T *fbase = reinterpret_cast<const T *>(a.f) + (long long)a.cf * a.cs_f;
     T *obase = reinterpret_cast<T *>(a.out) + (long long)a.co * a.cs_o;
     const long long bstride = a.G.bstride;
-    const T dx0 = (T)a.dv[0][0], dx1 = (T)a.dv[0][1], dy0 = (T)a.dv[1][0], dy1 = (T)a.dv[1][1];
-    const T dz0 = (T)a.dv[2][0], dz1 = (T)a.dv[2][1];
+    // global index g on axis d (at most two cells past the grid of n cells)
+    // to the cell it reads: periodic wrap, mirror or edge cell (m = the
+    // factor it is read with), or none (in = false; the face's constant cv)
+    auto face_map = [&](int d, int g, int n, bool &in, T &cv, T &m) -> int {
+        if (g >= 0 && g < n)
+            return g;
+        const int side = g < 0 ? 0 : 1;
+        const int rule = a.fm[d][side];
+        if (rule == FM_PERIODIC)
+            return g < 0 ? g + n : g - n;
+        if (rule == FM_CONST) {
+            if (in)
+                cv = (T)a.fv[d][side];
+            in = false;
+            return 0;
+        }
+        if constexpr (MIR) {
+            if (rule == FM_MIRROR) {
+                m = m * (T)a.fv[d][side];
+                return g < 0 ? -1 - g : 2 * n - 1 - g;
+            }
+            return g < 0 ? 0 : n - 1;
+        }
+        return 0;
+    };
 
     // global plane p of this block's column (at most two planes past the
-    // grid) as the offset of its cells from a grid base: a periodic z wraps;
-    // past a Dirichlet face zin is false and every cell of the plane reads
-    // that face's value zc (uniform over the workgroup: one division per
-    // plane, in scalar registers)
+    // grid) as the offset of its cells from a grid base, by the z faces'
+    // rules: in false past a constant face (every cell of the plane reads
+    // c), m the factor of a mirror face (uniform over the workgroup: one
+    // division per plane, in scalar registers)
     struct ZPart {
         long long off;
-        int g;  // bz * NZ + p (before wrapping)
+        int g;  // bz * NZ + p (before the face rule)
         int lz; // plane within its block
         bool in;
         T c;
+        T m;
     };
     const long long zblk = (long long)nbx * nby * bstride;
     auto zpart = [&](int p) -> ZPart {
@@ -129,15 +165,10 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         r.lz = 0;
         r.in = true;
         r.c = (T)0;
-        if (gz < 0 || gz >= NGZ) {
-            if (per2) {
-                gz = gz < 0 ? gz + NGZ : gz - NGZ;
-            } else {
-                r.in = false;
-                r.c = gz < 0 ? dz0 : dz1;
-                return r;
-            }
-        }
+        r.m = (T)1;
+        gz = face_map(2, gz, NGZ, r.in, r.c, r.m);
+        if (!r.in)
+            return r;
         const int qz = gz / NZ, lz = gz - qz * NZ;
         r.off = qz * zblk + SXY * lz;
         r.lz = lz;
@@ -172,6 +203,13 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
 #pragma unroll
             for (int r = 0; r < R; ++r)
                 Q[r] = *reinterpret_cast<const V *>(pl + x0 + NX * (y0 + ty + r * ROWS));
+            if (MIR && z.m != (T)1) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v)
+                        Q[r].v[v] = Q[r].v[v] * z.m;
+            }
         } else {
             const T c = z.c;
 #pragma unroll
@@ -235,32 +273,16 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
     int shx[HS1], shy[HS1];
     long long shoff[HS1];
     bool shin[HS1];
-    T shcv[HS1];
-    // x/y part of the grid offset of tile cell (x, y) (periodic axes
-    // wrapped); past a Dirichlet x or y face, in = false and cv = its value
-    auto xy_offset = [&](int x, int y, bool &in, T &cv) -> long long {
-        int gx = bx * NX + x, gy = by * NX + y0 + y;
+    T shcv[HS1], shm[HS1];
+    // x/y part of the grid offset of the cell that tile cell (x, y) reads by
+    // the x and y face rules, with its factor m; past a constant face, in =
+    // false and cv = the face's value
+    auto xy_offset = [&](int x, int y, bool &in, T &cv, T &m) -> long long {
         in = true;
         cv = (T)0;
-        if (gx < 0 || gx >= NGX) {
-            if (per0) {
-                gx = gx < 0 ? gx + NGX : gx - NGX;
-            } else {
-                in = false;
-                cv = gx < 0 ? dx0 : dx1;
-                gx = 0;
-            }
-        }
-        if (gy < 0 || gy >= NGY) {
-            if (per1) {
-                gy = gy < 0 ? gy + NGY : gy - NGY;
-            } else {
-                if (in)
-                    cv = gy < 0 ? dy0 : dy1;
-                in = false;
-                gy = 0;
-            }
-        }
+        m = (T)1;
+        const int gx = face_map(0, bx * NX + x, NGX, in, cv, m);
+        const int gy = face_map(1, by * NX + y0 + y, NGY, in, cv, m);
         const int qx = gx / NX, lx = gx - qx * NX;
         const int qy = gy / NX, ly = gy - qy * NX;
         return ((long long)qx + (long long)nbx * qy) * bstride + lx + (long long)NX * ly;
@@ -289,12 +311,27 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
             }
             shx[h] = x;
             shy[h] = y;
-            shoff[h] = xy_offset(x, y, shin[h], shcv[h]);
+            shoff[h] = xy_offset(x, y, shin[h], shcv[h], shm[h]);
         }
     }
     bool hin;
-    T hcv;
-    const long long hoff = xy_offset(hx, hy, hin, hcv);
+    T hcv, hm;
+    const long long hoff = xy_offset(hx, hy, hin, hcv, hm);
+    // an x-side pair is one vector load unless a mirror face reverses it
+    long long hoff2 = hoff + 1;
+    if (role == 1) {
+        bool in2;
+        T cv2, m2;
+        hoff2 = xy_offset(hx + 1, hy, in2, cv2, m2);
+    }
+    const bool pair_vec = !MIR || hoff2 == hoff + 1;
+    // tile cell (x, y) past a mirror or edge face: its u' is written by the
+    // thread that computes the cell it reads
+    auto mirrored = [&](int x, int y) -> bool {
+        const int gx = bx * NX + x, gy = by * NX + y0 + y;
+        const bool ox = gx < 0 || gx >= NGX, oy = gy < 0 || gy >= NGY;
+        return (ox && a.fm[0][gx < 0 ? 0 : 1] >= FM_MIRROR) || (oy && a.fm[1][gy < 0 ? 0 : 1] >= FM_MIRROR);
+    };
     // the held halo cells of plane p; the loads read valid addresses for
     // every role, also past a Dirichlet face
     auto gather = [&](const ZPart &z, HV &H) {
@@ -302,14 +339,20 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
 #pragma unroll
             for (int h = 0; h < HS1; ++h) {
                 const T v = ubase[shoff[h] + z.off];
-                H.v[h] = tid + h * WG < NH1 ? (shin[h] ? (z.in ? v : z.c) : shcv[h]) : (T)0;
+                const T vm = MIR ? v * shm[h] * z.m : v;
+                H.v[h] = tid + h * WG < NH1 ? (shin[h] ? (z.in ? vm : z.c) : shcv[h]) : (T)0;
             }
         } else {
             const T *q = ubase + hoff + z.off;
             if (role == 1) {
-                const VecT<T, 2> t = load_vec<T, 2>(q);
-                H.v[0] = t.v[0];
-                H.v[1] = t.v[1];
+                if (pair_vec) {
+                    const VecT<T, 2> t = load_vec<T, 2>(q);
+                    H.v[0] = t.v[0];
+                    H.v[1] = t.v[1];
+                } else {
+                    H.v[0] = q[0];
+                    H.v[1] = ubase[hoff2 + z.off];
+                }
             } else if (role == 2) {
                 H = load_vec<T, VEC>(q);
             } else if (role == 3) {
@@ -317,7 +360,7 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
             }
 #pragma unroll
             for (int v = 0; v < VEC; ++v)
-                H.v[v] = hin ? (z.in ? H.v[v] : z.c) : hcv;
+                H.v[v] = hin ? (z.in ? (MIR ? H.v[v] * hm * z.m : H.v[v]) : z.c) : hcv;
         }
     };
     auto put_halo = [&](T *A, const HV &H) {
@@ -338,8 +381,28 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
             }
         }
     };
-    // past a Dirichlet face the ring's u' is that face's constant
+    // past a constant face the ring's u' is that face's constant; past a
+    // mirror or edge face the thread that computes the cell it reads
+    // writes it (ring_skip)
     const bool ring_out = SCALAR ? !shin[0] : !hin;
+    const bool ring_skip =
+        MIR && (SCALAR ? mirrored(shx[0], shy[0]) : (role == 1 ? mirrored(hx + ringv, hy) : mirrored(hx, hy)));
+    // the ghost u' this thread writes for a mirror or edge face next to
+    // the tile: the factor (0: none) for the cell left of its first cell
+    // (x0 = 0), right of its last (x0 = NX - VEC), and for the row below
+    // row 0 / above row TY - 1 of its cells
+    auto ghost_factor = [&](int d, int side, bool at_edge) -> T {
+        if (!MIR || !at_edge || a.fm[d][side] < FM_MIRROR)
+            return (T)0;
+        return a.fm[d][side] == FM_MIRROR ? (T)a.fv[d][side] : (T)1;
+    };
+    const T gxl = ghost_factor(0, 0, bx == 0 && x0 == 0);
+    const T gxr = ghost_factor(0, 1, bx == nbx - 1 && x0 + VEC == NX);
+    const T gyl = ghost_factor(1, 0, by == 0 && y0 == 0);
+    const T gyh = ghost_factor(1, 1, by == nby - 1 && y0 + TY == NX);
+    // sweep two at the first / last plane of a z mirror or edge face reads
+    // its ghost u' as this factor times u' of that plane
+    const T gzl = ghost_factor(2, 0, true), gzh = ghost_factor(2, 1, true);
     // f at the held ring cells (planes of the tile only: z inside the grid)
     auto fetch_fring = [&](const ZPart &z, V &Fr) {
         if constexpr (SCALAR) {
@@ -437,7 +500,9 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
         const int gzp = bz * NZ + p;
         const bool ghost = !per2 && (gzp < 0 || gzp >= NGZ);
         if (ghost) {
-            const T c = gzp < 0 ? dz0 : dz1;
+            // (past a mirror or edge face the second sweep does not read it)
+            const int side = gzp < 0 ? 0 : 1;
+            const T c = a.fm[2][side] == FM_CONST ? (T)a.fv[2][side] : (T)0;
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -465,18 +530,33 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
                 }
                 P2[r] = res;
                 *reinterpret_cast<V *>(B + (y + 1) * PA + VEC + x0) = res;
+                // ghost u' past a mirror or edge face, for the tile's planes
+                if (MIR && j >= 1 && j <= nzt) {
+                    if (gxl != (T)0)
+                        B[(y + 1) * PA + VEC - 1] = res.v[0] * gxl;
+                    if (gxr != (T)0)
+                        B[(y + 1) * PA + VEC + NX] = res.v[VEC - 1] * gxr;
+                    if ((y == 0 && gyl != (T)0) || (y == TY - 1 && gyh != (T)0)) {
+                        const T gf = y == 0 ? gyl : gyh;
+                        V gv;
+#pragma unroll
+                        for (int v = 0; v < VEC; ++v)
+                            gv.v[v] = res.v[v] * gf;
+                        *reinterpret_cast<V *>(B + (y == 0 ? 0 : TY + 1) * PA + VEC + x0) = gv;
+                    }
+                }
             }
             // the ring, for planes of the tile
             if constexpr (SCALAR) {
                 // the ring cell of thread tid < NR1 is its halo slot 0
-                if (j >= 1 && j <= nzt && tid < NR1) {
+                if (j >= 1 && j <= nzt && tid < NR1 && !ring_skip) {
                     const int rx = shx[0], ry = shy[0];
                     const T *o = A + (ry + 2) * PA + VEC + rx;
                     B[(ry + 1) * PA + VEC + rx] =
                         ring_out ? shcv[0]
                                  : upd(H0.v[0], o[-1], o[1], o[-PA], o[PA], RZ.v[0], H1.v[0], FR.v[0], cm, cp, wd);
                 }
-            } else if (j >= 1 && j <= nzt && nring > 0) {
+            } else if (j >= 1 && j <= nzt && nring > 0 && !ring_skip) {
                 const T *o = A + (hy + 2) * PA + VEC + hx;
                 T *ob = B + (hy + 1) * PA + VEC + hx;
                 if (role == 1) {
@@ -507,6 +587,8 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
             const T wd = w / diag;
             const int q = p - 1;
             T *oq = obase + blk * bstride + SXY * q;
+            const int gq = bz * NZ + q;
+            const T fzl = gq == 0 ? gzl : (T)0, fzh = gq == NGZ - 1 ? gzh : (T)0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int y = ty + r * ROWS;
@@ -521,7 +603,9 @@ __global__ __launch_bounds__(WG) void k_jacobi2(const J2Args a)
                     const T c = P1[r].v[v];
                     const T xm = v == 0 ? xl : P1[r].v[v - 1];
                     const T xp = v == VEC - 1 ? xr : P1[r].v[v + 1];
-                    res.v[v] = upd(c, xm, xp, ym.v[v], yp.v[v], P0[r].v[v], P2[r].v[v], FQ[r].v[v], cm, cp, wd);
+                    const T zlo = (MIR && fzl != (T)0) ? c * fzl : P0[r].v[v];
+                    const T zhi = (MIR && fzh != (T)0) ? c * fzh : P2[r].v[v];
+                    res.v[v] = upd(c, xm, xp, ym.v[v], yp.v[v], zlo, zhi, FQ[r].v[v], cm, cp, wd);
                 }
                 store_nt<T, VEC>(oq + x0 + NX * (y0 + y), res);
             }
@@ -565,7 +649,14 @@ template <typename T, int NX, int TY, int WG>
 static bool j2_launch(const J2Args &a, int ntiles, hipStream_t st)
 {
     if constexpr (j2_valid<T, NX, TY, WG>()) {
-        hipLaunchKernelGGL((k_jacobi2<T, NX, TY, WG>), dim3(ntiles), dim3(WG), 0, st, a);
+        bool mir = false;
+        for (int d = 0; d < 3; ++d)
+            for (int s = 0; s < 2; ++s)
+                mir = mir || a.fm[d][s] >= FM_MIRROR;
+        if (mir)
+            hipLaunchKernelGGL((k_jacobi2<T, NX, TY, WG, true>), dim3(ntiles), dim3(WG), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_jacobi2<T, NX, TY, WG, false>), dim3(ntiles), dim3(WG), 0, st, a);
         return true;
     }
     return false;
@@ -679,29 +770,31 @@ int jacobi2_get_tune(int dtype, int edge, int *wg, int *zchunk, int *split)
     return CNV_OK;
 }
 
-// The grid's faces as the two-sweep kernel needs them (component comp):
-// every axis either periodic or with a Dirichlet constant on both faces, the
-// same on every block of a face, and no other BC on any block.  Fills per
-// and dv; false otherwise.
-bool jacobi2_faces(cnv_grid_s *g, int comp, int per[3], double dv[3][2])
+// The grid's domain faces as the two-sweep kernel needs them (component
+// comp): every block's BC list resolved for the sweep's stencil (-1, 2)
+// gives neighbour copies on every face inside the domain and, on each domain
+// face, the same rule on every block of that face: a periodic neighbour on
+// both sides of an axis, or on each side a Dirichlet constant, a Symmetry
+// mirror of nonzero sign, or an Absorbing edge copy.  Fills fm and fv;
+// false otherwise.
+bool jacobi2_faces(cnv_grid_s *g, int comp, int fm[3][2], double fv[3][2])
 {
     const int *nb = g->desc.nblocks;
     std::lock_guard<std::mutex> lk(g->mtx);
     const auto &lists = g->bcs[comp];
     const int sb[3] = {-1, -1, -1}, se[3] = {2, 2, 2};
-    bool set[3][2] = {{false, false}, {false, false}, {false, false}};
     int kind0[3][2];
+    double val0[3][2];
     for (int d = 0; d < 3; ++d)
-        for (int s = 0; s < 2; ++s)
+        for (int s = 0; s < 2; ++s) {
             kind0[d][s] = -1;
+            val0[d][s] = 0.0;
+        }
     for (long long b = 0; b < g->nblocks_total; ++b) {
         const int bi[3] = {(int)(b % nb[0]), (int)((b / nb[0]) % nb[1]), (int)(b / ((long long)nb[0] * nb[1]))};
         std::vector<DevBC> tmp;
-        for (const auto &x : lists[(size_t)b]) {
-            if (x.kind != CNV_BC_DIRICHLET && x.kind != CNV_BC_BASE)
-                return false;
+        for (const auto &x : lists[(size_t)b])
             tmp.push_back({x.kind, x.dir, x.side, 0, x.value});
-        }
         int kind[6];
         double val[6];
         resolve_faces(tmp.data(), (int)tmp.size(), sb, se, kind, val);
@@ -713,13 +806,13 @@ bool jacobi2_faces(cnv_grid_s *g, int comp, int per[3], double dv[3][2])
                     return false;
                 continue;
             }
-            if (kind[f] != FACE_NBR && kind[f] != FACE_DIRICHLET)
+            if (kind[f] == FACE_UNSET || (kind[f] == FACE_SYMMETRY && val[f] == 0.0))
                 return false;
-            if (!set[d][s]) {
-                set[d][s] = true;
+            const double v = (kind[f] == FACE_DIRICHLET || kind[f] == FACE_SYMMETRY) ? val[f] : 0.0;
+            if (kind0[d][s] < 0) {
                 kind0[d][s] = kind[f];
-                dv[d][s] = kind[f] == FACE_DIRICHLET ? val[f] : 0.0;
-            } else if (kind[f] != kind0[d][s] || (kind[f] == FACE_DIRICHLET && val[f] != dv[d][s])) {
+                val0[d][s] = v;
+            } else if (kind[f] != kind0[d][s] || v != val0[d][s]) {
                 return false;
             }
         }
@@ -728,12 +821,17 @@ bool jacobi2_faces(cnv_grid_s *g, int comp, int per[3], double dv[3][2])
         const bool p0 = kind0[d][0] == FACE_NBR, p1 = kind0[d][1] == FACE_NBR;
         if (p0 != p1)
             return false;
-        per[d] = p0 ? 1 : 0;
-        if (per[d])
-            dv[d][0] = dv[d][1] = 0.0;
+        for (int s = 0; s < 2; ++s) {
+            const int k = kind0[d][s];
+            fm[d][s] = k == FACE_NBR ? FM_PERIODIC : k == FACE_DIRICHLET ? FM_CONST : k == FACE_SYMMETRY ? FM_MIRROR
+                                                                                                        : FM_EDGE;
+            fv[d][s] = val0[d][s];
+        }
     }
-    // a periodic z wraps planes two cells past the grid with one addition
-    if (per[2] && nb[2] * g->desc.block_cells[2] < 2)
+    // a periodic or mirror z reads planes two cells past the grid from
+    // inside it with one addition
+    const int ngz = nb[2] * g->desc.block_cells[2];
+    if (ngz < 2 && (fm[2][0] == FM_PERIODIC || fm[2][0] == FM_MIRROR || fm[2][1] == FM_MIRROR))
         return false;
     return true;
 }
@@ -749,9 +847,9 @@ bool jacobi2_applies(cnv_grid_s *u, int cu, cnv_grid_s *f, cnv_grid_s *out)
         return false;
     if (u->desc.entity != CNV_CELL)
         return false;
-    int per[3];
-    double dv[3][2];
-    return jacobi2_faces(u, cu, per, dv);
+    int fm[3][2];
+    double fv[3][2];
+    return jacobi2_faces(u, cu, fm, fv);
 }
 
 int launch_jacobi2(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out, int co, double omega,
@@ -763,9 +861,9 @@ int launch_jacobi2(cnv_grid_s *u, int cu, cnv_grid_s *f, int cf, cnv_grid_s *out
     if (f->desc.dtype != u->desc.dtype || out->desc.dtype != u->desc.dtype)
         return fail(CNV_ERR_SIZE, "u, f and out must share a dtype");
     J2Args a;
-    if (!jacobi2_faces(u, cu, a.per, a.dv))
-        return fail(CNV_ERR_UNSUPPORTED,
-                    "two-sweep Jacobi needs every axis periodic or with one Dirichlet constant per domain face");
+    if (!jacobi2_faces(u, cu, a.fm, a.fv))
+        return fail(CNV_ERR_UNSUPPORTED, "two-sweep Jacobi needs every domain face periodic, Dirichlet, Symmetry "
+                                         "(nonzero sign) or Absorbing, the same on every block of the face");
     {
         std::lock_guard<std::mutex> lk(u->mtx);
         int rc = ensure_zmetric(u);

@@ -285,10 +285,11 @@ int cnv_jacobi_sweep(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int c
 int cnv_jacobi_defect(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid r, int cr, double *sumsq, cnv_stream s);
 /* No counterpart in the reference (see cnv_jacobi_sweep).  Two sweeps in
  * one launch: out = S(S(u)), S the sweep above, bit-identical to two calls
- * of cnv_jacobi_sweep, reading u and f once.  Needs every axis of u's BC
- * lists (component cu) periodic or with one Dirichlet constant per domain
- * face (the same on every block of the face, no other BC), x/y block edges
- * of 16, 32 or 64, and u, f, out of one dtype; CNV_ERR_UNSUPPORTED
+ * of cnv_jacobi_sweep, reading u and f once.  Needs u's BC lists (component
+ * cu) to give each domain face one rule on all its blocks: periodic on both
+ * sides of an axis, or per side Dirichlet, Symmetry (nonzero sign) or
+ * Absorbing (BC/Dirichlet.h, BC/Symmetry.h, BC/Absorbing.h); x/y block
+ * edges of 16, 32 or 64; u, f, out of one dtype; CNV_ERR_UNSUPPORTED
  * otherwise.  out must not alias u or f. */
 int cnv_jacobi_sweep2(cnv_grid u, int cu, cnv_grid f, int cf, cnv_grid out, int co, double omega, cnv_stream s);
 /* Tile of the two-sweep launch per (dtype, x/y block edge): workgroup size,

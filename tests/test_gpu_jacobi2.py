@@ -1,8 +1,8 @@
 """GPU parity of the two-sweep Jacobi launch (cnv_jacobi_sweep2).
 
 Two sweeps in one launch must equal two single sweeps bit for bit: checked
-against two oracle sweeps (periodic, Dirichlet-constant and mixed periodic /
-Dirichlet faces on 3 x 2 x 2 blocks whose spacings differ per direction, a
+against two oracle sweeps (periodic, Dirichlet, Symmetry and Absorbing faces
+and mixes of them on 3 x 2 x 2 blocks whose spacings differ per direction, a
 tanh-stretched z), against two GPU single sweeps for every workgroup size and
 z-planes-per-tile of each block edge (tiles split in z, ragged last tile), on
 blocks taller than wide, on axes of one block and blocks of 1 to 3 planes, and
@@ -19,20 +19,26 @@ from tests.helpers import assert_same, face_bc_lists
 
 pytestmark = pytest.mark.gpu
 
-# face BC sets as (kind, dir, side, value): 2 Dirichlet
+# face BC sets as (kind, dir, side, value): 1 Absorbing, 2 Dirichlet, 3 Symmetry (value = sign)
 BC2 = {
     "periodic": [],
     "dirichlet": [(2, d, s, 0.25 * (1 + d) * (1 if s else -1)) for d in range(3) for s in (0, 1)],
     "x_periodic": [(2, 1, 0, 0.5), (2, 1, 1, -0.75), (2, 2, 0, 1.0), (2, 2, 1, 0.0)],
     "z_periodic": [(2, 0, 0, 0.0), (2, 0, 1, 2.0), (2, 1, 0, -1.0), (2, 1, 1, 0.0)],
+    "symmetric": [(3, 0, 0, 1.0), (3, 0, 1, -1.0), (3, 1, 0, -1.0), (3, 1, 1, -1.0), (3, 2, 0, 1.0),
+                  (3, 2, 1, -1.0)],
+    "absorbing": [(1, d, s, 0.0) for d in range(3) for s in (0, 1)],
+    "mixed_all": [(3, 1, 0, -1.0), (1, 1, 1, 0.0), (2, 2, 0, 0.5), (3, 2, 1, 1.0)],
+    "z_mirror": [(2, 0, 0, 0.25), (2, 0, 1, -0.5), (1, 2, 0, 0.0), (3, 2, 1, -1.0)],
 }
+MAKE = {1: lambda d, s, v: cnv.Absorbing(d, s), 2: lambda d, s, v: cnv.Dirichlet(d, s, v),
+        3: lambda d, s, v: cnv.Symmetry(d, s, v)}
 
 
 def bgrid(nb, n, dtype, bcs, zf):
     g = cnv.Cartesian(nb, n, dtype=dtype)
     for (k, d, s, v) in bcs:
-        assert k == 2
-        g.addFaceBC(cnv.Dirichlet(d, s, v))
+        g.addFaceBC(MAKE[k](d, s, v))
     cnv.set_zfaces(g, zf)
     return g
 
@@ -76,11 +82,12 @@ def tiles2():
     return [(wg, zc, ys) for wg in (64, 128, 256, 512, 1024) for zc in (8, 16, 24, 32, 64) for ys in (1, 2, 4)]
 
 
+@pytest.mark.parametrize("bcname", ["x_periodic", "mixed_all"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("edge", [16, 32, 64])
-def test_every_sweep2_tile_matches_two_sweeps(gpu, dtype, edge):
+def test_every_sweep2_tile_matches_two_sweeps(gpu, dtype, edge, bcname):
     nb, n = (2, 2, 2), (edge,) * 3
-    bcs = BC2["x_periodic"]
+    bcs = BC2[bcname]
     u, f, out, mid, _, _, zf = setup(nb, n, dtype, bcs, 5 * edge)
     ref = bgrid(nb, n, dtype, bcs, zf)
     cnv.jacobi_sweep(u, f, mid, 2.0 / 3.0)
@@ -169,9 +176,9 @@ def test_sweep2_refuses_other_bcs(gpu):
         return x
 
     cases = [
-        lambda x: [x.addFaceBC(cnv.Absorbing(d, s)) for d in range(3) for s in (0, 1)],
-        lambda x: [x.addFaceBC(cnv.Symmetry(0, s, 1.0)) for s in (0, 1)],
-        lambda x: x.addFaceBC(cnv.Dirichlet(1, 0, 1.0)),  # one Dirichlet face, the other unwritten
+        lambda x: [x.addFaceBC(cnv.Symmetry(0, s, 0.0)) for s in (0, 1)],  # a mirror of sign 0
+        lambda x: x.addFaceBC(cnv.Dirichlet(1, 0, 1.0)),  # one Dirichlet face, the other periodic
+        lambda x: x.addFaceBC(cnv.Absorbing(2, 1)),        # one Absorbing face, the other periodic
     ]
     for add in cases:
         u, f, out = g(add), g(add), g(add)
@@ -190,6 +197,11 @@ THIN = [
     ((2, 2, 1), (16, 16, 2), "dirichlet"),
     ((2, 1, 1), (16, 16, 1), "dirichlet"),
     ((1, 2, 3), (32, 32, 3), "z_periodic"),
+    ((1, 1, 1), (16, 16, 16), "symmetric"),
+    ((2, 2, 1), (16, 16, 2), "absorbing"),
+    ((2, 1, 1), (16, 16, 1), "absorbing"),
+    ((1, 3, 2), (16, 16, 8), "mixed_all"),
+    ((2, 2, 3), (32, 32, 2), "z_mirror"),
 ]
 
 
