@@ -6,8 +6,9 @@ and mixes of them on 3 x 2 x 2 blocks whose spacings differ per direction, a
 tanh-stretched z), against two GPU single sweeps for every workgroup size and
 z-planes-per-tile of each block edge (tiles split in z, ragged last tile), on
 blocks taller than wide, on axes of one block and blocks of 1 to 3 planes, and
-on the 512^3 grid of 64^3 blocks.  BC lists the launch does not cover are
-refused."""
+on the 512^3 grid of 64^3 blocks.  One cycle of the solver (refined and
+all-FP64) equals the same cycle done with single sweeps.  BC lists the launch
+does not cover are refused."""
 import numpy as np
 import pytest
 
@@ -238,3 +239,45 @@ def test_sweep2_refuses_single_plane_periodic_z(gpu):
     u, f, out = (bgrid(nb, n, np.float32, [], zf) for _ in range(3))
     with pytest.raises(cnv.CnvError):
         cnv.jacobi_sweep2(u, f, out, 2.0 / 3.0)
+
+
+@pytest.mark.parametrize("refine", [False, True], ids=["fp64", "refined"])
+@pytest.mark.parametrize("S", [2, 4, 6, 10])
+@pytest.mark.parametrize("bcs", ["dirichlet", "mixed_all"])
+def test_solver_cycle_equals_single_sweeps(gpu, refine, S, bcs):
+    """One cycle of cnv_jacobi_solve, whose sweeps run two per launch (and,
+    in the all-FP64 solve, two single sweeps when S / 2 is odd), leaves u
+    bit-identical to the same cycle done with single sweeps through the
+    public entry points."""
+    nb, n = (2, 2, 2), (16, 16, 16)
+    N = 32
+    zf = stretch.tanh_faces(N, 2.0)
+    rng = np.random.Generator(np.random.PCG64(S + 7 * refine))
+    u0 = rng.standard_normal((N, N, N))
+    f0 = rng.standard_normal((N, N, N))
+    hom = [(k, d, s, 0.0 if k == 2 else v) for (k, d, s, v) in BC2[bcs]]
+    u, f, uref = (bgrid(nb, n, np.float64, BC2[bcs], zf) for _ in range(3))
+    u.upload_global(0, u0)
+    uref.upload_global(0, u0)
+    f.upload_global(0, f0)
+    if refine:
+        work = cnv.Cartesian(nb, n, dtype=np.float32, ncomp=3)
+        cnv.set_zfaces(work, zf)
+        # e (comps 0, 1) with the homogeneous BCs, r (comp 2)
+        wref = bgrid(nb, n, np.float32, hom, zf)
+        e = [bgrid(nb, n, np.float32, hom, zf) for _ in range(2)]
+        cnv.jacobi_defect(uref, f, wref)
+        clear(e[0])
+        for k in range(S):
+            cnv.jacobi_sweep(e[k % 2], wref, e[1 - k % 2], 2.0 / 3.0)
+        cnv.add_correction(uref, e[S % 2])
+    else:
+        work = cnv.Cartesian(nb, n, dtype=np.float64, ncomp=2)
+        cnv.set_zfaces(work, zf)
+        w = bgrid(nb, n, np.float64, BC2[bcs], zf)
+        for k in range(S):
+            a, b = (uref, w) if k % 2 == 0 else (w, uref)
+            cnv.jacobi_sweep(a, f, b, 2.0 / 3.0)
+    cyc, sw, rel, _ = cnv.jacobi_solve(u, f, work, refine=refine, sweeps_per_cycle=S, max_cycles=1, tol=0.0)
+    assert cyc == 1 and sw == S
+    assert_same(u.download(0), uref.download(0), "solver cycle of %d sweeps" % S)
